@@ -1,0 +1,22 @@
+"""CPU oracle for the per-RK-stage update of FVHOE (test infrastructure only).
+
+This package is the checker, never the product: only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it.  It restates, op for op and in the same floating-point order, the
+reference's NumPy path (citations are ``file:line`` into jpalafou/FVHOE).
+
+Pinning (what checks each function besides the restatement itself):
+
+* stencil weights          -- exact-rational moment conditions (tests/test_oracle_closed_forms.py)
+* stencil sweeps, centers, averages -- polynomial exactness + brute force + golden stages
+* primitives/conservatives -- round trip + golden stages
+* sound speed / hydro_dt   -- closed form + golden stages
+* llf                      -- Toro's textbook form (consistency, upwind limit) + golden stages
+* hllc                     -- consistency, supersonic limits, mirror symmetry + golden stages
+* MUSCL/minmod/moncen      -- brute force on tiny inputs + golden force_trouble stages
+* detect_troubled_cells, SED, broadcast -- golden vectors written by tests/golden/make_golden.py
+* boundary conditions      -- brute-force index maps + golden stages
+* whole stage / SSP-RK3 trajectories -- golden vectors written by tests/golden/make_golden.py
+"""
+
+from oracle.fvhoe_oracle import *  # noqa: F401,F403

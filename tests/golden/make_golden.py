@@ -1,0 +1,268 @@
+"""Write the golden fixtures under tests/golden/ from the reference itself.
+
+Run in the build container only (the reference is not available elsewhere):
+
+    python tests/golden/make_golden.py
+
+It imports jpalafou/FVHOE from /root/reference and records, for small
+seeded inputs, what the reference computes for the functions that no closed
+form or invariant pins: whole stages (`EulerSolver.hydrodynamics`, which
+composes hydrofluxes + revise_fluxes + euler_equation), SSP-RK trajectories,
+`detect_troubled_cells`, the smooth-extrema detectors and the convex face
+broadcast.  Only inputs and outputs are written (compressed .npz).
+"""
+
+from __future__ import annotations
+
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, REPO)
+sys.path.insert(0, "/root/reference")
+
+from fvhoe.boundary_conditions import BoundaryCondition  # noqa: E402
+from fvhoe.initial_conditions import InitialCondition as RefIC  # noqa: E402
+from fvhoe.slope_limiting import broadcast_to_troubled_interfaces, detect_troubled_cells  # noqa: E402
+from fvhoe.smooth_extrema_detection import (  # noqa: E402
+    compute_1d_smooth_extrema_detector,
+    compute_2d_smooth_extrema_detector,
+    compute_3d_smooth_extrema_detector,
+)
+from fvhoe.solver import EulerSolver  # noqa: E402
+
+from fvhoe_amd import initial_conditions as ics  # noqa: E402
+
+sys.path.insert(0, HERE)
+
+OUT = HERE
+
+
+class ArrayIC(RefIC):
+    """Hands a fixed conservative fv-average array to the reference solver."""
+
+    def base_ic(self, arr):
+        return lambda x, y, z: arr.copy()
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def make_solver(u0, n, p, bc, **kw):
+    kw.setdefault("gamma", 1.4)
+    return EulerSolver(
+        w0=ArrayIC(u0), nx=n[0], ny=n[1], nz=n[2], px=p[0], py=p[1], pz=p[2],
+        bc=BoundaryCondition(x=bc[0], y=bc[1], z=bc[2]), conservative_ic=True, fv_ic=True,
+        progress_bar=False, **kw)
+
+
+def capture_troubles(solver, sink: list):
+    def log(trouble=None, NAD_violation_magnitude=None, PAD_violation_magnitude=None, reset=False):
+        if not reset:
+            sink.append((np.asarray(trouble).astype(np.int8), np.asarray(NAD_violation_magnitude),
+                         np.asarray(PAD_violation_magnitude)))
+    solver.log_troubles = log
+
+
+def rough_field(rng, shape, nd):
+    """Smooth positive field plus a few jumps: exercises NAD, SED and PAD."""
+    grids = np.meshgrid(*[np.linspace(0, 1, s, endpoint=False) for s in shape], indexing="ij")
+    r = sum(g * (i + 1.3) for i, g in enumerate(grids[:nd]))
+    base = 1.0 + 0.3 * np.sin(2 * np.pi * r + rng.uniform(0, 6.28))
+    jump = (grids[0] > 0.55).astype(float) * rng.uniform(0.3, 1.0)
+    return base + jump + 0.01 * rng.standard_normal(shape)
+
+
+def random_state(rng, n, nd, shock=True):
+    """Conservative averages of a random smooth state with a pressure/density jump."""
+    w = np.empty((5,) + tuple(n))
+    w[0] = rough_field(rng, n, nd) if shock else 1.0 + 0.2 * rng.standard_normal(n) ** 2
+    w[1] = 0.3 * np.sin(np.linspace(0, 6.28, w[1].size)).reshape(n) + 0.05 * rng.standard_normal(n)
+    w[2] = 0.2 * rng.standard_normal(n) if nd > 1 else 0.05 * rng.standard_normal(n)
+    w[3] = 0.2 * rng.standard_normal(n) if nd > 2 else 0.0
+    w[4] = rough_field(rng, n, nd) * 1.5 if shock else 1.0
+    return ics.primitives_to_conservatives(w, 1.4)
+
+
+from cases import STAGE_CASES, build_stage_cases  # noqa: E402
+
+
+def run_stage_cases():
+    out = {}
+    for name, n, p, bc, warm, seed, u0, kw in STAGE_CASES:
+        rng = np.random.default_rng(seed)
+        nd = sum(1 for k in n if k > 1)
+        if u0 is None:
+            shape = tuple(k for k in n)
+            u0 = random_state(rng, shape, nd)
+        s = make_solver(u0, n, p, bc, **kw)
+        if warm:
+            s.ssprk3(n=warm)
+        u_in = np.array(s.am("u"))
+        logs = []
+        if kw.get("a_posteriori_slope_limiting"):
+            capture_troubles(s, logs)
+        dt, dudt = s.hydrodynamics(u_in.copy())
+        rec = dict(u=u_in, dt=np.float64(dt), dudt=np.asarray(dudt))
+        if logs:
+            rec["trouble"], rec["nad_mag"], rec["pad_mag"] = logs[0]
+        out[name] = rec
+        print(f"stage {name}: dt={dt:.6g} trouble={int(logs[0][0].sum()) if logs else '-'}", flush=True)
+    np.savez_compressed(os.path.join(OUT, "stages.npz"),
+                        **{f"{k}/{f}": v for k, rec in out.items() for f, v in rec.items()})
+
+
+# --------------------------------------------------------------- trajectories
+
+
+def run_trajectories():
+    P = ("periodic", "periodic")
+    O = ("outflow", "outflow")
+    rec = {}
+
+    # Sod N=256, p=3, HLLC, SSP-RK3, a-posteriori, outflow, to t=0.245
+    u0 = ics.sod_shock_tube(256, 3)
+    s = make_solver(u0, (256, 1, 1), (3, 0, 0), (O, P, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    logs = []
+    capture_troubles(s, logs)
+    s.ssprk3(0.245)
+    rec["sod/u0"] = u0
+    rec["sod/u_final"] = np.array(s.am("u"))
+    rec["sod/w_final"] = s.snapshots[-1]["w"]
+    rec["sod/t"] = np.float64(s.t)
+    rec["sod/timestamps"] = np.array(s.timestamps)
+    rec["sod/trouble_counts"] = np.array([int(t[0].sum()) for t in logs])
+    print(f"sod: steps={s.step_count} t={s.t}", flush=True)
+
+    # Kelvin-Helmholtz 256^2 periodic, 3 steps: packed trouble bitmaps per stage
+    n = 256
+    u0 = ics.kelvin_helmholtz(n, seed=0)
+    s = make_solver(u0, (n, n, 1), (3, 3, 0), (P, P, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    logs = []
+    capture_troubles(s, logs)
+    s.ssprk3(n=3)
+    uf = np.array(s.am("u"))
+    rec["kh256/u0_sha"] = np.array(sha(u0))
+    rec["kh256/u_final_sha"] = np.array(sha(uf))
+    rec["kh256/u_final_rows"] = uf[:, ::37, :, 0]
+    rec["kh256/u_final_sum"] = uf.sum(axis=(1, 2, 3))
+    rec["kh256/u_final_abssum"] = np.abs(uf).sum(axis=(1, 2, 3))
+    rec["kh256/timestamps"] = np.array(s.timestamps)
+    rec["kh256/trouble_bits"] = np.stack([np.packbits(t[0].astype(bool).ravel()) for t in logs])
+    print(f"kh256: t={s.t} troubles={[int(t[0].sum()) for t in logs]}", flush=True)
+
+    # 3-D density wave 16^3, p=7, LLF, no limiter, 2 steps
+    u0 = ics.density_wave_3d(16, 7)
+    s = make_solver(u0, (16, 16, 16), (7, 7, 7), (P, P, P), CFL=0.8, riemann_solver="llf")
+    s.ssprk3(n=2)
+    rec["wave3d/u0"] = u0
+    rec["wave3d/u_final"] = np.array(s.am("u"))
+    rec["wave3d/timestamps"] = np.array(s.timestamps)
+
+    # four quadrants 64^2, outflow, p=3, HLLC, a-posteriori, 3 steps
+    u0 = ics.four_quadrants(64, seed=200)
+    s = make_solver(u0, (64, 64, 1), (3, 3, 0), (O, O, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    logs = []
+    capture_troubles(s, logs)
+    s.ssprk3(n=3)
+    rec["quad64/u0"] = u0
+    rec["quad64/u_final"] = np.array(s.am("u"))
+    rec["quad64/timestamps"] = np.array(s.timestamps)
+    rec["quad64/trouble_bits"] = np.stack([np.packbits(t[0].astype(bool).ravel()) for t in logs])
+
+    # Shu-Osher-type instance N=512, p=5, outflow, 6 steps
+    u0 = ics.shu_osher_instance(512, 0.2, 5.0)
+    s = make_solver(u0, (512, 1, 1), (5, 0, 0), (O, P, P), x=(-5, 5), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    s.ssprk3(n=6)
+    rec["shuosher/u0"] = u0
+    rec["shuosher/u_final"] = np.array(s.am("u"))
+    rec["shuosher/timestamps"] = np.array(s.timestamps)
+
+    # the other steppers on a small 1-D problem
+    for integ in ("euler", "ssprk2", "rk4"):
+        u0 = ics.sod_shock_tube(64, 2)
+        s = make_solver(u0, (64, 1, 1), (2, 0, 0), (O, P, P), riemann_solver="hllc",
+                        a_posteriori_slope_limiting=True)
+        getattr(s, integ)(n=4)
+        rec[f"{integ}/u0"] = u0
+        rec[f"{integ}/u_final"] = np.array(s.am("u"))
+        rec[f"{integ}/timestamps"] = np.array(s.timestamps)
+    np.savez_compressed(os.path.join(OUT, "trajectories.npz"), **rec)
+
+
+# ----------------------------------------------------------- detection pieces
+
+
+def run_detection():
+    rec = {}
+    rng = np.random.default_rng(1234)
+    shapes = {"x": (46, 1, 1), "xy": (22, 19, 1), "xyz": (12, 11, 10)}
+    for dims, shp in shapes.items():
+        nd = len(dims)
+        w = np.stack([rough_field(rng, shp, nd) for _ in range(5)])
+        w[1:4] -= 1.2
+        ws = w + 0.02 * rng.standard_normal(w.shape)
+        ws[:, ws.shape[1] // 3] += 0.3  # a plateau jump
+        ws[4, 5] = -0.1  # a negative pressure for PAD
+        rec[f"in_{dims}/w"] = w
+        rec[f"in_{dims}/ws"] = ws
+    i = 0
+    for dims in ("x", "xy", "xyz"):
+        w, ws = rec[f"in_{dims}/w"], rec[f"in_{dims}/ws"]
+        for mode in ("global", "local"):
+            for rt in ("relative", "absolute"):
+                for nv in (None, ("rho", "P")):
+                    for pad in (None, {"rho": (0.0, np.inf), "P": (0.0, np.inf)}, {"P": (0.5, 2.0), "rho": (0.8, 3.0)}):
+                        for sed in (False, True):
+                            eps = (1e-2, 1e-5, 0.1)[i % 3]
+                            t, nm, pm = detect_troubled_cells(
+                                u=w, u_candidate=ws, dims=dims, NAD_eps=eps, mode=mode, range_type=rt,
+                                NAD_vars=nv, PAD_bounds=pad, SED=sed, SED_eps=1e-10)
+                            key = f"d{i}"
+                            rec[f"{key}/trouble"] = np.asarray(t).astype(np.int8)
+                            rec[f"{key}/nad_mag"] = np.asarray(nm)
+                            rec[f"{key}/pad_mag"] = np.asarray(pm)
+                            rec[f"{key}/meta"] = np.array(repr(dict(
+                                dims=dims, mode=mode, range_type=rt, NAD_vars=nv, PAD=pad, SED=sed, eps=eps)))
+                            i += 1
+    # smooth-extrema detectors, with flat patches (dv == 0 branch)
+    for dims, shp in (("x", (40, 1, 1)), ("xy", (20, 18, 1)), ("xz", (18, 1, 16)), ("xyz", (12, 11, 10))):
+        nd = len(dims)
+        u = np.stack([rough_field(rng, shp, nd) for _ in range(5)])
+        u[:, :6] = 1.0
+        if nd == 1:
+            a = compute_1d_smooth_extrema_detector(u, dim=dims, eps=1e-10)
+        elif nd == 2:
+            a = compute_2d_smooth_extrema_detector(u, dims=dims, eps=1e-10)
+        else:
+            a = compute_3d_smooth_extrema_detector(u, eps=1e-10)
+        rec[f"sed_{dims}/u"] = u
+        rec[f"sed_{dims}/alpha"] = a
+    # convex broadcast (1-D and 2-D, periodic and not)
+    for dims, shp in (("x", (30, 1, 1)), ("xy", (14, 12, 1))):
+        t = (rng.uniform(size=shp) < 0.12).astype(np.float64)
+        t[0] = 1.0
+        for per in (False, True):
+            tx, ty, tz = broadcast_to_troubled_interfaces(t, dims=dims, convex=True, periodic_x=per,
+                                                          periodic_y=per, periodic_z=per)
+            k = f"convex_{dims}_{int(per)}"
+            rec[f"{k}/trouble"] = t
+            rec[f"{k}/tx"], rec[f"{k}/ty"], rec[f"{k}/tz"] = tx, ty, tz
+    np.savez_compressed(os.path.join(OUT, "detection.npz"), **rec)
+    print(f"detection: {i} combos", flush=True)
+
+
+if __name__ == "__main__":
+    build_stage_cases()
+    run_stage_cases()
+    run_detection()
+    run_trajectories()
