@@ -1,0 +1,169 @@
+"""ctypes binding of libfvhoe_hip.so (the C-ABI in include/fvhoe_hip.h).
+
+There is no fallback: if the library is missing, or no GPU is visible when a
+kernel is called, this module raises.  torch provides device memory and the
+current stream only.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_longlong, c_ulonglong, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfvhoe_hip.so")
+
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH = 0, 1, 2, 3
+STENCIL = {"l": 0, "c": 1, "r": 2, "t": 3}
+RIEMANN = {"advection_upwind": 0, "llf": 1, "hllc": 2}
+LIMITER = {None: 0, "minmod": 1, "moncen": 2}
+BC = {"periodic": 0, "outflow": 1, "reflective": 2, "free": 3}
+RK_AXPY, RK_SSPRK3_STAGE3, RK_SSPRK3_FINAL, RK_SSPRK2_FINAL, RK_HALF_AXPY, RK_RK4_FINAL = range(6)
+
+
+class FvhoeStageConfig(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int * 3), ("p", c_int * 3), ("h", c_double * 3),
+        ("gamma", c_double), ("cfl", c_double), ("csq_floor", c_double),
+        ("bc", c_int * 6), ("riemann", c_int), ("a_posteriori", c_int), ("limiter", c_int),
+        ("force_trouble", c_int), ("nad_eps", c_double), ("nad_mode", c_int), ("nad_range", c_int),
+        ("nad_vars", c_int), ("pad_var", c_int), ("pad_lo", c_double), ("pad_hi", c_double),
+        ("sed", c_int), ("sed_eps", c_double), ("fixed_dt_on", c_int), ("fixed_dt", c_double),
+    ]
+
+
+_P = c_void_p
+_SIGNATURES = {
+    "fvhoe_version": (c_int, []),
+    "fvhoe_stencil_weights": (c_int, [c_int, c_int, POINTER(c_double), POINTER(c_int)]),
+    "fvhoe_stencil_sweep": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
+    "fvhoe_compute_primitives": (c_int, [_P, _P, c_int, c_longlong, c_double, _P]),
+    "fvhoe_compute_conservatives": (c_int, [_P, _P, c_int, c_longlong, c_double, _P]),
+    "fvhoe_compute_sound_speed": (c_int, [_P, _P, c_longlong, c_double, c_double, _P]),
+    "fvhoe_hydro_dt": (c_int, [_P, c_longlong, c_double, c_int, c_double, c_double, c_double, _P, _P, _P]),
+    "fvhoe_riemann": (c_int, [_P, _P, _P, c_int, c_longlong, c_int, c_int, c_double, c_double, _P]),
+    "fvhoe_riemann_faces": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_double, c_double, _P]),
+    "fvhoe_muscl": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
+    "fvhoe_bc_apply": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, POINTER(c_int), _P]),
+    "fvhoe_euler_equation": (c_int, [_P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_double, c_double,
+                                     c_double, _P]),
+    "fvhoe_minmax": (c_int, [_P, c_int, c_longlong, _P, _P, _P]),
+    "fvhoe_detect_troubled_cells": (c_int, [_P, _P, c_int, c_int, c_int, c_int, _P, c_int, c_int, c_double,
+                                            c_int, c_int, c_double, c_double, c_int, c_double, _P, _P, _P, _P]),
+    "fvhoe_smooth_extrema_detector": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_double, _P]),
+    "fvhoe_broadcast_simple": (c_int, [_P, _P, _P, _P, c_int, c_int, c_int, c_int, _P]),
+    "fvhoe_shift_max": (c_int, [_P, POINTER(c_longlong), c_longlong, _P, POINTER(c_longlong), c_longlong,
+                                c_double, c_int, _P, POINTER(c_longlong), c_longlong, POINTER(c_int), _P]),
+    "fvhoe_blend": (c_int, [_P, _P, _P, c_int, c_longlong, _P]),
+    "fvhoe_rk_combine": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_longlong, _P]),
+    "fvhoe_set_scalar": (c_int, [_P, c_double, _P]),
+    "fvhoe_hit_target": (c_int, [_P, _P, _P, c_double, c_int, _P]),
+    "fvhoe_advance_time": (c_int, [_P, _P, _P]),
+    "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
+    "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
+    "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m fvhoe_amd.csrc.build` "
+                "(or __graft_entry__.build()); there is no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGNATURES)
+
+
+def check(status: int, what: str):
+    if status == OK:
+        return
+    if status == ERR_ARG:
+        raise ValueError(f"{what}: invalid argument")
+    if status == ERR_UNSUPPORTED:
+        raise NotImplementedError(f"{what}: unsupported configuration")
+    raise RuntimeError(f"{what}: kernel launch failed")
+
+
+def call(name: str, *args):
+    check(getattr(lib(), name)(*args), name)
+
+
+# ---------------------------------------------------------------- torch glue
+
+_torch = None
+
+
+def torch():
+    global _torch
+    if _torch is None:
+        import torch as t
+
+        _torch = t
+    return _torch
+
+
+def device():
+    t = torch()
+    if not t.cuda.is_available():
+        raise RuntimeError("fvhoe_amd needs a ROCm GPU: torch.cuda.is_available() is False")
+    return t.device("cuda", t.cuda.current_device())
+
+
+def stream():
+    return c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> c_void_p:
+    return c_void_p(t.data_ptr()) if t is not None else c_void_p(0)
+
+
+def as_device(x, dtype=None):
+    """(device tensor, was_numpy): contiguous float64 on the GPU."""
+    t = torch()
+    dtype = dtype or t.float64
+    if isinstance(x, np.ndarray):
+        return t.from_numpy(np.ascontiguousarray(x)).to(device=device(), dtype=dtype), True
+    if not isinstance(x, t.Tensor):
+        x = t.as_tensor(np.asarray(x))
+        return x.to(device=device(), dtype=dtype).contiguous(), True
+    if x.device.type != "cuda":
+        return x.to(device=device(), dtype=dtype).contiguous(), True
+    return x.to(dtype=dtype).contiguous(), False
+
+
+def empty(shape, dtype=None):
+    t = torch()
+    return t.empty(tuple(int(s) for s in shape), dtype=dtype or t.float64, device=device())
+
+
+def zeros(shape, dtype=None):
+    t = torch()
+    return t.zeros(tuple(int(s) for s in shape), dtype=dtype or t.float64, device=device())
+
+
+def back(x, was_numpy: bool):
+    return x.cpu().numpy() if was_numpy else x
+
+
+def shape4(x):
+    s = tuple(x.shape)
+    if len(s) != 4:
+        raise ValueError(f"expected a 4-D (nvars, nx, ny, nz) array, got shape {s}")
+    return s

@@ -1,0 +1,709 @@
+// Elementary kernels: one HIP kernel per reference array function, behind
+// the C-ABI declared in include/fvhoe_hip.h.  Arrays are contiguous
+// (nvars, n0, n1, n2) float64 in the reference's layout.
+#include "fvhoe_common.h"
+#include "fvhoe_hip.h"
+
+using namespace fvhoe;
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline int grid_for(long long n) {
+  long long b = (n + TPB - 1) / TPB;
+  if (b > 65536) b = 65536;
+  return (int)(b < 1 ? 1 : b);
+}
+
+inline int launch_status() { return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH; }
+
+struct Weights9 {
+  double w[9];
+  int k;
+};
+
+__global__ void k_sweep(const double* __restrict__ u, double* __restrict__ out, int nv, int n0, int n1, int n2,
+                        int axis, Weights9 w) {
+  const int m0 = axis == 1 ? n0 - (w.k - 1) : n0;
+  const int m1 = axis == 2 ? n1 - (w.k - 1) : n1;
+  const int m2 = axis == 3 ? n2 - (w.k - 1) : n2;
+  const long long total = (long long)nv * m0 * m1 * m2;
+  const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long r = idx;
+    const int k = r % m2; r /= m2;
+    const int j = r % m1; r /= m1;
+    const int i = r % m0;
+    const int v = (int)(r / m0);
+    const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
+    double acc = 0.0;
+    for (int t = 0; t < w.k; ++t)
+      if (w.w[t] != 0.0) acc = acc + w.w[t] * u[base + t * s];
+    out[idx] = acc;
+  }
+}
+
+__global__ void k_prims(const double* __restrict__ u, double* __restrict__ w, int nv, long long nc, double gamma) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+    double uu[5], ww[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) uu[q] = u[q * nc + c];
+    prims_from_cons(uu, ww, gamma);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[q * nc + c] = ww[q];
+    for (int q = 5; q < nv; ++q) w[q * nc + c] = u[q * nc + c] / uu[0];
+  }
+}
+
+__global__ void k_cons(const double* __restrict__ w, double* __restrict__ u, int nv, long long nc, double gamma) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+    double uu[5], ww[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) ww[q] = w[q * nc + c];
+    cons_from_prims(ww, uu, gamma);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) u[q * nc + c] = uu[q];
+    for (int q = 5; q < nv; ++q) u[q * nc + c] = ww[0] * w[q * nc + c];
+  }
+}
+
+__global__ void k_sound(const double* __restrict__ w, double* __restrict__ c, long long nc, double gamma,
+                        double csq_floor) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x)
+    c[i] = sound_speed(w[i], w[4 * nc + i], gamma, csq_floor);
+}
+
+__global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// block max of a non-negative double, one atomic per block (on the raw bits:
+// the IEEE order of non-negative doubles is the unsigned order of their bits)
+__device__ __forceinline__ void block_atomic_max_pos(double x, unsigned long long* acc) {
+  __shared__ double red[TPB / 64];
+  for (int off = 32; off > 0; off >>= 1) x = dmax(x, __shfl_down(x, off, 64));
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) red[wave] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = dmax(m, red[i]);
+    atomicMax(acc, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+__global__ void k_dt_partial(const double* __restrict__ w, long long nc, int ndim, double gamma, double csq_floor,
+                             unsigned long long* acc) {
+  double m = 0.0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
+    const double c = sound_speed(w[i], w[4 * nc + i], gamma, csq_floor);
+    const double s = (fabs(w[nc + i]) + fabs(w[2 * nc + i])) + fabs(w[3 * nc + i]);
+    m = dmax(m, s + (double)ndim * c);
+  }
+  block_atomic_max_pos(m, acc);
+}
+
+__global__ void k_dt_final(const unsigned long long* acc, double cfl_h, double* dt) {
+  dt[0] = cfl_h / __longlong_as_double((long long)acc[0]);
+}
+
+template <int D>
+__global__ void k_riemann_pt(const double* __restrict__ wl, const double* __restrict__ wr, double* __restrict__ out,
+                             int nv, long long nc, int solver, double gamma, double csq_floor) {
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+    double a[5], b[5], F[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) { a[q] = wl[q * nc + c]; b[q] = wr[q * nc + c]; }
+    double vg = 0.0;
+    riemann<D>(solver, a, b, F, gamma, csq_floor, &vg);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) out[q * nc + c] = F[q];
+    if (nv > 5) {
+      const double rv = F[0];
+      if (solver == RS_HLLC) {  // passives upwinded by the Godunov velocity (fvhoe/riemann_solvers.py:186-195)
+        const double v = vg;
+        for (int q = 5; q < nv; ++q) {
+          const double up = v > 0.0 ? wl[q * nc + c] : (v < 0.0 ? wr[q * nc + c] : 0.0);
+          out[q * nc + c] = rv * up;
+        }
+      } else {
+        // llf / upwind on passives: conservative rho*s, flux v*rho*s
+        const double vl = a[D + 1], vr = b[D + 1];
+        for (int q = 5; q < nv; ++q) {
+          const double ul = a[0] * wl[q * nc + c], ur = b[0] * wr[q * nc + c];
+          const double fl = vl * ul, fr = vr * ur;
+          if (solver == RS_LLF) {
+            const double sl = fabs(vl) + sound_speed(a[0], a[4], gamma, csq_floor);
+            const double sr = fabs(vr) + sound_speed(b[0], b[4], gamma, csq_floor);
+            out[q * nc + c] = 0.5 * (fl + fr) - 0.5 * dmax(sl, sr) * (ur - ul);
+          } else {
+            out[q * nc + c] = vl > 0.0 ? fl : (vl < 0.0 ? fr : 0.0);
+          }
+        }
+      }
+    }
+  }
+}
+
+// out[f] = RS(wr_face[f], wl_face[f + e_axis]); out shape = face arrays' shape with axis shortened by 1
+template <int D>
+__global__ void k_riemann_faces(const double* __restrict__ wr_face, const double* __restrict__ wl_face,
+                                double* __restrict__ out, int n0, int n1, int n2, int axis, int solver, double gamma,
+                                double csq_floor) {
+  const int m0 = axis == 1 ? n0 - 1 : n0, m1 = axis == 2 ? n1 - 1 : n1, m2 = axis == 3 ? n2 - 1 : n2;
+  const long long nin = (long long)n0 * n1 * n2, nout = (long long)m0 * m1 * m2;
+  const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nout;
+       c += (long long)gridDim.x * blockDim.x) {
+    long long r = c;
+    const int k = r % m2; r /= m2;
+    const int j = r % m1;
+    const int i = (int)(r / m1);
+    const long long base = ((long long)i * n1 + j) * n2 + k;
+    double a[5], b[5], F[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) { a[q] = wr_face[q * nin + base]; b[q] = wl_face[q * nin + base + s]; }
+    riemann<D>(solver, a, b, F, gamma, csq_floor);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) out[q * nout + c] = F[q];
+  }
+}
+
+__global__ void k_muscl(const double* __restrict__ u, double* __restrict__ left, double* __restrict__ right, int nv,
+                        int n0, int n1, int n2, int axis, int limiter) {
+  const int m0 = axis == 1 ? n0 - 2 : n0, m1 = axis == 2 ? n1 - 2 : n1, m2 = axis == 3 ? n2 - 2 : n2;
+  const long long total = (long long)nv * m0 * m1 * m2;
+  const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long r = idx;
+    const int k = r % m2; r /= m2;
+    const int j = r % m1; r /= m1;
+    const int i = r % m0;
+    const int v = (int)(r / m0);
+    const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
+    const double u0 = u[base], u1 = u[base + s], u2 = u[base + 2 * s];
+    const double sl = limited_slope(limiter, u1 - u0, u2 - u1);
+    left[idx] = u1 - 0.5 * sl;
+    right[idx] = u1 + 0.5 * sl;
+  }
+}
+
+struct BcArgs {
+  int n[3], g[3], lo[3], hi[3];
+};
+
+__global__ void k_bc_apply(const double* __restrict__ u, double* __restrict__ out, int nv, BcArgs a) {
+  const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
+  const long long total = (long long)nv * N0 * N1 * N2;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long r = idx;
+    const int k = r % N2; r /= N2;
+    const int j = r % N1; r /= N1;
+    const int i = r % N0;
+    const int v = (int)(r / N0);
+    bool f0, f1, f2;
+    const int si = bc_source(i, a.n[0], a.g[0], a.lo[0], a.hi[0], &f0);
+    const int sj = bc_source(j, a.n[1], a.g[1], a.lo[1], a.hi[1], &f1);
+    const int sk = bc_source(k, a.n[2], a.g[2], a.lo[2], a.hi[2], &f2);
+    double x = u[(((long long)v * a.n[0] + si) * a.n[1] + sj) * a.n[2] + sk];
+    if ((f0 && v == 1) || (f1 && v == 2) || (f2 && v == 3)) x = -x;
+    out[idx] = x;
+  }
+}
+
+// dudt = 0.0 + cx*(F[i+1]-F[i]) + cy*(G[j+1]-G[j]) + cz*(H[k+1]-H[k]), c = -(1/h)  (fvhoe/solver.py:744-750)
+__global__ void k_euler_eq(const double* __restrict__ F, const double* __restrict__ G, const double* __restrict__ H,
+                           double* __restrict__ dudt, int nv, int n0, int n1, int n2, int active, double cx,
+                           double cy, double cz) {
+  const long long total = (long long)nv * n0 * n1 * n2;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long r = idx;
+    const int k = r % n2; r /= n2;
+    const int j = r % n1; r /= n1;
+    const int i = r % n0;
+    const int v = (int)(r / n0);
+    double acc = 0.0;
+    if (active & 1) {
+      const long long b = (((long long)v * (n0 + 1) + i) * n1 + j) * n2 + k;
+      acc = acc + cx * (F[b + (long long)n1 * n2] - F[b]);
+    }
+    if (active & 2) {
+      const long long b = (((long long)v * n0 + i) * (n1 + 1) + j) * n2 + k;
+      acc = acc + cy * (G[b + n2] - G[b]);
+    }
+    if (active & 4) {
+      const long long b = (((long long)v * n0 + i) * n1 + j) * (n2 + 1) + k;
+      acc = acc + cz * (H[b + 1] - H[b]);
+    }
+    dudt[idx] = acc;
+  }
+}
+
+__global__ void k_init_minmax(unsigned long long* acc, int nv) {
+  const int i = threadIdx.x;
+  if (i < 2 * nv) acc[i] = (i & 1) ? 0ull : ~0ull;  // (min, max) pairs
+}
+
+__global__ void k_minmax_partial(const double* __restrict__ u, long long nc, unsigned long long* acc) {
+  const int v = blockIdx.y;
+  unsigned long long mx = 0ull, mn = ~0ull;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
+    const unsigned long long b = ordered_bits(u[v * nc + i]);
+    mx = b > mx ? b : mx;
+    mn = b < mn ? b : mn;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long a = __shfl_down(mx, off, 64), c = __shfl_down(mn, off, 64);
+    mx = a > mx ? a : mx;
+    mn = c < mn ? c : mn;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(&acc[2 * v + 1], mx);
+    atomicMin(&acc[2 * v], mn);
+  }
+}
+
+__global__ void k_minmax_final(const unsigned long long* acc, double* out, int nv) {
+  const int i = threadIdx.x;
+  if (i < 2 * nv) out[i] = from_ordered_bits(acc[i]);
+}
+
+struct DetectArgs {
+  int N[3];        // padded extents
+  int dims;        // bit d set: axis d active (3 ghosts)
+  int mode;        // 0 global, 1 local
+  int range_type;  // 0 relative, 1 absolute
+  int nad_vars;    // bit mask over the 5 variables
+  int pad_var;     // -1: no PAD
+  int sed;
+  double eps, pad_lo, pad_hi, sed_eps;
+};
+
+__device__ __forceinline__ double sed_alpha_cell(const double* __restrict__ x, long long base, const long long st[3],
+                                                 int dims, double eps) {
+  double alpha = 0.0;
+  bool first = true;
+  for (int d = 0; d < 3; ++d) {
+    if (!(dims & (1 << d))) continue;
+    const long long s = st[d];
+    const double a = sed_alpha_1d([&](int o) { return x[base + o * s]; }, eps);
+    alpha = first ? a : dmin(alpha, a);
+    first = false;
+  }
+  return alpha;
+}
+
+// NAD / PAD / SED on one interior cell (fvhoe/slope_limiting.py:125-244)
+__global__ void k_detect(const double* __restrict__ w, const double* __restrict__ ws,
+                         const double* __restrict__ range_mm, signed char* __restrict__ trouble,
+                         double* __restrict__ nad_mag, double* __restrict__ pad_mag, DetectArgs a) {
+  const int g0 = (a.dims & 1) ? 3 : 0, g1 = (a.dims & 2) ? 3 : 0, g2 = (a.dims & 4) ? 3 : 0;
+  const int n0 = a.N[0] - 2 * g0, n1 = a.N[1] - 2 * g1, n2 = a.N[2] - 2 * g2;
+  const long long nc = (long long)n0 * n1 * n2;
+  const long long NC = (long long)a.N[0] * a.N[1] * a.N[2];
+  const long long st[3] = {(long long)a.N[1] * a.N[2], a.N[2], 1};
+  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+    long long r = c;
+    const int k = r % n2; r /= n2;
+    const int j = r % n1;
+    const int i = (int)(r / n1);
+    const long long base = ((long long)(i + g0) * a.N[1] + (j + g1)) * a.N[2] + (k + g2);
+    bool nad = false;
+    double ind_min = 0.0;
+    for (int v = 0; v < 5; ++v) {
+      const double* wv = w + v * NC;
+      double M = wv[base], m = wv[base];
+      for (int di = -(g0 > 0); di <= (g0 > 0); ++di)
+        for (int dj = -(g1 > 0); dj <= (g1 > 0); ++dj)
+          for (int dk = -(g2 > 0); dk <= (g2 > 0); ++dk) {
+            const double x = wv[base + di * st[0] + dj * st[1] + dk * st[2]];
+            M = dmax(M, x);
+            m = dmin(m, x);
+          }
+      const double uc = ws[v * NC + base];
+      double ind;
+      bool tv;
+      if (a.range_type == 0) {
+        const double rng = a.mode == 0 ? range_mm[2 * v + 1] - range_mm[2 * v] : M - m;
+        ind = dmin(uc - m, M - uc);
+        tv = ind < -a.eps * rng;
+      } else {
+        const double lo = a.mode == 0 ? (1.0 - a.eps) * range_mm[2 * v] : (1.0 - a.eps) * m;
+        const double hi = a.mode == 0 ? (1.0 + a.eps) * range_mm[2 * v + 1] : (1.0 + a.eps) * M;
+        ind = dmin(uc - lo, hi - uc);
+        tv = ind < 0.0;
+      }
+      if (tv && a.sed) tv = sed_alpha_cell(ws + v * NC, base, st, a.dims, a.sed_eps) < 1.0;
+      if (!((a.nad_vars >> v) & 1)) tv = false;
+      nad = nad || tv;
+      ind_min = v == 0 ? ind : dmin(ind_min, ind);
+    }
+    nad_mag[c] = nad ? -ind_min : 0.0;
+    double pind = 0.0;
+    if (a.pad_var >= 0) {
+      const double x = ws[a.pad_var * NC + base];
+      pind = dmin(x - a.pad_lo, a.pad_hi - x);
+    }
+    const bool pad = pind < 0.0;
+    pad_mag[c] = pad ? -pind : 0.0;
+    trouble[c] = (pad || nad) ? 1 : 0;
+  }
+}
+
+__global__ void k_sed(const double* __restrict__ u, double* __restrict__ alpha, int nv, int N0, int N1, int N2,
+                      int dims, double eps) {
+  const int g0 = (dims & 1) ? 3 : 0, g1 = (dims & 2) ? 3 : 0, g2 = (dims & 4) ? 3 : 0;
+  const int n0 = N0 - 2 * g0, n1 = N1 - 2 * g1, n2 = N2 - 2 * g2;
+  const long long nc = (long long)n0 * n1 * n2, NC = (long long)N0 * N1 * N2;
+  const long long st[3] = {(long long)N1 * N2, N2, 1};
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nc * nv;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const int v = (int)(idx / nc);
+    long long r = idx % nc;
+    const int k = r % n2; r /= n2;
+    const int j = r % n1;
+    const int i = (int)(r / n1);
+    const long long base = ((long long)(i + g0) * N1 + (j + g1)) * N2 + (k + g2);
+    alpha[idx] = sed_alpha_cell(u + v * NC, base, st, dims, eps);
+  }
+}
+
+__global__ void k_broadcast_simple(const signed char* __restrict__ t, double* __restrict__ tx, double* __restrict__ ty,
+                                   double* __restrict__ tz, int n0, int n1, int n2, int dims) {
+  // theta on faces = max of the flags of the cells the face touches (fvhoe/slope_limiting.py:366-380)
+  const long long fx = (long long)(n0 + 1) * n1 * n2, fy = (long long)n0 * (n1 + 1) * n2,
+                  fz = (long long)n0 * n1 * (n2 + 1);
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < fx + fy + fz;
+       idx += (long long)gridDim.x * blockDim.x) {
+    if (idx < fx) {
+      long long r = idx;
+      const int k = r % n2; r /= n2;
+      const int j = r % n1;
+      const int i = (int)(r / n1);
+      double th = 0.0;
+      if (dims & 1) {
+        const int a = i > 0 ? t[((long long)(i - 1) * n1 + j) * n2 + k] : 0;
+        const int b = i < n0 ? t[((long long)i * n1 + j) * n2 + k] : 0;
+        th = (double)(a > b ? a : b);
+      }
+      tx[idx] = th;
+    } else if (idx < fx + fy) {
+      long long r = idx - fx;
+      const int k = r % n2; r /= n2;
+      const int j = r % (n1 + 1);
+      const int i = (int)(r / (n1 + 1));
+      double th = 0.0;
+      if (dims & 2) {
+        const int a = j > 0 ? t[((long long)i * n1 + (j - 1)) * n2 + k] : 0;
+        const int b = j < n1 ? t[((long long)i * n1 + j) * n2 + k] : 0;
+        th = (double)(a > b ? a : b);
+      }
+      ty[idx - fx] = th;
+    } else {
+      long long r = idx - fx - fy;
+      const int k = r % (n2 + 1); r /= (n2 + 1);
+      const int j = r % n1;
+      const int i = (int)(r / n1);
+      double th = 0.0;
+      if (dims & 4) {
+        const int a = k > 0 ? t[((long long)i * n1 + j) * n2 + (k - 1)] : 0;
+        const int b = k < n2 ? t[((long long)i * n1 + j) * n2 + k] : 0;
+        th = (double)(a > b ? a : b);
+      }
+      tz[idx - fx - fy] = th;
+    }
+  }
+}
+
+struct ShiftMaxArgs {
+  int ext[3];                       // region extents
+  long long os[3], ss[3], ts[3];    // strides of out, S1, S2
+  long long ooff, soff, toff;       // element offsets of the region's origin
+  double coef;
+  int gt;                           // S1 term is coef*(S1 > 0) instead of coef*S1
+};
+
+// out[r] = max(coef * g(S1[r]), S2[r]) over a 3-D region
+__global__ void k_shift_max(double* out, const double* __restrict__ S1, const double* __restrict__ S2, ShiftMaxArgs a) {
+  const long long total = (long long)a.ext[0] * a.ext[1] * a.ext[2];
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    long long r = idx;
+    const int k = r % a.ext[2]; r /= a.ext[2];
+    const int j = r % a.ext[1];
+    const int i = (int)(r / a.ext[1]);
+    const double s = S1[a.soff + i * a.ss[0] + j * a.ss[1] + k * a.ss[2]];
+    const double x = a.coef * (a.gt ? (s > 0.0 ? 1.0 : 0.0) : s);
+    const double y = S2[a.toff + i * a.ts[0] + j * a.ts[1] + k * a.ts[2]];
+    out[a.ooff + i * a.os[0] + j * a.os[1] + k * a.os[2]] = dmax(x, y);
+  }
+}
+
+// F = (1 - theta) * F + theta * Fl, theta broadcast over variables (fvhoe/solver.py:729-731)
+__global__ void k_blend(double* __restrict__ F, const double* __restrict__ Fl, const double* __restrict__ th, int nv,
+                        long long nf) {
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nv * nf;
+       idx += (long long)gridDim.x * blockDim.x) {
+    const double t = th[idx % nf];
+    F[idx] = (1.0 - t) * F[idx] + t * Fl[idx];
+  }
+}
+
+// Runge-Kutta combinations in the reference's k-form (fvhoe/ode.py:245-298); dt read from device memory
+__global__ void k_rk(int kind, double* __restrict__ out, const double* __restrict__ u, const double* __restrict__ k0,
+                     const double* __restrict__ k1, const double* __restrict__ k2, const double* __restrict__ k3,
+                     const double* __restrict__ dtp, long long n) {
+  const double dt = dtp[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    double r;
+    switch (kind) {
+      case FVHOE_RK_AXPY: r = u[i] + dt * k0[i]; break;
+      case FVHOE_RK_SSPRK3_STAGE3: r = u[i] + 0.25 * dt * k0[i] + 0.25 * dt * k1[i]; break;
+      case FVHOE_RK_SSPRK3_FINAL: r = u[i] + (1.0 / 6.0) * dt * (k0[i] + k1[i] + 4.0 * k2[i]); break;
+      case FVHOE_RK_SSPRK2_FINAL: r = 0.5 * u[i] + 0.5 * (k0[i] + dt * k1[i]); break;  // k0 holds u1
+      case FVHOE_RK_HALF_AXPY: r = u[i] + 0.5 * dt * k0[i]; break;
+      case FVHOE_RK_RK4_FINAL: r = u[i] + (1.0 / 6.0) * dt * (k0[i] + 2.0 * k1[i] + 2.0 * k2[i] + k3[i]); break;
+      default: r = 0.0;
+    }
+    out[i] = r;
+  }
+}
+
+__global__ void k_set_scalar(double* p, double v) { p[0] = v; }
+
+__global__ void k_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target) {
+  const double dt = dt_stage[0];
+  dt_step[0] = (has_target && target - t[0] < dt) ? target - t[0] : dt;
+}
+
+__global__ void k_advance_time(double* t, const double* dt) { t[0] = t[0] + dt[0]; }
+
+}  // namespace
+
+// ------------------------------------------------------------------- C-ABI
+
+extern "C" {
+
+int fvhoe_version(void) { return FVHOE_ABI_VERSION; }
+
+int fvhoe_stencil_weights(int p, int kind, double* out, int* n) {
+  if (p < 0 || p > 8 || kind < 0 || kind > 3 || !out || !n) return FVHOE_ERR_ARG;
+  *n = ntaps(p);
+  for (int i = 0; i < *n; ++i) out[i] = weight(kind, p, i);
+  return FVHOE_OK;
+}
+
+int fvhoe_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                        void* stream) {
+  if (!u || !out || nv < 1 || axis < 1 || axis > 3 || kind < 0 || kind > 3) return FVHOE_ERR_ARG;
+  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  Weights9 w{};
+  w.k = ntaps(p);
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
+  if (na < w.k) return FVHOE_ERR_ARG;
+  for (int i = 0; i < w.k; ++i) w.w[i] = weight(kind, p, i);
+  const long long total = (long long)nv * n0 * n1 * n2;
+  k_sweep<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, n0, n1, n2, axis, w);
+  return launch_status();
+}
+
+int fvhoe_compute_primitives(const double* u, double* w, int nv, long long ncell, double gamma, void* stream) {
+  if (!u || !w || nv < 5 || ncell < 0) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  k_prims<<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(u, w, nv, ncell, gamma);
+  return launch_status();
+}
+
+int fvhoe_compute_conservatives(const double* w, double* u, int nv, long long ncell, double gamma, void* stream) {
+  if (!u || !w || nv < 5 || ncell < 0) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  k_cons<<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(w, u, nv, ncell, gamma);
+  return launch_status();
+}
+
+int fvhoe_compute_sound_speed(const double* w, double* c, long long ncell, double gamma, double csq_floor,
+                              void* stream) {
+  if (!w || !c || ncell < 0) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  k_sound<<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(w, c, ncell, gamma, csq_floor);
+  return launch_status();
+}
+
+int fvhoe_hydro_dt(const double* w, long long ncell, double h, int ndim, double CFL, double gamma, double csq_floor,
+                   double* dt_out, unsigned long long* scratch, void* stream) {
+  if (!w || !dt_out || !scratch || ncell <= 0) return FVHOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  k_fill_u64<<<1, 64, 0, s>>>(scratch, 1, 0ull);
+  k_dt_partial<<<grid_for(ncell) < 2048 ? grid_for(ncell) : 2048, TPB, 0, s>>>(w, ncell, ndim, gamma, csq_floor,
+                                                                              scratch);
+  k_dt_final<<<1, 1, 0, s>>>(scratch, CFL * h, dt_out);
+  return launch_status();
+}
+
+int fvhoe_riemann(const double* wl, const double* wr, double* out, int nv, long long ncell, int solver, int dim,
+                  double gamma, double csq_floor, void* stream) {
+  if (!wl || !wr || !out || nv < 5 || dim < 0 || dim > 2 || solver < 0 || solver > 2) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  const int g = grid_for(ncell);
+  if (dim == 0) k_riemann_pt<0><<<g, TPB, 0, s>>>(wl, wr, out, nv, ncell, solver, gamma, csq_floor);
+  else if (dim == 1) k_riemann_pt<1><<<g, TPB, 0, s>>>(wl, wr, out, nv, ncell, solver, gamma, csq_floor);
+  else k_riemann_pt<2><<<g, TPB, 0, s>>>(wl, wr, out, nv, ncell, solver, gamma, csq_floor);
+  return launch_status();
+}
+
+int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* out, int n0, int n1, int n2, int axis,
+                        int solver, double gamma, double csq_floor, void* stream) {
+  if (!wr_face || !wl_face || !out || axis < 1 || axis > 3 || solver < 0 || solver > 2) return FVHOE_ERR_ARG;
+  const long long nout = (long long)(axis == 1 ? n0 - 1 : n0) * (axis == 2 ? n1 - 1 : n1) * (axis == 3 ? n2 - 1 : n2);
+  if (nout <= 0) return nout == 0 ? FVHOE_OK : FVHOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int g = grid_for(nout);
+  if (axis == 1) k_riemann_faces<0><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
+  else if (axis == 2) k_riemann_faces<1><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
+  else k_riemann_faces<2><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
+  return launch_status();
+}
+
+int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
+                void* stream) {
+  if (!u || !left || !right || axis < 1 || axis > 3 || limiter < 0 || limiter > 2) return FVHOE_ERR_ARG;
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
+  if (na < 3) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * n0 * n1 * n2;
+  k_muscl<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, left, right, nv, n0, n1, n2, axis, limiter);
+  return launch_status();
+}
+
+int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                   const int* bc6, void* stream) {
+  if (!u || !out || !bc6 || nv < 1) return FVHOE_ERR_ARG;
+  BcArgs a{{n0, n1, n2}, {g0, g1, g2}, {bc6[0], bc6[2], bc6[4]}, {bc6[1], bc6[3], bc6[5]}};
+  for (int d = 0; d < 3; ++d) {
+    if (a.g[d] < 0 || a.n[d] < 1) return FVHOE_ERR_ARG;
+    if (a.g[d] > 0 && a.g[d] > a.n[d]) return FVHOE_ERR_ARG;  // slabs must come from the interior
+    if (a.lo[d] < 0 || a.lo[d] > 3 || a.hi[d] < 0 || a.hi[d] > 3) return FVHOE_ERR_UNSUPPORTED;
+  }
+  const long long total = (long long)nv * (n0 + 2 * g0) * (n1 + 2 * g1) * (n2 + 2 * g2);
+  k_bc_apply<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, a);
+  return launch_status();
+}
+
+int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
+                         int n2, int active, double cx, double cy, double cz, void* stream) {
+  if (!dudt || nv < 1 || ((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * n0 * n1 * n2;
+  k_euler_eq<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz);
+  return launch_status();
+}
+
+int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream) {
+  if (!u || !out || !scratch || nv < 1 || nv > 64 || ncell <= 0) return FVHOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  k_init_minmax<<<1, 128, 0, s>>>(scratch, nv);
+  dim3 grid(grid_for(ncell) < 1024 ? grid_for(ncell) : 1024, nv);
+  k_minmax_partial<<<grid, TPB, 0, s>>>(u, ncell, scratch);
+  k_minmax_final<<<1, 128, 0, s>>>(scratch, out, nv);
+  return launch_status();
+}
+
+int fvhoe_detect_troubled_cells(const double* w, const double* ws, int N0, int N1, int N2, int dims,
+                                const double* range_minmax, int mode, int range_type, double eps, int nad_vars,
+                                int pad_var, double pad_lo, double pad_hi, int sed, double sed_eps,
+                                signed char* trouble, double* nad_mag, double* pad_mag, void* stream) {
+  if (!w || !ws || !trouble || !nad_mag || !pad_mag || dims < 1 || dims > 7) return FVHOE_ERR_ARG;
+  if (mode < 0 || mode > 1 || range_type < 0 || range_type > 1 || pad_var > 4) return FVHOE_ERR_ARG;
+  if (mode == 0 && !range_minmax) return FVHOE_ERR_ARG;
+  DetectArgs a{{N0, N1, N2}, dims, mode, range_type, nad_vars, pad_var, sed, eps, pad_lo, pad_hi, sed_eps};
+  const long long nc = (long long)(N0 - ((dims & 1) ? 6 : 0)) * (N1 - ((dims & 2) ? 6 : 0)) * (N2 - ((dims & 4) ? 6 : 0));
+  if (nc < 0) return FVHOE_ERR_ARG;
+  if (nc == 0) return FVHOE_OK;
+  k_detect<<<grid_for(nc), TPB, 0, (hipStream_t)stream>>>(w, ws, range_minmax, trouble, nad_mag, pad_mag, a);
+  return launch_status();
+}
+
+int fvhoe_smooth_extrema_detector(const double* u, double* alpha, int nv, int N0, int N1, int N2, int dims,
+                                  double eps, void* stream) {
+  if (!u || !alpha || nv < 1 || dims < 1 || dims > 7) return FVHOE_ERR_ARG;
+  const long long nc = (long long)(N0 - ((dims & 1) ? 6 : 0)) * (N1 - ((dims & 2) ? 6 : 0)) * (N2 - ((dims & 4) ? 6 : 0));
+  if (nc < 0) return FVHOE_ERR_ARG;
+  if (nc == 0) return FVHOE_OK;
+  k_sed<<<grid_for(nc * nv), TPB, 0, (hipStream_t)stream>>>(u, alpha, nv, N0, N1, N2, dims, eps);
+  return launch_status();
+}
+
+int fvhoe_broadcast_simple(const signed char* trouble, double* tx, double* ty, double* tz, int n0, int n1, int n2,
+                           int dims, void* stream) {
+  if (!trouble || !tx || !ty || !tz) return FVHOE_ERR_ARG;
+  const long long total = (long long)(n0 + 1) * n1 * n2 + (long long)n0 * (n1 + 1) * n2 + (long long)n0 * n1 * (n2 + 1);
+  k_broadcast_simple<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(trouble, tx, ty, tz, n0, n1, n2, dims);
+  return launch_status();
+}
+
+int fvhoe_shift_max(double* out, const long long* out_strides, long long out_offset, const double* s1,
+                    const long long* s1_strides, long long s1_offset, double coef, int gt, const double* s2,
+                    const long long* s2_strides, long long s2_offset, const int* extent, void* stream) {
+  if (!out || !s1 || !s2 || !out_strides || !s1_strides || !s2_strides || !extent) return FVHOE_ERR_ARG;
+  ShiftMaxArgs a{};
+  for (int d = 0; d < 3; ++d) {
+    a.ext[d] = extent[d];
+    a.os[d] = out_strides[d];
+    a.ss[d] = s1_strides[d];
+    a.ts[d] = s2_strides[d];
+    if (extent[d] < 0) return FVHOE_ERR_ARG;
+  }
+  a.ooff = out_offset;
+  a.soff = s1_offset;
+  a.toff = s2_offset;
+  a.coef = coef;
+  a.gt = gt;
+  const long long total = (long long)a.ext[0] * a.ext[1] * a.ext[2];
+  if (total == 0) return FVHOE_OK;
+  k_shift_max<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(out, s1, s2, a);
+  return launch_status();
+}
+
+int fvhoe_blend(double* F, const double* Fl, const double* theta, int nv, long long nface, void* stream) {
+  if (!F || !Fl || !theta || nv < 1 || nface < 0) return FVHOE_ERR_ARG;
+  if (nface == 0) return FVHOE_OK;
+  k_blend<<<grid_for(nv * nface), TPB, 0, (hipStream_t)stream>>>(F, Fl, theta, nv, nface);
+  return launch_status();
+}
+
+int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, const double* k1, const double* k2,
+                     const double* k3, const double* dt, long long n, void* stream) {
+  if (!out || !u || !k0 || !dt || n < 0 || kind < 0 || kind > FVHOE_RK_RK4_FINAL) return FVHOE_ERR_ARG;
+  if ((kind == FVHOE_RK_SSPRK3_STAGE3 || kind == FVHOE_RK_SSPRK2_FINAL) && !k1) return FVHOE_ERR_ARG;
+  if (kind == FVHOE_RK_SSPRK3_FINAL && (!k1 || !k2)) return FVHOE_ERR_ARG;
+  if (kind == FVHOE_RK_RK4_FINAL && (!k1 || !k2 || !k3)) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_rk<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(kind, out, u, k0, k1, k2, k3, dt, n);
+  return launch_status();
+}
+
+int fvhoe_set_scalar(double* p, double v, void* stream) {
+  if (!p) return FVHOE_ERR_ARG;
+  k_set_scalar<<<1, 1, 0, (hipStream_t)stream>>>(p, v);
+  return launch_status();
+}
+
+int fvhoe_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target,
+                     void* stream) {
+  if (!dt_step || !dt_stage || !t) return FVHOE_ERR_ARG;
+  k_hit_target<<<1, 1, 0, (hipStream_t)stream>>>(dt_step, dt_stage, t, target, has_target);
+  return launch_status();
+}
+
+int fvhoe_advance_time(double* t, const double* dt, void* stream) {
+  if (!t || !dt) return FVHOE_ERR_ARG;
+  k_advance_time<<<1, 1, 0, (hipStream_t)stream>>>(t, dt);
+  return launch_status();
+}
+
+}  // extern "C"

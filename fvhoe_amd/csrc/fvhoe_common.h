@@ -1,0 +1,354 @@
+// Shared device code for the FVHOE per-stage kernels (gfx950).
+//
+// Every arithmetic expression below keeps the reference's floating-point
+// operation order (the NumPy expressions cited next to each function), and
+// the library is compiled with -ffp-contract=off, so FP64 results match the
+// reference bit for bit.  Division and sqrt are the compiler's IEEE
+// correctly rounded FP64 operations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+
+namespace fvhoe {
+
+// ---------------------------------------------------------------------------
+// stencil weights (fvhoe/stencils.py:25-255).  Each entry is a rational
+// written as num/den and folded at compile time with round-to-nearest, which
+// is the double the reference's `num / den` literals evaluate to.
+// kind: 0 = left face "l", 1 = centre "c", 2 = right face "r" (reversed l),
+//       3 = transverse (averages from centre values).
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int reach(int p) { return (p + 1) / 2; }
+__host__ __device__ constexpr int ntaps(int p) { return 2 * reach(p) + 1; }
+
+__host__ __device__ constexpr double w_left(int p, int i) {
+  switch (p) {
+    case 0: return 1.0;
+    case 1: { const double w[] = {1.0 / 4.0, 1.0, -1.0 / 4.0}; return w[i]; }
+    case 2: { const double w[] = {1.0 / 3.0, 5.0 / 6.0, -1.0 / 6.0}; return w[i]; }
+    case 3: { const double w[] = {-1.0 / 24.0, 5.0 / 12.0, 5.0 / 6.0, -1.0 / 4.0, 1.0 / 24.0}; return w[i]; }
+    case 4: { const double w[] = {-1.0 / 20.0, 9.0 / 20.0, 47.0 / 60.0, -13.0 / 60.0, 1.0 / 30.0}; return w[i]; }
+    case 5: { const double w[] = {1.0 / 120.0, -1.0 / 12.0, 59.0 / 120.0, 47.0 / 60.0, -31.0 / 120.0, 1.0 / 15.0,
+                                  -1.0 / 120.0}; return w[i]; }
+    case 6: { const double w[] = {1.0 / 105.0, -19.0 / 210.0, 107.0 / 210.0, 319.0 / 420.0, -101.0 / 420.0,
+                                  5.0 / 84.0, -1.0 / 140.0}; return w[i]; }
+    case 7: { const double w[] = {-1.0 / 560.0, 17.0 / 840.0, -97.0 / 840.0, 449.0 / 840.0, 319.0 / 420.0,
+                                  -223.0 / 840.0, 71.0 / 840.0, -1.0 / 56.0, 1.0 / 560.0}; return w[i]; }
+    case 8: { const double w[] = {-1.0 / 504.0, 11.0 / 504.0, -61.0 / 504.0, 275.0 / 504.0, 1879.0 / 2520.0,
+                                  -641.0 / 2520.0, 199.0 / 2520.0, -41.0 / 2520.0, 1.0 / 630.0}; return w[i]; }
+  }
+  return 0.0;
+}
+
+__host__ __device__ constexpr double w_centre(int p, int i) {
+  switch (p) {
+    case 0: return 1.0;
+    case 1: { const double w[] = {0.0, 1.0, 0.0}; return w[i]; }
+    case 2: { const double w[] = {-1.0 / 24.0, 13.0 / 12.0, -1.0 / 24.0}; return w[i]; }
+    case 3: { const double w[] = {0.0, -1.0 / 24.0, 13.0 / 12.0, -1.0 / 24.0, 0.0}; return w[i]; }
+    case 4: { const double w[] = {3.0 / 640.0, -29.0 / 480.0, 1067.0 / 960.0, -29.0 / 480.0, 3.0 / 640.0}; return w[i]; }
+    case 5: { const double w[] = {0.0, 3.0 / 640.0, -29.0 / 480.0, 1067.0 / 960.0, -29.0 / 480.0, 3.0 / 640.0, 0.0};
+              return w[i]; }
+    case 6: { const double w[] = {-5.0 / 7168.0, 159.0 / 17920.0, -7621.0 / 107520.0, 30251.0 / 26880.0,
+                                  -7621.0 / 107520.0, 159.0 / 17920.0, -5.0 / 7168.0}; return w[i]; }
+    case 7: { const double w[] = {0.0, -5.0 / 7168.0, 159.0 / 17920.0, -7621.0 / 107520.0, 30251.0 / 26880.0,
+                                  -7621.0 / 107520.0, 159.0 / 17920.0, -5.0 / 7168.0, 0.0}; return w[i]; }
+    case 8: { const double w[] = {35.0 / 294912.0, -425.0 / 258048.0, 31471.0 / 2580480.0, -100027.0 / 1290240.0,
+                                  5851067.0 / 5160960.0, -100027.0 / 1290240.0, 31471.0 / 2580480.0,
+                                  -425.0 / 258048.0, 35.0 / 294912.0}; return w[i]; }
+  }
+  return 0.0;
+}
+
+__host__ __device__ constexpr double w_transverse(int p, int i) {
+  switch (p) {
+    case 0: return 1.0;
+    case 1: { const double w[] = {0.0, 1.0, 0.0}; return w[i]; }
+    case 2: { const double w[] = {1.0 / 24.0, 11.0 / 12.0, 1.0 / 24.0}; return w[i]; }
+    case 3: { const double w[] = {0.0, 1.0 / 24.0, 11.0 / 12.0, 1.0 / 24.0, 0.0}; return w[i]; }
+    case 4: { const double w[] = {-17.0 / 5760.0, 77.0 / 1440.0, 863.0 / 960.0, 77.0 / 1440.0, -17.0 / 5760.0};
+              return w[i]; }
+    case 5: { const double w[] = {0.0, -17.0 / 5760.0, 77.0 / 1440.0, 863.0 / 960.0, 77.0 / 1440.0, -17.0 / 5760.0,
+                                  0.0}; return w[i]; }
+    case 6: { const double w[] = {367.0 / 967680.0, -281.0 / 53760.0, 6361.0 / 107520.0, 215641.0 / 241920.0,
+                                  6361.0 / 107520.0, -281.0 / 53760.0, 367.0 / 967680.0}; return w[i]; }
+    case 7: { const double w[] = {0.0, 367.0 / 967680.0, -281.0 / 53760.0, 6361.0 / 107520.0, 215641.0 / 241920.0,
+                                  6361.0 / 107520.0, -281.0 / 53760.0, 367.0 / 967680.0, 0.0}; return w[i]; }
+    case 8: { const double w[] = {-27859.0 / 464486400.0, 49879.0 / 58060800.0, -801973.0 / 116121600.0,
+                                  3629953.0 / 58060800.0, 41208059.0 / 46448640.0, 3629953.0 / 58060800.0,
+                                  -801973.0 / 116121600.0, 49879.0 / 58060800.0, -27859.0 / 464486400.0};
+              return w[i]; }
+  }
+  return 0.0;
+}
+
+enum { KIND_L = 0, KIND_C = 1, KIND_R = 2, KIND_T = 3 };
+
+__host__ __device__ constexpr double weight(int kind, int p, int i) {
+  return kind == KIND_L   ? w_left(p, i)
+         : kind == KIND_R ? w_left(p, ntaps(p) - 1 - i)
+         : kind == KIND_C ? w_centre(p, i)
+                          : w_transverse(p, i);
+}
+
+// compile-time loop: f(std::integral_constant<int, I>) for I = 0..N-1
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// out = sum_i w_i * x(i), accumulated from +0.0 left to right, zero taps
+// skipped (bitwise neutral for finite data: the accumulator is never -0.0
+// once it starts from +0.0), as fvhoe/fv.py:135-139.
+template <int KIND, int P, typename Get>
+__device__ __forceinline__ double sweep(Get&& x) {
+  double acc = 0.0;
+  static_for<ntaps(P)>([&](auto I) {
+    constexpr double w = weight(KIND, P, decltype(I)::value);
+    if constexpr (w != 0.0) acc = acc + w * x(decltype(I)::value);
+  });
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// thermodynamics (fvhoe/hydro.py:146-210); w = (rho, vx, vy, vz, P)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void prims_from_cons(const double u[5], double w[5], double gamma) {
+  const double rho = u[0];
+  w[0] = rho;
+  w[1] = u[1] / rho;
+  w[2] = u[2] / rho;
+  w[3] = u[3] / rho;
+  const double msum = ((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]);
+  w[4] = (gamma - 1.0) * (u[4] - 0.5 * msum);
+}
+
+// total energy of compute_conservatives: P/(gamma-1) + 0.5*sum(m*v), m = rho*v
+__device__ __forceinline__ double energy(const double w[5], double gamma) {
+  const double mx = w[0] * w[1], my = w[0] * w[2], mz = w[0] * w[3];
+  return w[4] / (gamma - 1.0) + 0.5 * (((mx * w[1]) + (my * w[2])) + (mz * w[3]));
+}
+
+__device__ __forceinline__ double sound_speed(double rho, double P, double gamma, double csq_floor) {
+  const double csq = gamma * P / rho;
+  return sqrt(csq > csq_floor ? csq : csq_floor);
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// Riemann solvers (fvhoe/riemann_solvers.py).  D = normal axis 0/1/2;
+// wl is the state left of the face, wr right of it.
+// ---------------------------------------------------------------------------
+enum { RS_UPWIND = 0, RS_LLF = 1, RS_HLLC = 2 };
+
+// HLLC "David variation", fvhoe/riemann_solvers.py:110-183
+template <int D>
+__device__ __forceinline__ void hllc(const double wl[5], const double wr[5], double F[5], double gamma,
+                                     double csq_floor, double* v_godunov = nullptr) {
+  const double El = energy(wl, gamma), Er = energy(wr, gamma);
+  const double cl = sound_speed(wl[0], wl[4], gamma, csq_floor);
+  const double cr = sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const double cmax = dmax(cl, cr);
+  const double vl = wl[D + 1], vr = wr[D + 1];
+  const double rhol = wl[0], rhor = wr[0], Pl = wl[4], Pr = wr[4];
+  const double Sl = dmin(vl, vr) - cmax;
+  const double Sr = dmax(vl, vr) + cmax;
+  const double rcL = rhol * (vl - Sl);
+  const double rcR = rhor * (Sr - vr);
+  const double vs = (rcL * vl + rcR * vr + Pl - Pr) / (rcL + rcR);
+  const double Ps = (rcL * Pr + rcR * Pl + rcL * rcR * (vl - vr)) / (rcL + rcR);
+  double r, v, P, E;
+  if (Sl > 0.0) {
+    r = rhol; v = vl; P = Pl; E = El;
+  } else if (vs > 0.0) {
+    r = rhol * (Sl - vl) / (Sl - vs);
+    E = ((Sl - vl) * El - Pl * vl + Ps * vs) / (Sl - vs);
+    v = vs; P = Ps;
+  } else if (Sr > 0.0) {
+    r = rhor * (Sr - vr) / (Sr - vs);
+    E = ((Sr - vr) * Er - Pr * vr + Ps * vs) / (Sr - vs);
+    v = vs; P = Ps;
+  } else {
+    r = rhor; v = vr; P = Pr; E = Er;
+  }
+  const double rv = r * v;
+  F[0] = rv;
+#pragma unroll
+  for (int k = 1; k < 4; ++k) {
+    if (k == D + 1) {
+      F[k] = rv * v + P;
+    } else {
+      const double up = v > 0.0 ? wl[k] : (v < 0.0 ? wr[k] : 0.0);
+      F[k] = rv * up;
+    }
+  }
+  F[4] = v * (E + P);
+  if (v_godunov) *v_godunov = v;
+}
+
+// physical flux along D (fvhoe/hydro.py:213-246)
+template <int D>
+__device__ __forceinline__ void phys_flux(const double w[5], const double u[5], double F[5], bool pressure) {
+  const double v = w[D + 1];
+  F[0] = v * w[0];
+  F[1] = v * u[1];
+  F[2] = v * u[2];
+  F[3] = v * u[3];
+  F[4] = v * u[4];
+  if (pressure) {
+    F[D + 1] = F[D + 1] + w[4];
+    F[4] = F[4] + v * w[4];
+  }
+}
+
+__device__ __forceinline__ void cons_from_prims(const double w[5], double u[5], double gamma) {
+  u[0] = w[0];
+  u[1] = w[0] * w[1];
+  u[2] = w[0] * w[2];
+  u[3] = w[0] * w[3];
+  u[4] = w[4] / (gamma - 1.0) + 0.5 * (((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]));
+}
+
+// local Lax-Friedrichs (Toro p.331; fvhoe/riemann_solvers.py:68-88)
+template <int D>
+__device__ __forceinline__ void llf(const double wl[5], const double wr[5], double F[5], double gamma,
+                                    double csq_floor) {
+  double ul[5], ur[5], Fl[5], Fr[5];
+  cons_from_prims(wl, ul, gamma);
+  cons_from_prims(wr, ur, gamma);
+  phys_flux<D>(wl, ul, Fl, true);
+  phys_flux<D>(wr, ur, Fr, true);
+  const double sl = fabs(wl[D + 1]) + sound_speed(wl[0], wl[4], gamma, csq_floor);
+  const double sr = fabs(wr[D + 1]) + sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const double hs = 0.5 * dmax(sl, sr);
+#pragma unroll
+  for (int k = 0; k < 5; ++k) F[k] = 0.5 * (Fl[k] + Fr[k]) - hs * (ur[k] - ul[k]);
+}
+
+// upwinding without pressure (fvhoe/riemann_solvers.py:31-45)
+template <int D>
+__device__ __forceinline__ void upwind(const double wl[5], const double wr[5], double F[5], double gamma) {
+  double ul[5], ur[5], Fl[5], Fr[5];
+  cons_from_prims(wl, ul, gamma);
+  cons_from_prims(wr, ur, gamma);
+  phys_flux<D>(wl, ul, Fl, false);
+  phys_flux<D>(wr, ur, Fr, false);
+  const double v = wl[D + 1];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) F[k] = v > 0.0 ? Fl[k] : (v < 0.0 ? Fr[k] : 0.0);
+}
+
+template <int D>
+__device__ __forceinline__ void riemann(int solver, const double wl[5], const double wr[5], double F[5],
+                                        double gamma, double csq_floor, double* v_godunov = nullptr) {
+  if (solver == RS_HLLC) hllc<D>(wl, wr, F, gamma, csq_floor, v_godunov);
+  else if (solver == RS_LLF) llf<D>(wl, wr, F, gamma, csq_floor);
+  else upwind<D>(wl, wr, F, gamma);
+}
+
+// ---------------------------------------------------------------------------
+// limiters (fvhoe/slope_limiting.py:56-84)
+// ---------------------------------------------------------------------------
+enum { LIM_NONE = 0, LIM_MINMOD = 1, LIM_MONCEN = 2 };
+
+__device__ __forceinline__ double minmod(double dl, double dr) {
+  const double den = dl > 0.0 ? (dl > 1e-16 ? dl : 1e-16) : (dl < -1e-16 ? dl : -1e-16);
+  double r = dr / den;
+  r = r < 1.0 ? r : 1.0;
+  return (r > 0.0 ? r : 0.0) * dl;
+}
+
+__device__ __forceinline__ double dsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+
+__device__ __forceinline__ double moncen(double dl, double dr) {
+  const double dc = 0.5 * (dl + dr);
+  double s = dmin(fabs(2.0 * dl), fabs(2.0 * dr));
+  s = dsign(dc) * dmin(s, fabs(dc));
+  return dl * dr >= 0.0 ? s : 0.0;
+}
+
+__device__ __forceinline__ double limited_slope(int limiter, double dl, double dr) {
+  return limiter == LIM_MINMOD ? minmod(dl, dr) : limiter == LIM_MONCEN ? moncen(dl, dr) : 0.5 * (dl + dr);
+}
+
+// ---------------------------------------------------------------------------
+// smooth-extrema detector along one axis (fvhoe/smooth_extrema_detection.py:26-49)
+// x(o) returns the value at offset o in [-3, 3] from the cell.
+// ---------------------------------------------------------------------------
+template <typename Get>
+__device__ __forceinline__ double sed_alpha_1d(Get&& x, double eps) {
+  double dU[5];
+#pragma unroll
+  for (int j = 0; j < 5; ++j) dU[j] = 0.5 * (x(j - 1) - x(j - 3));  // central differences at offsets -2..2
+  double a[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {  // alpha at offsets -1, 0, 1 -> dU index j+1
+    double dv = 0.5 * (0.5 * (dU[j + 2] - dU[j]));
+    dv = fabs(dv) <= eps ? (dv >= 0.0 ? eps : -eps) : dv;
+    const double vL = dU[j] - dU[j + 1];
+    double aL = -(dv < 0.0 ? (vL > 0.0 ? vL : 0.0) : (vL < 0.0 ? vL : 0.0)) / dv;
+    aL = aL < 1.0 ? aL : 1.0;
+    const double vR = dU[j + 2] - dU[j + 1];
+    double aR = (dv > 0.0 ? (vR > 0.0 ? vR : 0.0) : (vR < 0.0 ? vR : 0.0)) / dv;
+    aR = fabs(dv) <= eps ? 1.0 : aR;
+    aR = aR < aL ? aR : aL;
+    a[j] = dmin(aL, aR);
+  }
+  return dmin(dmin(a[2], a[1]), a[0]);
+}
+
+// ---------------------------------------------------------------------------
+// order-preserving double <-> uint64 maps for atomic min/max reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long ordered_bits(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ __forceinline__ double from_ordered_bits(unsigned long long b) {
+  const unsigned long long r = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
+#ifdef __HIP_DEVICE_COMPILE__
+  return __longlong_as_double((long long)r);
+#else
+  double d;
+  __builtin_memcpy(&d, &r, 8);
+  return d;
+#endif
+}
+
+// boundary kinds
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_FREE = 3 };
+
+// Source interior index and momentum sign for padded index a on an axis of
+// interior size n, pad g, boundary kinds (lo, hi) (fvhoe/boundary_conditions.py:60-151).
+__host__ __device__ __forceinline__ int bc_source(int a, int n, int g, int lo, int hi, bool* flip) {
+  *flip = false;
+  if (a < g) {
+    if (lo == BC_PERIODIC) return n - g + a;
+    if (lo == BC_FREE) return 0;
+    *flip = (lo == BC_REFLECTIVE);
+    return g - 1 - a;
+  }
+  if (a >= n + g) {
+    const int b = a - n - g;
+    if (hi == BC_PERIODIC) return b;
+    if (hi == BC_FREE) return n - 1;
+    *flip = (hi == BC_REFLECTIVE);
+    return n - 1 - b;
+  }
+  return a - g;
+}
+
+// Same map for a signed interior-relative index c in [-g, n+g).
+__host__ __device__ __forceinline__ int bc_map(int c, int n, int g, int lo, int hi, bool* flip) {
+  return bc_source(c + g, n, g, lo, hi, flip);
+}
+
+}  // namespace fvhoe

@@ -1,0 +1,189 @@
+/* C-ABI of libfvhoe_hip.so: the MI355X (gfx950) per-RK-stage update of
+ * FVHOE's finite-volume Euler solver.
+ *
+ * Conventions
+ *  - Arrays are device pointers to contiguous float64 data in the reference's
+ *    layout (nvars, n0, n1, n2), C order; variable order rho, vx|mx, vy|my,
+ *    vz|mz, P|E, passives (fvhoe/hydro.py:36-63).
+ *  - Axes are numbered 1..3 as the reference's array axes (fvhoe/fv.py:106).
+ *  - `stream` is a hipStream_t (NULL = the default stream).  Calls enqueue
+ *    work and return; nothing here synchronises with the host.
+ *  - The library allocates no device memory: outputs and scratch are the
+ *    caller's.
+ *  - Return: FVHOE_OK, or an error code; the Python layer maps
+ *    FVHOE_ERR_ARG to ValueError, FVHOE_ERR_UNSUPPORTED to
+ *    NotImplementedError and FVHOE_ERR_LAUNCH to RuntimeError, the exception
+ *    types the reference raises for the same conditions.
+ */
+#ifndef FVHOE_HIP_H
+#define FVHOE_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FVHOE_ABI_VERSION 1
+
+enum {
+  FVHOE_OK = 0,
+  FVHOE_ERR_ARG = 1,
+  FVHOE_ERR_UNSUPPORTED = 2,
+  FVHOE_ERR_LAUNCH = 3
+};
+
+/* stencil kinds, Riemann solvers, limiters, boundary kinds */
+enum { FVHOE_STENCIL_L = 0, FVHOE_STENCIL_C = 1, FVHOE_STENCIL_R = 2, FVHOE_STENCIL_T = 3 };
+enum { FVHOE_RS_UPWIND = 0, FVHOE_RS_LLF = 1, FVHOE_RS_HLLC = 2 };
+enum { FVHOE_LIM_NONE = 0, FVHOE_LIM_MINMOD = 1, FVHOE_LIM_MONCEN = 2 };
+enum { FVHOE_BC_PERIODIC = 0, FVHOE_BC_OUTFLOW = 1, FVHOE_BC_REFLECTIVE = 2, FVHOE_BC_FREE = 3 };
+
+/* Runge-Kutta combinations (fvhoe/ode.py:245-298), dt read from device */
+enum {
+  FVHOE_RK_AXPY = 0,           /* u + dt*k0                                   (ode.py:251,263,277,293) */
+  FVHOE_RK_SSPRK3_STAGE3 = 1,  /* u + 0.25*dt*k0 + 0.25*dt*k1                 (ode.py:278) */
+  FVHOE_RK_SSPRK3_FINAL = 2,   /* u + (1/6)*dt*(k0 + k1 + 4*k2)               (ode.py:279) */
+  FVHOE_RK_SSPRK2_FINAL = 3,   /* 0.5*u + 0.5*(k0 + dt*k1), k0 holding u1     (ode.py:265) */
+  FVHOE_RK_HALF_AXPY = 4,      /* u + 0.5*dt*k0                               (ode.py:291-292) */
+  FVHOE_RK_RK4_FINAL = 5       /* u + (1/6)*dt*(k0 + 2*k1 + 2*k2 + k3)        (ode.py:294) */
+};
+
+/* ---------------------------------------------------------------- host-only */
+
+int fvhoe_version(void);
+
+/* Weights of get_conservative_interpolation_stencil_weights (kind l/c/r) and
+ * get_transverse_reconstruction_stencil_weights (kind t), fvhoe/stencils.py:25-255.
+ * out holds >= 9 doubles; *n receives the tap count 2*ceil(p/2)+1. */
+int fvhoe_stencil_weights(int p, int kind, double* out, int* n);
+
+/* ------------------------------------------------------- elementary kernels */
+
+/* stencil_sweep / conservative_interpolation / transverse_reconstruction
+ * (fvhoe/fv.py:106-172): out is u's shape shortened by 2*ceil(p/2) on axis. */
+int fvhoe_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                        void* stream);
+
+/* compute_primitives / compute_conservatives (fvhoe/hydro.py:146-196) */
+int fvhoe_compute_primitives(const double* u, double* w, int nv, long long ncell, double gamma, void* stream);
+int fvhoe_compute_conservatives(const double* w, double* u, int nv, long long ncell, double gamma, void* stream);
+
+/* compute_sound_speed (fvhoe/hydro.py:199-210); w has >= 5 variables of ncell values */
+int fvhoe_compute_sound_speed(const double* w, double* c, long long ncell, double gamma, double csq_floor,
+                              void* stream);
+
+/* hydro_dt (fvhoe/hydro.py:270-294): *dt_out = CFL*h / max(|vx|+|vy|+|vz| + ndim*c);
+ * dt_out and scratch (1 x u64) are device memory. */
+int fvhoe_hydro_dt(const double* w, long long ncell, double h, int ndim, double CFL, double gamma, double csq_floor,
+                   double* dt_out, unsigned long long* scratch, void* stream);
+
+/* advection_upwind / llf / hllc (fvhoe/riemann_solvers.py:13,48,91), pointwise
+ * on contiguous (nv, ncell) primitive states; dim 0/1/2 = x/y/z. */
+int fvhoe_riemann(const double* wl, const double* wr, double* out, int nv, long long ncell, int solver, int dim,
+                  double gamma, double csq_floor, void* stream);
+
+/* The Riemann call of hydrofluxes (fvhoe/solver.py:575-606): out[f] =
+ * RS(wr_face[f], wl_face[f+1]) along axis, nv = 5. */
+int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* out, int n0, int n1, int n2, int axis,
+                        int solver, double gamma, double csq_floor, void* stream);
+
+/* MUSCL_interpolations with minmod / moncen / no limiter (fvhoe/slope_limiting.py:24-84) */
+int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
+                void* stream);
+
+/* BoundaryCondition.apply for periodic / outflow / reflective / free
+ * (fvhoe/boundary_conditions.py:306-402): out has shape (nv, n0+2g0, n1+2g1, n2+2g2);
+ * bc6 = kinds of (x lo, x hi, y lo, y hi, z lo, z hi). */
+int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                   const int* bc6, void* stream);
+
+/* EulerSolver.euler_equation (fvhoe/solver.py:734-751); c* = -(1/h*); active bit d = axis d+1 */
+int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
+                         int n2, int active, double cx, double cy, double cz, void* stream);
+
+/* per-variable (min, max) over ncell values: out[2v] = min, out[2v+1] = max
+ * (the global range of fvhoe/slope_limiting.py:168-170,184-185); scratch 2*nv x u64 */
+int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream);
+
+/* detect_troubled_cells (fvhoe/slope_limiting.py:87-244) on padded primitive
+ * averages w, ws of shape (5, N0, N1, N2) with 3 ghosts on each active axis
+ * (dims bit d = axis d+1).  range_minmax = fvhoe_minmax(w) output (mode global).
+ * mode 0 global / 1 local; range_type 0 relative / 1 absolute; nad_vars bit
+ * mask; pad_var = variable of the last PAD key (-1: none).  Outputs are the
+ * interior cells. */
+int fvhoe_detect_troubled_cells(const double* w, const double* ws, int N0, int N1, int N2, int dims,
+                                const double* range_minmax, int mode, int range_type, double eps, int nad_vars,
+                                int pad_var, double pad_lo, double pad_hi, int sed, double sed_eps,
+                                signed char* trouble, double* nad_mag, double* pad_mag, void* stream);
+
+/* compute_{1,2,3}d_smooth_extrema_detector (fvhoe/smooth_extrema_detection.py:8-95) */
+int fvhoe_smooth_extrema_detector(const double* u, double* alpha, int nv, int N0, int N1, int N2, int dims,
+                                  double eps, void* stream);
+
+/* broadcast_to_troubled_interfaces, simple form (fvhoe/slope_limiting.py:364-380) */
+int fvhoe_broadcast_simple(const signed char* trouble, double* tx, double* ty, double* tz, int n0, int n1, int n2,
+                           int dims, void* stream);
+
+/* out[r] = max(coef * g(s1[r]), s2[r]) over a strided 3-D region, g(x) = x or
+ * (x > 0); the update step of the convex broadcast (fvhoe/slope_limiting.py:387-442) */
+int fvhoe_shift_max(double* out, const long long* out_strides, long long out_offset, const double* s1,
+                    const long long* s1_strides, long long s1_offset, double coef, int gt, const double* s2,
+                    const long long* s2_strides, long long s2_offset, const int* extent, void* stream);
+
+/* F = (1 - theta)*F + theta*Fl, theta broadcast over variables (fvhoe/solver.py:729-731) */
+int fvhoe_blend(double* F, const double* Fl, const double* theta, int nv, long long nface, void* stream);
+
+/* Runge-Kutta combinations, FVHOE_RK_* kinds; dt is a device scalar */
+int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, const double* k1, const double* k2,
+                     const double* k3, const double* dt, long long n, void* stream);
+
+/* device scalars: *p = v; *dt_step = _hit_target(t, dt_stage, target) (fvhoe/ode.py:11-22); *t += *dt */
+int fvhoe_set_scalar(double* p, double v, void* stream);
+int fvhoe_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target,
+                     void* stream);
+int fvhoe_advance_time(double* t, const double* dt, void* stream);
+
+/* ------------------------------------------------------------ fused stage */
+
+/* Everything one stage needs: the EulerSolver arguments that reach
+ * hydrodynamics (fvhoe/solver.py:42-84). */
+typedef struct FvhoeStageConfig {
+  int n[3];             /* interior cells per axis */
+  int p[3];             /* polynomial degree per axis (inactive axis: n = 1, p = 0) */
+  double h[3];          /* mesh spacing */
+  double gamma, cfl, csq_floor;
+  int bc[6];            /* FVHOE_BC_* for x lo, x hi, y lo, y hi, z lo, z hi */
+  int riemann;          /* FVHOE_RS_* */
+  int a_posteriori;     /* a_posteriori_slope_limiting */
+  int limiter;          /* fallback slope limiter, FVHOE_LIM_* */
+  int force_trouble;
+  double nad_eps;
+  int nad_mode;         /* 0 global, 1 local */
+  int nad_range;        /* 0 relative, 1 absolute */
+  int nad_vars;         /* bit mask over rho, vx, vy, vz, P */
+  int pad_var;          /* variable of the last PAD key, -1 for none */
+  double pad_lo, pad_hi;
+  int sed;
+  double sed_eps;
+  int fixed_dt_on;
+  double fixed_dt;
+} FvhoeStageConfig;
+
+/* 1 if fvhoe_stage handles cfg (1-D along x, or 2-D in x-y, nvars = 5,
+ * simple face broadcast), else 0. */
+int fvhoe_stage_supported(const FvhoeStageConfig* cfg);
+
+/* Bytes of device workspace fvhoe_stage needs for cfg. */
+int fvhoe_stage_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes);
+
+/* One EulerSolver.hydrodynamics call (fvhoe/solver.py:429-454) fused into a
+ * few kernels: reads u (5, n0, n1, n2), writes dudt (same shape) and the
+ * stage's CFL dt to *dt_stage (device).  trouble (n0*n1*n2 int8, may be NULL)
+ * receives the a-posteriori flags.  workspace: fvhoe_stage_workspace_bytes. */
+int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
+                void* workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FVHOE_HIP_H */
