@@ -1,0 +1,69 @@
+"""Stencil sweeps on device arrays (fvhoe/fv.py:106-224).
+
+Each function takes a 4-D (nvars, nx, ny, nz) float64 array -- a torch
+tensor on the GPU, or a NumPy array, in which case the result comes back as
+NumPy -- and runs one HIP kernel per sweep.
+"""
+
+from __future__ import annotations
+
+from fvhoe_amd import _lib
+from fvhoe_amd.stencils import get_stencil_size
+
+
+def _as4(u):
+    """Device tensor viewed as 4-D plus the axis shift for arrays of lower rank."""
+    x, was_np = _lib.as_device(u)
+    shift = 4 - x.dim()
+    if shift < 0:
+        raise ValueError(f"arrays of rank {x.dim()} > 4 are not supported")
+    return x.reshape((1,) * shift + tuple(x.shape)), shift, was_np, tuple(x.shape)
+
+
+def stencil_sweep(u, mode: str, p: int, axis: int, pos: str = None):
+    """out = sum_i w_i u[i : len-(k-1)+i] along axis (fvhoe/fv.py:106-140)."""
+    if mode == "interpolation":
+        if pos not in ("l", "c", "r"):
+            raise ValueError(f"Invalid position '{pos}'.")
+        kind = _lib.STENCIL[pos]
+    elif mode == "reconstruction":
+        kind = _lib.STENCIL["t"]
+    else:
+        raise ValueError(f"{mode=}")
+    if p < 0 or p > 8:
+        raise NotImplementedError(f"p={p}")
+    x, shift, was_np, shape = _as4(u)
+    a4 = axis + shift
+    k = 2 * get_stencil_size(p) + 1
+    oshape = list(x.shape)
+    oshape[a4] -= k - 1
+    if oshape[a4] < 1:
+        raise ValueError(f"axis {axis} of length {x.shape[a4]} is shorter than the {k}-point stencil")
+    out = _lib.empty(oshape)
+    _lib.call("fvhoe_stencil_sweep", _lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, a4, _lib.stream())
+    out = out.reshape(tuple(oshape[shift:]))
+    return _lib.back(out, was_np)
+
+
+def conservative_interpolation(u, p: int, axis: int, pos: str = "c"):
+    """fvhoe/fv.py:143-159."""
+    return stencil_sweep(u, "interpolation", p, axis, pos)
+
+
+def transverse_reconstruction(u, p: int, axis: int):
+    """fvhoe/fv.py:162-172."""
+    return stencil_sweep(u, "reconstruction", p, axis)
+
+
+def interpolate_cell_centers(u, p=(0, 0, 0)):
+    """c-sweeps x, then y, then z (fvhoe/fv.py:175-197)."""
+    out = conservative_interpolation(u, p[0], 1, "c")
+    out = conservative_interpolation(out, p[1], 2, "c")
+    return conservative_interpolation(out, p[2], 3, "c")
+
+
+def interpolate_fv_averages(u, p=(0, 0, 0)):
+    """transverse sweeps x, then y, then z (fvhoe/fv.py:200-224)."""
+    out = transverse_reconstruction(u, p[0], 1)
+    out = transverse_reconstruction(out, p[1], 2)
+    return transverse_reconstruction(out, p[2], 3)
