@@ -65,6 +65,8 @@ _SIGNATURES = {
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
     "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),
+    "fvhoe_stage_timing": (c_int, [c_int]),
+    "fvhoe_stage_timing_read": (c_int, [POINTER(c_double), POINTER(c_longlong)]),
 }
 
 _lib = None

@@ -456,7 +456,7 @@ __global__ void k_blend(double* __restrict__ F, const double* __restrict__ Fl, c
 }
 
 // Runge-Kutta combinations in the reference's k-form (fvhoe/ode.py:245-298); dt read from device memory
-__global__ void k_rk(int kind, double* __restrict__ out, const double* __restrict__ u, const double* __restrict__ k0,
+__global__ void k_rk(int kind, double* out, const double* u, const double* __restrict__ k0,
                      const double* __restrict__ k1, const double* __restrict__ k2, const double* __restrict__ k3,
                      const double* __restrict__ dtp, long long n) {
   const double dt = dtp[0];

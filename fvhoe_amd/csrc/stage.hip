@@ -463,6 +463,36 @@ __global__ void k_stage_dt(const unsigned long long* dt_acc, const unsigned long
 
 // ------------------------------------------------------------------ host side
 
+// Optional per-kernel event timing (bench instrumentation): each launch of a
+// stage kernel is bracketed by a pair of events from a fixed pool.
+enum { TK_PRIMS = 0, TK_FLUX_X = 1, TK_FLUX_Y = 2, TK_PRIMS_CAND = 3, TK_DETECT = 4, TK_FINAL = 5, TK_COUNT = 6 };
+constexpr int TIMING_POOL = 8192;
+struct Timing {
+  bool on = false;
+  int used = 0;
+  int kind[TIMING_POOL];
+  hipEvent_t a[TIMING_POOL], b[TIMING_POOL];
+  bool created = false;
+  double ms[TK_COUNT] = {0, 0, 0, 0, 0, 0};
+  long long count[TK_COUNT] = {0, 0, 0, 0, 0, 0};
+};
+Timing g_timing;
+
+struct TimedLaunch {
+  int slot = -1;
+  hipStream_t s;
+  TimedLaunch(int k, hipStream_t st) : s(st) {
+    if (g_timing.on && g_timing.used < TIMING_POOL) {
+      slot = g_timing.used++;
+      g_timing.kind[slot] = k;
+      (void)hipEventRecord(g_timing.a[slot], s);
+    }
+  }
+  ~TimedLaunch() {
+    if (slot >= 0) (void)hipEventRecord(g_timing.b[slot], s);
+  }
+};
+
 struct Layout {
   int two_d, p, G, GE, n0, n1, NW0, NW1, NS0, NS1;
   size_t offW, offF, offG, offWS, offFlags, offScal, total;
@@ -546,6 +576,7 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
 
   {
     const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
+    TimedLaunch tl(TK_PRIMS, s);
     k_prims<P, PY, TX, TY, SrcU><<<t0 * t1, TPB, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
@@ -554,11 +585,14 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
   if (TWO_D) {
     constexpr int TA0 = 16, TB0 = 32, TA1 = 32, TB1 = 16;
     const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
-    k_flux<0, P, PY, TA0, TB0><<<tx * ty, TPB, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->riemann, c->gamma, c->csq_floor);
+    { TimedLaunch tl(TK_FLUX_X, s);
+    k_flux<0, P, PY, TA0, TB0><<<tx * ty, TPB, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->riemann, c->gamma, c->csq_floor); }
     const int fy = (L.n1 + 1 + TA1 - 1) / TA1, rx = (L.n0 + TB1 - 1) / TB1;
-    k_flux<1, PY, P, TA1, TB1><<<fy * rx, TPB, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->riemann, c->gamma, c->csq_floor);
+    { TimedLaunch tl(TK_FLUX_Y, s);
+    k_flux<1, PY, P, TA1, TB1><<<fy * rx, TPB, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->riemann, c->gamma, c->csq_floor); }
   } else {
     const int tx = (L.n0 + 1 + 255) / 256;
+    TimedLaunch tl(TK_FLUX_X, s);
     k_flux<0, P, 0, 256, 1><<<tx, TPB, 0, s>>>(W, L.NW0, L.NW1, g, F, 1, c->riemann, c->gamma, c->csq_floor);
   }
 
@@ -571,15 +605,19 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
     gs.ge1 = TWO_D ? 3 : 0;
     const int t0 = (L.NS0 + TX - 1) / TX, t1 = (L.NS1 + TY - 1) / TY;
     Reduce none{};
+    { TimedLaunch tl(TK_PRIMS_CAND, s);
     k_prims<P, PY, TX, TY, SrcUstar<TWO_D>><<<t0 * t1, TPB, 0, s>>>(SrcUstar<TWO_D>{u, F, G, dt_stage}, gs, WS,
-                                                                      L.NS0, L.NS1, t1, none, c->gamma);
+                                                                      L.NS0, L.NS1, t1, none, c->gamma); }
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
-    k_detect<<<gridc, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, any);
-    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, flags, any, SrcU{u}, g, fp, dudt);
+    { TimedLaunch tl(TK_DETECT, s);
+    k_detect<<<gridc, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, any); }
+    { TimedLaunch tl(TK_FINAL, s);
+    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, flags, any, SrcU{u}, g, fp, dudt); }
     if (trouble_out && hipMemcpyAsync(trouble_out, flags, (size_t)nc, hipMemcpyDeviceToDevice, s) != hipSuccess)
       return FVHOE_ERR_LAUNCH;
   } else {
+    TimedLaunch tl(TK_FINAL, s);
     k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, nullptr, any, SrcU{u}, g, fp, dudt);
   }
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
@@ -616,6 +654,34 @@ int fvhoe_stage_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long*
   if (st != FVHOE_OK) return st;
   if (!bytes) return FVHOE_ERR_ARG;
   *bytes = L.total;
+  return FVHOE_OK;
+}
+
+int fvhoe_stage_timing(int on) {
+  if (on && !g_timing.created) {
+    for (int i = 0; i < TIMING_POOL; ++i) {
+      if (hipEventCreate(&g_timing.a[i]) != hipSuccess || hipEventCreate(&g_timing.b[i]) != hipSuccess)
+        return FVHOE_ERR_LAUNCH;
+    }
+    g_timing.created = true;
+  }
+  g_timing.on = on != 0;
+  g_timing.used = 0;
+  for (int k = 0; k < TK_COUNT; ++k) { g_timing.ms[k] = 0.0; g_timing.count[k] = 0; }
+  return FVHOE_OK;
+}
+
+int fvhoe_stage_timing_read(double* ms, long long* count) {
+  if (!ms || !count) return FVHOE_ERR_ARG;
+  for (int i = 0; i < g_timing.used; ++i) {
+    float t = 0.0f;
+    if (hipEventSynchronize(g_timing.b[i]) != hipSuccess) return FVHOE_ERR_LAUNCH;
+    if (hipEventElapsedTime(&t, g_timing.a[i], g_timing.b[i]) != hipSuccess) return FVHOE_ERR_LAUNCH;
+    g_timing.ms[g_timing.kind[i]] += t;
+    g_timing.count[g_timing.kind[i]] += 1;
+  }
+  g_timing.used = 0;
+  for (int k = 0; k < TK_COUNT; ++k) { ms[k] = g_timing.ms[k]; count[k] = g_timing.count[k]; }
   return FVHOE_OK;
 }
 

@@ -1,0 +1,184 @@
+"""Time integration with device-resident state (fvhoe/ode.py).
+
+The steppers keep the reference's k-form arithmetic (fvhoe/ode.py:245-298) in
+one HIP kernel per combination; dt never leaves the device inside a step.
+The time loop reads the step's dt back once per step (the reference syncs
+twice per stage), or not at all for a fixed step count.
+"""
+
+from __future__ import annotations
+
+import time
+from abc import ABC, abstractmethod
+from typing import Any, Optional
+
+import numpy as np
+
+from fvhoe_amd import _lib
+
+
+def _hit_target(t: float, dt: float, target: float = None) -> float:
+    """fvhoe/ode.py:11-22."""
+    if target is not None and target - t < dt:
+        return target - t
+    return dt
+
+
+class ODE(ABC):
+    """u' = f(t, u) with u a device tensor (fvhoe/ode.py:25-298)."""
+
+    def __init__(self, u0, progress_bar: bool = False):
+        self.t = 0.0
+        self.timestamps = [0.0]
+        self.step_count = 0
+        self.u = _lib.as_device(u0)[0].clone()
+        self.snapshots = []
+        self.snapshot_times = []
+        self.print_progress_bar = bool(progress_bar)
+        self.timer = {"TOTAL": 0.0, "SNAPSHOTS": 0.0}
+        n = self.u.numel()
+        self._k = [_lib.empty(self.u.shape) for _ in range(4)]
+        self._s = _lib.empty(self.u.shape)
+        self._scal = _lib.zeros((8,))  # dt_stage x4, dt_step, t, spare
+        self._n = n
+
+    # ------------------------------------------------------------ to override
+    @abstractmethod
+    def stage(self, u, k_out, dt_out) -> None:
+        """k_out = dudt(u); dt_out[0] = this stage's CFL dt (device)."""
+
+    def f(self, t: float, u):
+        """(dt, dudt) with dt a Python float (fvhoe/ode.py:234-243)."""
+        k = _lib.empty(u.shape)
+        dt = _lib.empty((1,))
+        self.stage(u, k, dt)
+        return float(dt.item()), k
+
+    def snapshot(self):
+        self.snapshots.append(None)
+        self.snapshot_times.append(self.t)
+
+    def step_helper_function(self):
+        pass
+
+    def read_snapshots(self) -> bool:
+        return False
+
+    def write_snapshots(self, overwrite: bool = False):
+        pass
+
+    # ---------------------------------------------------------------- steppers
+    def _combine(self, kind, out, u, k0, k1=None, k2=None, k3=None, dt=None):
+        _lib.call("fvhoe_rk_combine", kind, _lib.ptr(out), _lib.ptr(u), _lib.ptr(k0), _lib.ptr(k1), _lib.ptr(k2),
+                  _lib.ptr(k3), _lib.ptr(dt), self._n, _lib.stream())
+
+    def _device_step(self, target: Optional[float], t_dev):
+        """One step of self.integrator on self.u; returns the dt_step scalar view."""
+        u, k, s = self.u, self._k, self._s
+        sc = self._scal
+        dts, dt_step = sc[0:1], sc[4:5]
+        name = self.integrator
+        self.stage(u, k[0], dts)
+        _lib.call("fvhoe_hit_target", _lib.ptr(dt_step), _lib.ptr(dts), _lib.ptr(t_dev),
+                  float(target) if target is not None else 0.0, int(target is not None), _lib.stream())
+        if name == "euler":
+            self._combine(_lib.RK_AXPY, u, u, k[0], dt=dt_step)
+        elif name == "ssprk2":
+            self._combine(_lib.RK_AXPY, s, u, k[0], dt=dt_step)
+            self.stage(s, k[1], sc[1:2])
+            self._combine(_lib.RK_SSPRK2_FINAL, u, u, s, k[1], dt=dt_step)
+        elif name == "ssprk3":
+            self._combine(_lib.RK_AXPY, s, u, k[0], dt=dt_step)
+            self.stage(s, k[1], sc[1:2])
+            self._combine(_lib.RK_SSPRK3_STAGE3, s, u, k[0], k[1], dt=dt_step)
+            self.stage(s, k[2], sc[2:3])
+            self._combine(_lib.RK_SSPRK3_FINAL, u, u, k[0], k[1], k[2], dt=dt_step)
+        elif name == "rk4":
+            self._combine(_lib.RK_HALF_AXPY, s, u, k[0], dt=dt_step)
+            self.stage(s, k[1], sc[1:2])
+            self._combine(_lib.RK_HALF_AXPY, s, u, k[1], dt=dt_step)
+            self.stage(s, k[2], sc[2:3])
+            self._combine(_lib.RK_AXPY, s, u, k[2], dt=dt_step)
+            self.stage(s, k[3], sc[3:4])
+            self._combine(_lib.RK_RK4_FINAL, u, u, k[0], k[1], k[2], k[3], dt=dt_step)
+        else:
+            raise ValueError(name)
+        _lib.call("fvhoe_advance_time", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.stream())
+        return dt_step
+
+    def take_step(self, target_time: float = None):
+        """fvhoe/ode.py:183-193: one step, dt read back to advance the host clock."""
+        t_dev = self._scal[5:6]
+        _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
+        dt = float(self._device_step(target_time, t_dev).item())
+        self.t += dt
+        self.timestamps.append(self.t)
+        self.step_count += 1
+        self.step_helper_function()
+
+    def advance(self, n: int):
+        """n steps with no host synchronisation (time kept on the device)."""
+        t_dev = self._scal[5:6]
+        _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
+        for _ in range(n):
+            self._device_step(None, t_dev)
+            self.step_count += 1
+            self.step_helper_function()
+        self.t = float(t_dev.item())
+        self.timestamps.append(self.t)
+
+    def integrate(self, T: float = None, n: int = None, exact: bool = True, downbeats: Any = None,
+                  log_every_step: bool = False, snapshot_dir: str = None, overwrite: bool = False) -> None:
+        """fvhoe/ode.py:94-174."""
+        if T is None:
+            self.snapshot()
+            t0 = time.time()
+            for _ in range(n):
+                self.take_step()
+            _lib.torch().cuda.synchronize()
+            self.timer["TOTAL"] += time.time() - t0
+            self.snapshot()
+            return
+        save = snapshot_dir is not None
+        if save:
+            import os
+
+            self.snapshot_dir = os.path.normpath(snapshot_dir)
+            if not overwrite and self.read_snapshots():
+                return
+        if downbeats is None:
+            downbeats = []
+        elif isinstance(downbeats, np.ndarray):
+            downbeats = downbeats.tolist()
+        downbeats = sorted(set([0] + list(downbeats) + [T]))[1:]
+        target = downbeats.pop(0) if exact else None
+        self.snapshot()
+        t0 = time.time()
+        while self.t < T:
+            self.take_step(target_time=target)
+            if self.t == target or self.t >= T or log_every_step:
+                ts = time.time()
+                self.snapshot()
+                self.timer["SNAPSHOTS"] += time.time() - ts
+                if downbeats and self.t == target:
+                    target = downbeats.pop(0)
+        _lib.torch().cuda.synchronize()
+        self.timer["TOTAL"] += time.time() - t0
+        if save:
+            self.write_snapshots(overwrite)
+
+    def euler(self, *args, **kwargs):
+        self.integrator = "euler"
+        self.integrate(*args, **kwargs)
+
+    def ssprk2(self, *args, **kwargs):
+        self.integrator = "ssprk2"
+        self.integrate(*args, **kwargs)
+
+    def ssprk3(self, *args, **kwargs):
+        self.integrator = "ssprk3"
+        self.integrate(*args, **kwargs)
+
+    def rk4(self, *args, **kwargs):
+        self.integrator = "rk4"
+        self.integrate(*args, **kwargs)

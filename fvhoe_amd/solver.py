@@ -1,0 +1,383 @@
+"""EulerSolver on MI355X (fvhoe/solver.py:41-1001).
+
+Same constructor arguments, steppers and snapshot format as the reference.
+Each RK stage runs either
+
+* the fused stage (`fvhoe_stage` in the C-ABI: a handful of LDS-tiled HIP
+  kernels) for 1-D runs along x and 2-D runs in x-y with 5 variables, the same
+  degree on both axes and the simple face broadcast, or
+* the composed stage, which follows hydrofluxes / revise_fluxes /
+  euler_equation (fvhoe/solver.py:456-751) call for call, one HIP kernel per
+  array function, for every other configuration (3-D, passives, mixed
+  degrees, convex blending, x-z / y-z planes).
+
+Both produce the reference's FP64 results bit for bit.
+"""
+
+from __future__ import annotations
+
+import json
+import os
+import pickle
+import shutil
+from typing import Callable, Dict, Iterable, Tuple
+
+import numpy as np
+
+from fvhoe_amd import _lib
+from fvhoe_amd.boundary_conditions import BoundaryCondition
+from fvhoe_amd.fv import conservative_interpolation, interpolate_cell_centers, interpolate_fv_averages
+from fvhoe_amd.fv import transverse_reconstruction
+from fvhoe_amd.hydro import HydroState, compute_primitives, hydro_dt_device
+from fvhoe_amd.initial_conditions import InitialCondition, fv_average, primitives_to_conservatives, uniform_mesh
+from fvhoe_amd.ode import ODE
+from fvhoe_amd.slope_limiting import (MUSCL_interpolations, broadcast_to_troubled_interfaces, detect_troubled_cells,
+                                      nad_mask, pad_spec)
+from fvhoe_amd.stencils import get_stencil_size
+
+
+class EulerSolver(ODE):
+    def __init__(
+        self,
+        w0: InitialCondition = None,
+        w0_passives: Dict[str, Callable] = None,
+        nx: int = 1, ny: int = 1, nz: int = 1,
+        px: int = 0, py: int = 0, pz: int = 0,
+        x: Tuple[float, float] = (0, 1), y: Tuple[float, float] = (0, 1), z: Tuple[float, float] = (0, 1),
+        CFL: float = 0.8,
+        fixed_dt: float = None,
+        gamma: float = 5 / 3,
+        bc: BoundaryCondition = None,
+        riemann_solver: str = "hllc",
+        conservative_ic: bool = False,
+        fv_ic: bool = False,
+        fixed_primitive_variables: Iterable = None,
+        a_posteriori_slope_limiting: bool = False,
+        slope_limiter: str = "minmod",
+        force_trouble: bool = False,
+        NAD: float = 1e-2,
+        NAD_mode: str = "global",
+        NAD_range: str = "relative",
+        NAD_vars: tuple = None,
+        PAD: dict = None,
+        SED: bool = True,
+        SED_tolerance: float = 1e-10,
+        convex: bool = False,
+        density_floor: float = None,
+        pressure_floor: float = None,
+        csq_floor: float = 1e-10,
+        progress_bar: bool = True,
+        snapshots_as_fv_averages: bool = True,
+        snapshot_helper_function: callable = None,
+        slab_buffer_size: int = 30,
+        cupy: bool = False,
+        fused: bool = None,
+    ):
+        if w0 is None:
+            raise ValueError("w0 (an InitialCondition) is required")
+        if riemann_solver not in _lib.RIEMANN:
+            raise TypeError(f"Invalid Riemann solver {riemann_solver}")
+        if slope_limiter not in _lib.LIMITER:
+            raise ValueError(f"Unknown slope limiter: {slope_limiter}")
+        self.csq_floor = csq_floor
+        self.density_floor = density_floor  # stored, never applied (as in the reference)
+        self.pressure_floor = pressure_floor
+        self.gamma = gamma
+        self.riemann_solver_name = riemann_solver
+        self.snapshot_helper_function = snapshot_helper_function
+        self.snapshots_as_fv_averages = snapshots_as_fv_averages
+        self.CFL, self.fixed_dt = CFL, fixed_dt
+        self.x_domain, self.y_domain, self.z_domain = x, y, z
+        self.n = (nx, ny, nz)
+        self.p = (px, py, pz)
+        self.h = ((x[1] - x[0]) / nx, (y[1] - y[0]) / ny, (z[1] - z[0]) / nz)
+        self.X, self.Y, self.Z = uniform_mesh(self.n, x, y, z)
+        self.x, self.y, self.z = self.X[:, 0, 0], self.Y[0, :, 0], self.Z[0, 0, :]
+        self.xdim = not (nx == 1 and px == 0)
+        self.ydim = not (ny == 1 and py == 0)
+        self.zdim = not (nz == 1 and pz == 0)
+        self.active = (self.xdim, self.ydim, self.zdim)
+        self.dims = "".join(d for d, a in zip("xyz", self.active) if a)
+        self.ndim = len(self.dims)
+
+        passives = tuple(w0_passives.keys()) if w0_passives is not None else ()
+        self.hydro_state = HydroState(passive_scalars=passives)
+        self.nvars = self.hydro_state.nvars
+        ic = w0.build_ic(passives, w0_passives)
+        if conservative_ic and fv_ic:
+            u0 = ic(self.X, self.Y, self.Z)
+        elif fv_ic:
+            raise ValueError("Can't have finite volume averages without conservative IC.")
+        else:
+            point = ic if conservative_ic else (lambda xx, yy, zz: primitives_to_conservatives(ic(xx, yy, zz), gamma))
+            u0 = fv_average(point, self.X, self.Y, self.Z, self.h, self.p)
+        super().__init__(u0, progress_bar=progress_bar)
+        self.f_evaluation_count = 0
+
+        self.bc = BoundaryCondition() if bc is None else BoundaryCondition(x=bc.x, y=bc.y, z=bc.z)
+        self.a_posteriori_slope_limiting = a_posteriori_slope_limiting
+        self.slope_limiter = slope_limiter
+        self.force_trouble = force_trouble
+        self.NAD, self.NAD_mode, self.NAD_range = NAD, NAD_mode, NAD_range
+        self.NAD_vars = NAD_vars if isinstance(NAD_vars, tuple) or NAD_vars is None else tuple(NAD_vars)
+        self.PAD = {"rho": (0.0, np.inf), "P": (0.0, np.inf)} if PAD is None else PAD
+        self.SED, self.SED_tolerance, self.convex = SED, SED_tolerance, convex
+        self.trouble_counter = 1  # fvhoe/solver.py:400
+        nx_, ny_, nz_ = self.n
+        self.log = {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag", "mean PAD mag")}
+
+        self._cfg = self._stage_config()
+        supported = bool(_lib.lib().fvhoe_stage_supported(self._cfg)) and self.nvars == 5 and not convex
+        self.fused = supported if fused is None else (bool(fused) and supported)
+        if fused and not supported:
+            raise NotImplementedError("the fused stage does not cover this configuration")
+        self._ws = None
+        self._trouble = _lib.empty(self.n, dtype=_lib.torch().int8) if a_posteriori_slope_limiting else None
+
+    # ------------------------------------------------------------------ config
+    def _stage_config(self):
+        c = _lib.FvhoeStageConfig()
+        c.n[:] = list(self.n)
+        c.p[:] = list(self.p)
+        c.h[:] = list(self.h)
+        c.gamma, c.cfl, c.csq_floor = float(self.gamma), float(self.CFL), float(self.csq_floor)
+        c.bc[:] = self.bc.codes()
+        c.riemann = _lib.RIEMANN[self.riemann_solver_name]
+        c.a_posteriori = int(bool(self.a_posteriori_slope_limiting))
+        c.limiter = 3 if self.slope_limiter is None else _lib.LIMITER[self.slope_limiter]
+        c.force_trouble = int(bool(self.force_trouble))
+        c.nad_eps = float(self.NAD)
+        if self.NAD_mode not in ("global", "local"):
+            raise ValueError(f"Unknown mode: {self.NAD_mode}")
+        if self.NAD_range not in ("relative", "absolute"):
+            raise ValueError(f"Unknown range_type: {self.NAD_range}")
+        c.nad_mode = 0 if self.NAD_mode == "global" else 1
+        c.nad_range = 0 if self.NAD_range == "relative" else 1
+        c.nad_vars = nad_mask(self.NAD_vars)
+        c.pad_var, c.pad_lo, c.pad_hi = pad_spec(self.PAD)
+        c.sed = int(bool(self.SED))
+        c.sed_eps = float(self.SED_tolerance)
+        c.fixed_dt_on = int(self.fixed_dt is not None)
+        c.fixed_dt = float(self.fixed_dt or 0.0)
+        return c
+
+    # ------------------------------------------------------------------- stage
+    def stage(self, u, k_out, dt_out) -> None:
+        """One RHS evaluation into k_out, this stage's dt into dt_out[0] (device)."""
+        self.f_evaluation_count += 1
+        if self.fused:
+            if self._ws is None:
+                nbytes = _lib.ctypes.c_ulonglong(0)
+                _lib.call("fvhoe_stage_workspace_bytes", _lib.ctypes.byref(self._cfg), _lib.ctypes.byref(nbytes))
+                self._ws = _lib.empty((nbytes.value + 256,), dtype=_lib.torch().uint8)
+            base = self._ws.data_ptr()
+            off = (-base) % 256
+            _lib.call("fvhoe_stage", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out), _lib.ptr(dt_out),
+                      _lib.ptr(self._trouble), _lib.ctypes.c_void_p(base + off), _lib.stream())
+            if self._trouble is not None:
+                self._log_troubles_device(self._trouble)
+        else:
+            dt, (F, G, H) = self.hydrofluxes(u, self.p)
+            if self.a_posteriori_slope_limiting:
+                F, G, H = self.revise_fluxes(u, F, G, H, dt, self.force_trouble)
+            self._euler_equation_into(F, G, H, k_out)
+            dt_out.copy_(dt)
+
+    def hydrodynamics(self, u):
+        """(dt, dudt) of one stage (fvhoe/solver.py:429-454)."""
+        x, was_np = _lib.as_device(u)
+        k = _lib.empty(x.shape)
+        dt = _lib.empty((1,))
+        self.stage(x, k, dt)
+        return float(dt.item()), _lib.back(k, was_np)
+
+    # ------------------------------------------------------- composed stage
+    def interpolate_primitives_from_conservatives(self, u, p, gw=(0, 0, 0), fv_average: bool = True):
+        """fvhoe/solver.py:753-792."""
+        cost = [(2 if fv_average else 1) * get_stencil_size(pi) for pi in p]
+        g = tuple(int(gw[i]) + cost[i] for i in range(3))
+        u_bc = self.bc.apply(u, g, t=self.t)
+        w = compute_primitives(self.hydro_state, interpolate_cell_centers(u_bc, p), self.gamma)
+        return interpolate_fv_averages(w, p) if fv_average else w
+
+    def hydrofluxes(self, u, p, slope_limiter: str = None):
+        """dt (device), (F, G, H) (fvhoe/solver.py:456-651)."""
+        if slope_limiter is None:
+            limit = False
+        elif max(p) <= 1:
+            limit = True
+        else:
+            raise ValueError("Slope limiting is only implemented for degree 1 polynomial interpolation.")
+        pmax = max(p)
+        cm = get_stencil_size(pmax)
+        gw = max(cm + 1, 2 * cm)
+        w_bc = self.interpolate_primitives_from_conservatives(u, p, tuple(gw if a else 0 for a in self.active))
+        if self.fixed_dt is None:
+            dt = hydro_dt_device(w_bc, min(self.h), self.ndim, self.CFL, self.gamma, self.csq_floor)
+        else:
+            dt = _lib.empty((1,))
+            dt.fill_(float(self.fixed_dt))
+        order = {0: (3, 2), 1: (1, 3), 2: (2, 1)}
+        tr_order = {0: (2, 3), 1: (1, 3), 2: (1, 2)}
+        out = []
+        rs = _lib.RIEMANN[self.riemann_solver_name]
+        for d in range(3):
+            if not self.active[d]:
+                out.append(None)
+                continue
+            a = d + 1
+            wt = w_bc
+            for ax in order[d]:
+                wt = conservative_interpolation(wt, p[ax - 1], ax, "c")
+            if limit and p[d] == 1:
+                wl_face, wr_face = MUSCL_interpolations(wt, a, slope_limiter)
+            else:
+                wl_face = conservative_interpolation(wt, p[d], a, "l")
+                wr_face = conservative_interpolation(wt, p[d], a, "r")
+            f = self._riemann_faces(wr_face, wl_face, a, rs)
+            for ax in tr_order[d]:
+                f = transverse_reconstruction(f, p[ax - 1], ax)
+            excess = [0, 0, 0, 0]
+            for e in range(3):
+                if self.active[e]:
+                    excess[e + 1] = gw - get_stencil_size(p[e]) - 1 if e == d else gw - 2 * get_stencil_size(p[e])
+            f = f[tuple(slice(x or None, -x or None) for x in excess)].contiguous()
+            out.append(f)
+        return dt, tuple(out)
+
+    def _riemann_faces(self, wr_face, wl_face, axis, rs):
+        """RS(wr_face[:-1], wl_face[1:]) along axis (fvhoe/solver.py:573-606)."""
+        nv, n0, n1, n2 = wr_face.shape
+        shape = [nv, n0, n1, n2]
+        shape[axis] -= 1
+        out = _lib.empty(shape)
+        if nv == 5:
+            _lib.call("fvhoe_riemann_faces", _lib.ptr(wr_face), _lib.ptr(wl_face), _lib.ptr(out), n0, n1, n2, axis, rs,
+                      float(self.gamma), float(self.csq_floor), _lib.stream())
+        else:
+            sl_l = [slice(None)] * 4
+            sl_r = [slice(None)] * 4
+            sl_l[axis] = slice(None, -1)
+            sl_r[axis] = slice(1, None)
+            a = wr_face[tuple(sl_l)].contiguous()
+            b = wl_face[tuple(sl_r)].contiguous()
+            nc = a[0].numel()
+            _lib.call("fvhoe_riemann", _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), nv, nc, rs, axis - 1,
+                      float(self.gamma), float(self.csq_floor), _lib.stream())
+        return out
+
+    def revise_fluxes(self, u, F, G, H, dt, force_trouble: bool = False):
+        """A-posteriori check and MUSCL fallback blend (fvhoe/solver.py:653-732);
+        dt is the stage's device dt.  Returns the revised (F, G, H)."""
+        L = self.euler_equation(F, G, H)
+        ustar = _lib.empty(u.shape)
+        _lib.call("fvhoe_rk_combine", _lib.RK_AXPY, _lib.ptr(ustar), _lib.ptr(u), _lib.ptr(L), None, None, None,
+                  _lib.ptr(dt), u.numel(), _lib.stream())
+        gw = tuple(3 if a else 0 for a in self.active)
+        w = self.interpolate_primitives_from_conservatives(u, self.p, gw)
+        ws = self.interpolate_primitives_from_conservatives(ustar, self.p, gw)
+        trouble, nad_mag, pad_mag = detect_troubled_cells(
+            w[:5].contiguous(), ws[:5].contiguous(), self.dims, self.NAD, self.NAD_mode, self.NAD_range,
+            self.NAD_vars, self.PAD, self.SED, self.SED_tolerance)
+        if force_trouble:
+            trouble = _lib.torch().ones_like(trouble)
+        self.log_troubles(trouble, nad_mag, pad_mag)
+        if bool(trouble.any().item()):
+            p1 = tuple(int(pi > 0) for pi in self.p)
+            _, (Fl, Gl, Hl) = self.hydrofluxes(u, p1, slope_limiter=self.slope_limiter)
+            per = self.bc.periodic()
+            tx, ty, tz = broadcast_to_troubled_interfaces(trouble, self.dims, self.convex, *per)
+            F = self._blend(F, Fl, tx)
+            G = self._blend(G, Gl, ty)
+            H = self._blend(H, Hl, tz)
+        return F, G, H
+
+    def _blend(self, F, Fl, th):
+        if F is None:
+            return None
+        out = F.clone()
+        th = th.contiguous()
+        _lib.call("fvhoe_blend", _lib.ptr(out), _lib.ptr(Fl), _lib.ptr(th), F.shape[0], th.numel(), _lib.stream())
+        return out
+
+    def _euler_equation_into(self, F, G, H, out):
+        nv, n0, n1, n2 = out.shape
+        active = sum(1 << d for d in range(3) if self.active[d])
+        _lib.call("fvhoe_euler_equation", _lib.ptr(F), _lib.ptr(G), _lib.ptr(H), _lib.ptr(out), nv, n0, n1, n2,
+                  active, -(1 / self.h[0]), -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
+
+    def euler_equation(self, F, G, H):
+        """dudt from face fluxes (fvhoe/solver.py:734-751)."""
+        out = _lib.empty((self.nvars, *self.n))
+        self._euler_equation_into(F, G, H, out)
+        return out
+
+    # ------------------------------------------------------------ bookkeeping
+    def log_troubles(self, trouble=None, NAD_violation_magnitude=None, PAD_violation_magnitude=None,
+                     reset: bool = False):
+        """Per-step means of trouble and violation magnitudes (fvhoe/solver.py:794-833)."""
+        lg = self.log
+        if reset:
+            if self.trouble_counter > 0:
+                for k in lg:
+                    lg[k] = lg[k] / self.trouble_counter
+                self.trouble_counter = 0
+            return
+        vals = {"mean trouble": trouble.to(_lib.torch().float64),
+                "mean NAD mag": NAD_violation_magnitude, "mean PAD mag": PAD_violation_magnitude}
+        for k, v in vals.items():
+            if v is None:
+                continue
+            lg[k] = v.clone() if self.trouble_counter == 0 else lg[k] + v
+        self.trouble_counter += 1
+
+    def _log_troubles_device(self, flags):
+        self.log_troubles(flags, None, None)
+
+    def step_helper_function(self):
+        if self.a_posteriori_slope_limiting:
+            self.log_troubles(reset=True)
+
+    def snapshot(self):
+        """fvhoe/solver.py:840-874."""
+        w = self.interpolate_primitives_from_conservatives(self.u, self.p, fv_average=self.snapshots_as_fv_averages)
+        rec = {"x": self.x, "y": self.y, "z": self.z, "t": self.t, "w": w.cpu().numpy()}
+        if self.a_posteriori_slope_limiting:
+            for key, name in (("mean trouble", "trouble"), ("mean NAD mag", "NAD violation magnitude"),
+                              ("mean PAD mag", "PAD violation magnitude")):
+                rec[name] = np.broadcast_to(self.log[key].cpu().numpy(), (5, *self.n)).copy()
+        self.snapshots.append(rec)
+        self.snapshot_times.append(self.t)
+        if self.snapshot_helper_function is not None:
+            self.snapshot_helper_function(self)
+
+    def read_snapshots(self) -> bool:
+        if os.path.exists(self.snapshot_dir):
+            with open(os.path.join(self.snapshot_dir, "arrs.pkl"), "rb") as f:
+                self.snapshots = pickle.load(f)
+            return True
+        return False
+
+    def write_snapshots(self, overwrite: bool):
+        """arrs.pkl, attrs.json, timing.txt (fvhoe/solver.py:889-953)."""
+        d = self.snapshot_dir
+        if os.path.exists(d) and not overwrite:
+            raise FileExistsError(f"Snapshot directory {d} already exists.")
+        if os.path.exists(d):
+            shutil.rmtree(d)
+        os.makedirs(d)
+        with open(os.path.join(d, "arrs.pkl"), "wb") as f:
+            pickle.dump(self.snapshots, f)
+        attrs = {k: v for k, v in self.__dict__.items()
+                 if isinstance(v, (int, float, str, bool, tuple, type(None))) and not k.startswith("_")}
+        attrs["PAD"] = {k: [{np.inf: "inf", -np.inf: "-inf"}.get(x, x) for x in v] for k, v in self.PAD.items()}
+        attrs["bc"] = self.bc.to_dict()
+        with open(os.path.join(d, "attrs.json"), "w") as f:
+            json.dump(dict(sorted(attrs.items(), key=lambda kv: (kv[0].lower(), kv[0]))), f, indent=4, default=str)
+        with open(os.path.join(d, "timing.txt"), "w") as f:
+            f.write("".join(f"{k}: {v:.6f} s\n" for k, v in self.timer.items()))
+
+    def run(self, *args, **kwargs):
+        """RK order matched to max(p), capped at RK4 (fvhoe/solver.py:955-974)."""
+        q = min(max(self.p), 3)
+        {0: self.euler, 1: self.ssprk2, 2: self.ssprk3, 3: self.rk4}[q](*args, **kwargs)

@@ -182,6 +182,16 @@ int fvhoe_stage_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long*
 int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
                 void* workspace, void* stream);
 
+/* Bench instrumentation: with timing on, every fused-stage kernel launch is
+ * bracketed by hipEvents on its stream.  fvhoe_stage_timing_read waits for
+ * the recorded events and returns, per kernel (prims, flux x, flux y,
+ * candidate prims, detect, final), accumulated milliseconds and launch
+ * counts (6 entries each).  fvhoe_stage_timing(on) also resets the sums.
+ * No counterpart in the reference (its Timer takes host wall-clock,
+ * fvhoe/timer.py:6-109). */
+int fvhoe_stage_timing(int on);
+int fvhoe_stage_timing_read(double* ms, long long* count);
+
 #ifdef __cplusplus
 }
 #endif

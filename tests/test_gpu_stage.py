@@ -1,0 +1,43 @@
+"""Per-stage parity on the GPU: every golden stage case the reference wrote
+(tests/golden/make_golden.py), through the fused stage where it applies and
+through the composed stage always, compared bit for bit (FP64)."""
+
+import numpy as np
+import pytest
+
+from cases import STAGE_CASES, build_stage_cases
+from gpu_helpers import assert_bits_equal, solver_from
+from helpers import group, load
+
+pytestmark = pytest.mark.gpu
+
+build_stage_cases()
+STAGES = load("stages.npz")
+
+
+def _run(case, fused):
+    name, n, p, bc, warm, seed, u0, kw = case
+    g = group(STAGES, name)
+    s = solver_from(g["u"], n, p, bc, fused=fused, **kw)
+    if fused and not s.fused:
+        pytest.skip("fused stage does not cover this configuration")
+    dt, dudt = s.hydrodynamics(s.u)
+    print(f"[{name} fused={s.fused}] dt={dt!r}", flush=True)
+    assert dt == float(g["dt"])
+    assert_bits_equal(dudt.cpu().numpy(), g["dudt"], name)
+    if "trouble" in g:
+        flags = s._trouble.cpu().numpy() if s.fused else s.log["mean trouble"].cpu().numpy()
+        assert np.array_equal(flags.astype(np.int8), g["trouble"].reshape(flags.shape))
+        if not s.fused:
+            assert_bits_equal(s.log["mean NAD mag"].cpu().numpy(), g["nad_mag"], name + " NAD")
+            assert_bits_equal(s.log["mean PAD mag"].cpu().numpy(), g["pad_mag"], name + " PAD")
+
+
+@pytest.mark.parametrize("case", STAGE_CASES, ids=[c[0] for c in STAGE_CASES])
+def test_composed_stage(case):
+    _run(case, fused=False)
+
+
+@pytest.mark.parametrize("case", STAGE_CASES, ids=[c[0] for c in STAGE_CASES])
+def test_fused_stage(case):
+    _run(case, fused=True)
