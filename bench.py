@@ -30,14 +30,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell for one launch of each fused-stage kernel (FP64, 5 variables)
-KERNELS = ["prims", "flux_x", "flux_y", "prims_candidate", "detect", "final"]
+KERNELS = ["prims", "flux_x", "flux_y", "prims_candidate", "detect", "fallback", "final"]
 BYTES_PER_CELL = {
     "prims": 40 + 40,             # read u, write W
     "flux_x": 40 + 40,            # read W, write F
     "flux_y": 40 + 40,            # read W, write G
     "prims_candidate": 120 + 40,  # read u, F, G, write WS
     "detect": 80 + 1,             # read W, WS, write flags
-    "final": 80 + 1 + 40,         # read F, G, flags, write dudt
+    "fallback": 0,                # work and traffic scale with the flagged cells only
+    "final": 80 + 40,             # read F, G, write dudt
 }
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
 
@@ -49,12 +50,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--cpu-stages", type=int, default=2)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run 3 device copies of 167,772,160 bytes (scale for HBM byte counters)")
     return ap.parse_args()
 
 
-def cpu_baseline(n: int):
-    """NumPy oracle, one stage of KH n^2 on one core."""
+def cpu_baseline(n: int, stages: int):
+    """NumPy oracle, `stages` stages of KH n^2 on one core."""
     import numpy as np
 
     import oracle as O
@@ -63,11 +67,12 @@ def cpu_baseline(n: int):
     u = ics.kelvin_helmholtz(n, seed=0)
     s = O.Scheme(n=(n, n, 1), p=(3, 3, 0), gamma=1.4, a_posteriori=True)
     t0 = time.perf_counter()
-    O.hydrodynamics(s, u)
+    for _ in range(stages):
+        O.hydrodynamics(s, u)
     dt = time.perf_counter() - t0
-    return {"value": round(n * n / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"one a-posteriori FP64 stage of Kelvin-Helmholtz {n}^2 p=3 HLLC (NumPy oracle), "
-                      f"{dt:.1f} s; cpu={os.cpu_count()} logical cores available, 1 used",
+    return {"value": round(n * n * stages / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{stages} a-posteriori FP64 stages of Kelvin-Helmholtz {n}^2 p=3 HLLC (NumPy oracle), "
+                      f"{dt:.1f} s; {os.cpu_count()} logical cores visible, 1 used",
             "numpy": np.__version__}
 
 
@@ -96,6 +101,14 @@ def main():
                     progress_bar=False)
     assert s.fused
     s.integrator = "ssprk3"
+    if a.copy_probe:
+        # a plain device copy of 167,772,160 bytes, three times: the scale for HBM byte counters
+        src = torch.ones(5 * 2048 * 2048, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     print(f"# warmup {a.warmup} steps at {n}^2", file=sys.stderr, flush=True)
     s.advance(a.warmup)
     torch.cuda.synchronize()
@@ -110,8 +123,8 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_total = start.elapsed_time(stop)
-    ms = (ctypes.c_double * 6)()
-    cnt = (ctypes.c_longlong * 6)()
+    ms = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_longlong * 7)()
     _lib.check(L.fvhoe_stage_timing_read(ms, cnt), "timing read")
     _lib.check(L.fvhoe_stage_timing(0), "timing")
     stages = 3 * a.steps
@@ -151,7 +164,7 @@ def main():
     }
     if not a.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(a.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(a.cpu_n, a.cpu_stages)
     print(json.dumps(out), flush=True)
 
 

@@ -93,9 +93,13 @@ struct SrcUstar {
   }
 };
 
+// Per-block partial reductions, 11 words per block: [0] max of
+// |vx|+|vy|+|vz|+ndim*c (raw bits of a non-negative double), [1 + 2v] / [2 + 2v]
+// ordered bits of the min / max of variable v.  Reduced by k_stage_reduce.
+constexpr int NPART = 11;
 struct Reduce {
-  unsigned long long* dt_acc;     // max of |vx|+|vy|+|vz|+ndim*c (raw bits, non-negative)
-  unsigned long long* range_acc;  // ordered-bits (min, max) per variable, or null
+  unsigned long long* partials;   // null: no reduction in this launch
+  int do_dt, do_range;
   int dt_lo0, dt_hi0, dt_lo1, dt_hi1;  // W index box of the dt reduction
   int rg_lo0, rg_hi0, rg_lo1, rg_hi1;  // W index box of the range reduction
   int ndim;
@@ -174,12 +178,12 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
     const int e = tid + q * TPB;
     if (e < TX * TY) {
       const int a = oa + e / TY, b = ob + e % TY;
-      if (red.dt_acc && a >= red.dt_lo0 && a < red.dt_hi0 && b >= red.dt_lo1 && b < red.dt_hi1) {
+      if (red.do_dt && a >= red.dt_lo0 && a < red.dt_hi0 && b >= red.dt_lo1 && b < red.dt_hi1) {
         const double c = sound_speed(wr[0][q], wr[4][q], red.gamma, red.csq_floor);
         const double s = (fabs(wr[1][q]) + fabs(wr[2][q])) + fabs(wr[3][q]);
         m = dmax(m, s + (double)red.ndim * c);
       }
-      if (red.range_acc && a >= red.rg_lo0 && a < red.rg_hi0 && b >= red.rg_lo1 && b < red.rg_hi1) {
+      if (red.do_range && a >= red.rg_lo0 && a < red.rg_hi0 && b >= red.rg_lo1 && b < red.rg_hi1) {
 #pragma unroll
         for (int v = 0; v < 5; ++v) {
           const unsigned long long o = ordered_bits(wr[v][q]);
@@ -189,23 +193,29 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
       }
     }
   }
-  if (red.dt_acc) {
-    for (int off = 32; off > 0; off >>= 1) m = dmax(m, __shfl_down(m, off, 64));
-    if ((tid & 63) == 0 && m > 0.0) atomicMax(red.dt_acc, (unsigned long long)__double_as_longlong(m));
-  }
-  if (red.range_acc) {
+  if (red.partials) {
+    __shared__ unsigned long long part[TPB / 64][NPART];
+    unsigned long long val[NPART];
+    val[0] = (unsigned long long)__double_as_longlong(m);
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      unsigned long long lo = rmin[v], hi = rmax[v];
+    for (int v = 0; v < 5; ++v) { val[1 + 2 * v] = rmin[v]; val[2 + 2 * v] = rmax[v]; }
+#pragma unroll
+    for (int k = 0; k < NPART; ++k) {
+      unsigned long long x = val[k];
       for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long a = __shfl_down(lo, off, 64), b = __shfl_down(hi, off, 64);
-        lo = a < lo ? a : lo;
-        hi = b > hi ? b : hi;
+        const unsigned long long y = __shfl_down(x, off, 64);
+        x = (k == 0 || (k & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
       }
-      if ((tid & 63) == 0) {
-        if (lo != ~0ull) atomicMin(&red.range_acc[2 * v], lo);
-        if (hi != 0ull) atomicMax(&red.range_acc[2 * v + 1], hi);
+      if ((tid & 63) == 0) part[tid >> 6][k] = x;
+    }
+    __syncthreads();
+    if (tid < NPART) {
+      unsigned long long x = part[0][tid];
+      for (int w = 1; w < TPB / 64; ++w) {
+        const unsigned long long y = part[w][tid];
+        x = (tid == 0 || (tid & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
       }
+      red.partials[(long long)blockIdx.x * NPART + tid] = x;
     }
   }
 }
@@ -289,11 +299,13 @@ struct DetectP {
 __global__ void __launch_bounds__(TPB) k_detect(const double* __restrict__ W, int NW0, int NW1,
                                                const double* __restrict__ WS, int NS0, int NS1, Geo g,
                                                const double* __restrict__ range_mm, DetectP a,
-                                               signed char* __restrict__ trouble, int* __restrict__ any) {
+                                               signed char* __restrict__ trouble, int* __restrict__ count,
+                                               int* __restrict__ list) {
   const long long nc = (long long)g.n0 * g.n1;
   const bool two_d = g.ge1 > 0;
-  int local_any = 0;
-  for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
+  const long long c = blockIdx.x * (long long)TPB + threadIdx.x;
+  int t = 0;
+  if (c < nc) {
     const int i = (int)(c / g.n1), j = (int)(c % g.n1);
     const long long wb = (long long)(i + g.ge0) * NW1 + (j + g.ge1);
     const long long sb = (long long)(i + 3) * NS1 + (j + (two_d ? 3 : 0));
@@ -344,11 +356,27 @@ __global__ void __launch_bounds__(TPB) k_detect(const double* __restrict__ W, in
       const double x = WS[a.pad_var * NSC + sb];
       pad = dmin(x - a.pad_lo, a.pad_hi - x) < 0.0;
     }
-    const int t = (a.force || pad || nad) ? 1 : 0;
+    t = (a.force || pad || nad) ? 1 : 0;
     trouble[c] = (signed char)t;
-    local_any |= t;
   }
-  if (__any(local_any) && (threadIdx.x & 63) == 0) atomicOr(any, 1);
+  // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
+  __shared__ int wbase[TPB / 64];
+  __shared__ int bbase;
+  const unsigned long long ball = __ballot(t);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wbase[wave] = __popcll(ball);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < TPB / 64; ++w) {
+      const int x = wbase[w];
+      wbase[w] = tot;
+      tot += x;
+    }
+    bbase = tot ? atomicAdd(count, tot) : 0;
+  }
+  __syncthreads();
+  if (t) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = (int)c;
 }
 
 // ---------------------------------------------------------------------------
@@ -395,77 +423,104 @@ __device__ void fallback_flux(const SrcU& src, const Geo& g, int i, int j, const
   riemann<D>(p.solver, wl, wr, F, p.gamma, p.csq_floor);
 }
 
+// MUSCL fluxes at the faces of every flagged cell, written over the
+// high-order fluxes (theta = 1 faces of the simple broadcast); a face shared by
+// two flagged cells is written twice with the same value.
 template <bool TWO_D>
-__global__ void __launch_bounds__(TPB) k_final(const double* __restrict__ F, const double* __restrict__ G,
-                                              const signed char* __restrict__ trouble, const int* __restrict__ any,
-                                              SrcU src, Geo g, FinalP p, double* __restrict__ dudt) {
+__global__ void __launch_bounds__(TPB) k_fallback(const int* __restrict__ count, const int* __restrict__ list,
+                                                 SrcU src, Geo g, FinalP p, double* __restrict__ F,
+                                                 double* __restrict__ G) {
+  const long long n = count[0];
+  constexpr int NF = TWO_D ? 4 : 2;
+  const long long nF = (long long)(g.n0 + 1) * g.n1, nG = (long long)g.n0 * (g.n1 + 1);
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n * NF;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int f = (int)(e / n);
+    const int c = list[e % n];
+    const int i = c / g.n1, j = c % g.n1;
+    double Fv[5];
+    if (f < 2) {
+      fallback_flux<0>(src, g, i + f, j, p, Fv);
+      const long long fi = (long long)(i + f) * g.n1 + j;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) F[q * nF + fi] = Fv[q];
+    } else if (TWO_D) {
+      fallback_flux<1>(src, g, i, j + (f - 2), p, Fv);
+      const long long gi = (long long)i * (g.n1 + 1) + j + (f - 2);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) G[q * nG + gi] = Fv[q];
+    }
+  }
+}
+
+// dudt = 0.0 + cx (F[i+1]-F[i]) + cy (G[j+1]-G[j])  (fvhoe/solver.py:744-750)
+template <bool TWO_D>
+__global__ void __launch_bounds__(TPB) k_final(const double* __restrict__ F, const double* __restrict__ G, Geo g,
+                                              double* __restrict__ dudt) {
   const long long nc = (long long)g.n0 * g.n1;
-  const bool revise = trouble && any[0];
+  const long long nF = (long long)(g.n0 + 1) * g.n1, nG = (long long)g.n0 * (g.n1 + 1);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
     const int i = (int)(c / g.n1), j = (int)(c % g.n1);
-    int tl = 0, tr = 0, tb = 0, tt = 0;
-    if (revise) {
-      const int t0 = trouble[c];
-      tl = t0 | (i > 0 ? trouble[c - g.n1] : 0);
-      tr = t0 | (i < g.n0 - 1 ? trouble[c + g.n1] : 0);
-      if (TWO_D) {
-        tb = t0 | (j > 0 ? trouble[c - 1] : 0);
-        tt = t0 | (j < g.n1 - 1 ? trouble[c + 1] : 0);
-      }
-    }
-    double Fl[5], Fr[5], Gb[5], Gt[5];
     const long long fi = (long long)i * g.n1 + j;
-    const long long nF = (long long)(g.n0 + 1) * g.n1, nG = (long long)g.n0 * (g.n1 + 1);
     const long long gi = (long long)i * (g.n1 + 1) + j;
-    if (tl) fallback_flux<0>(src, g, i, j, p, Fl);
-    else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) Fl[q] = F[q * nF + fi];
-    }
-    if (tr) fallback_flux<0>(src, g, i + 1, j, p, Fr);
-    else {
-#pragma unroll
-      for (int q = 0; q < 5; ++q) Fr[q] = F[q * nF + fi + g.n1];
-    }
-    if (TWO_D) {
-      if (tb) fallback_flux<1>(src, g, i, j, p, Gb);
-      else {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) Gb[q] = G[q * nG + gi];
-      }
-      if (tt) fallback_flux<1>(src, g, i, j + 1, p, Gt);
-      else {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) Gt[q] = G[q * nG + gi + 1];
-      }
-    }
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
-      double acc = 0.0 + g.cx * (Fr[q] - Fl[q]);
-      if (TWO_D) acc = acc + g.cy * (Gt[q] - Gb[q]);
+      double acc = 0.0 + g.cx * (F[q * nF + fi + g.n1] - F[q * nF + fi]);
+      if (TWO_D) acc = acc + g.cy * (G[q * nG + gi + 1] - G[q * nG + gi]);
       dudt[q * nc + c] = acc;
     }
   }
 }
 
-__global__ void k_stage_init(unsigned long long* dt_acc, unsigned long long* range_acc, int* any) {
-  const int t = threadIdx.x;
-  if (t == 0) { dt_acc[0] = 0ull; any[0] = 0; }
-  if (t < 10) range_acc[t] = (t & 1) ? 0ull : ~0ull;
+__global__ void k_stage_init(int* count) {
+  if (threadIdx.x == 0) count[0] = 0;
 }
 
-__global__ void k_stage_dt(const unsigned long long* dt_acc, const unsigned long long* range_acc, double* range,
-                           double cfl_h, int fixed_on, double fixed_dt, double* dt) {
-  const int t = threadIdx.x;
-  if (t == 0) dt[0] = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)dt_acc[0]);
-  if (t < 10) range[t] = from_ordered_bits(range_acc[t]);
+// final reduction of the prims partials: dt (fvhoe/hydro.py:290-293) and the
+// per-variable global (min, max) (fvhoe/slope_limiting.py:168-170)
+__global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long* __restrict__ partials, int nblocks,
+                                                       double* range, double cfl_h, int fixed_on, double fixed_dt,
+                                                       double* dt) {
+  __shared__ unsigned long long part[16][NPART];
+  const int tid = threadIdx.x;
+  unsigned long long val[NPART];
+  val[0] = 0ull;
+#pragma unroll
+  for (int k = 1; k < NPART; ++k) val[k] = (k & 1) ? ~0ull : 0ull;
+  for (int b = tid; b < nblocks; b += 1024) {
+#pragma unroll
+    for (int k = 0; k < NPART; ++k) {
+      const unsigned long long y = partials[(long long)b * NPART + k];
+      val[k] = (k == 0 || (k & 1) == 0) ? (y > val[k] ? y : val[k]) : (y < val[k] ? y : val[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NPART; ++k) {
+    unsigned long long x = val[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long y = __shfl_down(x, off, 64);
+      x = (k == 0 || (k & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
+    }
+    if ((tid & 63) == 0) part[tid >> 6][k] = x;
+  }
+  __syncthreads();
+  if (tid < NPART) {
+    unsigned long long x = part[0][tid];
+    for (int w = 1; w < 16; ++w) {
+      const unsigned long long y = part[w][tid];
+      x = (tid == 0 || (tid & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
+    }
+    if (tid == 0) dt[0] = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)x);
+    else range[tid - 1] = from_ordered_bits(x);
+  }
 }
 
 // ------------------------------------------------------------------ host side
 
 // Optional per-kernel event timing (bench instrumentation): each launch of a
 // stage kernel is bracketed by a pair of events from a fixed pool.
-enum { TK_PRIMS = 0, TK_FLUX_X = 1, TK_FLUX_Y = 2, TK_PRIMS_CAND = 3, TK_DETECT = 4, TK_FINAL = 5, TK_COUNT = 6 };
+enum { TK_PRIMS = 0, TK_FLUX_X = 1, TK_FLUX_Y = 2, TK_PRIMS_CAND = 3, TK_DETECT = 4, TK_FALLBACK = 5, TK_FINAL = 6,
+       TK_COUNT = 7 };
 constexpr int TIMING_POOL = 8192;
 struct Timing {
   bool on = false;
@@ -473,8 +528,8 @@ struct Timing {
   int kind[TIMING_POOL];
   hipEvent_t a[TIMING_POOL], b[TIMING_POOL];
   bool created = false;
-  double ms[TK_COUNT] = {0, 0, 0, 0, 0, 0};
-  long long count[TK_COUNT] = {0, 0, 0, 0, 0, 0};
+  double ms[TK_COUNT] = {};
+  long long count[TK_COUNT] = {};
 };
 Timing g_timing;
 
@@ -495,7 +550,8 @@ struct TimedLaunch {
 
 struct Layout {
   int two_d, p, G, GE, n0, n1, NW0, NW1, NS0, NS1;
-  size_t offW, offF, offG, offWS, offFlags, offScal, total;
+  int prims_blocks;
+  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -528,6 +584,12 @@ int make_layout(const FvhoeStageConfig* c, Layout* L) {
   L->offG = o; o = align256(o + (ay ? sizeof(double) * 5 * (size_t)L->n0 * (L->n1 + 1) : 0));
   L->offWS = o; o = align256(o + (c->a_posteriori ? sizeof(double) * 5 * (size_t)L->NS0 * L->NS1 : 0));
   L->offFlags = o; o = align256(o + (c->a_posteriori ? (size_t)L->n0 * L->n1 : 0));
+  L->offList = o; o = align256(o + (c->a_posteriori ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
+  {
+    const int tx = L->two_d ? 16 : 256, ty = L->two_d ? 32 : 1;
+    L->prims_blocks = ((L->NW0 + tx - 1) / tx) * ((L->NW1 + ty - 1) / ty);
+  }
+  L->offPart = o; o = align256(o + sizeof(unsigned long long) * NPART * (size_t)L->prims_blocks);
   L->offScal = o; o = align256(o + 256);
   L->total = o;
   return FVHOE_OK;
@@ -543,10 +605,10 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
   double* G = TWO_D ? (double*)(ws + L.offG) : nullptr;
   double* WS = (double*)(ws + L.offWS);
   signed char* flags = (signed char*)(ws + L.offFlags);
-  unsigned long long* dt_acc = (unsigned long long*)(ws + L.offScal);
-  unsigned long long* range_acc = dt_acc + 1;
-  double* range = (double*)(dt_acc + 11);
-  int* any = (int*)(dt_acc + 21);
+  unsigned long long* partials = (unsigned long long*)(ws + L.offPart);
+  double* range = (double*)(ws + L.offScal);
+  int* count = (int*)(range + 10);
+  int* list = (int*)(ws + L.offList);
 
   Geo g{};
   g.n0 = L.n0;
@@ -559,11 +621,12 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
 
   const bool apost = c->a_posteriori != 0;
   const bool need_range = apost && c->nad_mode == 0;
-  k_stage_init<<<1, 64, 0, s>>>(dt_acc, range_acc, any);
+  k_stage_init<<<1, 64, 0, s>>>(count);
 
   Reduce red{};
-  red.dt_acc = c->fixed_dt_on ? nullptr : dt_acc;
-  red.range_acc = need_range ? range_acc : nullptr;
+  red.partials = partials;
+  red.do_dt = !c->fixed_dt_on;
+  red.do_range = need_range;
   red.dt_lo0 = L.GE - L.G; red.dt_hi0 = L.GE + L.n0 + L.G;
   red.dt_lo1 = TWO_D ? L.GE - L.G : 0; red.dt_hi1 = TWO_D ? L.GE + L.n1 + L.G : 1;
   red.rg_lo0 = L.GE - 3; red.rg_hi0 = L.GE + L.n0 + 3;
@@ -580,7 +643,8 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
     k_prims<P, PY, TX, TY, SrcU><<<t0 * t1, TPB, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
-  k_stage_dt<<<1, 64, 0, s>>>(dt_acc, range_acc, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt, dt_stage);
+  k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
+                                    dt_stage);
 
   if (TWO_D) {
     constexpr int TA0 = 16, TB0 = 32, TA1 = 32, TB1 = 16;
@@ -610,15 +674,17 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
                                                                       L.NS0, L.NS1, t1, none, c->gamma); }
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
+    const int gridd = (int)((nc + TPB - 1) / TPB);
     { TimedLaunch tl(TK_DETECT, s);
-    k_detect<<<gridc, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, any); }
-    { TimedLaunch tl(TK_FINAL, s);
-    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, flags, any, SrcU{u}, g, fp, dudt); }
+    k_detect<<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list); }
+    { TimedLaunch tl(TK_FALLBACK, s);
+    k_fallback<TWO_D><<<1024, TPB, 0, s>>>(count, list, SrcU{u}, g, fp, F, G); }
     if (trouble_out && hipMemcpyAsync(trouble_out, flags, (size_t)nc, hipMemcpyDeviceToDevice, s) != hipSuccess)
       return FVHOE_ERR_LAUNCH;
-  } else {
+  }
+  {
     TimedLaunch tl(TK_FINAL, s);
-    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, nullptr, any, SrcU{u}, g, fp, dudt);
+    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, g, dudt);
   }
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }

@@ -185,8 +185,8 @@ int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, doub
 /* Bench instrumentation: with timing on, every fused-stage kernel launch is
  * bracketed by hipEvents on its stream.  fvhoe_stage_timing_read waits for
  * the recorded events and returns, per kernel (prims, flux x, flux y,
- * candidate prims, detect, final), accumulated milliseconds and launch
- * counts (6 entries each).  fvhoe_stage_timing(on) also resets the sums.
+ * candidate prims, detect, fallback, final), accumulated milliseconds and
+ * launch counts (7 entries each).  fvhoe_stage_timing(on) also resets the sums.
  * No counterpart in the reference (its Timer takes host wall-clock,
  * fvhoe/timer.py:6-109). */
 int fvhoe_stage_timing(int on);
