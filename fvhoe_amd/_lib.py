@@ -35,6 +35,14 @@ class FvhoeStageConfig(ctypes.Structure):
     ]
 
 
+class FvhoeRkFuse(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_int), ("out", c_void_p), ("u0", c_void_p), ("k0", c_void_p), ("k1", c_void_p),
+        ("k2", c_void_p), ("dt_step", c_void_p), ("set_dt_step", c_int), ("t", c_void_p),
+        ("target", c_double), ("has_target", c_int),
+    ]
+
+
 _P = c_void_p
 _SIGNATURES = {
     "fvhoe_version": (c_int, []),
@@ -65,6 +73,7 @@ _SIGNATURES = {
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
     "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),
+    "fvhoe_stage_rk": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, POINTER(FvhoeRkFuse), _P]),
     "fvhoe_stage_timing": (c_int, [c_int]),
     "fvhoe_stage_timing_read": (c_int, [POINTER(c_double), POINTER(c_longlong)]),
 }

@@ -32,6 +32,14 @@ __host__ __device__ constexpr int flux_ghosts(int p) {
   return reach(p) + 1 > 2 * reach(p) ? reach(p) + 1 : 2 * reach(p);  // gw of hydrofluxes, fvhoe/solver.py:493
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Give each XCD a contiguous run of tiles so neighbouring tiles (which share
+// halo data) are served by the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 struct Geo {
   int n0, n1;        // interior cells (n1 = 1 in 1-D)
   int ge0, ge1;      // ghost width of W / WS per axis (0 on an inactive axis)
@@ -109,37 +117,38 @@ struct Reduce {
 // ---------------------------------------------------------------------------
 // prims tile: W[a][b] for W indices a in [oa, oa+TX), b in [ob, ob+TY).
 // ---------------------------------------------------------------------------
-template <int PX, int PY, int TX, int TY, typename Src>
-__global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restrict__ W, int NW0, int NW1,
+template <int PX, int PY, int TX, int TY, typename Src, int NT = TPB>
+__global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, double* __restrict__ W, int NW0, int NW1,
                                               int tiles1, Reduce red, double gamma) {
   constexpr int RCX = ereach_c(PX), RTX = ereach_t(PX), RCY = ereach_c(PY), RTY = ereach_t(PY);
   constexpr int HX0 = TX + 2 * (RCX + RTX), HY0 = TY + 2 * (RCY + RTY);
   constexpr int HX1 = TX + 2 * RTX, HY1 = TY + 2 * RTY;
-  constexpr int NPT = (TX * TY + TPB - 1) / TPB;
+  constexpr int NPT = (TX * TY + NT - 1) / NT;
   __shared__ double S0[HX0 * HY0];
   __shared__ double S1[HX1 * HY0];
   __shared__ double C[5][HX1 * HY1];
 
-  const int oa = (blockIdx.x / tiles1) * TX, ob = (blockIdx.x % tiles1) * TY;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int oa = (tile / tiles1) * TX, ob = (tile % tiles1) * TY;
   const int cx0 = oa - g.ge0 - (RCX + RTX), cy0 = ob - g.ge1 - (RCY + RTY);
   const int tid = threadIdx.x;
 
   for (int v = 0; v < 5; ++v) {
-    for (int e = tid; e < HX0 * HY0; e += TPB) S0[e] = src.get(g, v, cx0 + e / HY0, cy0 + e % HY0);
+    for (int e = tid; e < HX0 * HY0; e += NT) S0[e] = src.get(g, v, cx0 + e / HY0, cy0 + e % HY0);
     __syncthreads();
-    for (int e = tid; e < HX1 * HY0; e += TPB) {
+    for (int e = tid; e < HX1 * HY0; e += NT) {
       const int i = e / HY0, j = e % HY0;
       S1[e] = sweep<KIND_C, PX>([&](int t) { return S0[(i + RCX + t - reach(PX)) * HY0 + j]; });
     }
     __syncthreads();
-    for (int e = tid; e < HX1 * HY1; e += TPB) {
+    for (int e = tid; e < HX1 * HY1; e += NT) {
       const int i = e / HY1, j = e % HY1;
       const double x = sweep<KIND_C, PY>([&](int t) { return S1[i * HY0 + j + RCY + t - reach(PY)]; });
       C[v][e] = 0.0 + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
     }
     __syncthreads();
   }
-  for (int e = tid; e < HX1 * HY1; e += TPB) {
+  for (int e = tid; e < HX1 * HY1; e += NT) {
     double uu[5], ww[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) uu[q] = C[q][e];
@@ -150,14 +159,14 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
   __syncthreads();
   double wr[5][NPT];
   for (int v = 0; v < 5; ++v) {
-    for (int e = tid; e < TX * HY1; e += TPB) {
+    for (int e = tid; e < TX * HY1; e += NT) {
       const int i = e / HY1, j = e % HY1;
       S1[e] = sweep<KIND_T, PX>([&](int t) { return C[v][(i + RTX + t - reach(PX)) * HY1 + j]; });
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < NPT; ++q) {
-      const int e = tid + q * TPB;
+      const int e = tid + q * NT;
       if (e < TX * TY) {
         const int i = e / TY, j = e % TY;
         const double x = sweep<KIND_T, PY>([&](int t) { return S1[i * HY1 + j + RTY + t - reach(PY)]; });
@@ -175,7 +184,7 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
   for (int q = 0; q < 5; ++q) { rmin[q] = ~0ull; rmax[q] = 0ull; }
 #pragma unroll
   for (int q = 0; q < NPT; ++q) {
-    const int e = tid + q * TPB;
+    const int e = tid + q * NT;
     if (e < TX * TY) {
       const int a = oa + e / TY, b = ob + e % TY;
       if (red.do_dt && a >= red.dt_lo0 && a < red.dt_hi0 && b >= red.dt_lo1 && b < red.dt_hi1) {
@@ -194,7 +203,7 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
     }
   }
   if (red.partials) {
-    __shared__ unsigned long long part[TPB / 64][NPART];
+    __shared__ unsigned long long part[NT / 64][NPART];
     unsigned long long val[NPART];
     val[0] = (unsigned long long)__double_as_longlong(m);
 #pragma unroll
@@ -211,7 +220,7 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
     __syncthreads();
     if (tid < NPART) {
       unsigned long long x = part[0][tid];
-      for (int w = 1; w < TPB / 64; ++w) {
+      for (int w = 1; w < NT / 64; ++w) {
         const unsigned long long y = part[w][tid];
         x = (tid == 0 || (tid & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
       }
@@ -225,8 +234,8 @@ __global__ void __launch_bounds__(TPB) k_prims(Src src, Geo g, double* __restric
 // fa), rows b in [b0, b0+TB).  D = 1: y faces, rows along x.  "normal" is the
 // face-normal axis with degree PN, "trans" the other with degree PT.
 // ---------------------------------------------------------------------------
-template <int D, int PN, int PT, int TA, int TB>
-__global__ void __launch_bounds__(TPB) k_flux(const double* __restrict__ W, int NW0, int NW1, Geo g,
+template <int D, int PN, int PT, int TA, int TB, int NT = TPB>
+__global__ void __launch_bounds__(NT) k_flux(const double* __restrict__ W, int NW0, int NW1, Geo g,
                                              double* __restrict__ Fout, int tilesB, int solver, double gamma,
                                              double csq_floor) {
   constexpr int EL = reach(PN);
@@ -239,20 +248,21 @@ __global__ void __launch_bounds__(TPB) k_flux(const double* __restrict__ W, int 
 
   const int nN = D == 0 ? g.n0 : g.n1, nT = D == 0 ? g.n1 : g.n0;
   const int geN = D == 0 ? g.ge0 : g.ge1, geT = D == 0 ? g.ge1 : g.ge0;
-  const int f0 = (blockIdx.x / tilesB) * TA, b0 = (blockIdx.x % tilesB) * TB;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int f0 = (tile / tilesB) * TA, b0 = (tile % tilesB) * TB;
   const int tid = threadIdx.x;
   // W index of normal cell (f0 - 1 - EL) and transverse cell (b0 - RC - RT)
   const int wn0 = f0 - 1 - EL + geN, wt0 = b0 - RC - RT + geT;
 
   for (int v = 0; v < 5; ++v) {
-    for (int e = tid; e < SZ_S0; e += TPB) {
+    for (int e = tid; e < SZ_S0; e += NT) {
       int r, c;
       if (D == 0) { r = e / HT0; c = e % HT0; } else { c = e / HN; r = e % HN; }  // contiguous W axis fastest
       const int a = D == 0 ? wn0 + r : wt0 + c, b = D == 0 ? wt0 + c : wn0 + r;
       SF[r * HT0 + c] = (a < NW0 && b < NW1) ? W[((long long)v * NW0 + a) * NW1 + b] : 0.0;
     }
     __syncthreads();
-    for (int e = tid; e < HN * HT1; e += TPB) {
+    for (int e = tid; e < HN * HT1; e += NT) {
       const int r = e / HT1, c = e % HT1;
       const double x = sweep<KIND_C, PT>([&](int t) { return SF[r * HT0 + c + RC + t - reach(PT)]; });
       X[v][e] = 0.0 + x;  // degree-0 z sweep precedes/follows (fvhoe/solver.py:524,541)
@@ -261,7 +271,7 @@ __global__ void __launch_bounds__(TPB) k_flux(const double* __restrict__ W, int 
   }
   // Riemann problems at faces f0+fr, transverse rows c in [0, HT1)
   double* FC = SF;
-  for (int e = tid; e < TA * HT1; e += TPB) {
+  for (int e = tid; e < TA * HT1; e += NT) {
     const int fr = e / HT1, c = e % HT1;
     double wl[5], wr[5], F[5];
 #pragma unroll
@@ -276,7 +286,7 @@ __global__ void __launch_bounds__(TPB) k_flux(const double* __restrict__ W, int 
   }
   __syncthreads();
   const int nF0 = D == 0 ? g.n0 + 1 : g.n0, nF1 = D == 0 ? g.n1 : g.n1 + 1;
-  for (int e = tid; e < 5 * TA * TB; e += TPB) {
+  for (int e = tid; e < 5 * TA * TB; e += NT) {
     int q, fr, c;
     if (D == 0) { q = e / (TA * TB); fr = (e / TB) % TA; c = e % TB; }
     else { q = e / (TA * TB); c = (e / TA) % TB; fr = e % TA; }
@@ -453,10 +463,33 @@ __global__ void __launch_bounds__(TPB) k_fallback(const int* __restrict__ count,
   }
 }
 
-// dudt = 0.0 + cx (F[i+1]-F[i]) + cy (G[j+1]-G[j])  (fvhoe/solver.py:744-750)
+// Runge-Kutta combination written by the stage's last kernel; the stage's own
+// dudt is the last k operand of the kind (fvhoe/ode.py:245-298).
+struct RkArgs {
+  int kind;                     // FVHOE_RK_*, -1 = none
+  double* out;
+  const double *u0, *k0, *k1, *k2;
+  const double* dt;             // step dt (device)
+};
+
+__device__ __forceinline__ double rk_combine(const RkArgs& r, long long i, double k_this) {
+  const double dt = r.dt[0];
+  switch (r.kind) {
+    case FVHOE_RK_AXPY: return r.u0[i] + dt * k_this;
+    case FVHOE_RK_SSPRK3_STAGE3: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
+    case FVHOE_RK_SSPRK3_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + r.k1[i] + 4.0 * k_this);
+    case FVHOE_RK_SSPRK2_FINAL: return 0.5 * r.u0[i] + 0.5 * (r.k0[i] + dt * k_this);
+    case FVHOE_RK_HALF_AXPY: return r.u0[i] + 0.5 * dt * k_this;
+    case FVHOE_RK_RK4_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 2.0 * r.k1[i] + 2.0 * r.k2[i] + k_this);
+  }
+  return 0.0;
+}
+
+// dudt = 0.0 + cx (F[i+1]-F[i]) + cy (G[j+1]-G[j])  (fvhoe/solver.py:744-750), then the
+// optional RK combination.  rk.out may alias the stage input or rk.u0 (pointwise).
 template <bool TWO_D>
 __global__ void __launch_bounds__(TPB) k_final(const double* __restrict__ F, const double* __restrict__ G, Geo g,
-                                              double* __restrict__ dudt) {
+                                              double* dudt, RkArgs rk) {
   const long long nc = (long long)g.n0 * g.n1;
   const long long nF = (long long)(g.n0 + 1) * g.n1, nG = (long long)g.n0 * (g.n1 + 1);
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
@@ -467,7 +500,8 @@ __global__ void __launch_bounds__(TPB) k_final(const double* __restrict__ F, con
     for (int q = 0; q < 5; ++q) {
       double acc = 0.0 + g.cx * (F[q * nF + fi + g.n1] - F[q * nF + fi]);
       if (TWO_D) acc = acc + g.cy * (G[q * nG + gi + 1] - G[q * nG + gi]);
-      dudt[q * nc + c] = acc;
+      if (dudt) dudt[q * nc + c] = acc;
+      if (rk.kind >= 0) rk.out[q * nc + c] = rk_combine(rk, q * nc + c, acc);
     }
   }
 }
@@ -480,7 +514,8 @@ __global__ void k_stage_init(int* count) {
 // per-variable global (min, max) (fvhoe/slope_limiting.py:168-170)
 __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long* __restrict__ partials, int nblocks,
                                                        double* range, double cfl_h, int fixed_on, double fixed_dt,
-                                                       double* dt) {
+                                                       double* dt, double* dt_step, const double* t, double target,
+                                                       int has_target) {
   __shared__ unsigned long long part[16][NPART];
   const int tid = threadIdx.x;
   unsigned long long val[NPART];
@@ -510,7 +545,12 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
       const unsigned long long y = part[w][tid];
       x = (tid == 0 || (tid & 1) == 0) ? (y > x ? y : x) : (y < x ? y : x);
     }
-    if (tid == 0) dt[0] = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)x);
+    if (tid == 0) {
+      const double d = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)x);
+      dt[0] = d;
+      // _hit_target (fvhoe/ode.py:11-22) for the first stage of a step
+      if (dt_step) dt_step[0] = (has_target && target - t[0] < d) ? target - t[0] : d;
+    }
     else range[tid - 1] = from_ordered_bits(x);
   }
 }
@@ -596,10 +636,12 @@ int make_layout(const FvhoeStageConfig* c, Layout* L) {
 }
 
 template <int P, bool TWO_D>
-int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, double* dudt, double* dt_stage,
+int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const Layout& L, const double* u, double* dudt,
+              double* dt_stage,
               signed char* trouble_out, char* ws, hipStream_t s) {
   constexpr int PY = TWO_D ? P : 0;
   constexpr int TX = TWO_D ? 16 : 256, TY = TWO_D ? 32 : 1;
+  constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
   double* W = (double*)(ws + L.offW);
   double* F = (double*)(ws + L.offF);
   double* G = TWO_D ? (double*)(ws + L.offG) : nullptr;
@@ -640,20 +682,24 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
   {
     const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
     TimedLaunch tl(TK_PRIMS, s);
-    k_prims<P, PY, TX, TY, SrcU><<<t0 * t1, TPB, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
+    k_prims<P, PY, TX, TY, SrcU, NTILE><<<t0 * t1, NTILE, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
+  RkArgs rk{-1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  const bool set_step = rkf && rkf->set_dt_step;
+  if (rkf && rkf->kind >= 0) rk = RkArgs{rkf->kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
   k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
-                                    dt_stage);
+                                    dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
+                                    set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0);
 
   if (TWO_D) {
     constexpr int TA0 = 16, TB0 = 32, TA1 = 32, TB1 = 16;
     const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
     { TimedLaunch tl(TK_FLUX_X, s);
-    k_flux<0, P, PY, TA0, TB0><<<tx * ty, TPB, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->riemann, c->gamma, c->csq_floor); }
+    k_flux<0, P, PY, TA0, TB0, NTILE><<<tx * ty, NTILE, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->riemann, c->gamma, c->csq_floor); }
     const int fy = (L.n1 + 1 + TA1 - 1) / TA1, rx = (L.n0 + TB1 - 1) / TB1;
     { TimedLaunch tl(TK_FLUX_Y, s);
-    k_flux<1, PY, P, TA1, TB1><<<fy * rx, TPB, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->riemann, c->gamma, c->csq_floor); }
+    k_flux<1, PY, P, TA1, TB1, NTILE><<<fy * rx, NTILE, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->riemann, c->gamma, c->csq_floor); }
   } else {
     const int tx = (L.n0 + 1 + 255) / 256;
     TimedLaunch tl(TK_FLUX_X, s);
@@ -670,7 +716,7 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
     const int t0 = (L.NS0 + TX - 1) / TX, t1 = (L.NS1 + TY - 1) / TY;
     Reduce none{};
     { TimedLaunch tl(TK_PRIMS_CAND, s);
-    k_prims<P, PY, TX, TY, SrcUstar<TWO_D>><<<t0 * t1, TPB, 0, s>>>(SrcUstar<TWO_D>{u, F, G, dt_stage}, gs, WS,
+    k_prims<P, PY, TX, TY, SrcUstar<TWO_D>, NTILE><<<t0 * t1, NTILE, 0, s>>>(SrcUstar<TWO_D>{u, F, G, dt_stage}, gs, WS,
                                                                       L.NS0, L.NS1, t1, none, c->gamma); }
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
@@ -684,23 +730,24 @@ int run_stage(const FvhoeStageConfig* c, const Layout& L, const double* u, doubl
   }
   {
     TimedLaunch tl(TK_FINAL, s);
-    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, g, dudt);
+    k_final<TWO_D><<<gridc, TPB, 0, s>>>(F, G, g, dudt, rk);
   }
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
 template <bool TWO_D>
-int dispatch(int p, const FvhoeStageConfig* c, const Layout& L, const double* u, double* dudt, double* dt,
+int dispatch(int p, const FvhoeStageConfig* c, const FvhoeRkFuse* rk, const Layout& L, const double* u, double* dudt,
+             double* dt,
              signed char* t, char* ws, hipStream_t s) {
   switch (p) {
-    case 1: return run_stage<1, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 2: return run_stage<2, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 3: return run_stage<3, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 4: return run_stage<4, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 5: return run_stage<5, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 6: return run_stage<6, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 7: return run_stage<7, TWO_D>(c, L, u, dudt, dt, t, ws, s);
-    case 8: return run_stage<8, TWO_D>(c, L, u, dudt, dt, t, ws, s);
+    case 1: return run_stage<1, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 2: return run_stage<2, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 3: return run_stage<3, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 4: return run_stage<4, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 5: return run_stage<5, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 6: return run_stage<6, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 7: return run_stage<7, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
+    case 8: return run_stage<8, TWO_D>(c, rk, L, u, dudt, dt, t, ws, s);
   }
   return FVHOE_ERR_UNSUPPORTED;
 }
@@ -751,17 +798,32 @@ int fvhoe_stage_timing_read(double* ms, long long* count) {
   return FVHOE_OK;
 }
 
-int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
-                void* workspace, void* stream) {
+int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                   signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream) {
   Layout L;
   const int st = make_layout(cfg, &L);
   if (st != FVHOE_OK) return st;
-  if (!u || !dudt || !dt_stage || !workspace) return FVHOE_ERR_ARG;
+  if (!u || !dt_stage || !workspace) return FVHOE_ERR_ARG;
+  if (!dudt && !(rk && rk->kind >= 0)) return FVHOE_ERR_ARG;
+  if (rk && rk->kind >= 0) {
+    if (rk->kind > FVHOE_RK_RK4_FINAL || !rk->out || !rk->u0 || !rk->dt_step) return FVHOE_ERR_ARG;
+    if ((rk->kind == FVHOE_RK_SSPRK3_STAGE3 || rk->kind == FVHOE_RK_SSPRK2_FINAL) && !rk->k0) return FVHOE_ERR_ARG;
+    if (rk->kind == FVHOE_RK_SSPRK3_FINAL && (!rk->k0 || !rk->k1)) return FVHOE_ERR_ARG;
+    if (rk->kind == FVHOE_RK_RK4_FINAL && (!rk->k0 || !rk->k1 || !rk->k2)) return FVHOE_ERR_ARG;
+    if (rk->out == dudt) return FVHOE_ERR_ARG;
+  }
+  if (rk && rk->set_dt_step && (!rk->dt_step || !rk->t)) return FVHOE_ERR_ARG;
   if (((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
-  return L.two_d ? dispatch<true>(L.p, cfg, L, u, dudt, dt_stage, trouble, ws, s)
-                 : dispatch<false>(L.p, cfg, L, u, dudt, dt_stage, trouble, ws, s);
+  return L.two_d ? dispatch<true>(L.p, cfg, rk, L, u, dudt, dt_stage, trouble, ws, s)
+                 : dispatch<false>(L.p, cfg, rk, L, u, dudt, dt_stage, trouble, ws, s);
+}
+
+int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
+                void* workspace, void* stream) {
+  if (!dudt) return FVHOE_ERR_ARG;
+  return fvhoe_stage_rk(cfg, u, dudt, dt_stage, trouble, workspace, nullptr, stream);
 }
 
 }  // extern "C"

@@ -72,12 +72,39 @@ class ODE(ABC):
         _lib.call("fvhoe_rk_combine", kind, _lib.ptr(out), _lib.ptr(u), _lib.ptr(k0), _lib.ptr(k1), _lib.ptr(k2),
                   _lib.ptr(k3), _lib.ptr(dt), self._n, _lib.stream())
 
+    def stage_rk(self, u, k_out, dt_out, kind, out, u0, k0=None, k1=None, k2=None, dt_step=None,
+                 set_dt_step=False, t_dev=None, target=None) -> bool:
+        """Optional: a stage that also writes the RK combination (fused path).
+        Returns False when not available."""
+        return False
+
     def _device_step(self, target: Optional[float], t_dev):
         """One step of self.integrator on self.u; returns the dt_step scalar view."""
         u, k, s = self.u, self._k, self._s
         sc = self._scal
         dts, dt_step = sc[0:1], sc[4:5]
         name = self.integrator
+        first = dict(dt_step=dt_step, set_dt_step=True, t_dev=t_dev, target=target)
+        if self._fused_rk():
+            if name == "euler":
+                self.stage_rk(u, k[0], dts, _lib.RK_AXPY, u, u, **first)
+            elif name == "ssprk2":
+                self.stage_rk(u, k[0], dts, _lib.RK_AXPY, s, u, **first)
+                self.stage_rk(s, k[1], sc[1:2], _lib.RK_SSPRK2_FINAL, u, u, k0=s, dt_step=dt_step)
+            elif name == "ssprk3":
+                self.stage_rk(u, k[0], dts, _lib.RK_AXPY, s, u, **first)
+                self.stage_rk(s, k[1], sc[1:2], _lib.RK_SSPRK3_STAGE3, s, u, k0=k[0], dt_step=dt_step)
+                self.stage_rk(s, None, sc[2:3], _lib.RK_SSPRK3_FINAL, u, u, k0=k[0], k1=k[1], dt_step=dt_step)
+            elif name == "rk4":
+                self.stage_rk(u, k[0], dts, _lib.RK_HALF_AXPY, s, u, **first)
+                self.stage_rk(s, k[1], sc[1:2], _lib.RK_HALF_AXPY, s, u, dt_step=dt_step)
+                self.stage_rk(s, k[2], sc[2:3], _lib.RK_AXPY, s, u, dt_step=dt_step)
+                self.stage_rk(s, None, sc[3:4], _lib.RK_RK4_FINAL, u, u, k0=k[0], k1=k[1], k2=k[2],
+                              dt_step=dt_step)
+            else:
+                raise ValueError(name)
+            _lib.call("fvhoe_advance_time", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.stream())
+            return dt_step
         self.stage(u, k[0], dts)
         _lib.call("fvhoe_hit_target", _lib.ptr(dt_step), _lib.ptr(dts), _lib.ptr(t_dev),
                   float(target) if target is not None else 0.0, int(target is not None), _lib.stream())
@@ -105,6 +132,9 @@ class ODE(ABC):
             raise ValueError(name)
         _lib.call("fvhoe_advance_time", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.stream())
         return dt_step
+
+    def _fused_rk(self) -> bool:
+        return False
 
     def take_step(self, target_time: float = None):
         """fvhoe/ode.py:183-193: one step, dt read back to advance the host clock."""

@@ -166,14 +166,8 @@ class EulerSolver(ODE):
         """One RHS evaluation into k_out, this stage's dt into dt_out[0] (device)."""
         self.f_evaluation_count += 1
         if self.fused:
-            if self._ws is None:
-                nbytes = _lib.ctypes.c_ulonglong(0)
-                _lib.call("fvhoe_stage_workspace_bytes", _lib.ctypes.byref(self._cfg), _lib.ctypes.byref(nbytes))
-                self._ws = _lib.empty((nbytes.value + 256,), dtype=_lib.torch().uint8)
-            base = self._ws.data_ptr()
-            off = (-base) % 256
             _lib.call("fvhoe_stage", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out), _lib.ptr(dt_out),
-                      _lib.ptr(self._trouble), _lib.ctypes.c_void_p(base + off), _lib.stream())
+                      _lib.ptr(self._trouble), self._workspace(), _lib.stream())
             if self._trouble is not None:
                 self._log_troubles_device(self._trouble)
         else:
@@ -182,6 +176,39 @@ class EulerSolver(ODE):
                 F, G, H = self.revise_fluxes(u, F, G, H, dt, self.force_trouble)
             self._euler_equation_into(F, G, H, k_out)
             dt_out.copy_(dt)
+
+    def _fused_rk(self) -> bool:
+        return self.fused
+
+    def stage_rk(self, u, k_out, dt_out, kind, out, u0, k0=None, k1=None, k2=None, dt_step=None,
+                 set_dt_step=False, t_dev=None, target=None) -> bool:
+        """A fused stage whose last kernel also writes the RK combination `kind`
+        into `out` (fvhoe/ode.py:245-298), so the state is touched once per stage."""
+        self.f_evaluation_count += 1
+        rk = _lib.FvhoeRkFuse()
+        rk.kind = kind
+        rk.out, rk.u0 = out.data_ptr(), u0.data_ptr()
+        rk.k0 = k0.data_ptr() if k0 is not None else None
+        rk.k1 = k1.data_ptr() if k1 is not None else None
+        rk.k2 = k2.data_ptr() if k2 is not None else None
+        rk.dt_step = dt_step.data_ptr()
+        rk.set_dt_step = int(bool(set_dt_step))
+        rk.t = t_dev.data_ptr() if t_dev is not None else None
+        rk.target = float(target) if target is not None else 0.0
+        rk.has_target = int(target is not None)
+        _lib.call("fvhoe_stage_rk", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out), _lib.ptr(dt_out),
+                  _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk), _lib.stream())
+        if self._trouble is not None:
+            self._log_troubles_device(self._trouble)
+        return True
+
+    def _workspace(self):
+        if self._ws is None:
+            nbytes = _lib.ctypes.c_ulonglong(0)
+            _lib.call("fvhoe_stage_workspace_bytes", _lib.ctypes.byref(self._cfg), _lib.ctypes.byref(nbytes))
+            self._ws = _lib.empty((nbytes.value + 256,), dtype=_lib.torch().uint8)
+        base = self._ws.data_ptr()
+        return _lib.ctypes.c_void_p(base + (-base) % 256)
 
     def hydrodynamics(self, u):
         """(dt, dudt) of one stage (fvhoe/solver.py:429-454)."""

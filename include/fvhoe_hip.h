@@ -182,6 +182,30 @@ int fvhoe_stage_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long*
 int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
                 void* workspace, void* stream);
 
+/* The Runge-Kutta step fused into a stage: the stage's last kernel also
+ * writes out = combine(kind, u0, k0, k1, k2, this stage's dudt, *dt_step),
+ * this stage's dudt taking the place of the kind's last k (k0 for AXPY and
+ * HALF_AXPY, k1 for SSPRK3_STAGE3 and SSPRK2_FINAL, k2 for SSPRK3_FINAL, k3
+ * for RK4_FINAL).  With set_dt_step, *dt_step = _hit_target(*t, this stage's
+ * dt, target) first (fvhoe/ode.py:11-22, 274-279).  out may alias u0 or the
+ * stage input; dudt may be NULL when a combine is given. */
+typedef struct FvhoeRkFuse {
+  int kind;
+  double* out;
+  const double* u0;
+  const double* k0;
+  const double* k1;
+  const double* k2;
+  double* dt_step;
+  int set_dt_step;
+  const double* t;
+  double target;
+  int has_target;
+} FvhoeRkFuse;
+
+int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                   signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream);
+
 /* Bench instrumentation: with timing on, every fused-stage kernel launch is
  * bracketed by hipEvents on its stream.  fvhoe_stage_timing_read waits for
  * the recorded events and returns, per kernel (prims, flux x, flux y,
