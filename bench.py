@@ -1,19 +1,21 @@
 """Benchmark: FVHOE per-RK-stage update on one MI355X.
 
 Workload: periodic 2-D Kelvin-Helmholtz, 2048^2 cells, p=3, HLLC, SSP-RK3,
-a-posteriori NAD/PAD/SED check with MUSCL fallback, FP64 (gamma 1.4, CFL 0.8,
+a-posteriori NAD/PAD/SED check with MUSCL fallback (gamma 1.4, CFL 0.8,
 seeded IC).  A step is one SSP-RK3 step = 3 stages over the whole grid; the
 state lives in HBM for the whole timed region and the host never synchronises
-inside it.
+inside it.  Two forms are timed from the same initial state: the plain FP64
+path (bit-identical to the reference) and the FP32 bulk form (FP32 sweeps and
+Riemann solves; FP64 pressure recovery, trouble tests and RK accumulation).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32] [--no-cpu]
 
 Prints one JSON line.  `value` is Mcell-updates/s per RK stage (cells x
-stages / s).  `roofline` is for the kernel that takes the most time,
-from HIP events around each launch on the stream it runs on; `stage_roofline`
-uses the algorithmic bytes of a whole a-posteriori FP64 stage.
-`cpu_baseline` times the NumPy oracle (one thread) on one stage of the same
-workload at 1024^2.
+stages / s) of the last form listed.  `roofline` is for the kernel that takes
+the most time, from HIP events around each launch on the stream it runs on;
+`stage_roofline` uses the algorithmic bytes of a whole a-posteriori FP64
+stage.  `cpu_baseline` times the NumPy oracle (one thread) on two stages of
+the same workload.
 """
 
 from __future__ import annotations
@@ -38,8 +40,11 @@ BYTES_PER_CELL = {
     "prims_candidate": 120 + 40,  # read u, F, G, write WS
     "detect": 80 + 1,             # read W, WS, write flags
     "fallback": 0,                # work and traffic scale with the flagged cells only
-    "final": 80 + 40,             # read F, G, write dudt
+    "final": 80 + 40 + 80,        # read F, G, write dudt; RK combine reads u0 (+ k) and writes the next state
 }
+# bytes of the arrays stored as the bulk type (W, F, G, WS) per cell for each kernel, FP64
+BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 120, "detect": 80, "fallback": 0,
+              "final": 80}
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
 
 
@@ -49,6 +54,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--forms", default="fp64,fp32",
+                    help="comma list of precision forms to time; the last one is the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-stages", type=int, default=2)
@@ -76,6 +83,54 @@ def cpu_baseline(n: int, stages: int):
             "numpy": np.__version__}
 
 
+def run_form(n, form, steps, warmup, u0):
+    """Time `steps` SSP-RK3 steps of one precision form; returns (result dict, final state)."""
+    import torch
+
+    from fvhoe_amd import _lib
+    from fvhoe_amd import initial_conditions as ics
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+    from fvhoe_amd.solver import EulerSolver
+
+    class KH(ics.InitialCondition):
+        def base_ic(self):
+            return lambda x, y, z: u0
+
+    s = EulerSolver(w0=KH(), nx=n, ny=n, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
+                    bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
+                    progress_bar=False, bulk_precision=form)
+    assert s.fused
+    s.integrator = "ssprk3"
+    print(f"# {form}: warmup {warmup} steps at {n}^2", file=sys.stderr, flush=True)
+    s.advance(warmup)
+    torch.cuda.synchronize()
+    L = _lib.lib()
+    _lib.check(L.fvhoe_stage_timing(1), "timing")
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    s.advance(steps)
+    stop.record()
+    torch.cuda.synchronize()
+    ms_total = start.elapsed_time(stop)
+    ms = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_longlong * 7)()
+    _lib.check(L.fvhoe_stage_timing_read(ms, cnt), "timing read")
+    _lib.check(L.fvhoe_stage_timing(0), "timing")
+    cells = n * n
+    stages = 3 * steps
+    per_kernel = {}
+    for i, k in enumerate(KERNELS):
+        if cnt[i]:
+            avg = ms[i] / cnt[i]
+            b = BYTES_PER_CELL[k] - (BULK_BYTES[k] / 2 if form == "fp32" else 0)
+            per_kernel[k] = {"ms_avg": avg, "launches": cnt[i], "bytes_per_cell": b,
+                             "algorithmic_GBs": b * cells / (avg / 1e3) / 1e9}
+    print(f"# {form}: {ms_total / steps:.3f} ms/step", file=sys.stderr, flush=True)
+    return {"value": cells * stages / (ms_total / 1e3) / 1e6, "ms_per_step": ms_total / steps,
+            "kernels": per_kernel, "t": s.t}, s.u
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -84,23 +139,10 @@ def main():
         sys.exit(2)
     import torch
 
-    from fvhoe_amd import _lib
     from fvhoe_amd import initial_conditions as ics
-    from fvhoe_amd.boundary_conditions import BoundaryCondition
-    from fvhoe_amd.solver import EulerSolver
 
     n = a.n
     u0 = ics.kelvin_helmholtz(n, seed=0)
-
-    class KH(ics.InitialCondition):
-        def base_ic(self):
-            return lambda x, y, z: u0
-
-    s = EulerSolver(w0=KH(), nx=n, ny=n, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
-                    bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
-                    progress_bar=False)
-    assert s.fused
-    s.integrator = "ssprk3"
     if a.copy_probe:
         # a plain device copy of 167,772,160 bytes, three times: the scale for HBM byte counters
         src = torch.ones(5 * 2048 * 2048, dtype=torch.float64, device="cuda")
@@ -109,58 +151,51 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    print(f"# warmup {a.warmup} steps at {n}^2", file=sys.stderr, flush=True)
-    s.advance(a.warmup)
-    torch.cuda.synchronize()
-    L = _lib.lib()
-    _lib.check(L.fvhoe_stage_timing(1), "timing")
-    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start.record()
-    s.advance(a.steps)
-    stop.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = start.elapsed_time(stop)
-    ms = (ctypes.c_double * 7)()
-    cnt = (ctypes.c_longlong * 7)()
-    _lib.check(L.fvhoe_stage_timing_read(ms, cnt), "timing read")
-    _lib.check(L.fvhoe_stage_timing(0), "timing")
-    stages = 3 * a.steps
+    names = a.forms.split(",")
+    forms, states = {}, {}
+    for form in names:
+        forms[form], states[form] = run_form(n, form, a.steps, a.warmup, u0)
+    head = forms[names[-1]]
     cells = n * n
-    value = cells * stages / (ms_total / 1e3) / 1e6
-    per_kernel = {k: {"ms_avg": ms[i] / cnt[i], "launches": cnt[i]} for i, k in enumerate(KERNELS) if cnt[i]}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["ms_avg"])
-    dom_bytes = BYTES_PER_CELL[dom] * cells
-    achieved = dom_bytes / (per_kernel[dom]["ms_avg"] / 1e3) / 1e9
-    for k, v in per_kernel.items():
-        v["algorithmic_GBs"] = round(BYTES_PER_CELL[k] * cells / (v["ms_avg"] / 1e3) / 1e9, 1)
-        v["ms_avg"] = round(v["ms_avg"], 4)
-    stage_gbs = STAGE_BYTES_APOST_FP64 * cells * stages / (ms_total / 1e3) / 1e9
+    stages = 3 * a.steps
+    pk = head["kernels"]
+    dom = max(pk, key=lambda k: pk[k]["ms_avg"])
+    achieved = pk[dom]["algorithmic_GBs"]
+    stage_gbs = STAGE_BYTES_APOST_FP64 * head["value"] * 1e6 / 1e9
+    extra = {}
+    if "fp64" in states and "fp32" in states:
+        d = (states["fp32"][0] - states["fp64"][0]).abs().mean().item()
+        extra = {"l1_density_fp32_vs_fp64": d,
+                 "speedup_fp32_vs_fp64": round(forms["fp32"]["value"] / forms["fp64"]["value"], 3)}
+    for f in forms.values():
+        for v in f["kernels"].values():
+            v["ms_avg"] = round(v["ms_avg"], 4)
+            v["algorithmic_GBs"] = round(v["algorithmic_GBs"], 1)
+        f["value"] = round(f["value"], 2)
+        f["ms_per_step"] = round(f["ms_per_step"], 4)
     out = {
         "metric": "Mcell-updates/s per RK stage, learned+FP32 vs plain FP64, at matched L1 error",
-        "value": round(value, 2),
+        "value": head["value"],
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms_total / a.steps, 4),
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32" if names[-1] == "fp32" else "f64",
         "data": "synthetic",
-        "config": {"workload": f"kelvin_helmholtz_{n}x{n}_p3_hllc_aposteriori_ssprk3", "form": "plain FP64",
+        "config": {"workload": f"kelvin_helmholtz_{n}x{n}_p3_hllc_aposteriori_ssprk3", "form": names[-1],
                    "cells": cells, "stages_timed": stages, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "bytes_per_cell": BYTES_PER_CELL[dom]},
+                     "bytes_per_cell": pk[dom]["bytes_per_cell"]},
         "stage_roofline": {"bytes_per_cell_stage": round(STAGE_BYTES_APOST_FP64, 1),
                            "achieved": round(stage_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(stage_gbs / HBM_PEAK_GBS, 4)},
-        "kernels": per_kernel,
-        "wall_s": round(wall, 3),
+        "forms": forms,
+        **extra,
     }
     if not a.no_cpu:
         print("# cpu baseline", file=sys.stderr, flush=True)

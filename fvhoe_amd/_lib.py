@@ -32,6 +32,7 @@ class FvhoeStageConfig(ctypes.Structure):
         ("force_trouble", c_int), ("nad_eps", c_double), ("nad_mode", c_int), ("nad_range", c_int),
         ("nad_vars", c_int), ("pad_var", c_int), ("pad_lo", c_double), ("pad_hi", c_double),
         ("sed", c_int), ("sed_eps", c_double), ("fixed_dt_on", c_int), ("fixed_dt", c_double),
+        ("bulk_fp32", c_int),
     ]
 
 

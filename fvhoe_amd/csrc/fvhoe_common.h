@@ -117,31 +117,38 @@ __device__ __forceinline__ double sweep(Get&& x) {
 }
 
 // ---------------------------------------------------------------------------
-// thermodynamics (fvhoe/hydro.py:146-210); w = (rho, vx, vy, vz, P)
+// thermodynamics (fvhoe/hydro.py:146-210); w = (rho, vx, vy, vz, P).
+// T is the arithmetic type: double for the reference-exact path, float for
+// the FP32 bulk form (where pressure recovery still runs in double).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void prims_from_cons(const double u[5], double w[5], double gamma) {
-  const double rho = u[0];
+template <typename T>
+__device__ __forceinline__ void prims_from_cons(const T u[5], T w[5], T gamma) {
+  const T rho = u[0];
   w[0] = rho;
   w[1] = u[1] / rho;
   w[2] = u[2] / rho;
   w[3] = u[3] / rho;
-  const double msum = ((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]);
-  w[4] = (gamma - 1.0) * (u[4] - 0.5 * msum);
+  const T msum = ((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]);
+  w[4] = (gamma - T(1)) * (u[4] - T(0.5) * msum);
 }
 
 // total energy of compute_conservatives: P/(gamma-1) + 0.5*sum(m*v), m = rho*v
-__device__ __forceinline__ double energy(const double w[5], double gamma) {
-  const double mx = w[0] * w[1], my = w[0] * w[2], mz = w[0] * w[3];
-  return w[4] / (gamma - 1.0) + 0.5 * (((mx * w[1]) + (my * w[2])) + (mz * w[3]));
+template <typename T>
+__device__ __forceinline__ T energy(const T w[5], T gamma) {
+  const T mx = w[0] * w[1], my = w[0] * w[2], mz = w[0] * w[3];
+  return w[4] / (gamma - T(1)) + T(0.5) * (((mx * w[1]) + (my * w[2])) + (mz * w[3]));
 }
 
-__device__ __forceinline__ double sound_speed(double rho, double P, double gamma, double csq_floor) {
-  const double csq = gamma * P / rho;
+template <typename T>
+__device__ __forceinline__ T sound_speed(T rho, T P, T gamma, T csq_floor) {
+  const T csq = gamma * P / rho;
   return sqrt(csq > csq_floor ? csq : csq_floor);
 }
 
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
 
 // ---------------------------------------------------------------------------
 // Riemann solvers (fvhoe/riemann_solvers.py).  D = normal axis 0/1/2;
@@ -150,43 +157,43 @@ __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : 
 enum { RS_UPWIND = 0, RS_LLF = 1, RS_HLLC = 2 };
 
 // HLLC "David variation", fvhoe/riemann_solvers.py:110-183
-template <int D>
-__device__ __forceinline__ void hllc(const double wl[5], const double wr[5], double F[5], double gamma,
-                                     double csq_floor, double* v_godunov = nullptr) {
-  const double El = energy(wl, gamma), Er = energy(wr, gamma);
-  const double cl = sound_speed(wl[0], wl[4], gamma, csq_floor);
-  const double cr = sound_speed(wr[0], wr[4], gamma, csq_floor);
-  const double cmax = dmax(cl, cr);
-  const double vl = wl[D + 1], vr = wr[D + 1];
-  const double rhol = wl[0], rhor = wr[0], Pl = wl[4], Pr = wr[4];
-  const double Sl = dmin(vl, vr) - cmax;
-  const double Sr = dmax(vl, vr) + cmax;
-  const double rcL = rhol * (vl - Sl);
-  const double rcR = rhor * (Sr - vr);
-  const double vs = (rcL * vl + rcR * vr + Pl - Pr) / (rcL + rcR);
-  const double Ps = (rcL * Pr + rcR * Pl + rcL * rcR * (vl - vr)) / (rcL + rcR);
-  double r, v, P, E;
-  if (Sl > 0.0) {
+template <int D, typename T>
+__device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor,
+                                     T* v_godunov = nullptr) {
+  const T El = energy(wl, gamma), Er = energy(wr, gamma);
+  const T cl = sound_speed(wl[0], wl[4], gamma, csq_floor);
+  const T cr = sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const T cmax = dmax(cl, cr);
+  const T vl = wl[D + 1], vr = wr[D + 1];
+  const T rhol = wl[0], rhor = wr[0], Pl = wl[4], Pr = wr[4];
+  const T Sl = dmin(vl, vr) - cmax;
+  const T Sr = dmax(vl, vr) + cmax;
+  const T rcL = rhol * (vl - Sl);
+  const T rcR = rhor * (Sr - vr);
+  const T vs = (rcL * vl + rcR * vr + Pl - Pr) / (rcL + rcR);
+  const T Ps = (rcL * Pr + rcR * Pl + rcL * rcR * (vl - vr)) / (rcL + rcR);
+  T r, v, P, E;
+  if (Sl > T(0)) {
     r = rhol; v = vl; P = Pl; E = El;
-  } else if (vs > 0.0) {
+  } else if (vs > T(0)) {
     r = rhol * (Sl - vl) / (Sl - vs);
     E = ((Sl - vl) * El - Pl * vl + Ps * vs) / (Sl - vs);
     v = vs; P = Ps;
-  } else if (Sr > 0.0) {
+  } else if (Sr > T(0)) {
     r = rhor * (Sr - vr) / (Sr - vs);
     E = ((Sr - vr) * Er - Pr * vr + Ps * vs) / (Sr - vs);
     v = vs; P = Ps;
   } else {
     r = rhor; v = vr; P = Pr; E = Er;
   }
-  const double rv = r * v;
+  const T rv = r * v;
   F[0] = rv;
 #pragma unroll
   for (int k = 1; k < 4; ++k) {
     if (k == D + 1) {
       F[k] = rv * v + P;
     } else {
-      const double up = v > 0.0 ? wl[k] : (v < 0.0 ? wr[k] : 0.0);
+      const T up = v > T(0) ? wl[k] : (v < T(0) ? wr[k] : T(0));
       F[k] = rv * up;
     }
   }
@@ -195,9 +202,9 @@ __device__ __forceinline__ void hllc(const double wl[5], const double wr[5], dou
 }
 
 // physical flux along D (fvhoe/hydro.py:213-246)
-template <int D>
-__device__ __forceinline__ void phys_flux(const double w[5], const double u[5], double F[5], bool pressure) {
-  const double v = w[D + 1];
+template <int D, typename T>
+__device__ __forceinline__ void phys_flux(const T w[5], const T u[5], T F[5], bool pressure) {
+  const T v = w[D + 1];
   F[0] = v * w[0];
   F[1] = v * u[1];
   F[2] = v * u[2];
@@ -209,48 +216,56 @@ __device__ __forceinline__ void phys_flux(const double w[5], const double u[5], 
   }
 }
 
-__device__ __forceinline__ void cons_from_prims(const double w[5], double u[5], double gamma) {
+template <typename T>
+__device__ __forceinline__ void cons_from_prims(const T w[5], T u[5], T gamma) {
   u[0] = w[0];
   u[1] = w[0] * w[1];
   u[2] = w[0] * w[2];
   u[3] = w[0] * w[3];
-  u[4] = w[4] / (gamma - 1.0) + 0.5 * (((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]));
+  u[4] = w[4] / (gamma - T(1)) + T(0.5) * (((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]));
 }
 
 // local Lax-Friedrichs (Toro p.331; fvhoe/riemann_solvers.py:68-88)
-template <int D>
-__device__ __forceinline__ void llf(const double wl[5], const double wr[5], double F[5], double gamma,
-                                    double csq_floor) {
-  double ul[5], ur[5], Fl[5], Fr[5];
+template <int D, typename T>
+__device__ __forceinline__ void llf(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor) {
+  T ul[5], ur[5], Fl[5], Fr[5];
   cons_from_prims(wl, ul, gamma);
   cons_from_prims(wr, ur, gamma);
   phys_flux<D>(wl, ul, Fl, true);
   phys_flux<D>(wr, ur, Fr, true);
-  const double sl = fabs(wl[D + 1]) + sound_speed(wl[0], wl[4], gamma, csq_floor);
-  const double sr = fabs(wr[D + 1]) + sound_speed(wr[0], wr[4], gamma, csq_floor);
-  const double hs = 0.5 * dmax(sl, sr);
+  const T sl = fabs(wl[D + 1]) + sound_speed(wl[0], wl[4], gamma, csq_floor);
+  const T sr = fabs(wr[D + 1]) + sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const T hs = T(0.5) * dmax(sl, sr);
 #pragma unroll
-  for (int k = 0; k < 5; ++k) F[k] = 0.5 * (Fl[k] + Fr[k]) - hs * (ur[k] - ul[k]);
+  for (int k = 0; k < 5; ++k) F[k] = T(0.5) * (Fl[k] + Fr[k]) - hs * (ur[k] - ul[k]);
 }
 
 // upwinding without pressure (fvhoe/riemann_solvers.py:31-45)
-template <int D>
-__device__ __forceinline__ void upwind(const double wl[5], const double wr[5], double F[5], double gamma) {
-  double ul[5], ur[5], Fl[5], Fr[5];
+template <int D, typename T>
+__device__ __forceinline__ void upwind(const T wl[5], const T wr[5], T F[5], T gamma) {
+  T ul[5], ur[5], Fl[5], Fr[5];
   cons_from_prims(wl, ul, gamma);
   cons_from_prims(wr, ur, gamma);
   phys_flux<D>(wl, ul, Fl, false);
   phys_flux<D>(wr, ur, Fr, false);
-  const double v = wl[D + 1];
+  const T v = wl[D + 1];
 #pragma unroll
-  for (int k = 0; k < 5; ++k) F[k] = v > 0.0 ? Fl[k] : (v < 0.0 ? Fr[k] : 0.0);
+  for (int k = 0; k < 5; ++k) F[k] = v > T(0) ? Fl[k] : (v < T(0) ? Fr[k] : T(0));
 }
 
-template <int D>
-__device__ __forceinline__ void riemann(int solver, const double wl[5], const double wr[5], double F[5],
-                                        double gamma, double csq_floor, double* v_godunov = nullptr) {
+template <int D, typename T>
+__device__ __forceinline__ void riemann(int solver, const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor,
+                                        T* v_godunov = nullptr) {
   if (solver == RS_HLLC) hllc<D>(wl, wr, F, gamma, csq_floor, v_godunov);
   else if (solver == RS_LLF) llf<D>(wl, wr, F, gamma, csq_floor);
+  else upwind<D>(wl, wr, F, gamma);
+}
+
+// compile-time solver choice
+template <int RS, int D, typename T>
+__device__ __forceinline__ void riemann_t(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor) {
+  if constexpr (RS == RS_HLLC) hllc<D>(wl, wr, F, gamma, csq_floor);
+  else if constexpr (RS == RS_LLF) llf<D>(wl, wr, F, gamma, csq_floor);
   else upwind<D>(wl, wr, F, gamma);
 }
 
@@ -259,24 +274,29 @@ __device__ __forceinline__ void riemann(int solver, const double wl[5], const do
 // ---------------------------------------------------------------------------
 enum { LIM_NONE = 0, LIM_MINMOD = 1, LIM_MONCEN = 2 };
 
-__device__ __forceinline__ double minmod(double dl, double dr) {
-  const double den = dl > 0.0 ? (dl > 1e-16 ? dl : 1e-16) : (dl < -1e-16 ? dl : -1e-16);
-  double r = dr / den;
-  r = r < 1.0 ? r : 1.0;
-  return (r > 0.0 ? r : 0.0) * dl;
+template <typename T>
+__device__ __forceinline__ T minmod(T dl, T dr) {
+  const T lo = T(1e-16);
+  const T den = dl > T(0) ? (dl > lo ? dl : lo) : (dl < -lo ? dl : -lo);
+  T r = dr / den;
+  r = r < T(1) ? r : T(1);
+  return (r > T(0) ? r : T(0)) * dl;
 }
 
-__device__ __forceinline__ double dsign(double x) { return x > 0.0 ? 1.0 : (x < 0.0 ? -1.0 : x); }
+template <typename T>
+__device__ __forceinline__ T dsign(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : x); }
 
-__device__ __forceinline__ double moncen(double dl, double dr) {
-  const double dc = 0.5 * (dl + dr);
-  double s = dmin(fabs(2.0 * dl), fabs(2.0 * dr));
+template <typename T>
+__device__ __forceinline__ T moncen(T dl, T dr) {
+  const T dc = T(0.5) * (dl + dr);
+  T s = dmin(fabs(T(2) * dl), fabs(T(2) * dr));
   s = dsign(dc) * dmin(s, fabs(dc));
-  return dl * dr >= 0.0 ? s : 0.0;
+  return dl * dr >= T(0) ? s : T(0);
 }
 
-__device__ __forceinline__ double limited_slope(int limiter, double dl, double dr) {
-  return limiter == LIM_MINMOD ? minmod(dl, dr) : limiter == LIM_MONCEN ? moncen(dl, dr) : 0.5 * (dl + dr);
+template <typename T>
+__device__ __forceinline__ T limited_slope(int limiter, T dl, T dr) {
+  return limiter == LIM_MINMOD ? minmod(dl, dr) : limiter == LIM_MONCEN ? moncen(dl, dr) : T(0.5) * (dl + dr);
 }
 
 // ---------------------------------------------------------------------------

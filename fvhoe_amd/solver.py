@@ -72,6 +72,7 @@ class EulerSolver(ODE):
         slab_buffer_size: int = 30,
         cupy: bool = False,
         fused: bool = None,
+        bulk_precision: str = "fp64",
     ):
         if w0 is None:
             raise ValueError("w0 (an InitialCondition) is required")
@@ -79,6 +80,11 @@ class EulerSolver(ODE):
             raise TypeError(f"Invalid Riemann solver {riemann_solver}")
         if slope_limiter not in _lib.LIMITER:
             raise ValueError(f"Unknown slope limiter: {slope_limiter}")
+        if bulk_precision not in ("fp64", "fp32"):
+            raise ValueError(f"bulk_precision must be 'fp64' or 'fp32', got {bulk_precision!r}")
+        # "fp32": reconstruction sweeps and Riemann fluxes in FP32 (fused stage only); pressure recovery,
+        # trouble tests, divergence and the RK accumulation stay FP64
+        self.bulk_precision = bulk_precision
         self.csq_floor = csq_floor
         self.density_floor = density_floor  # stored, never applied (as in the reference)
         self.pressure_floor = pressure_floor
@@ -131,6 +137,8 @@ class EulerSolver(ODE):
         self.fused = supported if fused is None else (bool(fused) and supported)
         if fused and not supported:
             raise NotImplementedError("the fused stage does not cover this configuration")
+        if bulk_precision == "fp32" and not self.fused:
+            raise NotImplementedError("the FP32 bulk form runs on the fused stage (1-D x or 2-D x-y, 5 variables)")
         self._ws = None
         self._trouble = _lib.empty(self.n, dtype=_lib.torch().int8) if a_posteriori_slope_limiting else None
 
@@ -159,6 +167,7 @@ class EulerSolver(ODE):
         c.sed_eps = float(self.SED_tolerance)
         c.fixed_dt_on = int(self.fixed_dt is not None)
         c.fixed_dt = float(self.fixed_dt or 0.0)
+        c.bulk_fp32 = int(self.bulk_precision == "fp32")
         return c
 
     # ------------------------------------------------------------------- stage

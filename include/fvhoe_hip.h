@@ -166,6 +166,8 @@ typedef struct FvhoeStageConfig {
   double sed_eps;
   int fixed_dt_on;
   double fixed_dt;
+  int bulk_fp32;        /* 1: FP32 sweeps, Riemann solves and W/F/G storage; FP64 pressure
+                           recovery, trouble tests, divergence and RK accumulation */
 } FvhoeStageConfig;
 
 /* 1 if fvhoe_stage handles cfg (1-D along x, or 2-D in x-y, nvars = 5,
