@@ -178,11 +178,32 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   const int tid = threadIdx.x;
   const bool inside = cx0 >= 0 && cx0 + HX0 <= g.n0 && cy0 >= 0 && cy0 + HY0 <= g.n1;
 
+  // all five variables' tile elements are fetched up front (one memory latency per block)
+  constexpr int NE = (HX0 * HY0 + NT - 1) / NT;
+  T pre[5][NE];
+  {
+    constexpr int DR = NT / HY0, DC = NT % HY0;
+    int r = tid / HY0, c = tid % HY0;
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      if (tid + k * NT < HX0 * HY0) {
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          pre[v][k] = T(inside ? src.at(g, v, cx0 + r, cy0 + c) : src.get(g, v, cx0 + r, cy0 + c));
+      }
+      r += DR;
+      c += DC;
+      if (c >= HY0) {
+        c -= HY0;
+        ++r;
+      }
+    }
+  }
+#pragma unroll
   for (int v = 0; v < 5; ++v) {
-    if (inside)
-      tile_for<HX0, HY0, NT>([&](int e, int r, int c) { S0[e] = T(src.at(g, v, cx0 + r, cy0 + c)); });
-    else
-      tile_for<HX0, HY0, NT>([&](int e, int r, int c) { S0[e] = T(src.get(g, v, cx0 + r, cy0 + c)); });
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+      if (tid + k * NT < HX0 * HY0) S0[tid + k * NT] = pre[v][k];
     __syncthreads();
     tile_for<HX1, HY0, NT>([&](int e, int i, int j) {
       S1[e] = sweepT<KIND_C, PX, T>([&](int t) { return S0[(i + RCX + t - reach(PX)) * HY0 + j]; });
@@ -291,19 +312,38 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
   // W index of normal cell (f0 - 1 - EL) and transverse cell (b0 - RC - RT); both >= 0
   const int wn0 = f0 - 1 - EL + geN, wt0 = b0 - RC - RT + geT;
 
-  for (int v = 0; v < 5; ++v) {
-    const T* Wv = W + v * NW0 * NW1;
-    if (D == 0) {
-      tile_for<HN, HT0, NT>([&](int e, int r, int c) {
-        const int a = min(wn0 + r, NW0 - 1), b = min(wt0 + c, NW1 - 1);  // clamped reads feed unused outputs only
-        S0[e] = Wv[a * NW1 + b];
-      });
-    } else {
-      tile_for<HT0, HN, NT>([&](int e, int c, int r) {  // contiguous W axis (y = normal) fastest
-        const int a = min(wt0 + c, NW0 - 1), b = min(wn0 + r, NW1 - 1);
-        S0[r * HT0 + c] = Wv[a * NW1 + b];
-      });
+  // all five variables' tile elements are fetched up front (one memory latency per block);
+  // for D = 1 the contiguous W axis (y = normal) runs fastest over the threads
+  constexpr int NE = (HN * HT0 + NT - 1) / NT;
+  constexpr int WFAST = D == 0 ? HT0 : HN;
+  T pre[5][NE];
+  int dst[NE];
+  {
+    constexpr int DR = NT / WFAST, DC = NT % WFAST;
+    int r = tid / WFAST, c = tid % WFAST;  // (slow, fast) index pair
+#pragma unroll
+    for (int k = 0; k < NE; ++k) {
+      if (tid + k * NT < HN * HT0) {
+        // clamped reads feed unused outputs only
+        const int a = D == 0 ? min(wn0 + r, NW0 - 1) : min(wt0 + r, NW0 - 1);
+        const int b = D == 0 ? min(wt0 + c, NW1 - 1) : min(wn0 + c, NW1 - 1);
+        dst[k] = D == 0 ? r * HT0 + c : c * HT0 + r;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) pre[v][k] = W[(v * NW0 + a) * NW1 + b];
+      }
+      r += DR;
+      c += DC;
+      if (c >= WFAST) {
+        c -= WFAST;
+        ++r;
+      }
     }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+#pragma unroll
+    for (int k = 0; k < NE; ++k)
+      if (tid + k * NT < HN * HT0) S0[dst[k]] = pre[v][k];
     __syncthreads();
     tile_for<HN, HT1, NT>([&](int e, int r, int c) {
       const T x = sweepT<KIND_C, PT, T>([&](int t) { return S0[r * HT0 + c + RC + t - reach(PT)]; });
@@ -382,7 +422,8 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
     const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
     const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
     const int NWC = NW0 * NW1, NSC = NS0 * NS1;
-    bool nad = false;
+    bool tvv[5];
+#pragma unroll
     for (int v = 0; v < 5; ++v) {
       const T* wv = W + v * NWC;
       double M = (double)wv[wb], m = M;
@@ -414,8 +455,12 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
         ind = dmin(uc - lo, hi - uc);
         tv = ind < 0.0;
       }
-      if (!((a.nad_vars >> v) & 1)) tv = false;
-      if (tv && a.sed) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
+      tvv[v] = tv && ((a.nad_vars >> v) & 1);
+    }
+    bool nad = false;
+    for (int v = 0; v < 5; ++v) {
+      bool tv = tvv[v];
+      if (tv && a.sed && !nad) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
         const T* sv = WS + v * NSC;
         double alpha = sed_alpha_1d([&](int o) { return (double)sv[sb + o * NS1]; }, a.sed_eps);
         if (two_d) alpha = dmin(alpha, sed_alpha_1d([&](int o) { return (double)sv[sb + o]; }, a.sed_eps));
