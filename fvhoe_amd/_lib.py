@@ -68,6 +68,7 @@ _SIGNATURES = {
                                 c_double, c_int, _P, POINTER(c_longlong), c_longlong, POINTER(c_int), _P]),
     "fvhoe_blend": (c_int, [_P, _P, _P, c_int, c_longlong, _P]),
     "fvhoe_rk_combine": (c_int, [c_int, _P, _P, _P, _P, _P, _P, _P, c_longlong, _P]),
+    "fvhoe_log_troubles": (c_int, [_P, _P, c_longlong, c_int, c_double, _P]),
     "fvhoe_set_scalar": (c_int, [_P, c_double, _P]),
     "fvhoe_hit_target": (c_int, [_P, _P, _P, c_double, c_int, _P]),
     "fvhoe_advance_time": (c_int, [_P, _P, _P]),

@@ -475,6 +475,15 @@ __global__ void k_rk(int kind, double* out, const double* u, const double* __res
   }
 }
 
+// trouble log (fvhoe/solver.py:794-833): 0: dst = t, 1: dst = dst + t, 2: dst = dst / d
+__global__ void k_log(double* __restrict__ dst, const signed char* __restrict__ t, long long n, int mode, double d) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    if (mode == 0) dst[i] = (double)t[i];
+    else if (mode == 1) dst[i] = dst[i] + (double)t[i];
+    else dst[i] = dst[i] / d;
+  }
+}
+
 __global__ void k_set_scalar(double* p, double v) { p[0] = v; }
 
 __global__ void k_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target) {
@@ -684,6 +693,14 @@ int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, c
   if (kind == FVHOE_RK_RK4_FINAL && (!k1 || !k2 || !k3)) return FVHOE_ERR_ARG;
   if (n == 0) return FVHOE_OK;
   k_rk<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(kind, out, u, k0, k1, k2, k3, dt, n);
+  return launch_status();
+}
+
+int fvhoe_log_troubles(double* mean, const signed char* trouble, long long n, int mode, double divisor,
+                       void* stream) {
+  if (!mean || n < 0 || mode < 0 || mode > 2 || (mode < 2 && !trouble)) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_log<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, trouble, n, mode, divisor);
   return launch_status();
 }
 

@@ -145,10 +145,13 @@ __device__ __forceinline__ T sound_speed(T rho, T P, T gamma, T csq_floor) {
   return sqrt(csq > csq_floor ? csq : csq_floor);
 }
 
+// max / min as the hardware v_max / v_min: for non-NaN operands they equal
+// np.maximum / np.minimum except possibly in the sign of a zero result, which
+// no caller's result depends on (differences, comparisons, alpha < 1 tests).
 template <typename T>
-__device__ __forceinline__ T dmax(T a, T b) { return a > b ? a : b; }
+__device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
 template <typename T>
-__device__ __forceinline__ T dmin(T a, T b) { return a < b ? a : b; }
+__device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
 
 // ---------------------------------------------------------------------------
 // Riemann solvers (fvhoe/riemann_solvers.py).  D = normal axis 0/1/2;

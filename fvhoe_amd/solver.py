@@ -131,6 +131,9 @@ class EulerSolver(ODE):
         self.trouble_counter = 1  # fvhoe/solver.py:400
         nx_, ny_, nz_ = self.n
         self.log = {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag", "mean PAD mag")}
+        self._zero_like_log = _lib.zeros((nx_, ny_, nz_))
+        self._one = _lib.zeros((1,))
+        self._one.fill_(1.0)
 
         self._cfg = self._stage_config()
         supported = bool(_lib.lib().fvhoe_stage_supported(self._cfg)) and self.nvars == 5 and not convex
@@ -351,20 +354,26 @@ class EulerSolver(ODE):
     # ------------------------------------------------------------ bookkeeping
     def log_troubles(self, trouble=None, NAD_violation_magnitude=None, PAD_violation_magnitude=None,
                      reset: bool = False):
-        """Per-step means of trouble and violation magnitudes (fvhoe/solver.py:794-833)."""
+        """Per-step means of trouble and violation magnitudes (fvhoe/solver.py:794-833);
+        the counter starts at 1 (fvhoe/solver.py:400)."""
         lg = self.log
+        n = lg["mean trouble"].numel()
         if reset:
             if self.trouble_counter > 0:
                 for k in lg:
-                    lg[k] = lg[k] / self.trouble_counter
+                    _lib.call("fvhoe_log_troubles", _lib.ptr(lg[k]), None, n, 2, float(self.trouble_counter),
+                              _lib.stream())
                 self.trouble_counter = 0
             return
-        vals = {"mean trouble": trouble.to(_lib.torch().float64),
-                "mean NAD mag": NAD_violation_magnitude, "mean PAD mag": PAD_violation_magnitude}
-        for k, v in vals.items():
+        t8 = trouble if trouble.dtype == _lib.torch().int8 else trouble.to(_lib.torch().int8).contiguous()
+        _lib.call("fvhoe_log_troubles", _lib.ptr(lg["mean trouble"]), _lib.ptr(t8), n,
+                  0 if self.trouble_counter == 0 else 1, 1.0, _lib.stream())
+        for k, v in (("mean NAD mag", NAD_violation_magnitude), ("mean PAD mag", PAD_violation_magnitude)):
             if v is None:
                 continue
-            lg[k] = v.clone() if self.trouble_counter == 0 else lg[k] + v
+            _lib.call("fvhoe_rk_combine", _lib.RK_AXPY, _lib.ptr(lg[k]),
+                      _lib.ptr(self._zero_like_log if self.trouble_counter == 0 else lg[k]), _lib.ptr(v.contiguous()),
+                      None, None, None, _lib.ptr(self._one), n, _lib.stream())
         self.trouble_counter += 1
 
     def _log_troubles_device(self, flags):

@@ -136,6 +136,11 @@ int fvhoe_blend(double* F, const double* Fl, const double* theta, int nv, long l
 int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, const double* k1, const double* k2,
                      const double* k3, const double* dt, long long n, void* stream);
 
+/* EulerSolver.log_troubles (fvhoe/solver.py:794-833) on device: mode 0
+ * mean = trouble, 1 mean += trouble, 2 mean /= divisor (trouble unused). */
+int fvhoe_log_troubles(double* mean, const signed char* trouble, long long n, int mode, double divisor,
+                       void* stream);
+
 /* device scalars: *p = v; *dt_step = _hit_target(t, dt_stage, target) (fvhoe/ode.py:11-22); *t += *dt */
 int fvhoe_set_scalar(double* p, double v, void* stream);
 int fvhoe_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target,
