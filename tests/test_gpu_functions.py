@@ -1,0 +1,124 @@
+"""Each drop-in function of fvhoe_amd (the reference's module-level API) on
+the GPU against the oracle on seeded inputs, bit for bit."""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from fvhoe_amd import boundary_conditions as bcm
+from fvhoe_amd import fv, hydro, riemann_solvers, slope_limiting, smooth_extrema_detection
+from gpu_helpers import assert_bits_equal
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(7)
+
+
+def state(shape, rng=RNG):
+    w = np.empty((5,) + shape)
+    w[0] = 0.5 + rng.random(shape)
+    w[1:4] = rng.standard_normal((3,) + shape) * 0.5
+    w[4] = 0.3 + rng.random(shape)
+    return w
+
+
+@pytest.mark.parametrize("p", range(9))
+@pytest.mark.parametrize("axis", [1, 2, 3])
+@pytest.mark.parametrize("pos", ["l", "c", "r"])
+def test_conservative_interpolation(p, axis, pos):
+    u = RNG.standard_normal((5, 19, 18, 17))
+    assert_bits_equal(fv.conservative_interpolation(u, p, axis, pos), O.conservative_interpolation(u, p, axis, pos))
+
+
+@pytest.mark.parametrize("p", range(9))
+def test_transverse_and_composites(p):
+    u = RNG.standard_normal((5, 20, 19, 18))
+    assert_bits_equal(fv.transverse_reconstruction(u, p, 2), O.transverse_reconstruction(u, p, 2))
+    pp = (p, max(p - 1, 0), min(p + 1, 8))
+    assert_bits_equal(fv.interpolate_cell_centers(u, pp), O.interpolate_cell_centers(u, pp))
+    assert_bits_equal(fv.interpolate_fv_averages(u, pp), O.interpolate_fv_averages(u, pp))
+
+
+def test_thermodynamics():
+    w = state((13, 12, 11))
+    hs = hydro.HydroState()
+    u = O.compute_conservatives(w, 1.4)
+    assert_bits_equal(hydro.compute_conservatives(hs, w, 1.4), u)
+    assert_bits_equal(hydro.compute_primitives(hs, u, 1.4), O.compute_primitives(u, 1.4))
+    assert_bits_equal(hydro.compute_sound_speed(w, 1.4, 1e-10), O.compute_sound_speed(w, 1.4, 1e-10))
+    assert hydro.hydro_dt(w, 0.01, 3, 0.8, 1.4, 1e-10) == O.hydro_dt(w, 0.01, 3, 0.8, 1.4, 1e-10)
+
+
+@pytest.mark.parametrize("name", ["hllc", "llf", "advection_upwind"])
+@pytest.mark.parametrize("dim", ["x", "y", "z"])
+def test_riemann_solvers(name, dim):
+    wl, wr = state((40, 3, 2)), state((40, 3, 2))
+    wr[:, :10] = wl[:, :10]            # equal states
+    wl[1 + "xyz".index(dim), 10:15] = 3.0  # supersonic to the right
+    wr[1 + "xyz".index(dim), 15:20] = -3.0  # supersonic to the left
+    got = getattr(riemann_solvers, name)(hydro.HydroState(), (wl, wr), 1.4, dim, 1e-10)
+    assert_bits_equal(got, O.RIEMANN[name](wl, wr, 1.4, dim, 1e-10), name)
+
+
+@pytest.mark.parametrize("limiter", ["minmod", "moncen", None])
+@pytest.mark.parametrize("axis", [1, 2, 3])
+def test_muscl(limiter, axis):
+    u = RNG.standard_normal((5, 12, 11, 10))
+    u[:, 3:5] = 0.25  # flat stretch: zero slopes
+    gl, gr = slope_limiting.MUSCL_interpolations(u, axis, limiter)
+    ol, orr = O.MUSCL_interpolations(u, axis, limiter)
+    assert_bits_equal(gl, ol)
+    assert_bits_equal(gr, orr)
+
+
+@pytest.mark.parametrize("dims,shape", [("x", (30, 1, 1)), ("xy", (16, 14, 1)), ("xz", (14, 1, 13)),
+                                        ("xyz", (11, 10, 9))])
+def test_smooth_extrema_detector(dims, shape):
+    u = RNG.standard_normal((5,) + shape)
+    u[:, :5] = 1.0
+    if len(dims) == 1:
+        a, b = smooth_extrema_detection.compute_1d_smooth_extrema_detector(u, dims), \
+            O.compute_1d_smooth_extrema_detector(u, dims)
+    elif len(dims) == 2:
+        a, b = smooth_extrema_detection.compute_2d_smooth_extrema_detector(u, dims), \
+            O.compute_2d_smooth_extrema_detector(u, dims)
+    else:
+        a, b = smooth_extrema_detection.compute_3d_smooth_extrema_detector(u), O.compute_3d_smooth_extrema_detector(u)
+    assert_bits_equal(a, b)
+
+
+@pytest.mark.parametrize("mode", ["global", "local"])
+@pytest.mark.parametrize("range_type", ["relative", "absolute"])
+@pytest.mark.parametrize("dims,shape", [("x", (40, 1, 1)), ("xy", (18, 17, 1)), ("xyz", (11, 10, 9))])
+def test_detect_troubled_cells(mode, range_type, dims, shape):
+    w = state(shape)
+    ws = w + 0.05 * RNG.standard_normal(w.shape)
+    ws[4, 4] = -0.2
+    pad = {"rho": (0.0, np.inf), "P": (0.0, np.inf)}
+    got = slope_limiting.detect_troubled_cells(w, ws, dims, 1e-3, mode, range_type, ("rho", "P", "vx"), pad, True)
+    ref = O.detect_troubled_cells(w, ws, dims, 1e-3, mode, range_type, ("rho", "P", "vx"), pad, True)
+    assert np.array_equal(np.asarray(got[0]), np.asarray(ref[0]).astype(np.int64))
+    assert_bits_equal(got[1], ref[1])
+    assert_bits_equal(got[2], ref[2])
+
+
+@pytest.mark.parametrize("convex", [False, True])
+@pytest.mark.parametrize("dims,shape", [("x", (25, 1, 1)), ("xy", (12, 11, 1)), ("yz", (1, 12, 10))])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_broadcast(convex, dims, shape, periodic):
+    t = (RNG.random(shape) < 0.15).astype(np.int64)
+    t.flat[0] = 1
+    per = (periodic,) * 3
+    got = slope_limiting.broadcast_to_troubled_interfaces(t, dims, convex, *per)
+    ref = O.broadcast_to_troubled_interfaces(t, dims, convex, per)
+    for a, b in zip(got, ref):
+        assert_bits_equal(a, b)
+
+
+@pytest.mark.parametrize("kinds", [("periodic", "periodic"), ("outflow", "outflow"), ("reflective", "reflective"),
+                                   ("free", "free"), ("outflow", "reflective")])
+def test_boundary_conditions(kinds):
+    u = RNG.standard_normal((5, 9, 8, 7))
+    gw = (3, 2, 4)
+    bc = bcm.BoundaryCondition(x=kinds, y=kinds[::-1], z="periodic")
+    ref = O.apply_bc(u, gw, (kinds, kinds[::-1], ("periodic", "periodic")))
+    assert_bits_equal(bc.apply(u, gw), ref)
