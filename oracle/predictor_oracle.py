@@ -1,0 +1,115 @@
+"""NumPy reference of the troubled-cell predictor (test infrastructure only).
+
+The predictor has no counterpart in FVHOE.  Network: three valid 5x5
+convolutions 4 -> 16 -> 16 -> 1 channels with ReLU after the first two, on
+feature maps padded by 6 cells; weights are used rounded to bfloat16 (round
+to nearest even), activations and accumulation in float64 here.  Loss:
+class-weighted binary cross-entropy on logits, mean over pixels.
+
+Pinned by finite differences of the loss (tests/test_predictor_oracle.py).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+K = 5
+CH = (4, 16, 16, 1)
+HALO = 6
+
+
+def param_shapes():
+    return [("w1", (16, 4, K, K)), ("b1", (16,)), ("w2", (16, 16, K, K)), ("b2", (16,)),
+            ("w3", (1, 16, K, K)), ("b3", (1,))]
+
+
+def n_params() -> int:
+    return sum(int(np.prod(s)) for _, s in param_shapes())
+
+
+def unpack(theta: np.ndarray) -> dict:
+    out, o = {}, 0
+    for name, shape in param_shapes():
+        n = int(np.prod(shape))
+        out[name] = theta[o:o + n].reshape(shape)
+        o += n
+    return out
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """float32 -> nearest bfloat16 (ties to even) -> float32."""
+    b = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    lsb = (b >> 16) & 1
+    r = ((b + 0x7FFF + lsb) >> 16) << 16
+    return r.astype(np.uint32).view(np.float32)
+
+
+def conv_valid(x: np.ndarray, w: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """x (B, Cin, H, W), w (Cout, Cin, K, K) -> (B, Cout, H-K+1, W-K+1), cross-correlation."""
+    B, C, H, W = x.shape
+    Ho, Wo = H - K + 1, W - K + 1
+    out = np.zeros((B, w.shape[0], Ho, Wo))
+    for dy in range(K):
+        for dx in range(K):
+            patch = x[:, :, dy:dy + Ho, dx:dx + Wo]
+            out += np.einsum("bchw,oc->bohw", patch, w[:, :, dy, dx])
+    return out + b[None, :, None, None]
+
+
+def forward(theta: np.ndarray, x: np.ndarray, rounding: bool = True):
+    """Logits (B, H-12, W-12).  rounding=False uses the float weights as they are
+    (for finite-difference checks of backward)."""
+    p = {k: (bf16_round(v) if (rounding and k.startswith("w")) else v).astype(np.float64)
+         for k, v in unpack(theta).items()}
+    z1 = conv_valid(x, p["w1"], p["b1"])
+    a1 = np.maximum(z1, 0)
+    z2 = conv_valid(a1, p["w2"], p["b2"])
+    a2 = np.maximum(z2, 0)
+    z3 = conv_valid(a2, p["w3"], p["b3"])[:, 0]
+    return z3, (x, z1, a1, z2, a2, p)
+
+
+def loss(z: np.ndarray, y: np.ndarray, pos_weight: float) -> float:
+    sp = np.logaddexp(0, -z)  # softplus(-z)
+    l = pos_weight * y * sp + (1 - y) * (z + sp)
+    return float(l.mean())
+
+
+def backward(z, cache, y, pos_weight):
+    x, z1, a1, z2, a2, p = cache
+    n = z.size
+    s = 1 / (1 + np.exp(-z))
+    dz = (pos_weight * y * (s - 1) + (1 - y) * s) / n
+    g = {}
+
+    def conv_grads(inp, dout, w):
+        B, C, H, W = inp.shape
+        Ho, Wo = dout.shape[2], dout.shape[3]
+        dw = np.zeros_like(w)
+        din = np.zeros_like(inp)
+        for dy in range(K):
+            for dx in range(K):
+                patch = inp[:, :, dy:dy + Ho, dx:dx + Wo]
+                dw[:, :, dy, dx] = np.einsum("bohw,bchw->oc", dout, patch)
+                din[:, :, dy:dy + Ho, dx:dx + Wo] += np.einsum("bohw,oc->bchw", dout, w[:, :, dy, dx])
+        return dw, dout.sum(axis=(0, 2, 3)), din
+
+    d3 = dz[:, None]
+    g["w3"], g["b3"], da2 = conv_grads(a2, d3, p["w3"])
+    d2 = da2 * (z2 > 0)
+    g["w2"], g["b2"], da1 = conv_grads(a1, d2, p["w2"])
+    d1 = da1 * (z1 > 0)
+    g["w1"], g["b1"], _ = conv_grads(x, d1, p["w1"])
+    return np.concatenate([g[name].ravel() for name, _ in param_shapes()])
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    parts = []
+    for name, shape in param_shapes():
+        if name.startswith("w"):
+            fan_in = shape[1] * K * K
+            parts.append(rng.standard_normal(shape).ravel() * np.sqrt(2.0 / fan_in))
+        else:
+            parts.append(np.zeros(int(np.prod(shape))))
+    return np.concatenate(parts).astype(np.float32)
