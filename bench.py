@@ -4,11 +4,14 @@ Workload: periodic 2-D Kelvin-Helmholtz, 2048^2 cells, p=3, HLLC, SSP-RK3,
 a-posteriori NAD/PAD/SED check with MUSCL fallback (gamma 1.4, CFL 0.8,
 seeded IC).  A step is one SSP-RK3 step = 3 stages over the whole grid; the
 state lives in HBM for the whole timed region and the host never synchronises
-inside it.  Two forms are timed from the same initial state: the plain FP64
-path (bit-identical to the reference) and the FP32 bulk form (FP32 sweeps and
-Riemann solves; FP64 pressure recovery, trouble tests and RK accumulation).
+inside it.  Three forms are timed from the same initial state: the plain FP64
+path (bit-identical to the reference), the FP32 bulk form (FP32 sweeps and
+Riemann solves; FP64 pressure recovery, trouble tests and RK accumulation),
+and the learned form (FP32 bulk plus the troubled-cell predictor, run once
+per step, whose pre-flagged cells get MUSCL fluxes before the candidate; the
+plain check still runs on every cell).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32,learned] [--no-cpu]
 
 Prints one JSON line.  `value` is Mcell-updates/s per RK stage (cells x
 stages / s) of the last form listed.  `roofline` is for the kernel that takes
@@ -32,7 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes per cell for one launch of each fused-stage kernel (FP64, 5 variables)
-KERNELS = ["prims", "flux_x", "flux_y", "prims_candidate", "detect", "fallback", "final"]
+KERNELS = ["prims", "flux_x", "flux_y", "prims_candidate", "detect", "fallback", "final", "predict"]
 BYTES_PER_CELL = {
     "prims": 40 + 40,             # read u, write W
     "flux_x": 40 + 40,            # read W, write F
@@ -41,10 +44,11 @@ BYTES_PER_CELL = {
     "detect": 80 + 1,             # read W, WS, write flags
     "fallback": 0,                # work and traffic scale with the flagged cells only
     "final": 80 + 40 + 80,        # read F, G, write dudt; RK combine reads u0 (+ k) and writes the next state
+    "predict": 40 + 16 + 16 + 1,  # features: read W (rho, vx, vy, P + range), write X (4 floats); network: read X, write flags
 }
 # bytes of the arrays stored as the bulk type (W, F, G, WS) per cell for each kernel, FP64
 BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 120, "detect": 80, "fallback": 0,
-              "final": 80}
+              "final": 80, "predict": 40}
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
 
 
@@ -54,8 +58,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--forms", default="fp64,fp32",
-                    help="comma list of precision forms to time; the last one is the headline")
+    ap.add_argument("--forms", default="fp64,fp32,learned",
+                    help="comma list of forms to time (fp64, fp32, learned); the last one is the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-stages", type=int, default=2)
@@ -84,21 +88,24 @@ def cpu_baseline(n: int, stages: int):
 
 
 def run_form(n, form, steps, warmup, u0):
-    """Time `steps` SSP-RK3 steps of one precision form; returns (result dict, final state)."""
+    """Time `steps` SSP-RK3 steps of one form ("fp64", "fp32", or "learned" = FP32 bulk with the
+    troubled-cell predictor); returns (result dict, final state)."""
     import torch
 
     from fvhoe_amd import _lib
     from fvhoe_amd import initial_conditions as ics
     from fvhoe_amd.boundary_conditions import BoundaryCondition
+    from fvhoe_amd.predictor import DEFAULT_WEIGHTS, TroubledCellPredictor
     from fvhoe_amd.solver import EulerSolver
 
     class KH(ics.InitialCondition):
         def base_ic(self):
             return lambda x, y, z: u0
 
+    pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form == "learned" else None
     s = EulerSolver(w0=KH(), nx=n, ny=n, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
                     bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
-                    progress_bar=False, bulk_precision=form)
+                    progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred)
     assert s.fused
     s.integrator = "ssprk3"
     print(f"# {form}: warmup {warmup} steps at {n}^2", file=sys.stderr, flush=True)
@@ -113,8 +120,8 @@ def run_form(n, form, steps, warmup, u0):
     stop.record()
     torch.cuda.synchronize()
     ms_total = start.elapsed_time(stop)
-    ms = (ctypes.c_double * 7)()
-    cnt = (ctypes.c_longlong * 7)()
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_longlong * 8)()
     _lib.check(L.fvhoe_stage_timing_read(ms, cnt), "timing read")
     _lib.check(L.fvhoe_stage_timing(0), "timing")
     cells = n * n
@@ -123,12 +130,17 @@ def run_form(n, form, steps, warmup, u0):
     for i, k in enumerate(KERNELS):
         if cnt[i]:
             avg = ms[i] / cnt[i]
-            b = BYTES_PER_CELL[k] - (BULK_BYTES[k] / 2 if form == "fp32" else 0)
+            b = BYTES_PER_CELL[k] - (BULK_BYTES[k] / 2 if form != "fp64" else 0)
             per_kernel[k] = {"ms_avg": avg, "launches": cnt[i], "bytes_per_cell": b,
                              "algorithmic_GBs": b * cells / (avg / 1e3) / 1e9}
     print(f"# {form}: {ms_total / steps:.3f} ms/step", file=sys.stderr, flush=True)
-    return {"value": cells * stages / (ms_total / 1e3) / 1e6, "ms_per_step": ms_total / steps,
-            "kernels": per_kernel, "t": s.t}, s.u
+    out = {"value": cells * stages / (ms_total / 1e3) / 1e6, "ms_per_step": ms_total / steps,
+           "kernels": per_kernel, "t": s.t}
+    if pred is not None:
+        st = [int(v) for v in s.predictor_stats.tolist()]
+        out["predictor"] = {"preflagged": st[0], "second_pass": st[1], "preflag_unflagged": st[2],
+                            "logit_threshold": pred.logit_threshold}
+    return out, s.u
 
 
 def main():
@@ -163,10 +175,10 @@ def main():
     achieved = pk[dom]["algorithmic_GBs"]
     stage_gbs = STAGE_BYTES_APOST_FP64 * head["value"] * 1e6 / 1e9
     extra = {}
-    if "fp64" in states and "fp32" in states:
-        d = (states["fp32"][0] - states["fp64"][0]).abs().mean().item()
-        extra = {"l1_density_fp32_vs_fp64": d,
-                 "speedup_fp32_vs_fp64": round(forms["fp32"]["value"] / forms["fp64"]["value"], 3)}
+    for f in ("fp32", "learned"):
+        if "fp64" in states and f in states:
+            extra[f"l1_density_{f}_vs_fp64"] = (states[f][0] - states["fp64"][0]).abs().mean().item()
+            extra[f"speedup_{f}_vs_fp64"] = round(forms[f]["value"] / forms["fp64"]["value"], 3)
     for f in forms.values():
         for v in f["kernels"].values():
             v["ms_avg"] = round(v["ms_avg"], 4)
@@ -184,7 +196,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if names[-1] == "fp32" else "f64",
+        "dtype": "f64" if names[-1] == "fp64" else "f32",
         "data": "synthetic",
         "config": {"workload": f"kelvin_helmholtz_{n}x{n}_p3_hllc_aposteriori_ssprk3", "form": names[-1],
                    "cells": cells, "stages_timed": stages, "parallelism": "single GPU"},

@@ -9,7 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_double, c_int, c_longlong, c_ulonglong, c_void_p
+from ctypes import POINTER, c_double, c_float, c_int, c_longlong, c_ulonglong, c_void_p
 
 import numpy as np
 
@@ -43,6 +43,14 @@ class FvhoeRkFuse(ctypes.Structure):
         ("target", c_double), ("has_target", c_int),
     ]
 
+
+class FvhoePredict(ctypes.Structure):
+    _fields_ = [("packed", c_void_p), ("logit_threshold", c_float), ("infer", c_int), ("stats", c_void_p)]
+
+
+PRED_NPARAMS = 8433
+PRED_PACKED_BYTES = 20480
+PRED_HALO = 6
 
 _P = c_void_p
 _SIGNATURES = {
@@ -78,6 +86,16 @@ _SIGNATURES = {
     "fvhoe_stage_rk": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, POINTER(FvhoeRkFuse), _P]),
     "fvhoe_stage_timing": (c_int, [c_int]),
     "fvhoe_stage_timing_read": (c_int, [POINTER(c_double), POINTER(c_longlong)]),
+    "fvhoe_pred_pack": (c_int, [_P, _P, _P]),
+    "fvhoe_pred_infer": (c_int, [_P, c_int, c_int, _P, c_float, _P, _P, _P]),
+    "fvhoe_pred_features": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P]),
+    "fvhoe_pred_conv_forward": (c_int, [_P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
+    "fvhoe_pred_conv_backward": (c_int, [_P, _P, _P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, _P]),
+    "fvhoe_pred_loss": (c_int, [_P, _P, c_float, c_longlong, _P, _P, _P]),
+    "fvhoe_pred_adam": (c_int, [_P, _P, _P, _P, c_int, c_float, c_float, c_float, c_float, c_int, _P]),
+    "fvhoe_stage_predicted_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
+    "fvhoe_stage_predicted": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, POINTER(FvhoeRkFuse),
+                                      POINTER(FvhoePredict), _P]),
 }
 
 _lib = None

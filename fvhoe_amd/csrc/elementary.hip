@@ -84,13 +84,13 @@ __global__ void k_fill_u64(unsigned long long* p, int n, unsigned long long v) {
 // the IEEE order of non-negative doubles is the unsigned order of their bits)
 __device__ __forceinline__ void block_atomic_max_pos(double x, unsigned long long* acc) {
   __shared__ double red[TPB / 64];
-  for (int off = 32; off > 0; off >>= 1) x = dmax(x, __shfl_down(x, off, 64));
+  for (int off = 32; off > 0; off >>= 1) x = np_max(x, __shfl_down(x, off, 64));
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0) red[wave] = x;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = red[0];
-    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = dmax(m, red[i]);
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = np_max(m, red[i]);
     atomicMax(acc, (unsigned long long)__double_as_longlong(m));
   }
 }
@@ -101,7 +101,7 @@ __global__ void k_dt_partial(const double* __restrict__ w, long long nc, int ndi
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
     const double c = sound_speed(w[i], w[4 * nc + i], gamma, csq_floor);
     const double s = (fabs(w[nc + i]) + fabs(w[2 * nc + i])) + fabs(w[3 * nc + i]);
-    m = dmax(m, s + (double)ndim * c);
+    m = np_max(m, s + (double)ndim * c);
   }
   block_atomic_max_pos(m, acc);
 }
@@ -138,7 +138,7 @@ __global__ void k_riemann_pt(const double* __restrict__ wl, const double* __rest
           if (solver == RS_LLF) {
             const double sl = fabs(vl) + sound_speed(a[0], a[4], gamma, csq_floor);
             const double sr = fabs(vr) + sound_speed(b[0], b[4], gamma, csq_floor);
-            out[q * nc + c] = 0.5 * (fl + fr) - 0.5 * dmax(sl, sr) * (ur - ul);
+            out[q * nc + c] = 0.5 * (fl + fr) - 0.5 * np_max(sl, sr) * (ur - ul);
           } else {
             out[q * nc + c] = vl > 0.0 ? fl : (vl < 0.0 ? fr : 0.0);
           }
@@ -292,8 +292,8 @@ __device__ __forceinline__ double sed_alpha_cell(const double* __restrict__ x, l
   for (int d = 0; d < 3; ++d) {
     if (!(dims & (1 << d))) continue;
     const long long s = st[d];
-    const double a = sed_alpha_1d([&](int o) { return x[base + o * s]; }, eps);
-    alpha = first ? a : dmin(alpha, a);
+    const double a = sed_alpha_1d<true>([&](int o) { return x[base + o * s]; }, eps);
+    alpha = first ? a : np_min(alpha, a);
     first = false;
   }
   return alpha;
@@ -323,32 +323,32 @@ __global__ void k_detect(const double* __restrict__ w, const double* __restrict_
         for (int dj = -(g1 > 0); dj <= (g1 > 0); ++dj)
           for (int dk = -(g2 > 0); dk <= (g2 > 0); ++dk) {
             const double x = wv[base + di * st[0] + dj * st[1] + dk * st[2]];
-            M = dmax(M, x);
-            m = dmin(m, x);
+            M = np_max(M, x);
+            m = np_min(m, x);
           }
       const double uc = ws[v * NC + base];
       double ind;
       bool tv;
       if (a.range_type == 0) {
         const double rng = a.mode == 0 ? range_mm[2 * v + 1] - range_mm[2 * v] : M - m;
-        ind = dmin(uc - m, M - uc);
+        ind = np_min(uc - m, M - uc);
         tv = ind < -a.eps * rng;
       } else {
         const double lo = a.mode == 0 ? (1.0 - a.eps) * range_mm[2 * v] : (1.0 - a.eps) * m;
         const double hi = a.mode == 0 ? (1.0 + a.eps) * range_mm[2 * v + 1] : (1.0 + a.eps) * M;
-        ind = dmin(uc - lo, hi - uc);
+        ind = np_min(uc - lo, hi - uc);
         tv = ind < 0.0;
       }
       if (tv && a.sed) tv = sed_alpha_cell(ws + v * NC, base, st, a.dims, a.sed_eps) < 1.0;
       if (!((a.nad_vars >> v) & 1)) tv = false;
       nad = nad || tv;
-      ind_min = v == 0 ? ind : dmin(ind_min, ind);
+      ind_min = v == 0 ? ind : np_min(ind_min, ind);
     }
     nad_mag[c] = nad ? -ind_min : 0.0;
     double pind = 0.0;
     if (a.pad_var >= 0) {
       const double x = ws[a.pad_var * NC + base];
-      pind = dmin(x - a.pad_lo, a.pad_hi - x);
+      pind = np_min(x - a.pad_lo, a.pad_hi - x);
     }
     const bool pad = pind < 0.0;
     pad_mag[c] = pad ? -pind : 0.0;
@@ -441,7 +441,7 @@ __global__ void k_shift_max(double* out, const double* __restrict__ S1, const do
     const double s = S1[a.soff + i * a.ss[0] + j * a.ss[1] + k * a.ss[2]];
     const double x = a.coef * (a.gt ? (s > 0.0 ? 1.0 : 0.0) : s);
     const double y = S2[a.toff + i * a.ts[0] + j * a.ts[1] + k * a.ts[2]];
-    out[a.ooff + i * a.os[0] + j * a.os[1] + k * a.os[2]] = dmax(x, y);
+    out[a.ooff + i * a.os[0] + j * a.os[1] + k * a.os[2]] = np_max(x, y);
   }
 }
 

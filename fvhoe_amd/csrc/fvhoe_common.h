@@ -153,6 +153,14 @@ __device__ __forceinline__ T dmax(T a, T b) { return fmax(a, b); }
 template <typename T>
 __device__ __forceinline__ T dmin(T a, T b) { return fmin(a, b); }
 
+// np.minimum / np.maximum exactly (NumPy 2.x on x86-64): a NaN operand
+// propagates, and equal operands (+0 and -0) give the second one.  Used where
+// the value itself is an output of a drop-in function.
+template <typename T>
+__device__ __forceinline__ T np_min(T a, T b) { return a != a ? a : (a < b ? a : b); }
+template <typename T>
+__device__ __forceinline__ T np_max(T a, T b) { return a != a ? a : (a > b ? a : b); }
+
 // ---------------------------------------------------------------------------
 // Riemann solvers (fvhoe/riemann_solvers.py).  D = normal axis 0/1/2;
 // wl is the state left of the face, wr right of it.
@@ -304,10 +312,13 @@ __device__ __forceinline__ T limited_slope(int limiter, T dl, T dr) {
 
 // ---------------------------------------------------------------------------
 // smooth-extrema detector along one axis (fvhoe/smooth_extrema_detection.py:26-49)
-// x(o) returns the value at offset o in [-3, 3] from the cell.
+// x(o) returns the value at offset o in [-3, 3] from the cell.  EXACT: the
+// minima as np.minimum (the alpha value is an output), else hardware min (only
+// alpha < 1 is used).
 // ---------------------------------------------------------------------------
-template <typename Get>
+template <bool EXACT = false, typename Get>
 __device__ __forceinline__ double sed_alpha_1d(Get&& x, double eps) {
+  auto mn = [](double a, double b) { return EXACT ? np_min(a, b) : dmin(a, b); };
   double dU[5];
 #pragma unroll
   for (int j = 0; j < 5; ++j) dU[j] = 0.5 * (x(j - 1) - x(j - 3));  // central differences at offsets -2..2
@@ -323,9 +334,9 @@ __device__ __forceinline__ double sed_alpha_1d(Get&& x, double eps) {
     double aR = (dv > 0.0 ? (vR > 0.0 ? vR : 0.0) : (vR < 0.0 ? vR : 0.0)) / dv;
     aR = fabs(dv) <= eps ? 1.0 : aR;
     aR = aR < aL ? aR : aL;
-    a[j] = dmin(aL, aR);
+    a[j] = mn(aL, aR);
   }
-  return dmin(dmin(a[2], a[1]), a[0]);
+  return mn(mn(a[2], a[1]), a[0]);
 }
 
 // ---------------------------------------------------------------------------

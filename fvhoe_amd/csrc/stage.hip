@@ -24,6 +24,7 @@
 // divergence, the candidate and the RK accumulation stay FP64.
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
+#include "predictor.h"
 
 using namespace fvhoe;
 
@@ -258,7 +259,7 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
       if (red.do_dt && a >= red.dt_lo0 && a < red.dt_hi0 && b >= red.dt_lo1 && b < red.dt_hi1) {
         const double c = sound_speed((double)wr[0][q], (double)wr[4][q], red.gamma, red.csq_floor);
         const double s = (fabs((double)wr[1][q]) + fabs((double)wr[2][q])) + fabs((double)wr[3][q]);
-        m = dmax(m, s + (double)red.ndim * c);
+        m = np_max(m, s + (double)red.ndim * c);  // a NaN wave speed gives a NaN dt, as np.max does
       }
       if (red.do_range && a >= red.rg_lo0 && a < red.rg_hi0 && b >= red.rg_lo1 && b < red.rg_hi1) {
 #pragma unroll
@@ -412,11 +413,12 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
                                                const T* __restrict__ WS, int NS0, int NS1, Geo g,
                                                const double* __restrict__ range_mm, DetectP a,
                                                signed char* __restrict__ trouble, int* __restrict__ count,
-                                               int* __restrict__ list) {
+                                               int* __restrict__ list, const signed char* __restrict__ pre,
+                                               int* __restrict__ fp_count) {
   const int nc = g.n0 * g.n1;
   const bool two_d = g.ge1 > 0;
   const int c = blockIdx.x * TPB + threadIdx.x;
-  int t = 0;
+  int t = 0, fp = 0;
   if (c < nc) {
     const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
     const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
@@ -474,7 +476,18 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
       pad = dmin(x - a.pad_lo, a.pad_hi - x) < 0.0;
     }
     t = (a.force || pad || nad) ? 1 : 0;
-    trouble[c] = (signed char)t;
+    if (pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
+      const int pf = pre[c];
+      trouble[c] = (signed char)(t | pf);
+      fp = pf && !t;
+      t = t && !pf;
+    } else {
+      trouble[c] = (signed char)t;
+    }
+  }
+  if (pre) {
+    const unsigned long long bf = __ballot(fp);
+    if ((threadIdx.x & 63) == 0 && bf) atomicAdd(fp_count, (int)__popcll(bf));
   }
   // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
   __shared__ int wbase[TPB / 64];
@@ -613,8 +626,45 @@ __global__ void __launch_bounds__(TPB) k_final(const T* __restrict__ F, const T*
   }
 }
 
-__global__ void k_stage_init(int* count) {
-  if (threadIdx.x == 0) count[0] = 0;
+// counters: [0] cells listed by the check, [1] pre-flagged cells, [2] pre-flagged cells the check cleared
+__global__ void k_stage_init(int* count, int reset_pre) {
+  if (threadIdx.x == 0) {
+    count[0] = 0;
+    count[2] = 0;
+    if (reset_pre) count[1] = 0;
+  }
+}
+
+__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats) {
+  if (threadIdx.x == 0) {
+    stats[0] += count[1];
+    stats[1] += count[0];
+    stats[2] += count[2];
+  }
+}
+
+// Predictor features: (rho, vx, vy, P) of W at the cells of the grid extended
+// by 6 per side (boundary conditions applied by index mapping), scaled to the
+// global range (w - min) / (max - min); 0 where max = min.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_features(const T* __restrict__ W, int NW0, int NW1, Geo g,
+                                                 const double* __restrict__ range, float* __restrict__ X) {
+  const int HX = g.n0 + 12, WX = g.n1 + 12, total = HX * WX;
+  const int e = blockIdx.x * TPB + threadIdx.x;
+  if (e >= total) return;
+  const int a = e / WX, b = e - (e / WX) * WX;
+  bool fx, fy;
+  const int sx = cell_map(a - 6, g.n0, g.bc[0], g.bc[1], &fx);
+  const int sy = cell_map(b - 6, g.n1, g.bc[2], g.bc[3], &fy);
+  const int wi = (sx + g.ge0) * NW1 + sy + g.ge1, NWC = NW0 * NW1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int v = k == 3 ? 4 : k;
+    double x = (double)W[v * NWC + wi];
+    if ((fx && v == 1) || (fy && v == 2)) x = -x;
+    const double lo = range[2 * v], rng = range[2 * v + 1] - lo;
+    X[k * total + e] = rng > 0.0 ? (float)((x - lo) / rng) : 0.0f;
+  }
 }
 
 // final reduction of the prims partials: dt (fvhoe/hydro.py:290-293, and
@@ -659,7 +709,7 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
 // Optional per-kernel event timing (bench instrumentation): each launch of a
 // stage kernel is bracketed by a pair of events from a fixed pool.
 enum { TK_PRIMS = 0, TK_FLUX_X = 1, TK_FLUX_Y = 2, TK_PRIMS_CAND = 3, TK_DETECT = 4, TK_FALLBACK = 5, TK_FINAL = 6,
-       TK_COUNT = 7 };
+       TK_PREDICT = 7, TK_COUNT = 8 };
 constexpr int TIMING_POOL = 8192;
 struct Timing {
   bool on = false;
@@ -688,15 +738,15 @@ struct TimedLaunch {
 };
 
 struct Layout {
-  int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks;
-  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, total;
+  int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred;
+  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offX, offPre, offPreList, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int TX2 = 16, TY2 = 32, TX1 = 256;
 
-int make_layout(const FvhoeStageConfig* c, Layout* L) {
+int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   if (!c) return FVHOE_ERR_ARG;
   const bool ax = !(c->n[0] == 1 && c->p[0] == 0), ay = !(c->n[1] == 1 && c->p[1] == 0),
              az = !(c->n[2] == 1 && c->p[2] == 0);
@@ -736,12 +786,20 @@ int make_layout(const FvhoeStageConfig* c, Layout* L) {
   }
   L->offPart = o; o = align256(o + sizeof(unsigned long long) * NPART * (size_t)L->prims_blocks);
   L->offScal = o; o = align256(o + 256);
+  L->pred = pred;
+  L->offX = L->offPre = L->offPreList = o;
+  if (pred) {  // predictor features, pre-flags and their list
+    if (!L->two_d || !c->a_posteriori) return FVHOE_ERR_UNSUPPORTED;
+    L->offX = o; o = align256(o + sizeof(float) * 4 * (size_t)(L->n0 + 12) * (L->n1 + 12));
+    L->offPre = o; o = align256(o + (size_t)L->n0 * L->n1);
+    L->offPreList = o; o = align256(o + sizeof(int) * (size_t)L->n0 * L->n1);
+  }
   L->total = o;
   return FVHOE_OK;
 }
 
 template <typename T, int P, bool TWO_D>
-int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed char* preflags, const Layout& L,
+int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePredict* pred, const Layout& L,
               const double* u, double* dudt, double* dt_stage, signed char* trouble_out, char* ws, hipStream_t s) {
   constexpr int PY = TWO_D ? P : 0;
   constexpr int TX = TWO_D ? TX2 : TX1, TY = TWO_D ? TY2 : 1;
@@ -766,8 +824,11 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed ch
   g.cy = -(1.0 / c->h[1]);
 
   const bool apost = c->a_posteriori != 0;
-  const bool need_range = apost && c->nad_mode == 0;
-  k_stage_init<<<1, 64, 0, s>>>(count);
+  const bool use_pred = TWO_D && apost && pred != nullptr;
+  const bool need_range = apost && (c->nad_mode == 0 || use_pred);
+  signed char* preflags = use_pred ? (signed char*)(ws + L.offPre) : nullptr;
+  int* pre_list = (int*)(ws + L.offPreList);
+  k_stage_init<<<1, 64, 0, s>>>(count, use_pred && pred->infer);
 
   Reduce red{};
   red.partials = partials;
@@ -795,6 +856,17 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed ch
   k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
                                     set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0);
+  if (use_pred && pred->infer) {
+    float* X = (float*)(ws + L.offX);
+    const int nx = (L.n0 + 12) * (L.n1 + 12);
+    {
+      TimedLaunch tl(TK_PREDICT, s);
+      k_features<T><<<(nx + TPB - 1) / TPB, TPB, 0, s>>>(W, L.NW0, L.NW1, g, range, X);
+      const int st = predictor_infer(X, L.n0, L.n1, pred->packed, pred->logit_threshold, preflags, nullptr,
+                                     count + 1, pre_list, s);
+      if (st != FVHOE_OK) return st;
+    }
+  }
 
   if (TWO_D) {
     constexpr int TA0 = 16, TB0 = 32, TA1 = 32, TB1 = 16;
@@ -819,6 +891,10 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed ch
 
   FinalP fp{c->riemann, c->limiter, c->gamma, c->csq_floor};
   const int nc = L.n0 * L.n1;
+  if (use_pred) {  // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate
+    TimedLaunch tl(TK_FALLBACK, s);
+    k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count + 1, pre_list, SrcU{u}, g, fp, F, G);
+  }
   const int gridc = (nc + TPB - 1) / TPB < 65536 ? (nc + TPB - 1) / TPB : 65536;
   if (apost) {
     Geo gs = g;
@@ -836,7 +912,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed ch
     const int gridd = (nc + TPB - 1) / TPB;
     {
       TimedLaunch tl(TK_DETECT, s);
-      k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list);
+      k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list,
+                                        preflags, count + 2);
     }
     {
       TimedLaunch tl(TK_FALLBACK, s);
@@ -844,17 +921,17 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const signed ch
     }
     if (trouble_out && hipMemcpyAsync(trouble_out, flags, (size_t)nc, hipMemcpyDeviceToDevice, s) != hipSuccess)
       return FVHOE_ERR_LAUNCH;
+    if (use_pred && pred->stats) k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats);
   }
   {
     TimedLaunch tl(TK_FINAL, s);
     k_final<TWO_D, T><<<gridc, TPB, 0, s>>>(F, G, g, dudt, rk);
   }
-  (void)preflags;
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
 template <typename T, bool TWO_D>
-int dispatch(int p, const FvhoeStageConfig* c, const FvhoeRkFuse* rk, const signed char* pre, const Layout& L,
+int dispatch(int p, const FvhoeStageConfig* c, const FvhoeRkFuse* rk, const FvhoePredict* pre, const Layout& L,
              const double* u, double* dudt, double* dt, signed char* t, char* ws, hipStream_t s) {
   switch (p) {
     case 1: return run_stage<T, 1, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
@@ -915,11 +992,14 @@ int fvhoe_stage_timing_read(double* ms, long long* count) {
   return FVHOE_OK;
 }
 
-int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
-                   signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream) {
+namespace {
+
+int stage_entry(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,
+                void* workspace, const FvhoeRkFuse* rk, const FvhoePredict* pred, void* stream) {
   Layout L;
-  const int st = make_layout(cfg, &L);
+  const int st = make_layout(cfg, &L, pred != nullptr);
   if (st != FVHOE_OK) return st;
+  if (pred && !pred->packed) return FVHOE_ERR_ARG;
   if (!u || !dt_stage || !workspace) return FVHOE_ERR_ARG;
   if (!dudt && !(rk && rk->kind >= 0)) return FVHOE_ERR_ARG;
   if (rk && rk->kind >= 0) {
@@ -934,10 +1014,56 @@ int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, d
   hipStream_t s = (hipStream_t)stream;
   char* ws = (char*)workspace;
   if (L.fp32)
-    return L.two_d ? dispatch<float, true>(L.p, cfg, rk, nullptr, L, u, dudt, dt_stage, trouble, ws, s)
-                   : dispatch<float, false>(L.p, cfg, rk, nullptr, L, u, dudt, dt_stage, trouble, ws, s);
-  return L.two_d ? dispatch<double, true>(L.p, cfg, rk, nullptr, L, u, dudt, dt_stage, trouble, ws, s)
-                 : dispatch<double, false>(L.p, cfg, rk, nullptr, L, u, dudt, dt_stage, trouble, ws, s);
+    return L.two_d ? dispatch<float, true>(L.p, cfg, rk, pred, L, u, dudt, dt_stage, trouble, ws, s)
+                   : dispatch<float, false>(L.p, cfg, rk, pred, L, u, dudt, dt_stage, trouble, ws, s);
+  return L.two_d ? dispatch<double, true>(L.p, cfg, rk, pred, L, u, dudt, dt_stage, trouble, ws, s)
+                 : dispatch<double, false>(L.p, cfg, rk, pred, L, u, dudt, dt_stage, trouble, ws, s);
+}
+
+}  // namespace
+
+int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                   signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream) {
+  return stage_entry(cfg, u, dudt, dt_stage, trouble, workspace, rk, nullptr, stream);
+}
+
+int fvhoe_stage_predicted_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes) {
+  Layout L;
+  const int st = make_layout(cfg, &L, true);
+  if (st != FVHOE_OK) return st;
+  if (!bytes) return FVHOE_ERR_ARG;
+  *bytes = L.total;
+  return FVHOE_OK;
+}
+
+int fvhoe_stage_predicted(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                          signed char* trouble, void* workspace, const FvhoeRkFuse* rk, const FvhoePredict* pred,
+                          void* stream) {
+  if (!pred) return FVHOE_ERR_ARG;
+  return stage_entry(cfg, u, dudt, dt_stage, trouble, workspace, rk, pred, stream);
+}
+
+int fvhoe_pred_features(const FvhoeStageConfig* cfg, const void* workspace, float* X, void* stream) {
+  Layout L;
+  const int st = make_layout(cfg, &L);
+  if (st != FVHOE_OK) return st;
+  if (!workspace || !X) return FVHOE_ERR_ARG;
+  if (!L.two_d || !cfg->a_posteriori || cfg->nad_mode != 0) return FVHOE_ERR_UNSUPPORTED;
+  const char* ws = (const char*)workspace;
+  Geo g{};
+  g.n0 = L.n0;
+  g.n1 = L.n1;
+  g.ge0 = L.GE;
+  g.ge1 = L.GE;
+  for (int k = 0; k < 4; ++k) g.bc[k] = cfg->bc[k];
+  const double* range = (const double*)(ws + L.offScal);
+  const int nx = (L.n0 + 12) * (L.n1 + 12);
+  hipStream_t s = (hipStream_t)stream;
+  if (L.fp32)
+    k_features<float><<<(nx + TPB - 1) / TPB, TPB, 0, s>>>((const float*)(ws + L.offW), L.NW0, L.NW1, g, range, X);
+  else
+    k_features<double><<<(nx + TPB - 1) / TPB, TPB, 0, s>>>((const double*)(ws + L.offW), L.NW0, L.NW1, g, range, X);
+  return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
 int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage, signed char* trouble,

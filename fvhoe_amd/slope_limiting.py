@@ -6,6 +6,8 @@ from __future__ import annotations
 import ctypes
 from typing import Dict, Optional, Tuple
 
+import numpy as np
+
 from fvhoe_amd import _lib
 
 _VAR = {"rho": 0, "vx": 1, "vy": 2, "vz": 3, "P": 4, "mx": 1, "my": 2, "mz": 3, "E": 4}
@@ -102,6 +104,19 @@ def _shift_max(out, s1, coef, gt, s2):
               _lib.ptr(s2), st[2], 0, ext, _lib.stream())
 
 
+def _np_assign(dst, src):
+    """dst[...] = src under NumPy's assignment broadcasting (ValueError where NumPy raises one)."""
+    try:
+        ok = np.broadcast_shapes(tuple(src.shape), tuple(dst.shape)) == tuple(dst.shape)
+    except ValueError:
+        ok = False
+    if not ok:
+        raise ValueError(f"could not broadcast input array from shape {tuple(src.shape)} into shape "
+                         f"{tuple(dst.shape)}")
+    if dst.numel():
+        dst.copy_(src.expand(dst.shape))
+
+
 def _convex_1d(t1):
     """fvhoe/slope_limiting.py:387-407 on a (n+4, 1, 1) device array."""
     th = t1.clone()
@@ -164,15 +179,17 @@ def broadcast_to_troubled_interfaces(trouble, dims: str = None, convex: bool = F
         a = _lib.zeros((nx + 4 if xd else 1, ny + 4 if yd else 1, nz + 4 if zd else 1))
         inner = tuple(slice(2, -2) if f else slice(None) for f in (xd, yd, zd))
         a[inner] = tr
+        # the reference's periodic fills index all three axes as padded; on an inactive axis the target is
+        # empty and NumPy's broadcasting rules decide between a no-op and a ValueError
         if periodic_x and xd:
-            a[:2, 2:-2, 2:-2] = tr[-2:]
-            a[-2:, 2:-2, 2:-2] = tr[:2]
+            _np_assign(a[:2, 2:-2, 2:-2], tr[-2:])
+            _np_assign(a[-2:, 2:-2, 2:-2], tr[:2])
         if periodic_y and yd:
-            a[2:-2, :2, 2:-2] = tr[:, -2:]
-            a[2:-2, -2:, 2:-2] = tr[:, :2]
+            _np_assign(a[2:-2, :2, 2:-2], tr[:, -2:])
+            _np_assign(a[2:-2, -2:, 2:-2], tr[:, :2])
         if periodic_z and zd:
-            a[2:-2, 2:-2, :2] = tr[2:-2, 2:-2, -2:]
-            a[2:-2, 2:-2, -2:] = tr[2:-2, 2:-2, :2]
+            _np_assign(a[2:-2, 2:-2, :2], tr[2:-2, 2:-2, -2:])
+            _np_assign(a[2:-2, 2:-2, -2:], tr[2:-2, 2:-2, :2])
         ax = [_AXIS[d] for d in dims]
         idx = [0, 0, 0]
         for k in ax:

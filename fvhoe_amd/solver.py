@@ -73,6 +73,8 @@ class EulerSolver(ODE):
         cupy: bool = False,
         fused: bool = None,
         bulk_precision: str = "fp64",
+        predictor=None,
+        predictor_every_stage: bool = False,
     ):
         if w0 is None:
             raise ValueError("w0 (an InitialCondition) is required")
@@ -144,6 +146,15 @@ class EulerSolver(ODE):
             raise NotImplementedError("the FP32 bulk form runs on the fused stage (1-D x or 2-D x-y, 5 variables)")
         self._ws = None
         self._trouble = _lib.empty(self.n, dtype=_lib.torch().int8) if a_posteriori_slope_limiting else None
+        # learned troubled-cell predictor (fvhoe_amd.predictor): pre-flagged cells get MUSCL fluxes before
+        # the candidate is formed; the plain check still judges every cell
+        self.predictor = predictor
+        self.predictor_every_stage = bool(predictor_every_stage)
+        self.predictor_stats = None
+        if predictor is not None:
+            if not (self.fused and self.ndim == 2 and self.xdim and self.ydim and a_posteriori_slope_limiting):
+                raise NotImplementedError("the predictor runs in the fused 2-D a-posteriori stage")
+            self.predictor_stats = _lib.zeros((3,), dtype=_lib.torch().int64)
 
     # ------------------------------------------------------------------ config
     def _stage_config(self):
@@ -177,7 +188,12 @@ class EulerSolver(ODE):
     def stage(self, u, k_out, dt_out) -> None:
         """One RHS evaluation into k_out, this stage's dt into dt_out[0] (device)."""
         self.f_evaluation_count += 1
-        if self.fused:
+        if self.predictor is not None:
+            _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),
+                      _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), None,
+                      _lib.ctypes.byref(self._predict_args(True)), _lib.stream())
+            self._log_troubles_device(self._trouble)
+        elif self.fused:
             _lib.call("fvhoe_stage", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out), _lib.ptr(dt_out),
                       _lib.ptr(self._trouble), self._workspace(), _lib.stream())
             if self._trouble is not None:
@@ -208,16 +224,38 @@ class EulerSolver(ODE):
         rk.t = t_dev.data_ptr() if t_dev is not None else None
         rk.target = float(target) if target is not None else 0.0
         rk.has_target = int(target is not None)
-        _lib.call("fvhoe_stage_rk", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out), _lib.ptr(dt_out),
-                  _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk), _lib.stream())
+        if self.predictor is not None:
+            pa = self._predict_args(bool(set_dt_step) or self.predictor_every_stage)
+            _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),
+                      _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk),
+                      _lib.ctypes.byref(pa), _lib.stream())
+        else:
+            _lib.call("fvhoe_stage_rk", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),
+                      _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk),
+                      _lib.stream())
         if self._trouble is not None:
             self._log_troubles_device(self._trouble)
         return True
 
+    def _predict_args(self, infer: bool):
+        pa = _lib.FvhoePredict()
+        pa.packed = self.predictor.packed.data_ptr()
+        pa.logit_threshold = float(self.predictor.logit_threshold)
+        pa.infer = int(bool(infer))
+        pa.stats = self.predictor_stats.data_ptr()
+        return pa
+
+    def predictor_features(self):
+        """Predictor features (4, nx+12, ny+12) float32 of the input of the last fused stage."""
+        X = _lib.empty((4, self.n[0] + 12, self.n[1] + 12), dtype=_lib.torch().float32)
+        _lib.call("fvhoe_pred_features", _lib.ctypes.byref(self._cfg), self._workspace(), _lib.ptr(X), _lib.stream())
+        return X
+
     def _workspace(self):
         if self._ws is None:
             nbytes = _lib.ctypes.c_ulonglong(0)
-            _lib.call("fvhoe_stage_workspace_bytes", _lib.ctypes.byref(self._cfg), _lib.ctypes.byref(nbytes))
+            fn = "fvhoe_stage_predicted_workspace_bytes" if self.predictor is not None else "fvhoe_stage_workspace_bytes"
+            _lib.call(fn, _lib.ctypes.byref(self._cfg), _lib.ctypes.byref(nbytes))
             self._ws = _lib.empty((nbytes.value + 256,), dtype=_lib.torch().uint8)
         base = self._ws.data_ptr()
         return _lib.ctypes.c_void_p(base + (-base) % 256)

@@ -216,12 +216,79 @@ int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, d
 /* Bench instrumentation: with timing on, every fused-stage kernel launch is
  * bracketed by hipEvents on its stream.  fvhoe_stage_timing_read waits for
  * the recorded events and returns, per kernel (prims, flux x, flux y,
- * candidate prims, detect, fallback, final), accumulated milliseconds and
- * launch counts (7 entries each).  fvhoe_stage_timing(on) also resets the sums.
+ * candidate prims, detect, fallback, final, predictor features + inference),
+ * accumulated milliseconds and launch counts (8 entries each).  fvhoe_stage_timing(on) also resets the sums.
  * No counterpart in the reference (its Timer takes host wall-clock,
  * fvhoe/timer.py:6-109). */
 int fvhoe_stage_timing(int on);
 int fvhoe_stage_timing_read(double* ms, long long* count);
+
+/* ------------------------------------------------------------------------
+ * Learned troubled-cell predictor.  No counterpart in the reference: it is the
+ * learned part of the stage.  A three-layer valid 5x5 convolutional network,
+ * 4 -> 16 -> 16 -> 1 channels, ReLU after the first two layers, on features
+ * X (4, n0+12, n1+12) float: the primitive fv averages (rho, vx, vy, P) of the
+ * stage input, (w - min) / (max - min) with the stage's global NAD range,
+ * extended by 6 ghost cells per side with the solver's boundary conditions.
+ * Parameters theta (FVHOE_PRED_NPARAMS float): w1 (16,4,5,5), b1 (16),
+ * w2 (16,16,5,5), b2 (16), w3 (1,16,5,5), b3 (1).  The network sees its
+ * weights rounded to bfloat16; inference runs on BF16 MFMA with FP32
+ * accumulation, training in FP32 (gradients w.r.t. the rounded weights are
+ * applied to the FP32 master copy).
+ * ---------------------------------------------------------------------- */
+#define FVHOE_PRED_NPARAMS 8433
+#define FVHOE_PRED_PACKED_BYTES 20480
+#define FVHOE_PRED_HALO 6
+
+/* theta -> packed inference weights (FVHOE_PRED_PACKED_BYTES, device). */
+int fvhoe_pred_pack(const float* theta, void* packed, void* stream);
+
+/* Network on X (4, n0+12, n1+12): flags (n0*n1 int8, may be NULL) = logit >
+ * logit_threshold, logits (n0*n1 float, may be NULL). */
+int fvhoe_pred_infer(const float* X, int n0, int n1, const void* packed, float logit_threshold,
+                     signed char* flags, float* logits, void* stream);
+
+/* Features X of the input of the last fvhoe_stage / fvhoe_stage_rk /
+ * fvhoe_stage_predicted call made with the same cfg and workspace (2-D,
+ * a-posteriori with the global NAD range). */
+int fvhoe_pred_features(const FvhoeStageConfig* cfg, const void* workspace, float* X, void* stream);
+
+/* Training kernels (FP32).  out (B, Cout, H-4, W-4) = conv(in (B, Cin, H, W),
+ * bf16(w) (Cout, Cin, 5, 5)) + bias, ReLU when relu != 0. */
+int fvhoe_pred_conv_forward(const float* in, const float* w, const float* bias, float* out, int B, int Cin,
+                            int Cout, int H, int W, int relu, void* stream);
+/* dw, db = gradients of the convolution given dout; din (may be NULL) =
+ * dL/d(in) masked by in > 0 (the ReLU of the layer that produced in). */
+int fvhoe_pred_conv_backward(const float* in, const float* dout, const float* w, float* din, float* dw, float* db,
+                             int B, int Cin, int Cout, int H, int W, void* stream);
+/* Class-weighted binary cross-entropy on logits, mean over n: dlogits, and
+ * *loss += the loss (device, caller zeroes it). */
+int fvhoe_pred_loss(const float* logits, const signed char* labels, float pos_weight, long long n,
+                    float* dlogits, float* loss, void* stream);
+/* One Adam update of theta (n values), step counted from 1. */
+int fvhoe_pred_adam(float* theta, const float* grad, float* m, float* v, int n, float lr, float beta1,
+                    float beta2, float eps, int step, void* stream);
+
+/* The stage with the predictor (2-D fused stage only).  With infer = 1 the
+ * network runs on this stage's input and its flags are kept in the
+ * workspace; with infer = 0 the flags of the last infer = 1 call are reused.
+ * Pre-flagged cells get MUSCL fluxes at their faces before the candidate is
+ * formed; the NAD/PAD/SED check then runs on every cell of that candidate
+ * and the faces of the cells it flags get MUSCL fluxes as well.  trouble
+ * receives pre-flags | check flags.  stats (device, may be NULL):
+ * [0] += pre-flagged cells, [1] += cells flagged by the check only,
+ * [2] += pre-flagged cells the check did not flag. */
+typedef struct FvhoePredict {
+  const void* packed;
+  float logit_threshold;
+  int infer;
+  long long* stats;
+} FvhoePredict;
+
+int fvhoe_stage_predicted_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes);
+int fvhoe_stage_predicted(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                          signed char* trouble, void* workspace, const FvhoeRkFuse* rk, const FvhoePredict* pred,
+                          void* stream);
 
 #ifdef __cplusplus
 }

@@ -56,15 +56,19 @@ def conv_valid(x: np.ndarray, w: np.ndarray, b: np.ndarray) -> np.ndarray:
     return out + b[None, :, None, None]
 
 
-def forward(theta: np.ndarray, x: np.ndarray, rounding: bool = True):
+def forward(theta: np.ndarray, x: np.ndarray, rounding: bool = True, act_bf16: bool = False):
     """Logits (B, H-12, W-12).  rounding=False uses the float weights as they are
-    (for finite-difference checks of backward)."""
+    (for finite-difference checks of backward).  act_bf16 also rounds the input
+    and the two hidden activations to bfloat16, as the inference kernel stores
+    them (its BF16 MFMA data path)."""
     p = {k: (bf16_round(v) if (rounding and k.startswith("w")) else v).astype(np.float64)
          for k, v in unpack(theta).items()}
+    act = (lambda a: bf16_round(a.astype(np.float32)).astype(np.float64)) if act_bf16 else (lambda a: a)
+    x = act(x)
     z1 = conv_valid(x, p["w1"], p["b1"])
-    a1 = np.maximum(z1, 0)
+    a1 = act(np.maximum(z1, 0))
     z2 = conv_valid(a1, p["w2"], p["b2"])
-    a2 = np.maximum(z2, 0)
+    a2 = act(np.maximum(z2, 0))
     z3 = conv_valid(a2, p["w3"], p["b3"])[:, 0]
     return z3, (x, z1, a1, z2, a2, p)
 
@@ -113,3 +117,25 @@ def init_params(seed: int = 0) -> np.ndarray:
         else:
             parts.append(np.zeros(int(np.prod(shape))))
     return np.concatenate(parts).astype(np.float32)
+
+
+def features(s, u: np.ndarray) -> np.ndarray:
+    """Predictor input for the 2-D stage input u: (4, n0+12, n1+12) float32.
+
+    The primitive fv averages of u with 3 ghost cells (fvhoe/solver.py:753-792
+    with gw=3, as in revise_fluxes, fvhoe/solver.py:681-687) give the global
+    (min, max) per variable (fvhoe/slope_limiting.py:168-170); the interior
+    averages of (rho, vx, vy, P), extended by 6 cells per side with the
+    scheme's boundary conditions, are scaled to (w - min) / (max - min)
+    (0 where max = min), in float64, then rounded to float32."""
+    from oracle import fvhoe_oracle as O
+
+    w3 = O.interpolate_primitives_from_conservatives(s, u, s.p, (3, 3, 0))
+    lo = w3.min(axis=(1, 2, 3))
+    hi = w3.max(axis=(1, 2, 3))
+    we = O.apply_bc(w3[:, 3:-3, 3:-3, :], (HALO, HALO, 0), s.bc)[..., 0]
+    out = []
+    for v in (0, 1, 2, 4):
+        rng = hi[v] - lo[v]
+        out.append((we[v] - lo[v]) / rng if rng > 0 else np.zeros_like(we[v]))
+    return np.stack(out).astype(np.float32)

@@ -108,8 +108,13 @@ def test_broadcast(convex, dims, shape, periodic):
     t = (RNG.random(shape) < 0.15).astype(np.int64)
     t.flat[0] = 1
     per = (periodic,) * 3
+    try:
+        ref = O.broadcast_to_troubled_interfaces(t, dims, convex, per)
+    except ValueError:  # the reference's periodic convex fill fails the same way for x-z / y-z planes
+        with pytest.raises(ValueError):
+            slope_limiting.broadcast_to_troubled_interfaces(t, dims, convex, *per)
+        return
     got = slope_limiting.broadcast_to_troubled_interfaces(t, dims, convex, *per)
-    ref = O.broadcast_to_troubled_interfaces(t, dims, convex, per)
     for a, b in zip(got, ref):
         assert_bits_equal(a, b)
 
