@@ -2,54 +2,28 @@
 // 5x5 convolutional network 4 -> 16 -> 16 -> 1 channels on normalized
 // primitive fv averages (rho, vx, vy, P), predicting the a-posteriori flags.
 //
-// Inference: one fused kernel per 16x16 tile of cells, layers 1 and 2 on
-// BF16 MFMA (v_mfma_f32_16x16x16_bf16, FP32 accumulation), layer 3 in FP32
-// SIMT; inputs and activations are BF16 in LDS.
+// Inference: the fused kernel of predictor_infer.h (layers 1 and 2 on BF16
+// MFMA with FP32 accumulation, layer 3 in FP32, activations in LDS as BF16)
+// on planar float features.
 // Training: FP32 SIMT forward / backward / Adam kernels with the weights
 // rounded to BF16 in the forward pass (the gradient is applied to the FP32
 // master weights).
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
 #include "predictor.h"
+#include "predictor_infer.h"
 
 using namespace fvhoe;
+using namespace fvhoe::pred;
 
 namespace {
 
-constexpr int KS = 5, KK = 25;
-constexpr int C0 = 4, C1 = 16, C2 = 16;
-constexpr int HALO = 6;
-constexpr int PT = 16;             // output tile edge
-constexpr int L0 = PT + 12, L1 = PT + 8, L2 = PT + 4;
-constexpr int K1STEPS = 7;         // ceil(25 taps * 4 channels / 16)
-
-// parameter vector layout (float32): w1 (16,4,5,5), b1 (16), w2 (16,16,5,5), b2 (16), w3 (1,16,5,5), b3 (1)
 constexpr int OW1 = 0, OB1 = OW1 + C1 * C0 * KK, OW2 = OB1 + C1, OB2 = OW2 + C2 * C1 * KK, OW3 = OB2 + C2,
               OB3 = OW3 + C2 * KK, NPARAM = OB3 + 1;
 static_assert(NPARAM == FVHOE_PRED_NPARAMS, "parameter count");
-
-// packed inference weights (bytes)
-constexpr int PW1 = 0;                                   // bf16 [7][16][16]  (k-step, n, k)
-constexpr int PW2 = PW1 + K1STEPS * 16 * 16 * 2;         // bf16 [25][16][16] (tap, n, cin)
-constexpr int PW3 = PW2 + KK * 16 * 16 * 2;              // f32 [16][25]      (bf16-rounded values)
-constexpr int PB1 = PW3 + C2 * KK * 4, PB2 = PB1 + 64, PB3 = PB2 + 64, PACKED = PB3 + 64;
 static_assert(PACKED <= FVHOE_PRED_PACKED_BYTES, "packed size");
 
-__host__ __device__ __forceinline__ unsigned short f2bf(float f) {  // round to nearest even
-#ifdef __HIP_DEVICE_COMPILE__
-  unsigned int b = __float_as_uint(f);
-#else
-  unsigned int b;
-  __builtin_memcpy(&b, &f, 4);
-#endif
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  return (unsigned short)(b >> 16);
-}
-__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
 __device__ __forceinline__ float bfround(float f) { return bf2f(f2bf(f)); }
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 // ---------------------------------------------------------------------------
 // weight packing for inference
@@ -61,140 +35,26 @@ __global__ void k_pack(const float* __restrict__ th, unsigned char* __restrict__
   float* b1 = (float*)(out + PB1);
   float* b2 = (float*)(out + PB2);
   float* b3 = (float*)(out + PB3);
-  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < K1STEPS * 256 + KK * 256 + C2 * KK + 33;
-       i += blockDim.x * gridDim.x) {
-    if (i < K1STEPS * 256) {
-      const int s = i / 256, n = (i / 16) % 16, k = i % 16;
-      const int t = 4 * s + k / 4, c = k % 4;
+  constexpr int N1 = K1STEPS * 16 * 32, N2 = K2STEPS * 16 * 32, N3 = C2 * KK;
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < N1 + N2 + N3 + 33; i += blockDim.x * gridDim.x) {
+    if (i < N1) {  // [step][n][k]: tap 8 step + k/4, channel k%4
+      const int s = i / 512, n = (i / 32) % 16, k = i % 32;
+      const int t = 8 * s + k / 4, c = k % 4;
       w1s[i] = t < KK ? f2bf(th[OW1 + (n * C0 + c) * KK + t]) : (unsigned short)0;
-    } else if (i < K1STEPS * 256 + KK * 256) {
-      const int j = i - K1STEPS * 256;
-      const int t = j / 256, n = (j / 16) % 16, c = j % 16;
-      w2s[j] = f2bf(th[OW2 + (n * C1 + c) * KK + t]);
-    } else if (i < K1STEPS * 256 + KK * 256 + C2 * KK) {
-      const int j = i - K1STEPS * 256 - KK * 256;
+    } else if (i < N1 + N2) {  // [step][n][k]: tap 2 step + k/16, channel k%16
+      const int j = i - N1;
+      const int s = j / 512, n = (j / 32) % 16, k = j % 32;
+      const int t = 2 * s + k / 16, c = k % 16;
+      w2s[j] = t < KK ? f2bf(th[OW2 + (n * C1 + c) * KK + t]) : (unsigned short)0;
+    } else if (i < N1 + N2 + N3) {
+      const int j = i - N1 - N2;
       w3s[j] = bf2f(f2bf(th[OW3 + j]));
     } else {
-      const int j = i - K1STEPS * 256 - KK * 256 - C2 * KK;
+      const int j = i - N1 - N2 - N3;
       if (j < 16) b1[j] = th[OB1 + j];
       else if (j < 32) b2[j - 16] = th[OB2 + j - 16];
       else b3[0] = th[OB3];
     }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// fused inference: X (4, n0+12, n1+12) -> logits / flags (n0, n1)
-// ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_infer(const float* __restrict__ X, int n0, int n1,
-                                              const unsigned char* __restrict__ pw, float thr,
-                                              signed char* __restrict__ flags, float* __restrict__ logits,
-                                              int* __restrict__ count, int* __restrict__ list, int tiles1) {
-  __shared__ __attribute__((aligned(16))) unsigned short xin[L0 * L0 * C0];
-  __shared__ __attribute__((aligned(16))) unsigned short a1[L1 * L1 * C1];
-  __shared__ __attribute__((aligned(16))) unsigned short a2[L2 * L2 * C2];
-  __shared__ __attribute__((aligned(16))) unsigned short w1s[K1STEPS * 256];
-  __shared__ __attribute__((aligned(16))) unsigned short w2s[KK * 256];
-  __shared__ float w3s[C2 * KK], b1[16], b2[16], b3;
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int oy0 = (blockIdx.x / tiles1) * PT, ox0 = (blockIdx.x % tiles1) * PT;
-  const int HX = n0 + 2 * HALO, WX = n1 + 2 * HALO;
-  // weights
-  for (int i = tid; i < K1STEPS * 256; i += 256) w1s[i] = ((const unsigned short*)(pw + PW1))[i];
-  for (int i = tid; i < KK * 256; i += 256) w2s[i] = ((const unsigned short*)(pw + PW2))[i];
-  for (int i = tid; i < C2 * KK; i += 256) w3s[i] = ((const float*)(pw + PW3))[i];
-  if (tid < 16) {
-    b1[tid] = ((const float*)(pw + PB1))[tid];
-    b2[tid] = ((const float*)(pw + PB2))[tid];
-  }
-  if (tid == 0) b3 = ((const float*)(pw + PB3))[0];
-  // input tile, channel-last bf16
-  for (int i = tid; i < L0 * L0 * C0; i += 256) {
-    const int c = i / (L0 * L0), p = i % (L0 * L0), y = p / L0, x = p % L0;
-    const int gy = oy0 + y, gx = ox0 + x;
-    const float v = (gy < HX && gx < WX) ? X[((long long)c * HX + gy) * WX + gx] : 0.0f;
-    xin[p * C0 + c] = f2bf(v);
-  }
-  __syncthreads();
-  const int mrow = lane & 15, kb = lane >> 4;
-  // layer 1: 24x24 pixels = 36 groups of 16, K = 25 taps x 4 channels in 7 steps of 16
-  for (int grp = wave; grp < (L1 * L1) / 16; grp += 4) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int p = grp * 16 + mrow, py = p / L1, px = p % L1;
-#pragma unroll
-    for (int s = 0; s < K1STEPS; ++s) {
-      const int t = 4 * s + kb;
-      bf16x4 a = {0, 0, 0, 0};
-      if (t < KK) {
-        const int dy = t / KS, dx = t % KS;
-        a = *(const bf16x4*)&xin[((py + dy) * L0 + (px + dx)) * C0];
-      }
-      const bf16x4 b = *(const bf16x4*)&w1s[(s * 16 + mrow) * 16 + kb * 4];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    // acc[j] = D[row 4*kb + j][col mrow]: pixel grp*16 + 4*kb + j, channel mrow
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int q = grp * 16 + 4 * kb + j;
-      const float v = acc[j] + b1[mrow];
-      a1[q * C1 + mrow] = f2bf(v > 0.f ? v : 0.f);
-    }
-  }
-  __syncthreads();
-  // layer 2: 20x20 pixels = 25 groups of 16, K = 16 channels per tap, 25 taps
-  for (int grp = wave; grp < (L2 * L2) / 16; grp += 4) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int p = grp * 16 + mrow, py = p / L2, px = p % L2;
-#pragma unroll 5
-    for (int t = 0; t < KK; ++t) {
-      const int dy = t / KS, dx = t % KS;
-      const bf16x4 a = *(const bf16x4*)&a1[((py + dy) * L1 + (px + dx)) * C1 + kb * 4];
-      const bf16x4 b = *(const bf16x4*)&w2s[(t * 16 + mrow) * 16 + kb * 4];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int q = grp * 16 + 4 * kb + j;
-      const float v = acc[j] + b2[mrow];
-      a2[q * C2 + mrow] = f2bf(v > 0.f ? v : 0.f);
-    }
-  }
-  __syncthreads();
-  // layer 3: one output pixel per thread, FP32
-  const int y = tid / PT, x = tid % PT;
-  float z = b3;
-  for (int t = 0; t < KK; ++t) {
-    const int dy = t / KS, dx = t % KS;
-    const unsigned short* ap = &a2[((y + dy) * L2 + (x + dx)) * C2];
-#pragma unroll
-    for (int c = 0; c < C2; ++c) z += w3s[c * KK + t] * bf2f(ap[c]);
-  }
-  const int gy = oy0 + y, gx = ox0 + x;
-  const bool inside = gy < n0 && gx < n1;
-  const int hit = (inside && z > thr) ? 1 : 0;
-  if (inside) {
-    const long long o = (long long)gy * n1 + gx;
-    if (flags) flags[o] = (signed char)hit;
-    if (logits) logits[o] = z;
-  }
-  if (list) {  // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
-    __shared__ int wbase[4];
-    __shared__ int bbase;
-    const unsigned long long ball = __ballot(hit);
-    if (lane == 0) wbase[wave] = __popcll(ball);
-    __syncthreads();
-    if (tid == 0) {
-      int tot = 0;
-      for (int w = 0; w < 4; ++w) {
-        const int c = wbase[w];
-        wbase[w] = tot;
-        tot += c;
-      }
-      bbase = tot ? atomicAdd(count, tot) : 0;
-    }
-    __syncthreads();
-    if (hit) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = gy * n1 + gx;
   }
 }
 
@@ -329,9 +189,7 @@ namespace fvhoe {
 
 int predictor_infer(const float* X, int n0, int n1, const void* packed, float thr, signed char* flags,
                     float* logits, int* count, int* list, hipStream_t s) {
-  const int t0 = (n0 + PT - 1) / PT, t1 = (n1 + PT - 1) / PT;
-  k_infer<<<t0 * t1, 256, 0, s>>>(X, n0, n1, (const unsigned char*)packed, thr, flags, logits, count, list, t1);
-  return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
+  return launch_infer(FeatPlanar{X, n0 + 2 * HALO, n1 + 2 * HALO}, n0, n1, packed, thr, flags, logits, count, list, s);
 }
 
 }  // namespace fvhoe

@@ -25,6 +25,7 @@
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
 #include "predictor.h"
+#include "predictor_infer.h"
 
 using namespace fvhoe;
 
@@ -667,6 +668,34 @@ __global__ void __launch_bounds__(TPB) k_features(const T* __restrict__ W, int N
   }
 }
 
+// The same features read straight from W inside the inference kernel; pixels
+// beyond the extended grid (the overhang of the last tiles) are 0.
+template <typename T>
+struct FeatW {
+  const T* __restrict__ W;
+  int NW0, NW1;
+  Geo g;
+  const double* __restrict__ range;
+  __device__ __forceinline__ void load(int a, int b, float f[4]) const {
+    if (a >= g.n0 + 12 || b >= g.n1 + 12) {
+      f[0] = f[1] = f[2] = f[3] = 0.0f;
+      return;
+    }
+    bool fx, fy;
+    const int sx = cell_map(a - 6, g.n0, g.bc[0], g.bc[1], &fx);
+    const int sy = cell_map(b - 6, g.n1, g.bc[2], g.bc[3], &fy);
+    const int wi = (sx + g.ge0) * NW1 + sy + g.ge1, NWC = NW0 * NW1;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int v = k == 3 ? 4 : k;
+      double x = (double)W[v * NWC + wi];
+      if ((fx && v == 1) || (fy && v == 2)) x = -x;
+      const double lo = range[2 * v], rng = range[2 * v + 1] - lo;
+      f[k] = rng > 0.0 ? (float)((x - lo) / rng) : 0.0f;
+    }
+  }
+};
+
 // final reduction of the prims partials: dt (fvhoe/hydro.py:290-293, and
 // _hit_target, fvhoe/ode.py:11-22, when asked) and the per-variable global
 // (min, max) (fvhoe/slope_limiting.py:168-170)
@@ -739,7 +768,7 @@ struct TimedLaunch {
 
 struct Layout {
   int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred;
-  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offX, offPre, offPreList, total;
+  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offPre, offPreList, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -787,10 +816,9 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offPart = o; o = align256(o + sizeof(unsigned long long) * NPART * (size_t)L->prims_blocks);
   L->offScal = o; o = align256(o + 256);
   L->pred = pred;
-  L->offX = L->offPre = L->offPreList = o;
-  if (pred) {  // predictor features, pre-flags and their list
+  L->offPre = L->offPreList = o;
+  if (pred) {  // pre-flags and their list
     if (!L->two_d || !c->a_posteriori) return FVHOE_ERR_UNSUPPORTED;
-    L->offX = o; o = align256(o + sizeof(float) * 4 * (size_t)(L->n0 + 12) * (L->n1 + 12));
     L->offPre = o; o = align256(o + (size_t)L->n0 * L->n1);
     L->offPreList = o; o = align256(o + sizeof(int) * (size_t)L->n0 * L->n1);
   }
@@ -856,16 +884,11 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
                                     set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0);
-  if (use_pred && pred->infer) {
-    float* X = (float*)(ws + L.offX);
-    const int nx = (L.n0 + 12) * (L.n1 + 12);
-    {
-      TimedLaunch tl(TK_PREDICT, s);
-      k_features<T><<<(nx + TPB - 1) / TPB, TPB, 0, s>>>(W, L.NW0, L.NW1, g, range, X);
-      const int st = predictor_infer(X, L.n0, L.n1, pred->packed, pred->logit_threshold, preflags, nullptr,
-                                     count + 1, pre_list, s);
-      if (st != FVHOE_OK) return st;
-    }
+  if (use_pred && pred->infer) {  // the network on this stage's W, features formed on load
+    TimedLaunch tl(TK_PREDICT, s);
+    const int st = pred::launch_infer(FeatW<T>{W, L.NW0, L.NW1, g, range}, L.n0, L.n1, pred->packed,
+                                      pred->logit_threshold, preflags, nullptr, count + 1, pre_list, s);
+    if (st != FVHOE_OK) return st;
   }
 
   if (TWO_D) {
