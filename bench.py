@@ -65,7 +65,45 @@ def parse():
     ap.add_argument("--cpu-stages", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 device copies of 167,772,160 bytes (scale for HBM byte counters)")
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble"],
+                    help="kh: the headline Kelvin-Helmholtz stage; ensemble: 512 uneven 1-D Shu-Osher runs to t=1.8")
+    ap.add_argument("--ensemble-t", type=float, default=1.8)
+    ap.add_argument("--ensemble-count", type=int, default=512)
     return ap.parse_args()
+
+
+def run_ensemble(a):
+    """512 Shu-Osher-type 1-D instances, N ~ U{256..8192}, p=5 HLLC a-posteriori, SSP-RK3 to t=1.8,
+    every step one captured graph over all instances; tile size from the autotuner."""
+    import torch
+
+    from fvhoe_amd.ensemble import shu_osher_ensemble_solver
+
+    forms = {}
+    for form in a.forms.split(","):
+        if form == "learned":
+            continue  # the predictor network is two-dimensional
+        e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision=form)
+        tune = e.autotune(a.ensemble_t, steps=10)
+        print(f"# ensemble {form}: {sum(e.n)} cells, autotune {tune} -> tile {e.tile}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        steps = e.run(a.ensemble_t)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        cu = e.cell_updates()
+        forms[form] = {"value": round(cu / wall / 1e6, 2), "seconds": round(wall, 3), "ensemble_steps": steps,
+                       "cell_updates": cu, "tile": e.tile, "autotune_ms_per_step": tune,
+                       "t_min": float(e.times().min()), "t_max": float(e.times().max())}
+        print(f"# ensemble {form}: {wall:.2f}s, {steps} steps", file=sys.stderr, flush=True)
+    head = forms[list(forms)[-1]]
+    out = {"metric": "Mcell-updates/s per RK stage (ensemble)", "value": head["value"], "unit": "Mcell-updates/s",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32" if list(forms)[-1] == "fp32" else "f64", "data": "synthetic",
+           "config": {"workload": f"shu_osher_ensemble_{a.ensemble_count}_p5_hllc_aposteriori_ssprk3_t{a.ensemble_t}",
+                      "form": list(forms)[-1], "parallelism": "single GPU"},
+           "forms": forms}
+    print(json.dumps(out), flush=True)
 
 
 def cpu_baseline(n: int, stages: int):
@@ -149,6 +187,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
                                    "and not built", "n_gpus": a.gpus}))
         sys.exit(2)
+    if a.workload == "ensemble":
+        run_ensemble(a)
+        return
     import torch
 
     from fvhoe_amd import initial_conditions as ics

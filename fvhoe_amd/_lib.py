@@ -93,6 +93,12 @@ _SIGNATURES = {
     "fvhoe_pred_conv_backward": (c_int, [_P, _P, _P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, _P]),
     "fvhoe_pred_loss": (c_int, [_P, _P, c_float, c_longlong, _P, _P, _P]),
     "fvhoe_pred_adam": (c_int, [_P, _P, _P, _P, c_int, c_float, c_float, c_float, c_float, c_int, _P]),
+    "fvhoe_ensemble_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int,
+                                               POINTER(c_ulonglong)]),
+    "fvhoe_ensemble_setup": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), POINTER(c_double), c_int,
+                                     _P, _P]),
+    "fvhoe_ensemble_step": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P, c_double,
+                                    _P, _P]),
     "fvhoe_stage_predicted_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage_predicted": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, POINTER(FvhoeRkFuse),
                                       POINTER(FvhoePredict), _P]),

@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 OUT = os.path.join(PKG, "libfvhoe_hip.so")
-SOURCES = ["elementary.hip", "stage.hip", "predictor.hip"]
+SOURCES = ["elementary.hip", "stage.hip", "predictor.hip", "ensemble.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -1,0 +1,643 @@
+// Batched ensemble of independent 1-D runs of uneven size (EulerSolver along
+// x, fvhoe/solver.py:41-1001, one per instance), advanced together: every
+// SSP-RK3 step of every active instance is one sequence of seven kernels per
+// stage, each launched once for the whole ensemble over a table of
+// (instance, tile) pairs, so that the work of one launch is balanced in tiles
+// of TILE cells whatever the instance sizes.  Each instance keeps its own CFL
+// dt, time and _hit_target (fvhoe/ode.py:11-22, 183-193); an instance whose
+// time has reached the target is skipped by every kernel.
+//
+// Kernels of one stage (the same arithmetic, in the same order, as the fused
+// 1-D stage, so the FP64 form reproduces the reference bit for bit):
+//   reset   : per-instance reduction words and fallback counters.
+//   prims   : u (BC-mapped on load) -> centres -> FP64 primitives -> fv averages
+//             W on the ghost-extended range, with per-instance CFL max and
+//             NAD range folded in by 64-bit atomics.
+//   scalars : per-instance dt (fvhoe/hydro.py:270-294), range, and in the
+//             first stage the step's dt and the active flag.
+//   flux    : l/r face values, Riemann solver (fvhoe/solver.py:519-649).
+//   prims   : again on the candidate u + dt L(F) -> WS.
+//   detect  : NAD/PAD/SED flags (fvhoe/slope_limiting.py:87-244) and a
+//             ballot compaction of the flagged cells per instance.
+//   fallback: MUSCL fluxes at the faces of flagged cells (fvhoe/solver.py:710-731).
+//   final   : divergence and the SSP-RK3 combination (fvhoe/ode.py:271-283).
+#include <vector>
+
+#include "fvhoe_common.h"
+#include "fvhoe_hip.h"
+
+using namespace fvhoe;
+
+namespace {
+
+__host__ __device__ constexpr int e_reach(int p) { return p == 1 ? 0 : (p % 2 ? reach(p) - 1 : reach(p)); }
+__host__ __device__ constexpr int e_ghosts(int p) { return reach(p) + 1 > 2 * reach(p) ? reach(p) + 1 : 2 * reach(p); }
+__host__ __device__ constexpr int e_gext(int p) { return e_ghosts(p) > 3 ? e_ghosts(p) : 3; }
+
+constexpr int NACC = 11;  // [0] max wave speed (raw bits), [1 + 2v] / [2 + 2v] ordered min / max of variable v
+
+template <int KIND, int P, typename T, typename Get>
+__device__ __forceinline__ T esweep(Get&& x) {
+  T acc = T(0);
+  static_for<ntaps(P)>([&](auto I) {
+    constexpr double w = weight(KIND, P, decltype(I)::value);
+    if constexpr (w != 0.0) acc = acc + T(w) * x(decltype(I)::value);
+  });
+  return acc;
+}
+
+__device__ __forceinline__ unsigned long long obits(double x) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double from_obits(unsigned long long b) {
+  const unsigned long long r = (b & 0x8000000000000000ull) ? (b & 0x7fffffffffffffffull) : ~b;
+  return __longlong_as_double((long long)r);
+}
+
+// per-instance tables and scalars, all device memory
+struct Ens {
+  int count, bc_lo, bc_hi, riemann, limiter, apost;
+  double gamma, cfl, csq_floor;
+  const int* n;                 // cells
+  const double* h;              // cell width
+  const long long* off_u;       // offset of instance i in u-shaped arrays (x 5 for the variable planes)
+  const long long* off_w;       // in W (n + 2 GE cells)
+  const long long* off_f;       // in F (n + 1 faces)
+  const long long* off_s;       // in WS (n + 6 cells)
+  double* dt_stage;
+  double* dt_step;
+  int* active;
+  unsigned long long* acc;      // NACC per instance
+  double* range;                // 10 per instance
+  int* cnt;                     // flagged cells per instance
+  // detection
+  int nad_mode, nad_range, nad_vars, pad_var, sed, force;
+  double nad_eps, pad_lo, pad_hi, sed_eps;
+};
+
+// u of instance i at (possibly ghost) cell c, BC applied (fvhoe/boundary_conditions.py:306-402)
+__device__ __forceinline__ double u_at(const double* __restrict__ u, int n, int v, int c, int lo, int hi) {
+  bool f;
+  const int s = bc_map(c, n, n, lo, hi, &f);
+  const double x = u[v * n + s];
+  return (f && v == 1) ? -x : x;
+}
+
+// candidate u + dt L(F) at a (possibly ghost) cell (fvhoe/solver.py:678); reflective ghosts negate it whole
+template <typename T>
+__device__ __forceinline__ double ustar_at(const double* __restrict__ u, const T* __restrict__ F, int n, double cx,
+                                           double dt, int v, int c, int lo, int hi) {
+  bool f;
+  const int s = bc_map(c, n, n, lo, hi, &f);
+  const double L = 0.0 + cx * ((double)F[v * (n + 1) + s + 1] - (double)F[v * (n + 1) + s]);
+  const double x = u[v * n + s] + dt * L;
+  return (f && v == 1) ? -x : x;
+}
+
+__global__ void k_ens_reset(Ens e) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= e.count) return;
+  e.acc[i * NACC] = 0ull;
+  for (int k = 1; k < NACC; ++k) e.acc[i * NACC + k] = (k & 1) ? ~0ull : 0ull;
+  e.cnt[i] = 0;
+}
+
+// W (CAND = false, from u; ghost width GE) or WS (CAND = true, from the candidate; ghost width 3)
+template <typename T, int P, int TILE, bool CAND>
+__global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restrict__ tiles,
+                                                   const double* __restrict__ u, const T* __restrict__ F,
+                                                   T* __restrict__ out) {
+  constexpr int RC = e_reach(P), RT = e_reach(P), GE = e_gext(P), G = e_ghosts(P);
+  constexpr int G0 = CAND ? 3 : GE;
+  constexpr int HU = TILE + 2 * (RC + RT), HC = TILE + 2 * RT;
+  __shared__ T SU[5][HU];
+  __shared__ T SC[5][HC];
+  const int2 tl = tiles[blockIdx.x];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], NO = n + 2 * G0;
+  const double cx = -(1.0 / e.h[i]);
+  const double dt = e.dt_stage[i];
+  const double* ui = u + 5 * e.off_u[i];
+  const T* Fi = CAND ? F + 5 * e.off_f[i] : nullptr;
+  const int c0 = tl.y - G0;  // first cell of the tile
+  const int tid = threadIdx.x;
+  const int span = G0 + RC + RT;  // cells beyond [-span, n + span) feed discarded outputs only
+  for (int k = tid; k < HU; k += TILE) {
+    const int c = min(c0 - RC - RT + k, n + span - 1);
+#pragma unroll
+    for (int v = 0; v < 5; ++v)
+      SU[v][k] = T(CAND ? ustar_at<T>(ui, Fi, n, cx, dt, v, c, e.bc_lo, e.bc_hi) : u_at(ui, n, v, c, e.bc_lo, e.bc_hi));
+  }
+  __syncthreads();
+  for (int k = tid; k < HC; k += TILE) {
+    double uu[5], ww[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      // x sweep, then the degree-0 sweeps of the inactive y and z axes (fvhoe/fv.py:175-196)
+      const T sx = esweep<KIND_C, P, T>([&](int t) { return SU[v][k + RC + t - reach(P)]; });
+      const T sy = esweep<KIND_C, 0, T>([&](int) { return sx; });
+      uu[v] = (double)(T(0) + sy);
+    }
+    prims_from_cons(uu, ww, e.gamma);
+#pragma unroll
+    for (int v = 0; v < 5; ++v) SC[v][k] = T(ww[v]);
+  }
+  __syncthreads();
+  const int a = tl.y + tid;  // output index (cell a - G0)
+  const bool valid = a < NO;
+  T w[5];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const T tx = esweep<KIND_T, P, T>([&](int t) { return SC[v][tid + RT + t - reach(P)]; });
+    const T ty = esweep<KIND_T, 0, T>([&](int) { return tx; });
+    w[v] = T(0) + ty;
+    if (valid) out[5 * (CAND ? e.off_s[i] : e.off_w[i]) + v * NO + a] = w[v];
+  }
+  if (CAND) return;
+  // per-instance CFL max over cells [-G, n+G) and NAD range over [-3, n+3)
+  const int c = a - GE;
+  unsigned long long val[NACC];
+  double m = 0.0;
+  val[0] = 0ull;
+#pragma unroll
+  for (int k = 1; k < NACC; ++k) val[k] = (k & 1) ? ~0ull : 0ull;
+  if (valid && c >= -G && c < n + G) {
+    const double cs = sound_speed((double)w[0], (double)w[4], e.gamma, e.csq_floor);
+    const double s = (fabs((double)w[1]) + fabs((double)w[2])) + fabs((double)w[3]);
+    m = np_max(m, s + 1.0 * cs);
+  }
+  val[0] = (unsigned long long)__double_as_longlong(m);
+  if (valid && c >= -3 && c < n + 3) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      val[1 + 2 * v] = obits((double)w[v]);
+      val[2 + 2 * v] = val[1 + 2 * v];
+    }
+  }
+  __shared__ unsigned long long part[TILE / 64][NACC];
+#pragma unroll
+  for (int k = 0; k < NACC; ++k) {
+    unsigned long long x = val[k];
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long y = __shfl_down(x, off, 64);
+      x = (k == 0 || (k & 1) == 0) ? (x > y ? x : y) : (x < y ? x : y);
+    }
+    if ((tid & 63) == 0) part[tid >> 6][k] = x;
+  }
+  __syncthreads();
+  if (tid < NACC) {
+    unsigned long long x = part[0][tid];
+    for (int wv = 1; wv < TILE / 64; ++wv) {
+      const unsigned long long y = part[wv][tid];
+      x = (tid == 0 || (tid & 1) == 0) ? (x > y ? x : y) : (x < y ? x : y);
+    }
+    if (tid == 0 || (tid & 1) == 0) atomicMax(&e.acc[i * NACC + tid], x);
+    else atomicMin(&e.acc[i * NACC + tid], x);
+  }
+}
+
+// per-instance dt, range; first stage: active flag and the step's dt
+__global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, double target) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= e.count) return;
+  if (first) e.active[i] = t[i] < target ? 1 : 0;
+  if (!e.active[i]) return;
+  const unsigned long long* a = e.acc + i * NACC;
+  const double d = (e.cfl * e.h[i]) / __longlong_as_double((long long)a[0]);
+  e.dt_stage[i] = d;
+  if (first) e.dt_step[i] = (target - t[i] < d) ? target - t[i] : d;
+  for (int k = 0; k < 10; ++k) e.range[i * 10 + k] = from_obits(a[1 + k]);
+}
+
+// faces fa in [0, n] of each instance: l/r values of W, Riemann solver
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_flux(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
+                                                  T* __restrict__ F) {
+  constexpr int GE = e_gext(P), EL = reach(P);
+  const int2 tl = tiles[blockIdx.x];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], fa = tl.y + threadIdx.x;
+  if (fa > n) return;
+  const T* Wi = W + 5 * e.off_w[i];
+  const int NW = n + 2 * GE;
+  T wl[5], wr[5], res[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    // degree-0 sweeps of the inactive axes around the face values (fvhoe/solver.py:522-568)
+    auto xc = [&](int c) {
+      const T x = esweep<KIND_C, 0, T>([&](int) { return Wi[q * NW + c + GE]; });
+      return T(0) + x;
+    };
+    wl[q] = esweep<KIND_R, P, T>([&](int t) { return xc(fa - 1 - EL + t); });
+    wr[q] = esweep<KIND_L, P, T>([&](int t) { return xc(fa - EL + t); });
+  }
+  riemann<0>(e.riemann, wl, wr, res, T(e.gamma), T(e.csq_floor));
+  T* Fi = F + 5 * e.off_f[i];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const T x = esweep<KIND_T, 0, T>([&](int) { return res[q]; });
+    Fi[q * (n + 1) + fa] = T(0) + x;
+  }
+}
+
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
+                                                    const T* __restrict__ WS, int* __restrict__ list) {
+  constexpr int GE = e_gext(P);
+  const int2 tl = tiles[blockIdx.x];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], c = tl.y + threadIdx.x;
+  int t = 0;
+  if (c < n) {
+    const T* Wi = W + 5 * e.off_w[i];
+    const T* Si = WS + 5 * e.off_s[i];
+    const int NW = n + 2 * GE, NS = n + 6;
+    const double* rg = e.range + 10 * i;
+    bool tvv[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const T* wv = Wi + v * NW + c + GE;
+      double M = (double)wv[0], m = M;
+#pragma unroll
+      for (int d = -1; d <= 1; ++d) {
+        const double x = (double)wv[d];
+        M = dmax(M, x);
+        m = dmin(m, x);
+      }
+      const double uc = (double)Si[v * NS + c + 3];
+      bool tv;
+      if (e.nad_range == 0) {
+        const double rng = e.nad_mode == 0 ? rg[2 * v + 1] - rg[2 * v] : M - m;
+        tv = dmin(uc - m, M - uc) < -e.nad_eps * rng;
+      } else {
+        const double lo = e.nad_mode == 0 ? (1.0 - e.nad_eps) * rg[2 * v] : (1.0 - e.nad_eps) * m;
+        const double hi = e.nad_mode == 0 ? (1.0 + e.nad_eps) * rg[2 * v + 1] : (1.0 + e.nad_eps) * M;
+        tv = dmin(uc - lo, hi - uc) < 0.0;
+      }
+      tvv[v] = tv && ((e.nad_vars >> v) & 1);
+    }
+    bool nad = false;
+    for (int v = 0; v < 5; ++v) {
+      bool tv = tvv[v];
+      if (tv && e.sed && !nad) {  // smooth-extrema detector only where NAD flags
+        const T* sv = Si + v * NS + c + 3;
+        tv = sed_alpha_1d([&](int o) { return (double)sv[o]; }, e.sed_eps) < 1.0;
+      }
+      nad = nad || tv;
+    }
+    bool pad = false;
+    if (e.pad_var >= 0) {
+      const double x = (double)Si[e.pad_var * NS + c + 3];
+      pad = dmin(x - e.pad_lo, e.pad_hi - x) < 0.0;
+    }
+    t = (e.force || pad || nad) ? 1 : 0;
+  }
+  __shared__ int wbase[TILE / 64];
+  __shared__ int bbase;
+  const unsigned long long ball = __ballot(t);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) wbase[wave] = __popcll(ball);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int w = 0; w < TILE / 64; ++w) {
+      const int x = wbase[w];
+      wbase[w] = tot;
+      tot += x;
+    }
+    bbase = tot ? atomicAdd(&e.cnt[i], tot) : 0;
+  }
+  __syncthreads();
+  if (t) list[e.off_u[i] + bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
+}
+
+// MUSCL p=1 flux at face fa of instance i from the stage input (fvhoe/solver.py:712-717)
+template <typename T>
+__device__ void ens_fallback_flux(const Ens& e, const double* __restrict__ ui, int n, int fa, T F[5]) {
+  T w[4][5];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    double uu[5], wd[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) uu[q] = 0.0 + u_at(ui, n, q, fa - 2 + k, e.bc_lo, e.bc_hi);
+    prims_from_cons(uu, wd, e.gamma);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) w[k][q] = T(0.0 + wd[q]);
+  }
+  T wl[5], wr[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    if (e.limiter == 3) {  // slope_limiter None: p=1 conservative interpolation
+      wl[q] = ((T(0) + T(-0.25) * w[0][q]) + T(1) * w[1][q]) + T(0.25) * w[2][q];
+      wr[q] = ((T(0) + T(0.25) * w[1][q]) + T(1) * w[2][q]) + T(-0.25) * w[3][q];
+    } else {
+      const T s1 = limited_slope(e.limiter, w[1][q] - w[0][q], w[2][q] - w[1][q]);
+      const T s2 = limited_slope(e.limiter, w[2][q] - w[1][q], w[3][q] - w[2][q]);
+      wl[q] = w[1][q] + T(0.5) * s1;
+      wr[q] = w[2][q] - T(0.5) * s2;
+    }
+  }
+  riemann<0>(e.riemann, wl, wr, F, T(e.gamma), T(e.csq_floor));
+}
+
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __restrict__ tiles,
+                                                      const double* __restrict__ u, const int* __restrict__ list,
+                                                      T* __restrict__ F) {
+  const int2 tl = tiles[blockIdx.x];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], cnt = e.cnt[i];
+  const double* ui = u + 5 * e.off_u[i];
+  T* Fi = F + 5 * e.off_f[i];
+  for (int k = threadIdx.x; k < 2 * TILE; k += TILE) {
+    const int j = tl.y + k / 2;
+    if (j >= cnt) continue;
+    const int fa = list[e.off_u[i] + j] + (k & 1);
+    T Fv[5];
+    ens_fallback_flux<T>(e, ui, n, fa, Fv);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Fi[q * (n + 1) + fa] = Fv[q];
+  }
+}
+
+// divergence (fvhoe/solver.py:744-750) and the SSP-RK3 combination of this stage
+// (fvhoe/ode.py:275-279): stage 0 -> k0, s = u + dt k0; stage 1 -> k1,
+// s = u + 0.25 dt k0 + 0.25 dt k1; stage 2 -> u + (1/6) dt (k0 + k1 + 4 k2).
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restrict__ tiles, const T* __restrict__ F,
+                                                   int stage, double* __restrict__ u, double* __restrict__ s,
+                                                   double* __restrict__ k0, double* __restrict__ k1) {
+  const int2 tl = tiles[blockIdx.x];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], c = tl.y + threadIdx.x;
+  if (c >= n) return;
+  const double cx = -(1.0 / e.h[i]);
+  const double dt = e.dt_step[i];
+  const T* Fi = F + 5 * e.off_f[i];
+  const long long b = 5 * e.off_u[i];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    const double L = 0.0 + cx * ((double)Fi[q * (n + 1) + c + 1] - (double)Fi[q * (n + 1) + c]);
+    const long long o = b + q * n + c;
+    if (stage == 0) {
+      k0[o] = L;
+      s[o] = u[o] + dt * L;
+    } else if (stage == 1) {
+      k1[o] = L;
+      s[o] = u[o] + 0.25 * dt * k0[o] + 0.25 * dt * L;
+    } else {
+      u[o] = u[o] + (1.0 / 6.0) * dt * (k0[o] + k1[o] + 4.0 * L);
+    }
+  }
+}
+
+__global__ void k_ens_advance(Ens e, double* __restrict__ t, int* __restrict__ steps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < e.count && e.active[i]) {
+    t[i] = t[i] + e.dt_step[i];
+    if (steps) steps[i] += 1;
+  }
+}
+
+// ------------------------------------------------------------------ host side
+
+size_t a256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+struct Plan {
+  int count, p, tile, fp32, GE;
+  long long cells, wcells, fcells, scells;
+  int tw, tf, ts, tc;  // tiles of each kind
+  size_t o_n, o_h, o_ou, o_ow, o_of, o_os, o_dts, o_dtp, o_act, o_acc, o_rng, o_cnt, o_tw, o_tf, o_ts, o_tc;
+  size_t o_W, o_F, o_WS, o_list, o_k0, o_k1, o_s, total;
+};
+
+int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan* P) {
+  if (!c || !n || count < 1) return FVHOE_ERR_ARG;
+  if (tile != 64 && tile != 128 && tile != 256) return FVHOE_ERR_ARG;
+  if (c->p[0] < 1 || c->p[0] > 8 || c->n[1] != 1 || c->p[1] != 0 || c->n[2] != 1 || c->p[2] != 0)
+    return FVHOE_ERR_UNSUPPORTED;
+  if (c->fixed_dt_on || c->bulk_fp32 < 0 || c->bulk_fp32 > 1) return FVHOE_ERR_UNSUPPORTED;
+  if (c->riemann < 0 || c->riemann > 2 || c->limiter < 0 || c->limiter > 3) return FVHOE_ERR_ARG;
+  P->count = count;
+  P->p = c->p[0];
+  P->tile = tile;
+  P->fp32 = c->bulk_fp32;
+  P->GE = e_gext(P->p);
+  const int pad = e_ghosts(P->p) + 2 * reach(P->p);
+  P->cells = P->wcells = P->fcells = P->scells = 0;
+  P->tw = P->tf = P->ts = P->tc = 0;
+  for (int i = 0; i < count; ++i) {
+    if (n[i] < pad || n[i] < 7) return FVHOE_ERR_UNSUPPORTED;
+    P->cells += n[i];
+    P->wcells += n[i] + 2 * P->GE;
+    P->fcells += n[i] + 1;
+    P->scells += n[i] + 6;
+    P->tw += (n[i] + 2 * P->GE + tile - 1) / tile;
+    P->tf += (n[i] + 1 + tile - 1) / tile;
+    P->ts += (n[i] + 6 + tile - 1) / tile;
+    P->tc += (n[i] + tile - 1) / tile;
+  }
+  if (5 * (P->wcells + P->scells) > 2000000000LL) return FVHOE_ERR_UNSUPPORTED;
+  const size_t esz = P->fp32 ? 4 : 8;
+  size_t o = 0;
+  P->o_n = o; o = a256(o + 4 * (size_t)count);
+  P->o_h = o; o = a256(o + 8 * (size_t)count);
+  P->o_ou = o; o = a256(o + 8 * (size_t)count);
+  P->o_ow = o; o = a256(o + 8 * (size_t)count);
+  P->o_of = o; o = a256(o + 8 * (size_t)count);
+  P->o_os = o; o = a256(o + 8 * (size_t)count);
+  P->o_dts = o; o = a256(o + 8 * (size_t)count);
+  P->o_dtp = o; o = a256(o + 8 * (size_t)count);
+  P->o_act = o; o = a256(o + 4 * (size_t)count);
+  P->o_acc = o; o = a256(o + 8 * NACC * (size_t)count);
+  P->o_rng = o; o = a256(o + 80 * (size_t)count);
+  P->o_cnt = o; o = a256(o + 4 * (size_t)count);
+  P->o_tw = o; o = a256(o + 8 * (size_t)P->tw);
+  P->o_tf = o; o = a256(o + 8 * (size_t)P->tf);
+  P->o_ts = o; o = a256(o + 8 * (size_t)P->ts);
+  P->o_tc = o; o = a256(o + 8 * (size_t)P->tc);
+  P->o_W = o; o = a256(o + esz * 5 * (size_t)P->wcells);
+  P->o_F = o; o = a256(o + esz * 5 * (size_t)P->fcells);
+  P->o_WS = o; o = a256(o + esz * 5 * (size_t)P->scells);
+  P->o_list = o; o = a256(o + 4 * (size_t)P->cells);
+  P->o_k0 = o; o = a256(o + 8 * 5 * (size_t)P->cells);
+  P->o_k1 = o; o = a256(o + 8 * 5 * (size_t)P->cells);
+  P->o_s = o; o = a256(o + 8 * 5 * (size_t)P->cells);
+  P->total = o;
+  return FVHOE_OK;
+}
+
+Ens make_ens(const FvhoeStageConfig* c, const Plan& P, char* ws) {
+  Ens e{};
+  e.count = P.count;
+  e.bc_lo = c->bc[0];
+  e.bc_hi = c->bc[1];
+  e.riemann = c->riemann;
+  e.limiter = c->limiter;
+  e.apost = c->a_posteriori;
+  e.gamma = c->gamma;
+  e.cfl = c->cfl;
+  e.csq_floor = c->csq_floor;
+  e.n = (const int*)(ws + P.o_n);
+  e.h = (const double*)(ws + P.o_h);
+  e.off_u = (const long long*)(ws + P.o_ou);
+  e.off_w = (const long long*)(ws + P.o_ow);
+  e.off_f = (const long long*)(ws + P.o_of);
+  e.off_s = (const long long*)(ws + P.o_os);
+  e.dt_stage = (double*)(ws + P.o_dts);
+  e.dt_step = (double*)(ws + P.o_dtp);
+  e.active = (int*)(ws + P.o_act);
+  e.acc = (unsigned long long*)(ws + P.o_acc);
+  e.range = (double*)(ws + P.o_rng);
+  e.cnt = (int*)(ws + P.o_cnt);
+  e.nad_mode = c->nad_mode;
+  e.nad_range = c->nad_range;
+  e.nad_vars = c->nad_vars;
+  e.pad_var = c->pad_var;
+  e.sed = c->sed;
+  e.force = c->force_trouble;
+  e.nad_eps = c->nad_eps;
+  e.pad_lo = c->pad_lo;
+  e.pad_hi = c->pad_hi;
+  e.sed_eps = c->sed_eps;
+  return e;
+}
+
+template <typename T, int P, int TILE>
+int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
+             hipStream_t s) {
+  Ens e = make_ens(c, PL, ws);
+  const int2* tw = (const int2*)(ws + PL.o_tw);
+  const int2* tf = (const int2*)(ws + PL.o_tf);
+  const int2* ts = (const int2*)(ws + PL.o_ts);
+  const int2* tc = (const int2*)(ws + PL.o_tc);
+  T* W = (T*)(ws + PL.o_W);
+  T* F = (T*)(ws + PL.o_F);
+  T* WS = (T*)(ws + PL.o_WS);
+  int* list = (int*)(ws + PL.o_list);
+  double* k0 = (double*)(ws + PL.o_k0);
+  double* k1 = (double*)(ws + PL.o_k1);
+  double* sv = (double*)(ws + PL.o_s);
+  const int gi = (PL.count + 255) / 256;
+  for (int st = 0; st < 3; ++st) {
+    const double* in = st == 0 ? u : sv;
+    k_ens_reset<<<gi, 256, 0, s>>>(e);
+    k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W);
+    k_ens_scalars<<<gi, 256, 0, s>>>(e, st == 0, t, target);
+    k_ens_flux<T, P, TILE><<<PL.tf, TILE, 0, s>>>(e, tf, W, F);
+    if (c->a_posteriori) {
+      k_ens_prims<T, P, TILE, true><<<PL.ts, TILE, 0, s>>>(e, ts, in, F, WS);
+      k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
+      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F);
+    }
+    k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1);
+  }
+  k_ens_advance<<<gi, 256, 0, s>>>(e, t, nsteps);
+  return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
+}
+
+template <typename T, int TILE>
+int by_p(int p, const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
+         hipStream_t s) {
+  switch (p) {
+    case 1: return run_step<T, 1, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 2: return run_step<T, 2, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 3: return run_step<T, 3, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 4: return run_step<T, 4, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 5: return run_step<T, 5, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 6: return run_step<T, 6, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 7: return run_step<T, 7, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 8: return run_step<T, 8, TILE>(c, PL, u, t, nsteps, target, ws, s);
+  }
+  return FVHOE_ERR_UNSUPPORTED;
+}
+
+template <typename T>
+int by_tile(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
+            hipStream_t s) {
+  switch (PL.tile) {
+    case 64: return by_p<T, 64>(PL.p, c, PL, u, t, nsteps, target, ws, s);
+    case 128: return by_p<T, 128>(PL.p, c, PL, u, t, nsteps, target, ws, s);
+    case 256: return by_p<T, 256>(PL.p, c, PL, u, t, nsteps, target, ws, s);
+  }
+  return FVHOE_ERR_ARG;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvhoe_ensemble_workspace_bytes(const FvhoeStageConfig* cfg, int count, const int* n, int tile,
+                                   unsigned long long* bytes) {
+  Plan P;
+  const int st = make_plan(cfg, count, n, tile, &P);
+  if (st != FVHOE_OK) return st;
+  if (!bytes) return FVHOE_ERR_ARG;
+  *bytes = P.total;
+  return FVHOE_OK;
+}
+
+int fvhoe_ensemble_setup(const FvhoeStageConfig* cfg, int count, const int* n, const double* h, int tile,
+                         void* workspace, void* stream) {
+  Plan P;
+  const int st = make_plan(cfg, count, n, tile, &P);
+  if (st != FVHOE_OK) return st;
+  if (!h || !workspace || ((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
+  for (int i = 0; i < count; ++i)
+    if (!(h[i] > 0.0)) return FVHOE_ERR_ARG;
+  // host tables, copied once (synchronously: the host arrays are released on return)
+  std::vector<char> host(P.o_W, 0);
+  int* hn = (int*)(host.data() + P.o_n);
+  double* hh = (double*)(host.data() + P.o_h);
+  long long* ou = (long long*)(host.data() + P.o_ou);
+  long long* ow = (long long*)(host.data() + P.o_ow);
+  long long* of = (long long*)(host.data() + P.o_of);
+  long long* os = (long long*)(host.data() + P.o_os);
+  int* act = (int*)(host.data() + P.o_act);
+  int2* tw = (int2*)(host.data() + P.o_tw);
+  int2* tf = (int2*)(host.data() + P.o_tf);
+  int2* ts = (int2*)(host.data() + P.o_ts);
+  int2* tc = (int2*)(host.data() + P.o_tc);
+  long long a = 0, b = 0, f = 0, sn = 0;
+  int kw = 0, kf = 0, ks = 0, kc = 0;
+  for (int i = 0; i < count; ++i) {
+    hn[i] = n[i];
+    hh[i] = h[i];
+    ou[i] = a;
+    ow[i] = b;
+    of[i] = f;
+    os[i] = sn;
+    act[i] = 1;
+    a += n[i];
+    b += n[i] + 2 * P.GE;
+    f += n[i] + 1;
+    sn += n[i] + 6;
+    for (int x = 0; x < n[i] + 2 * P.GE; x += tile) tw[kw++] = make_int2(i, x);
+    for (int x = 0; x < n[i] + 1; x += tile) tf[kf++] = make_int2(i, x);
+    for (int x = 0; x < n[i] + 6; x += tile) ts[ks++] = make_int2(i, x);
+    for (int x = 0; x < n[i]; x += tile) tc[kc++] = make_int2(i, x);
+  }
+  (void)stream;
+  if (hipMemcpy(workspace, host.data(), host.size(), hipMemcpyHostToDevice) != hipSuccess) return FVHOE_ERR_LAUNCH;
+  return FVHOE_OK;
+}
+
+int fvhoe_ensemble_step(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u, double* t,
+                        int* steps, double target, void* workspace, void* stream) {
+  Plan P;
+  const int st = make_plan(cfg, count, n, tile, &P);
+  if (st != FVHOE_OK) return st;
+  if (!u || !t || !workspace || ((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  return P.fp32 ? by_tile<float>(cfg, P, u, t, steps, target, ws, s)
+                 : by_tile<double>(cfg, P, u, t, steps, target, ws, s);
+}
+
+}  // extern "C"

@@ -290,6 +290,31 @@ int fvhoe_stage_predicted(const FvhoeStageConfig* cfg, const double* u, double* 
                           signed char* trouble, void* workspace, const FvhoeRkFuse* rk, const FvhoePredict* pred,
                           void* stream);
 
+/* ------------------------------------------------------------------------
+ * Batched ensemble of independent 1-D runs of uneven size, advanced together
+ * by SSP-RK3 (fvhoe/ode.py:271-283), each instance with its own CFL dt, time
+ * and _hit_target (fvhoe/ode.py:11-22, 183-193).  No counterpart in the
+ * reference, which runs one EulerSolver (fvhoe/solver.py:41-1001) at a time;
+ * every instance follows that solver's stage (fvhoe/solver.py:429-454) bit
+ * for bit in the FP64 form.  scheme: the shared settings (p[0], gamma, cfl,
+ * bc[0..1], riemann, a-posteriori options, bulk_fp32; n[1] = n[2] = 1,
+ * p[1] = p[2] = 0; n[0] and h are per instance).  tile: 64, 128 or 256 cells
+ * per workgroup.  u: the instances' conservative states, instance-major, each
+ * (5, n[i]) contiguous; t: per-instance times (device).
+ * ---------------------------------------------------------------------- */
+int fvhoe_ensemble_workspace_bytes(const FvhoeStageConfig* scheme, int count, const int* n, int tile,
+                                   unsigned long long* bytes);
+/* Writes the instance tables into the workspace (synchronous copy; n and h
+ * are host arrays). */
+int fvhoe_ensemble_setup(const FvhoeStageConfig* scheme, int count, const int* n, const double* h, int tile,
+                         void* workspace, void* stream);
+/* One SSP-RK3 step of every instance whose time is below target; the others
+ * are left untouched.  steps (device, may be NULL): per-instance step
+ * counters, incremented for the instances that stepped.  Enqueues only
+ * kernels, so it can be captured in a graph. */
+int fvhoe_ensemble_step(const FvhoeStageConfig* scheme, int count, const int* n, int tile, double* u, double* t,
+                        int* steps, double target, void* workspace, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
