@@ -50,6 +50,21 @@ BYTES_PER_CELL = {
 BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 120, "detect": 80, "fallback": 0,
               "final": 80, "predict": 40}
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
+BF16_PEAK_TFLOPS = 2500.0
+# useful multiply-adds of the predictor network per cell: 2 * 25 * (4*16 + 16*16 + 16*1)
+PREDICT_FLOP_PER_CELL = 2 * 25 * (4 * 16 + 16 * 16 + 16)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "kh2048_hbm_traffic.json")
+
+
+def measured_traffic(form, kernel):
+    """HBM bytes per launch of `kernel` in `form` from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    recorded in profiles/ (null when that kernel was not measured)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            rec = json.load(f)
+        return rec["forms"][form][kernel]["bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def parse():
@@ -212,8 +227,16 @@ def main():
     cells = n * n
     stages = 3 * a.steps
     pk = head["kernels"]
-    dom = max(pk, key=lambda k: pk[k]["ms_avg"])
+    # the dominant kernel: most time per step (average launch time x launches)
+    dom = max(pk, key=lambda k: pk[k]["ms_avg"] * pk[k]["launches"])
     achieved = pk[dom]["algorithmic_GBs"]
+    traffic = measured_traffic(names[-1], dom)
+    pred_roof = None
+    if "predict" in pk:
+        tf = PREDICT_FLOP_PER_CELL * cells / (pk["predict"]["ms_avg"] / 1e3) / 1e12
+        pred_roof = {"bound": "mfma", "kernel": "predict", "achieved": round(tf, 1), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(tf / BF16_PEAK_TFLOPS, 4),
+                     "flop_per_cell": PREDICT_FLOP_PER_CELL}
     stage_gbs = STAGE_BYTES_APOST_FP64 * head["value"] * 1e6 / 1e9
     extra = {}
     for f in ("fp32", "learned"):
@@ -241,9 +264,10 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"kelvin_helmholtz_{n}x{n}_p3_hllc_aposteriori_ssprk3", "form": names[-1],
                    "cells": cells, "stages_timed": stages, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_cell": pk[dom]["bytes_per_cell"]},
+        "predictor_roofline": pred_roof,
         "stage_roofline": {"bytes_per_cell_stage": round(STAGE_BYTES_APOST_FP64, 1),
                            "achieved": round(stage_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": round(stage_gbs / HBM_PEAK_GBS, 4)},

@@ -66,7 +66,8 @@ __device__ __forceinline__ float bf_hi(unsigned int w) { return __uint_as_float(
 struct FeatPlanar {
   const float* __restrict__ X;
   int HX, WX;
-  __device__ __forceinline__ void load(int gy, int gx, float f[4]) const {
+  __device__ __forceinline__ void init(double*) const {}
+  __device__ __forceinline__ void load(int gy, int gx, float f[4], const double*) const {
     const bool in = gy < HX && gx < WX;
 #pragma unroll
     for (int c = 0; c < 4; ++c) f[c] = in ? X[((long long)c * HX + gy) * WX + gx] : 0.0f;
@@ -80,7 +81,7 @@ constexpr int INFER_THREADS = 256;
 __device__ __forceinline__ int swz(int q, int k) { return q * 16 + 4 * k; }
 
 template <typename Feat>
-__global__ void __launch_bounds__(INFER_THREADS) k_infer(Feat feat, int n0, int n1,
+__global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, int n1,
                                                         const unsigned char* __restrict__ pw, float thr,
                                                         signed char* __restrict__ flags,
                                                         float* __restrict__ logits, int* __restrict__ count,
@@ -95,6 +96,8 @@ __global__ void __launch_bounds__(INFER_THREADS) k_infer(Feat feat, int n0, int 
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int mrow = lane & 15, kb = lane >> 4;
+  __shared__ double fconst[8];
+  feat.init(fconst);  // per-block constants of the feature map (visible after the first barrier)
   // B fragments for the whole kernel: lane (n = mrow, kb) holds B[8kb..8kb+7][n]
   bf16x8 wb1[K1STEPS], wb2[K2STEPS];
 #pragma unroll
@@ -126,7 +129,7 @@ __global__ void __launch_bounds__(INFER_THREADS) k_infer(Feat feat, int n0, int 
     for (int p = tid; p < L0 * L0; p += INFER_THREADS) {
       const int y = p / L0, x = p % L0;
       float f[4];
-      feat.load(oy0 + y, ox0 + x, f);
+      feat.load(oy0 + y, ox0 + x, f, fconst);
       const bf16x4v v = {(__bf16)f[0], (__bf16)f[1], (__bf16)f[2], (__bf16)f[3]};
       *(bf16x4v*)&xin[p * C0] = v;
     }

@@ -646,7 +646,7 @@ __global__ void k_pred_stats(const int* __restrict__ count, long long* __restric
 
 // Predictor features: (rho, vx, vy, P) of W at the cells of the grid extended
 // by 6 per side (boundary conditions applied by index mapping), scaled to the
-// global range (w - min) / (max - min); 0 where max = min.
+// global range as (w - min) * (1 / (max - min)) in FP64; 0 where max = min.
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_features(const T* __restrict__ W, int NW0, int NW1, Geo g,
                                                  const double* __restrict__ range, float* __restrict__ X) {
@@ -664,7 +664,7 @@ __global__ void __launch_bounds__(TPB) k_features(const T* __restrict__ W, int N
     double x = (double)W[v * NWC + wi];
     if ((fx && v == 1) || (fy && v == 2)) x = -x;
     const double lo = range[2 * v], rng = range[2 * v + 1] - lo;
-    X[k * total + e] = rng > 0.0 ? (float)((x - lo) / rng) : 0.0f;
+    X[k * total + e] = rng > 0.0 ? (float)((x - lo) * (1.0 / rng)) : 0.0f;
   }
 }
 
@@ -676,7 +676,16 @@ struct FeatW {
   int NW0, NW1;
   Geo g;
   const double* __restrict__ range;
-  __device__ __forceinline__ void load(int a, int b, float f[4]) const {
+  // per-variable (min, 1 / (max - min)) into sh[0..7], once per block
+  __device__ __forceinline__ void init(double* sh) const {
+    if (threadIdx.x < 4) {
+      const int k = threadIdx.x, v = k == 3 ? 4 : k;
+      const double lo = range[2 * v], rng = range[2 * v + 1] - lo;
+      sh[2 * k] = lo;
+      sh[2 * k + 1] = rng > 0.0 ? 1.0 / rng : 0.0;
+    }
+  }
+  __device__ __forceinline__ void load(int a, int b, float f[4], const double* sh) const {
     if (a >= g.n0 + 12 || b >= g.n1 + 12) {
       f[0] = f[1] = f[2] = f[3] = 0.0f;
       return;
@@ -690,8 +699,8 @@ struct FeatW {
       const int v = k == 3 ? 4 : k;
       double x = (double)W[v * NWC + wi];
       if ((fx && v == 1) || (fy && v == 2)) x = -x;
-      const double lo = range[2 * v], rng = range[2 * v + 1] - lo;
-      f[k] = rng > 0.0 ? (float)((x - lo) / rng) : 0.0f;
+      const double inv = sh[2 * k + 1];
+      f[k] = inv != 0.0 ? (float)((x - sh[2 * k]) * inv) : 0.0f;
     }
   }
 };

@@ -228,7 +228,7 @@ int fvhoe_stage_timing_read(double* ms, long long* count);
  * learned part of the stage.  A three-layer valid 5x5 convolutional network,
  * 4 -> 16 -> 16 -> 1 channels, ReLU after the first two layers, on features
  * X (4, n0+12, n1+12) float: the primitive fv averages (rho, vx, vy, P) of the
- * stage input, (w - min) / (max - min) with the stage's global NAD range,
+ * stage input, (w - min) * (1 / (max - min)) in FP64 with the stage's global NAD range,
  * extended by 6 ghost cells per side with the solver's boundary conditions.
  * Parameters theta (FVHOE_PRED_NPARAMS float): w1 (16,4,5,5), b1 (16),
  * w2 (16,16,5,5), b2 (16), w3 (1,16,5,5), b3 (1).  The network sees its

@@ -126,7 +126,7 @@ def features(s, u: np.ndarray) -> np.ndarray:
     with gw=3, as in revise_fluxes, fvhoe/solver.py:681-687) give the global
     (min, max) per variable (fvhoe/slope_limiting.py:168-170); the interior
     averages of (rho, vx, vy, P), extended by 6 cells per side with the
-    scheme's boundary conditions, are scaled to (w - min) / (max - min)
+    scheme's boundary conditions, are scaled to (w - min) * (1 / (max - min))
     (0 where max = min), in float64, then rounded to float32."""
     from oracle import fvhoe_oracle as O
 
@@ -137,5 +137,5 @@ def features(s, u: np.ndarray) -> np.ndarray:
     out = []
     for v in (0, 1, 2, 4):
         rng = hi[v] - lo[v]
-        out.append((we[v] - lo[v]) / rng if rng > 0 else np.zeros_like(we[v]))
+        out.append((we[v] - lo[v]) * (1.0 / rng) if rng > 0 else np.zeros_like(we[v]))
     return np.stack(out).astype(np.float32)

@@ -47,3 +47,36 @@ def test_conv_valid_brute_force():
                 ref = b[o] + sum(w[o, c, dy, dx] * x[0, c, i + dy, j + dx]
                                  for c in range(2) for dy in range(5) for dx in range(5))
                 assert abs(out[0, o, i, j] - ref) < 1e-12
+
+
+def test_features_invariants():
+    """The predictor features: the range-defining cells map to exactly 0 and to
+    1 within rounding; ghost cells follow the boundary maps (periodic wrap,
+    mirror, mirror with the normal velocity negated)."""
+    from oracle import fvhoe_oracle as O
+    from fvhoe_amd import initial_conditions as ics
+
+    n = 24
+    u = ics.four_quadrants(n, seed=3)
+    for kind in ("periodic", "outflow", "reflective"):
+        s = O.Scheme(n=(n, n, 1), p=(3, 3, 0), gamma=1.4, a_posteriori=True,
+                     bc=((kind, kind), (kind, kind), ("periodic", "periodic")))
+        X = PO.features(s, u)
+        assert X.shape == (4, n + 12, n + 12) and X.dtype == np.float32
+        w3 = O.interpolate_primitives_from_conservatives(s, u, s.p, (3, 3, 0))[..., 0]
+        core = X[:, 3:-3, 3:-3]  # the interior and 3 ghost cells per side
+        for k, v in enumerate((0, 1, 2, 4)):
+            if w3[v].max() > w3[v].min():
+                assert core[k].min() >= 0.0 and core[k].max() <= 1.0 + 1e-6
+        inner = X[:, 6:-6, 6:-6]
+        if kind == "periodic":
+            assert np.array_equal(X[:, :6, 6:-6], inner[:, -6:, :])
+        else:
+            mirrored = X[:, :6, 6:-6][:, ::-1, :]
+            same = (0, 2, 3) if kind == "reflective" else (0, 1, 2, 3)
+            assert np.array_equal(mirrored[list(same)], inner[list(same), :6, :])
+            if kind == "reflective":  # vx negated about the wall: f(-v) = 2 f(0) - f(v) in exact arithmetic
+                w = O.interpolate_primitives_from_conservatives(s, u, s.p, (0, 0, 0))[..., 0]
+                lo, hi = w3[1].min(), w3[1].max()
+                ref = ((-w[1, :6, :] - lo) * (1.0 / (hi - lo))).astype(np.float32)
+                assert np.array_equal(mirrored[1], ref)
