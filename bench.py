@@ -80,11 +80,81 @@ def parse():
     ap.add_argument("--cpu-stages", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 device copies of 167,772,160 bytes (scale for HBM byte counters)")
-    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble"],
-                    help="kh: the headline Kelvin-Helmholtz stage; ensemble: 512 uneven 1-D Shu-Osher runs to t=1.8")
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "kh-accuracy"],
+                    help="kh: the headline Kelvin-Helmholtz stage; ensemble: 512 uneven 1-D Shu-Osher runs to t=1.8; "
+                         "kh-accuracy: every form to --t-end against a 2x finer FP64 reference")
+    ap.add_argument("--t-end", type=float, default=0.8)
     ap.add_argument("--ensemble-t", type=float, default=1.8)
     ap.add_argument("--ensemble-count", type=int, default=512)
     return ap.parse_args()
+
+
+def run_kh_accuracy(a):
+    """L1 density error at t_end of each form at n^2 against plain FP64 at (2n)^2 started from the
+    piecewise-constant refinement of the same initial averages and averaged back over 2x2 blocks."""
+    import numpy as np
+    import torch
+
+    from fvhoe_amd import initial_conditions as ics
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+    from fvhoe_amd.predictor import DEFAULT_WEIGHTS, TroubledCellPredictor
+    from fvhoe_amd.solver import EulerSolver
+
+    n = a.n
+    u0 = ics.kelvin_helmholtz(n, seed=0)
+
+    def solver(u, m, form):
+        class IC(ics.InitialCondition):
+            def base_ic(self):
+                return lambda x, y, z: u
+
+        pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form == "learned" else None
+        s = EulerSolver(w0=IC(), nx=m, ny=m, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
+                        bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
+                        progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred)
+        s.integrator = "ssprk3"
+        return s
+
+    def run(s):
+        t0 = time.perf_counter()
+        steps = 0
+        while s.t < a.t_end:
+            s.take_step(a.t_end)
+            steps += 1
+            if steps % 500 == 0:
+                print(f"#   step {steps} t={s.t:.4f}", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        return steps, time.perf_counter() - t0
+
+    fine_u0 = np.repeat(np.repeat(u0, 2, axis=1), 2, axis=2)
+    print(f"# reference {2 * n}^2 fp64 to t={a.t_end}", file=sys.stderr, flush=True)
+    ref = solver(fine_u0, 2 * n, "fp64")
+    rsteps, rsec = run(ref)
+    r = ref.u[0, :, :, 0]
+    rho_ref = 0.25 * (r[0::2, 0::2] + r[1::2, 0::2] + r[0::2, 1::2] + r[1::2, 1::2])
+    del ref
+    out = {"workload": f"kelvin_helmholtz_{n}x{n}_p3_hllc_aposteriori_ssprk3_to_t{a.t_end}",
+           "reference": {"cells": (2 * n) ** 2, "steps": rsteps, "seconds": round(rsec, 2)}, "forms": {}}
+    dens = {}
+    for form in a.forms.split(","):
+        print(f"# {form} {n}^2 to t={a.t_end}", file=sys.stderr, flush=True)
+        s = solver(u0, n, form)
+        steps, sec = run(s)
+        rho = s.u[0, :, :, 0]
+        dens[form] = rho.clone()
+        rec = {"steps": steps, "seconds": round(sec, 2), "t": s.t,
+               "l1_density_error": float((rho - rho_ref).abs().mean().item())}
+        if s.predictor_stats is not None:
+            rec["predictor"] = dict(zip(("preflagged", "second_pass", "preflag_unflagged"),
+                                        [int(v) for v in s.predictor_stats.tolist()]))
+        out["forms"][form] = rec
+        del s
+    if "fp64" in out["forms"]:
+        e64 = out["forms"]["fp64"]["l1_density_error"]
+        for f, rec in out["forms"].items():
+            rec["error_ratio_vs_fp64"] = round(rec["l1_density_error"] / e64, 4)
+            rec["l1_density_vs_fp64"] = float((dens[f] - dens["fp64"]).abs().mean().item())
+    print(json.dumps(out), flush=True)
 
 
 def run_ensemble(a):
@@ -204,6 +274,9 @@ def main():
         sys.exit(2)
     if a.workload == "ensemble":
         run_ensemble(a)
+        return
+    if a.workload == "kh-accuracy":
+        run_kh_accuracy(a)
         return
     import torch
 

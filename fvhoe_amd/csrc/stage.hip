@@ -510,6 +510,114 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
   if (t) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
 }
 
+// 2-D detection over strips of R rows (x) per thread, one column (y) per
+// lane: the 3x3 max/min of W comes from per-row 3-wide maxima slid down the
+// strip (3.75 loads per cell and variable instead of 9).  The flags equal
+// k_detect's: max/min of the same nine values, in another order.
+template <typename T, int R>
+__global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, int NW0, int NW1,
+                                                     const T* __restrict__ WS, int NS0, int NS1, Geo g,
+                                                     const double* __restrict__ range_mm, DetectP a,
+                                                     signed char* __restrict__ trouble, int* __restrict__ count,
+                                                     int* __restrict__ list, const signed char* __restrict__ pre,
+                                                     int* __restrict__ fp_count, int jblocks) {
+  const int tid = threadIdx.x;
+  const int j = (blockIdx.x % jblocks) * TPB + tid, i0 = (blockIdx.x / jblocks) * R;
+  const bool col = j < g.n1;
+  const int NWC = NW0 * NW1, NSC = NS0 * NS1;
+  const int jc = col ? j : g.n1 - 1;  // idle lanes read a valid column, their results are dropped
+  unsigned nadbits[R];  // bit v: NAD test of variable v (masked by nad_vars)
+#pragma unroll
+  for (int k = 0; k < R; ++k) nadbits[k] = 0u;
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    double hM[R + 2], hm[R + 2];
+#pragma unroll
+    for (int r = 0; r < R + 2; ++r) {
+      const int i = min(i0 - 1 + r, g.n0);  // W row of cell i (ghosts: g.ge0 >= 3)
+      const T* wr = W + v * NWC + (i + g.ge0) * NW1 + jc + g.ge1;
+      const double x0 = (double)wr[-1], x1 = (double)wr[0], x2 = (double)wr[1];
+      hM[r] = dmax(dmax(x0, x1), x2);
+      hm[r] = dmin(dmin(x0, x1), x2);
+    }
+    const double rlo = range_mm[2 * v], rhi = range_mm[2 * v + 1];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int i = min(i0 + k, g.n0 - 1);
+      const double M = dmax(dmax(hM[k], hM[k + 1]), hM[k + 2]);
+      const double m = dmin(dmin(hm[k], hm[k + 1]), hm[k + 2]);
+      const double uc = (double)WS[v * NSC + (i + 3) * NS1 + jc + 3];
+      bool tv;
+      if (a.nad_range == 0) {
+        const double rng = a.nad_mode == 0 ? rhi - rlo : M - m;
+        tv = dmin(uc - m, M - uc) < -a.eps * rng;
+      } else {
+        const double lo = a.nad_mode == 0 ? (1.0 - a.eps) * rlo : (1.0 - a.eps) * m;
+        const double hi = a.nad_mode == 0 ? (1.0 + a.eps) * rhi : (1.0 + a.eps) * M;
+        tv = dmin(uc - lo, hi - uc) < 0.0;
+      }
+      if (tv && ((a.nad_vars >> v) & 1)) nadbits[k] |= 1u << v;
+    }
+  }
+  unsigned tbits = 0u, fpbits = 0u;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i = i0 + k;
+    if (!col || i >= g.n0) continue;
+    const int sb = (i + 3) * NS1 + j + 3;
+    bool nad = false;
+    for (int v = 0; v < 5; ++v) {
+      bool tv = (nadbits[k] >> v) & 1u;
+      if (tv && a.sed && !nad) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
+        const T* sv = WS + v * NSC;
+        double alpha = sed_alpha_1d([&](int o) { return (double)sv[sb + o * NS1]; }, a.sed_eps);
+        alpha = dmin(alpha, sed_alpha_1d([&](int o) { return (double)sv[sb + o]; }, a.sed_eps));
+        tv = alpha < 1.0;
+      }
+      nad = nad || tv;
+    }
+    bool pad = false;
+    if (a.pad_var >= 0) {
+      const double x = (double)WS[a.pad_var * NSC + sb];
+      pad = dmin(x - a.pad_lo, a.pad_hi - x) < 0.0;
+    }
+    int t = (a.force || pad || nad) ? 1 : 0;
+    const int c = i * g.n1 + j;
+    if (pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
+      const int pf = pre[c];
+      trouble[c] = (signed char)(t | pf);
+      if (pf && !t) fpbits |= 1u << k;
+      t = t && !pf;
+    } else {
+      trouble[c] = (signed char)t;
+    }
+    if (t) tbits |= 1u << k;
+  }
+  if (pre) {
+    int f = __popc(fpbits);
+    for (int off = 32; off > 0; off >>= 1) f += __shfl_down(f, off, 64);
+    if ((tid & 63) == 0 && f) atomicAdd(fp_count, f);
+  }
+  // compaction: block scan of the per-thread counts, one atomic per block
+  __shared__ int sc[TPB];
+  __shared__ int bbase;
+  const int cnt = __popc(tbits);
+  sc[tid] = cnt;
+  __syncthreads();
+  for (int off = 1; off < TPB; off <<= 1) {
+    const int x = tid >= off ? sc[tid - off] : 0;
+    __syncthreads();
+    sc[tid] += x;
+    __syncthreads();
+  }
+  if (tid == TPB - 1) bbase = sc[TPB - 1] ? atomicAdd(count, sc[TPB - 1]) : 0;
+  __syncthreads();
+  int pos = bbase + sc[tid] - cnt;
+#pragma unroll
+  for (int k = 0; k < R; ++k)
+    if ((tbits >> k) & 1u) list[pos++] = (i0 + k) * g.n1 + j;
+}
+
 // ---------------------------------------------------------------------------
 // MUSCL fallback fluxes and the final divergence
 // ---------------------------------------------------------------------------
@@ -942,9 +1050,15 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
     const int gridd = (nc + TPB - 1) / TPB;
+    constexpr int DR = 8;  // rows per thread of the 2-D detection
+    const int jblocks = (L.n1 + TPB - 1) / TPB, gridd2 = jblocks * ((L.n0 + DR - 1) / DR);
     {
       TimedLaunch tl(TK_DETECT, s);
-      k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list,
+      if (TWO_D)
+        k_detect_strip<T, DR><<<gridd2, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count,
+                                                      list, preflags, count + 2, jblocks);
+      else
+        k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list,
                                         preflags, count + 2);
     }
     {
