@@ -80,9 +80,10 @@ def parse():
     ap.add_argument("--cpu-stages", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 device copies of 167,772,160 bytes (scale for HBM byte counters)")
-    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "kh-accuracy"],
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "kh-accuracy", "predictor"],
                     help="kh: the headline Kelvin-Helmholtz stage; ensemble: 512 uneven 1-D Shu-Osher runs to t=1.8; "
-                         "kh-accuracy: every form to --t-end against a 2x finer FP64 reference")
+                         "kh-accuracy: every form to --t-end against a 2x finer FP64 reference; "
+                         "predictor: the inference kernel alone on random features")
     ap.add_argument("--t-end", type=float, default=0.8)
     ap.add_argument("--ensemble-t", type=float, default=1.8)
     ap.add_argument("--ensemble-count", type=int, default=512)
@@ -155,6 +156,40 @@ def run_kh_accuracy(a):
             rec["error_ratio_vs_fp64"] = round(rec["l1_density_error"] / e64, 4)
             rec["l1_density_vs_fp64"] = float((dens[f] - dens["fp64"]).abs().mean().item())
     print(json.dumps(out), flush=True)
+
+
+def run_predictor(a):
+    """The predictor's inference kernel alone: n^2 logits from uniform random planar features,
+    `steps` timed launches after `warmup`."""
+    import torch
+
+    from fvhoe_amd import _lib
+    from fvhoe_amd.predictor import DEFAULT_WEIGHTS, TroubledCellPredictor
+
+    n = a.n
+    m = TroubledCellPredictor.load(DEFAULT_WEIGHTS)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(0)
+    X = torch.rand((4, n + 12, n + 12), dtype=torch.float32, device="cuda", generator=g)
+    flags = torch.empty((n, n), dtype=torch.int8, device="cuda")
+
+    def launch():
+        _lib.call("fvhoe_pred_infer", _lib.ptr(X), n, n, _lib.ptr(m.packed), float(m.logit_threshold),
+                  _lib.ptr(flags), None, _lib.stream())
+
+    for _ in range(a.warmup):
+        launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(a.steps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / a.steps
+    tf = PREDICT_FLOP_PER_CELL * n * n / (ms / 1e3) / 1e12
+    print(json.dumps({"metric": "predictor inference", "value": round(tf, 1), "unit": "TFLOP/s", "ms": round(ms, 4),
+                      "cells": n * n, "frac_bf16_peak": round(tf / BF16_PEAK_TFLOPS, 4)}), flush=True)
 
 
 def run_ensemble(a):
@@ -277,6 +312,9 @@ def main():
         return
     if a.workload == "kh-accuracy":
         run_kh_accuracy(a)
+        return
+    if a.workload == "predictor":
+        run_predictor(a)
         return
     import torch
 

@@ -289,6 +289,149 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
 }
 
 // ---------------------------------------------------------------------------
+// 2-D prims over column strips: lane l of a wave owns W column b and R
+// consecutive W rows.  The x sweeps run down the strip in registers; the y
+// sweeps take their neighbours from adjacent lanes with DPP wavefront shifts,
+// so the kernel needs no LDS and no barrier outside the final reduction.  Each
+// wave yields 64 - 2H valid columns (H = c and t reaches).  Same operations in
+// the same order as k_prims, so W is identical.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int dpp_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(dpp_next(__float_as_int(x))); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(dpp_prev(__float_as_int(x))); }
+__device__ __forceinline__ double lane_next(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_next((int)(b & 0xffffffffll)), hi = dpp_next((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_prev(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_prev((int)(b & 0xffffffffll)), hi = dpp_prev((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// x of lane l + O (|O| <= 4)
+template <int O, typename T>
+__device__ __forceinline__ T lane_at(T x) {
+  if constexpr (O == 0) return x;
+  else if constexpr (O > 0) return lane_at<O - 1>(lane_next(x));
+  else return lane_at<O + 1>(lane_prev(x));
+}
+// stencil sweep whose taps are compile-time offsets (for lane_at)
+template <int KIND, int P, typename T, typename Get>
+__device__ __forceinline__ T sweepL(Get&& x) {
+  T acc = T(0);
+  static_for<ntaps(P)>([&](auto I) {
+    constexpr double w = weight(KIND, P, decltype(I)::value);
+    if constexpr (w != 0.0) acc = acc + T(w) * x(std::integral_constant<int, decltype(I)::value - reach(P)>{});
+  });
+  return acc;
+}
+
+__host__ __device__ constexpr int strip_halo(int p) { return ereach_c(p) + ereach_t(p); }
+// W rows per lane in k_prims_strip (register budget: 5 variables x (rows + 2) centres per lane)
+template <typename T>
+constexpr int strip_rows() { return sizeof(T) == 8 ? 4 : 8; }
+constexpr int STRIP_MAX_P = 3;  // degrees that use k_prims_strip (higher degrees: the LDS-tiled k_prims)
+
+template <typename T, int P, typename Src>
+__global__ void __launch_bounds__(256) k_prims_strip(Src src, Geo g, T* __restrict__ W, int NW0, int NW1,
+                                                       int cwins, Reduce red, double gamma) {
+  constexpr int RC = ereach_c(P), RT = ereach_t(P), H = RC + RT, VC = 64 - 2 * H;
+  constexpr int R = strip_rows<T>(), NU = R + 2 * H, NC = R + 2 * RT;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int win = blockIdx.x % cwins, rb = blockIdx.x / cwins;
+  const int b = win * VC - H + lane;   // W column of this lane
+  const int a0 = (rb * 4 + wave) * R;  // first W row of this wave
+  const int cy = b - g.ge1, cx0 = a0 - g.ge0;
+  const int cyl = min(max(cy, -(g.ge1 + H)), g.n1 + g.ge1 + H - 1);
+  const bool inside = cx0 - H >= 0 && cx0 + R - 1 + H < g.n0 && win * VC - H - g.ge1 >= 0 &&
+                      win * VC - H + 63 - g.ge1 < g.n1;
+  T C[5][NC];
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    T uc[NU];
+#pragma unroll
+    for (int r = 0; r < NU; ++r) {
+      const int cx = min(max(cx0 - H + r, -(g.ge0 + H)), g.n0 + g.ge0 + H - 1);
+      uc[r] = T(inside ? src.at(g, v, cx, cyl) : src.get(g, v, cx, cyl));
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      const T s1 = sweepT<KIND_C, P, T>([&](int t) { return uc[k + RC + t - reach(P)]; });
+      const T x = sweepL<KIND_C, P, T>([&](auto o) { return lane_at<decltype(o)::value>(s1); });
+      C[v][k] = T(0) + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
+    }
+  }
+  // pressure recovery in FP64 (fvhoe/hydro.py:146-170)
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    double uu[5], ww[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) uu[q] = (double)C[q][k];
+    prims_from_cons(uu, ww, gamma);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) C[q][k] = T(ww[q]);
+  }
+  const bool lane_ok = lane >= H && lane < 64 - H && b < NW1;
+  double m = 0.0;
+  unsigned long long val[NPART];
+  val[0] = 0ull;
+#pragma unroll
+  for (int k = 1; k < NPART; ++k) val[k] = (k & 1) ? ~0ull : 0ull;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    T w[5];
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const T tx = sweepT<KIND_T, P, T>([&](int t) { return C[v][i + RT + t - reach(P)]; });
+      const T y = sweepL<KIND_T, P, T>([&](auto o) { return lane_at<decltype(o)::value>(tx); });
+      w[v] = T(0) + y;  // z-axis sweep of degree 0 (fvhoe/fv.py:221-223)
+    }
+    const int a = a0 + i;
+    if (lane_ok && a < NW0) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) W[(v * NW0 + a) * NW1 + b] = w[v];
+      if (red.partials) {
+        if (red.do_dt && a >= red.dt_lo0 && a < red.dt_hi0 && b >= red.dt_lo1 && b < red.dt_hi1) {
+          const double c = sound_speed((double)w[0], (double)w[4], red.gamma, red.csq_floor);
+          const double sp = (fabs((double)w[1]) + fabs((double)w[2])) + fabs((double)w[3]);
+          m = np_max(m, sp + (double)red.ndim * c);  // a NaN wave speed gives a NaN dt, as np.max does
+        }
+        if (red.do_range && a >= red.rg_lo0 && a < red.rg_hi0 && b >= red.rg_lo1 && b < red.rg_hi1) {
+#pragma unroll
+          for (int v = 0; v < 5; ++v) {
+            const unsigned long long o = ordered_bits((double)w[v]);
+            val[1 + 2 * v] = red_op(1, val[1 + 2 * v], o);
+            val[2 + 2 * v] = red_op(2, val[2 + 2 * v], o);
+          }
+        }
+      }
+    }
+  }
+  if (!red.partials) return;
+  val[0] = (unsigned long long)__double_as_longlong(m);
+  __shared__ unsigned long long part[4][NPART];
+#pragma unroll
+  for (int k = 0; k < NPART; ++k) {
+    unsigned long long x = val[k];
+    for (int off = 32; off > 0; off >>= 1) x = red_op(k, x, __shfl_down(x, off, 64));
+    if (lane == 0) part[wave][k] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < NPART) {
+    unsigned long long x = part[0][threadIdx.x];
+    for (int w = 1; w < 4; ++w) x = red_op(threadIdx.x, x, part[w][threadIdx.x]);
+    red.partials[blockIdx.x * NPART + threadIdx.x] = x;
+  }
+}
+
+inline int strip_cwins(int p, int NW1) { return (NW1 + (64 - 2 * strip_halo(p)) - 1) / (64 - 2 * strip_halo(p)); }
+inline int strip_blocks(int p, int rows, int NW0, int NW1) {
+  return strip_cwins(p, NW1) * ((NW0 + 4 * rows - 1) / (4 * rows));
+}
+
+// ---------------------------------------------------------------------------
 // flux tile.  D = 0: x faces fa in [f0, f0+TA) (face fa = left face of cell
 // fa), rows b in [b0, b0+TB).  D = 1: y faces, rows along x.  "normal" is the
 // face-normal axis with degree PN, "trans" the other with degree PT.
@@ -926,9 +1069,12 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offWS = o; o = align256(o + (c->a_posteriori ? esz * 5 * (size_t)L->NS0 * L->NS1 : 0));
   L->offFlags = o; o = align256(o + (c->a_posteriori ? (size_t)L->n0 * L->n1 : 0));
   L->offList = o; o = align256(o + (c->a_posteriori ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
-  {
-    const int tx = L->two_d ? TX2 : TX1, ty = L->two_d ? TY2 : 1;
-    L->prims_blocks = ((L->NW0 + tx - 1) / tx) * ((L->NW1 + ty - 1) / ty);
+  if (L->two_d && L->p <= STRIP_MAX_P) {
+    L->prims_blocks = strip_blocks(L->p, L->fp32 ? strip_rows<float>() : strip_rows<double>(), L->NW0, L->NW1);
+  } else if (L->two_d) {
+    L->prims_blocks = ((L->NW0 + TX2 - 1) / TX2) * ((L->NW1 + TY2 - 1) / TY2);
+  } else {
+    L->prims_blocks = (L->NW0 + TX1 - 1) / TX1;
   }
   L->offPart = o; o = align256(o + sizeof(unsigned long long) * NPART * (size_t)L->prims_blocks);
   L->offScal = o; o = align256(o + 256);
@@ -990,9 +1136,14 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   red.csq_floor = c->csq_floor;
 
   {
-    const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
     TimedLaunch tl(TK_PRIMS, s);
-    k_prims<T, P, PY, TX, TY, SrcU, NTILE><<<t0 * t1, NTILE, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
+    if constexpr (TWO_D && P <= STRIP_MAX_P) {
+      k_prims_strip<T, P, SrcU><<<strip_blocks(P, strip_rows<T>(), L.NW0, L.NW1), 256, 0, s>>>(
+          SrcU{u}, g, W, L.NW0, L.NW1, strip_cwins(P, L.NW1), red, c->gamma);
+    } else {
+      const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
+      k_prims<T, P, PY, TX, TY, SrcU, NTILE><<<t0 * t1, NTILE, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
+    }
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
   RkArgs rk{-1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -1044,6 +1195,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     Reduce none{};
     {
       TimedLaunch tl(TK_PRIMS_CAND, s);
+      // the LDS tiles form each candidate value once per tile (the strips would form it (R + 2H) / R times)
       k_prims<T, P, PY, TX, TY, SrcUstar<TWO_D, T>, NTILE><<<t0 * t1, NTILE, 0, s>>>(
           SrcUstar<TWO_D, T>{u, F, G, dt_stage}, gs, WS, L.NS0, L.NS1, t1, none, c->gamma);
     }
