@@ -1099,7 +1099,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   T* F = (T*)(ws + L.offF);
   T* G = TWO_D ? (T*)(ws + L.offG) : nullptr;
   T* WS = (T*)(ws + L.offWS);
-  signed char* flags = (signed char*)(ws + L.offFlags);
+  // the flags go straight to the caller's array when one is given (no copy node in a captured graph)
+  signed char* flags = trouble_out ? trouble_out : (signed char*)(ws + L.offFlags);
   unsigned long long* partials = (unsigned long long*)(ws + L.offPart);
   double* range = (double*)(ws + L.offScal);
   int* count = (int*)(range + 10);
@@ -1217,8 +1218,6 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       TimedLaunch tl(TK_FALLBACK, s);
       k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count, list, SrcU{u}, g, fp, F, G);
     }
-    if (trouble_out && hipMemcpyAsync(trouble_out, flags, (size_t)nc, hipMemcpyDeviceToDevice, s) != hipSuccess)
-      return FVHOE_ERR_LAUNCH;
     if (use_pred && pred->stats) k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats);
   }
   {

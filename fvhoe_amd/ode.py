@@ -41,6 +41,7 @@ class ODE(ABC):
         self._s = _lib.empty(self.u.shape)
         self._scal = _lib.zeros((8,))  # dt_stage x4, dt_step, t, spare
         self._n = n
+        self._graphs = {}  # integrator -> (captured step, RHS evaluations per step)
 
     # ------------------------------------------------------------ to override
     @abstractmethod
@@ -146,16 +147,56 @@ class ODE(ABC):
         self.step_count += 1
         self.step_helper_function()
 
-    def advance(self, n: int):
-        """n steps with no host synchronisation (time kept on the device)."""
+    def advance(self, n: int, graph: bool = False):
+        """n steps with no host synchronisation (time kept on the device).  With graph=True the
+        step after the first is captured once in a HIP graph (per integrator) and replayed, which
+        removes the host launch work between kernels; the kernels and their arguments are the
+        same as in the eager steps, so the results are identical."""
         t_dev = self._scal[5:6]
         _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
-        for _ in range(n):
+        # the capture needs the steady state of one completed step (workspace allocated, counters reset)
+        have = (self.integrator, t_dev.data_ptr()) in self._graphs
+        eager = n if not graph else (0 if have else min(n, 1))
+        for _ in range(eager):
             self._device_step(None, t_dev)
             self.step_count += 1
             self.step_helper_function()
+        if n > eager:
+            g, per_step = self._step_graph(t_dev)
+            for _ in range(n - eager):
+                g.replay()
+            self.step_count += n - eager
+            if hasattr(self, "f_evaluation_count"):
+                self.f_evaluation_count += per_step * (n - eager)
         self.t = float(t_dev.item())
         self.timestamps.append(self.t)
+
+    def _step_graph(self, t_dev):
+        """One step of self.integrator (with its bookkeeping kernels) captured in a graph.  The
+        capture records the launches without running them; host-side counters touched while
+        recording are restored, and they return to the same values after every step, so the
+        captured launch arguments hold for every replay."""
+        key = (self.integrator, t_dev.data_ptr())
+        if key not in self._graphs:
+            torch = _lib.torch()
+            fe = getattr(self, "f_evaluation_count", None)
+            saved = {k: v for k, v in vars(self).items() if k in ("trouble_counter",)}
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.stream(s):
+                with torch.cuda.graph(g, stream=s):
+                    self._device_step(None, t_dev)
+                    self.step_helper_function()
+            torch.cuda.current_stream().wait_stream(s)
+            per_step = 0
+            if fe is not None:
+                per_step = self.f_evaluation_count - fe
+                self.f_evaluation_count = fe
+            for k, v in saved.items():
+                setattr(self, k, v)
+            self._graphs[key] = (g, per_step)
+        return self._graphs[key]
 
     def integrate(self, T: float = None, n: int = None, exact: bool = True, downbeats: Any = None,
                   log_every_step: bool = False, snapshot_dir: str = None, overwrite: bool = False) -> None:

@@ -1,0 +1,60 @@
+"""Graph-captured steps (ODE.advance(graph=True)) against eager steps: every
+replay, interleaved with eager steps and host synchronisations, must give the
+eager trajectory bit for bit (state, time, trouble log, predictor counters)."""
+
+import numpy as np
+import pytest
+
+from fvhoe_amd import _lib
+from fvhoe_amd import initial_conditions as ics
+from fvhoe_amd.predictor import TroubledCellPredictor
+from gpu_helpers import assert_bits_equal, solver_from
+
+pytestmark = pytest.mark.gpu
+
+P, O = ("periodic", "periodic"), ("outflow", "outflow")
+
+
+def _pair(kind, integ, **kw):
+    n = 64
+    u0 = ics.kelvin_helmholtz(n, seed=0) if kind == "kh" else ics.four_quadrants(n, seed=2)
+    bc = (P, P, P) if kind == "kh" else (O, O, P)
+    mk = lambda: solver_from(u0, (n, n, 1), (3, 3, 0), bc, riemann_solver="hllc",  # noqa: E731
+                             a_posteriori_slope_limiting=True, **kw)
+    a, b = mk(), mk()
+    a.integrator = b.integrator = integ
+    return a, b
+
+
+@pytest.mark.parametrize("kind,integ,form", [("kh", "ssprk3", "fp64"), ("kh", "rk4", "fp64"),
+                                             ("quad", "ssprk3", "fp32"), ("quad", "ssprk3", "learned")])
+def test_graph_replays_equal_eager_steps(kind, integ, form):
+    kw = {"bulk_precision": "fp64" if form == "fp64" else "fp32"}
+    if form == "learned":
+        kw["predictor"] = TroubledCellPredictor.load()
+    a, b = _pair(kind, integ, **kw)
+    torch = _lib.torch()
+
+    def check(tag):
+        torch.cuda.synchronize()
+        assert_bits_equal(b.u.cpu().numpy(), a.u.cpu().numpy(), f"u after {tag}")
+        assert a.t == b.t, tag
+        assert a.step_count == b.step_count
+        assert a.f_evaluation_count == b.f_evaluation_count
+        assert_bits_equal(b.log["mean trouble"].cpu().numpy(), a.log["mean trouble"].cpu().numpy(), "log")
+        if form == "learned":
+            assert np.array_equal(a.predictor_stats.cpu().numpy(), b.predictor_stats.cpu().numpy())
+
+    a.advance(3)
+    b.advance(3, graph=True)  # 1 eager step, capture, 2 replays
+    check("3 steps")
+    for k in range(3):  # replays one by one, with eager steps in between
+        a.advance(1)
+        b.advance(1, graph=True)
+        check(f"replay {k}")
+        a.advance(1)
+        b.advance(1)
+        check(f"eager {k}")
+    a.advance(4)
+    b.advance(4, graph=True)
+    check("4 more replays")
