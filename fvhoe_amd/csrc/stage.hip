@@ -443,10 +443,11 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
   constexpr int RC = ereach_c(PT), RT = ereach_t(PT);
   constexpr int HN = TA + 1 + 2 * EL;
   constexpr int HT0 = TB + 2 * (RC + RT), HT1 = TB + 2 * RT;
+  constexpr int HS = HT0 | 1;  // S0 row stride, odd: the transposed stores of D = 1 hit distinct banks
   constexpr int NR = (TA * HT1 + NT - 1) / NT;   // Riemann problems per thread
   static_assert(TA * HT1 <= HN * HT1, "flux results reuse the X buffer");
   __shared__ T X[5][HN * HT1];
-  __shared__ T S0[HN * HT0];
+  __shared__ T S0[HN * HS];
   const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
 
   const int nN = D == 0 ? g.n0 : g.n1, nT = D == 0 ? g.n1 : g.n0;
@@ -472,7 +473,7 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
         // clamped reads feed unused outputs only
         const int a = D == 0 ? min(wn0 + r, NW0 - 1) : min(wt0 + r, NW0 - 1);
         const int b = D == 0 ? min(wt0 + c, NW1 - 1) : min(wn0 + c, NW1 - 1);
-        dst[k] = D == 0 ? r * HT0 + c : c * HT0 + r;
+        dst[k] = D == 0 ? r * HS + c : c * HS + r;
 #pragma unroll
         for (int v = 0; v < 5; ++v) pre[v][k] = W[(v * NW0 + a) * NW1 + b];
       }
@@ -491,7 +492,7 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
       if (tid + k * NT < HN * HT0) S0[dst[k]] = pre[v][k];
     __syncthreads();
     tile_for<HN, HT1, NT>([&](int e, int r, int c) {
-      const T x = sweepT<KIND_C, PT, T>([&](int t) { return S0[r * HT0 + c + RC + t - reach(PT)]; });
+      const T x = sweepT<KIND_C, PT, T>([&](int t) { return S0[r * HS + c + RC + t - reach(PT)]; });
       X[v][e] = T(0) + x;  // degree-0 z sweep precedes/follows (fvhoe/solver.py:524,541)
     });
     __syncthreads();

@@ -134,6 +134,7 @@ class EulerSolver(ODE):
         nx_, ny_, nz_ = self.n
         self.log = {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag", "mean PAD mag")}
         self._zero_like_log = _lib.zeros((nx_, ny_, nz_))
+        self._log_written = {}  # log arrays that have received magnitudes (the fused stage logs flags only)
         self._one = _lib.zeros((1,))
         self._one.fill_(1.0)
 
@@ -399,6 +400,8 @@ class EulerSolver(ODE):
         if reset:
             if self.trouble_counter > 0:
                 for k in lg:
+                    if k != "mean trouble" and not self._log_written.get(k, False):
+                        continue  # never written since construction: dividing zeros changes nothing
                     _lib.call("fvhoe_log_troubles", _lib.ptr(lg[k]), None, n, 2, float(self.trouble_counter),
                               _lib.stream())
                 self.trouble_counter = 0
@@ -409,6 +412,7 @@ class EulerSolver(ODE):
         for k, v in (("mean NAD mag", NAD_violation_magnitude), ("mean PAD mag", PAD_violation_magnitude)):
             if v is None:
                 continue
+            self._log_written[k] = True
             _lib.call("fvhoe_rk_combine", _lib.RK_AXPY, _lib.ptr(lg[k]),
                       _lib.ptr(self._zero_like_log if self.trouble_counter == 0 else lg[k]), _lib.ptr(v.contiguous()),
                       None, None, None, _lib.ptr(self._one), n, _lib.stream())
