@@ -23,6 +23,29 @@ struct Weights9 {
   int k;
 };
 
+// (a, j, k) with idx = (a * m1 + j) * m2 + k, in 32-bit arithmetic when idx fits
+// (64-bit integer division is a long software sequence on the GPU)
+__device__ __forceinline__ void split3(long long idx, int m1, int m2, int* a, int* j, int* k) {
+  if (idx <= 0x7fffffffLL) {
+    const unsigned r = (unsigned)idx, q = r / (unsigned)m2, a32 = q / (unsigned)m1;
+    *k = (int)(r - q * (unsigned)m2);
+    *j = (int)(q - a32 * (unsigned)m1);
+    *a = (int)a32;
+  } else {
+    const long long q = idx / m2, aa = q / m1;
+    *k = (int)(idx - q * m2);
+    *j = (int)(q - aa * m1);
+    *a = (int)aa;
+  }
+}
+// (v, i, j, k) with idx = ((v * m0 + i) * m1 + j) * m2 + k
+__device__ __forceinline__ void split4(long long idx, int m0, int m1, int m2, int* v, int* i, int* j, int* k) {
+  int a;
+  split3(idx, m1, m2, &a, j, k);
+  *v = a / m0;
+  *i = a - *v * m0;
+}
+
 __global__ void k_sweep(const double* __restrict__ u, double* __restrict__ out, int nv, int n0, int n1, int n2,
                         int axis, Weights9 w) {
   const int m0 = axis == 1 ? n0 - (w.k - 1) : n0;
@@ -32,17 +55,32 @@ __global__ void k_sweep(const double* __restrict__ u, double* __restrict__ out, 
   const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    long long r = idx;
-    const int k = r % m2; r /= m2;
-    const int j = r % m1; r /= m1;
-    const int i = r % m0;
-    const int v = (int)(r / m0);
+    int v, i, j, k;
+    split4(idx, m0, m1, m2, &v, &i, &j, &k);
     const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
     double acc = 0.0;
     for (int t = 0; t < w.k; ++t)
       if (w.w[t] != 0.0) acc = acc + w.w[t] * u[base + t * s];
     out[idx] = acc;
   }
+}
+
+// The same sweep over a 2-D launch: blockIdx.y = v * m0 + i, x = j * m2 + k
+// (32-bit index math, taps unrolled; zero taps skipped as above).
+__global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, double* __restrict__ out, int n0, int n1,
+                                               int n2, int axis, Weights9 w, int m0, int m1, int m2) {
+  const int jk = blockIdx.x * blockDim.x + threadIdx.x;
+  if (jk >= m1 * m2) return;
+  const int vi = blockIdx.y;
+  const int v = vi / m0, i = vi - (vi / m0) * m0;
+  const int j = m2 == 1 ? jk : jk / m2, k = m2 == 1 ? 0 : jk - (jk / m2) * m2;
+  const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
+  const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
+  double acc = 0.0;
+#pragma unroll
+  for (int t = 0; t < 9; ++t)
+    if (t < w.k && w.w[t] != 0.0) acc = acc + w.w[t] * u[base + t * s];
+  out[(long long)vi * (m1 * m2) + jk] = acc;
 }
 
 __global__ void k_prims(const double* __restrict__ u, double* __restrict__ w, int nv, long long nc, double gamma) {
@@ -158,10 +196,8 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
   const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nout;
        c += (long long)gridDim.x * blockDim.x) {
-    long long r = c;
-    const int k = r % m2; r /= m2;
-    const int j = r % m1;
-    const int i = (int)(r / m1);
+    int i, j, k;
+    split3(c, m1, m2, &i, &j, &k);
     const long long base = ((long long)i * n1 + j) * n2 + k;
     double a[5], b[5], F[5];
 #pragma unroll
@@ -179,11 +215,8 @@ __global__ void k_muscl(const double* __restrict__ u, double* __restrict__ left,
   const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    long long r = idx;
-    const int k = r % m2; r /= m2;
-    const int j = r % m1; r /= m1;
-    const int i = r % m0;
-    const int v = (int)(r / m0);
+    int v, i, j, k;
+    split4(idx, m0, m1, m2, &v, &i, &j, &k);
     const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
     const double u0 = u[base], u1 = u[base + s], u2 = u[base + 2 * s];
     const double sl = limited_slope(limiter, u1 - u0, u2 - u1);
@@ -201,11 +234,8 @@ __global__ void k_bc_apply(const double* __restrict__ u, double* __restrict__ ou
   const long long total = (long long)nv * N0 * N1 * N2;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    long long r = idx;
-    const int k = r % N2; r /= N2;
-    const int j = r % N1; r /= N1;
-    const int i = r % N0;
-    const int v = (int)(r / N0);
+    int v, i, j, k;
+    split4(idx, N0, N1, N2, &v, &i, &j, &k);
     bool f0, f1, f2;
     const int si = bc_source(i, a.n[0], a.g[0], a.lo[0], a.hi[0], &f0);
     const int sj = bc_source(j, a.n[1], a.g[1], a.lo[1], a.hi[1], &f1);
@@ -223,11 +253,8 @@ __global__ void k_euler_eq(const double* __restrict__ F, const double* __restric
   const long long total = (long long)nv * n0 * n1 * n2;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    long long r = idx;
-    const int k = r % n2; r /= n2;
-    const int j = r % n1; r /= n1;
-    const int i = r % n0;
-    const int v = (int)(r / n0);
+    int v, i, j, k;
+    split4(idx, n0, n1, n2, &v, &i, &j, &k);
     double acc = 0.0;
     if (active & 1) {
       const long long b = (((long long)v * (n0 + 1) + i) * n1 + j) * n2 + k;
@@ -309,10 +336,8 @@ __global__ void k_detect(const double* __restrict__ w, const double* __restrict_
   const long long NC = (long long)a.N[0] * a.N[1] * a.N[2];
   const long long st[3] = {(long long)a.N[1] * a.N[2], a.N[2], 1};
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
-    long long r = c;
-    const int k = r % n2; r /= n2;
-    const int j = r % n1;
-    const int i = (int)(r / n1);
+    int i, j, k;
+    split3(c, n1, n2, &i, &j, &k);
     const long long base = ((long long)(i + g0) * a.N[1] + (j + g1)) * a.N[2] + (k + g2);
     bool nad = false;
     double ind_min = 0.0;
@@ -365,10 +390,8 @@ __global__ void k_sed(const double* __restrict__ u, double* __restrict__ alpha, 
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < nc * nv;
        idx += (long long)gridDim.x * blockDim.x) {
     const int v = (int)(idx / nc);
-    long long r = idx % nc;
-    const int k = r % n2; r /= n2;
-    const int j = r % n1;
-    const int i = (int)(r / n1);
+    int i, j, k;
+    split3(idx % nc, n1, n2, &i, &j, &k);
     const long long base = ((long long)(i + g0) * N1 + (j + g1)) * N2 + (k + g2);
     alpha[idx] = sed_alpha_cell(u + v * NC, base, st, dims, eps);
   }
@@ -382,10 +405,8 @@ __global__ void k_broadcast_simple(const signed char* __restrict__ t, double* __
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < fx + fy + fz;
        idx += (long long)gridDim.x * blockDim.x) {
     if (idx < fx) {
-      long long r = idx;
-      const int k = r % n2; r /= n2;
-      const int j = r % n1;
-      const int i = (int)(r / n1);
+      int i, j, k;
+      split3(idx, n1, n2, &i, &j, &k);
       double th = 0.0;
       if (dims & 1) {
         const int a = i > 0 ? t[((long long)(i - 1) * n1 + j) * n2 + k] : 0;
@@ -394,10 +415,8 @@ __global__ void k_broadcast_simple(const signed char* __restrict__ t, double* __
       }
       tx[idx] = th;
     } else if (idx < fx + fy) {
-      long long r = idx - fx;
-      const int k = r % n2; r /= n2;
-      const int j = r % (n1 + 1);
-      const int i = (int)(r / (n1 + 1));
+      int i, j, k;
+      split3(idx - fx, (n1 + 1), n2, &i, &j, &k);
       double th = 0.0;
       if (dims & 2) {
         const int a = j > 0 ? t[((long long)i * n1 + (j - 1)) * n2 + k] : 0;
@@ -406,10 +425,8 @@ __global__ void k_broadcast_simple(const signed char* __restrict__ t, double* __
       }
       ty[idx - fx] = th;
     } else {
-      long long r = idx - fx - fy;
-      const int k = r % (n2 + 1); r /= (n2 + 1);
-      const int j = r % n1;
-      const int i = (int)(r / n1);
+      int i, j, k;
+      split3(idx - fx - fy, n1, (n2 + 1), &i, &j, &k);
       double th = 0.0;
       if (dims & 4) {
         const int a = k > 0 ? t[((long long)i * n1 + j) * n2 + (k - 1)] : 0;
@@ -434,10 +451,8 @@ __global__ void k_shift_max(double* out, const double* __restrict__ S1, const do
   const long long total = (long long)a.ext[0] * a.ext[1] * a.ext[2];
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
-    long long r = idx;
-    const int k = r % a.ext[2]; r /= a.ext[2];
-    const int j = r % a.ext[1];
-    const int i = (int)(r / a.ext[1]);
+    int i, j, k;
+    split3(idx, a.ext[1], a.ext[2], &i, &j, &k);
     const double s = S1[a.soff + i * a.ss[0] + j * a.ss[1] + k * a.ss[2]];
     const double x = a.coef * (a.gt ? (s > 0.0 ? 1.0 : 0.0) : s);
     const double y = S2[a.toff + i * a.ts[0] + j * a.ts[1] + k * a.ts[2]];
@@ -518,7 +533,14 @@ int fvhoe_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, in
   if (na < w.k) return FVHOE_ERR_ARG;
   for (int i = 0; i < w.k; ++i) w.w[i] = weight(kind, p, i);
   const long long total = (long long)nv * n0 * n1 * n2;
-  k_sweep<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, n0, n1, n2, axis, w);
+  const int m0 = axis == 1 ? n0 - (w.k - 1) : n0, m1 = axis == 2 ? n1 - (w.k - 1) : n1,
+            m2 = axis == 3 ? n2 - (w.k - 1) : n2;
+  if ((long long)m1 * m2 >= 64 && (long long)m1 * m2 < (1ll << 30) && (long long)nv * m0 <= 65535) {
+    const dim3 grid((unsigned)((m1 * m2 + TPB - 1) / TPB), (unsigned)(nv * m0));
+    k_sweep2<<<grid, TPB, 0, (hipStream_t)stream>>>(u, out, n0, n1, n2, axis, w, m0, m1, m2);
+  } else {
+    k_sweep<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, n0, n1, n2, axis, w);
+  }
   return launch_status();
 }
 
