@@ -110,10 +110,11 @@ def run_kh_accuracy(a):
             def base_ic(self):
                 return lambda x, y, z: u
 
-        pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form == "learned" else None
+        pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form.startswith("learned") else None
         s = EulerSolver(w0=IC(), nx=m, ny=m, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
                         bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
-                        progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred)
+                        progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred,
+                        predictor_overlap=form == "learned-overlap")
         s.integrator = "ssprk3"
         return s
 
@@ -261,10 +262,11 @@ def run_form(n, form, steps, warmup, u0):
         def base_ic(self):
             return lambda x, y, z: u0
 
-    pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form == "learned" else None
+    pred = TroubledCellPredictor.load(DEFAULT_WEIGHTS) if form.startswith("learned") else None
     s = EulerSolver(w0=KH(), nx=n, ny=n, px=3, py=3, gamma=1.4, CFL=0.8, conservative_ic=True, fv_ic=True,
                     bc=BoundaryCondition(), riemann_solver="hllc", a_posteriori_slope_limiting=True,
-                    progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred)
+                    progress_bar=False, bulk_precision="fp64" if form == "fp64" else "fp32", predictor=pred,
+                    predictor_overlap=form == "learned-overlap")
     assert s.fused
     s.integrator = "ssprk3"
     print(f"# {form}: warmup {warmup} steps at {n}^2", file=sys.stderr, flush=True)
@@ -353,7 +355,7 @@ def main():
                      "flop_per_cell": PREDICT_FLOP_PER_CELL}
     stage_gbs = STAGE_BYTES_APOST_FP64 * head["value"] * 1e6 / 1e9
     extra = {}
-    for f in ("fp32", "learned"):
+    for f in ("fp32", "learned", "learned-overlap"):
         if "fp64" in states and f in states:
             extra[f"l1_density_{f}_vs_fp64"] = (states[f][0] - states["fp64"][0]).abs().mean().item()
             extra[f"speedup_{f}_vs_fp64"] = round(forms[f]["value"] / forms["fp64"]["value"], 3)

@@ -45,7 +45,8 @@ class FvhoeRkFuse(ctypes.Structure):
 
 
 class FvhoePredict(ctypes.Structure):
-    _fields_ = [("packed", c_void_p), ("logit_threshold", c_float), ("infer", c_int), ("stats", c_void_p)]
+    _fields_ = [("packed", c_void_p), ("logit_threshold", c_float), ("infer", c_int), ("stats", c_void_p),
+                ("overlap", c_int), ("side_stream", c_void_p), ("ev_fork", c_void_p), ("ev_join", c_void_p)]
 
 
 PRED_NPARAMS = 8433
@@ -99,6 +100,8 @@ _SIGNATURES = {
                                      _P, _P]),
     "fvhoe_ensemble_step": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P, c_double,
                                     _P, _P]),
+    "fvhoe_event_create": (c_int, [POINTER(c_void_p)]),
+    "fvhoe_event_destroy": (c_int, [_P]),
     "fvhoe_stage_predicted_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage_predicted": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, POINTER(FvhoeRkFuse),
                                       POINTER(FvhoePredict), _P]),
@@ -165,6 +168,26 @@ def device():
 
 def stream():
     return c_void_p(torch().cuda.current_stream().cuda_stream)
+
+
+class OverlapStreams:
+    """A side stream (default priority) and two HIP events for running work beside the caller's stream."""
+
+    def __init__(self):
+        import weakref
+
+        device()
+        self.stream = torch().cuda.Stream()
+        fork, join = c_void_p(0), c_void_p(0)
+        call("fvhoe_event_create", ctypes.byref(fork))
+        call("fvhoe_event_create", ctypes.byref(join))
+        self.fork, self.join = fork.value, join.value
+        weakref.finalize(self, _destroy_events, lib(), self.fork, self.join)
+
+
+def _destroy_events(library, *events):
+    for e in events:
+        library.fvhoe_event_destroy(c_void_p(e))
 
 
 def ptr(t) -> c_void_p:

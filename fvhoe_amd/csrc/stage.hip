@@ -888,11 +888,13 @@ __global__ void k_stage_init(int* count, int reset_pre) {
   }
 }
 
-__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats) {
+__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats, int with_pre) {
   if (threadIdx.x == 0) {
-    stats[0] += count[1];
+    if (with_pre) {
+      stats[0] += count[1];
+      stats[2] += count[2];
+    }
     stats[1] += count[0];
-    stats[2] += count[2];
   }
 }
 
@@ -1118,8 +1120,12 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
 
   const bool apost = c->a_posteriori != 0;
   const bool use_pred = TWO_D && apost && pred != nullptr;
+  const bool overlap = use_pred && pred->overlap;
+  // a stage that runs the network beside its own kernels applies no pre-flags itself
+  const bool pre_now = use_pred && !(overlap && pred->infer);
   const bool need_range = apost && (c->nad_mode == 0 || use_pred);
-  signed char* preflags = use_pred ? (signed char*)(ws + L.offPre) : nullptr;
+  signed char* preflags = pre_now ? (signed char*)(ws + L.offPre) : nullptr;
+  signed char* preflags_out = use_pred ? (signed char*)(ws + L.offPre) : nullptr;
   int* pre_list = (int*)(ws + L.offPreList);
   k_stage_init<<<1, 64, 0, s>>>(count, use_pred && pred->infer);
 
@@ -1155,10 +1161,20 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
                                     set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0);
   if (use_pred && pred->infer) {  // the network on this stage's W, features formed on load
-    TimedLaunch tl(TK_PREDICT, s);
-    const int st = pred::launch_infer(FeatW<T>{W, L.NW0, L.NW1, g, range}, L.n0, L.n1, pred->packed,
-                                      pred->logit_threshold, preflags, nullptr, count + 1, pre_list, s);
-    if (st != FVHOE_OK) return st;
+    hipStream_t ps = s;
+    if (overlap) {  // fork: the network runs beside the rest of this stage
+      ps = (hipStream_t)pred->side_stream;
+      if (hipEventRecord((hipEvent_t)pred->ev_fork, s) != hipSuccess ||
+          hipStreamWaitEvent(ps, (hipEvent_t)pred->ev_fork, 0) != hipSuccess)
+        return FVHOE_ERR_LAUNCH;
+    }
+    {
+      TimedLaunch tl(TK_PREDICT, ps);
+      const int st = pred::launch_infer(FeatW<T>{W, L.NW0, L.NW1, g, range}, L.n0, L.n1, pred->packed,
+                                        pred->logit_threshold, preflags_out, nullptr, count + 1, pre_list, ps);
+      if (st != FVHOE_OK) return st;
+    }
+    if (overlap && hipEventRecord((hipEvent_t)pred->ev_join, ps) != hipSuccess) return FVHOE_ERR_LAUNCH;
   }
 
   if (TWO_D) {
@@ -1184,7 +1200,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
 
   FinalP fp{c->riemann, c->limiter, c->gamma, c->csq_floor};
   const int nc = L.n0 * L.n1;
-  if (use_pred) {  // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate
+  if (pre_now) {  // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate
     TimedLaunch tl(TK_FALLBACK, s);
     k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count + 1, pre_list, SrcU{u}, g, fp, F, G);
   }
@@ -1219,12 +1235,15 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       TimedLaunch tl(TK_FALLBACK, s);
       k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count, list, SrcU{u}, g, fp, F, G);
     }
-    if (use_pred && pred->stats) k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats);
+    if (use_pred && pred->stats) k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats, pre_now ? 1 : 0);
   }
   {
     TimedLaunch tl(TK_FINAL, s);
     k_final<TWO_D, T><<<gridc, TPB, 0, s>>>(F, G, g, dudt, rk);
   }
+  // join: the caller's stream waits for the network before anything after this call
+  if (overlap && pred->infer && hipStreamWaitEvent(s, (hipEvent_t)pred->ev_join, 0) != hipSuccess)
+    return FVHOE_ERR_LAUNCH;
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
@@ -1298,6 +1317,7 @@ int stage_entry(const FvhoeStageConfig* cfg, const double* u, double* dudt, doub
   const int st = make_layout(cfg, &L, pred != nullptr);
   if (st != FVHOE_OK) return st;
   if (pred && !pred->packed) return FVHOE_ERR_ARG;
+  if (pred && pred->overlap && (!pred->side_stream || !pred->ev_fork || !pred->ev_join)) return FVHOE_ERR_ARG;
   if (!u || !dt_stage || !workspace) return FVHOE_ERR_ARG;
   if (!dudt && !(rk && rk->kind >= 0)) return FVHOE_ERR_ARG;
   if (rk && rk->kind >= 0) {
@@ -1323,6 +1343,19 @@ int stage_entry(const FvhoeStageConfig* cfg, const double* u, double* dudt, doub
 int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
                    signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream) {
   return stage_entry(cfg, u, dudt, dt_stage, trouble, workspace, rk, nullptr, stream);
+}
+
+int fvhoe_event_create(void** event) {
+  if (!event) return FVHOE_ERR_ARG;
+  hipEvent_t e;
+  if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return FVHOE_ERR_LAUNCH;
+  *event = (void*)e;
+  return FVHOE_OK;
+}
+
+int fvhoe_event_destroy(void* event) {
+  if (!event) return FVHOE_ERR_ARG;
+  return hipEventDestroy((hipEvent_t)event) == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
 int fvhoe_stage_predicted_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes) {

@@ -75,6 +75,7 @@ class EulerSolver(ODE):
         bulk_precision: str = "fp64",
         predictor=None,
         predictor_every_stage: bool = False,
+        predictor_overlap: bool = False,
     ):
         if w0 is None:
             raise ValueError("w0 (an InitialCondition) is required")
@@ -151,11 +152,17 @@ class EulerSolver(ODE):
         # the candidate is formed; the plain check still judges every cell
         self.predictor = predictor
         self.predictor_every_stage = bool(predictor_every_stage)
+        # the network of a step's first stage runs on a side stream beside that stage's other kernels;
+        # its flags apply from the step's second stage on
+        self.predictor_overlap = bool(predictor_overlap) and not self.predictor_every_stage
         self.predictor_stats = None
+        self._overlap = None
         if predictor is not None:
             if not (self.fused and self.ndim == 2 and self.xdim and self.ydim and a_posteriori_slope_limiting):
                 raise NotImplementedError("the predictor runs in the fused 2-D a-posteriori stage")
             self.predictor_stats = _lib.zeros((3,), dtype=_lib.torch().int64)
+            if self.predictor_overlap:
+                self._overlap = _lib.OverlapStreams()
 
     # ------------------------------------------------------------------ config
     def _stage_config(self):
@@ -226,7 +233,7 @@ class EulerSolver(ODE):
         rk.target = float(target) if target is not None else 0.0
         rk.has_target = int(target is not None)
         if self.predictor is not None:
-            pa = self._predict_args(bool(set_dt_step) or self.predictor_every_stage)
+            pa = self._predict_args(bool(set_dt_step) or self.predictor_every_stage, overlap=self._overlap)
             _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),
                       _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk),
                       _lib.ctypes.byref(pa), _lib.stream())
@@ -238,12 +245,16 @@ class EulerSolver(ODE):
             self._log_troubles_device(self._trouble)
         return True
 
-    def _predict_args(self, infer: bool):
+    def _predict_args(self, infer: bool, overlap=None):
         pa = _lib.FvhoePredict()
         pa.packed = self.predictor.packed.data_ptr()
         pa.logit_threshold = float(self.predictor.logit_threshold)
         pa.infer = int(bool(infer))
         pa.stats = self.predictor_stats.data_ptr()
+        if overlap is not None:
+            pa.overlap = 1
+            pa.side_stream = overlap.stream.cuda_stream
+            pa.ev_fork, pa.ev_join = overlap.fork, overlap.join
         return pa
 
     def predictor_features(self):

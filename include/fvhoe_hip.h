@@ -277,13 +277,28 @@ int fvhoe_pred_adam(float* theta, const float* grad, float* m, float* v, int n, 
  * and the faces of the cells it flags get MUSCL fluxes as well.  trouble
  * receives pre-flags | check flags.  stats (device, may be NULL):
  * [0] += pre-flagged cells, [1] += cells flagged by the check only,
- * [2] += pre-flagged cells the check did not flag. */
+ * [2] += pre-flagged cells the check did not flag.
+ * overlap = 1: a call with infer = 1 runs the network on side_stream,
+ * forked after the stage's reductions (ev_fork) and concurrent with the
+ * flux, candidate, check, fallback and final kernels of that call, which
+ * therefore apply no pre-flags; stream waits for the network (ev_join)
+ * before the call returns, so later calls (infer = 0) apply its flags.
+ * Calls with infer = 0 ignore overlap.  Events come from fvhoe_event_create;
+ * the side stream must have the default priority. */
 typedef struct FvhoePredict {
   const void* packed;
   float logit_threshold;
   int infer;
   long long* stats;
+  int overlap;
+  void* side_stream;
+  void* ev_fork;
+  void* ev_join;
 } FvhoePredict;
+
+/* hipEvent_t without timing, for the overlap fork / join (host-only). */
+int fvhoe_event_create(void** event);
+int fvhoe_event_destroy(void* event);
 
 int fvhoe_stage_predicted_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes);
 int fvhoe_stage_predicted(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,

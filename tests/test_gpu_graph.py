@@ -27,11 +27,13 @@ def _pair(kind, integ, **kw):
 
 
 @pytest.mark.parametrize("kind,integ,form", [("kh", "ssprk3", "fp64"), ("kh", "rk4", "fp64"),
-                                             ("quad", "ssprk3", "fp32"), ("quad", "ssprk3", "learned")])
+                                             ("quad", "ssprk3", "fp32"), ("quad", "ssprk3", "learned"),
+                                             ("quad", "ssprk3", "learned-overlap"), ("quad", "rk4", "learned-overlap")])
 def test_graph_replays_equal_eager_steps(kind, integ, form):
     kw = {"bulk_precision": "fp64" if form == "fp64" else "fp32"}
-    if form == "learned":
+    if form.startswith("learned"):
         kw["predictor"] = TroubledCellPredictor.load()
+        kw["predictor_overlap"] = form == "learned-overlap"
     a, b = _pair(kind, integ, **kw)
     torch = _lib.torch()
 
@@ -42,7 +44,7 @@ def test_graph_replays_equal_eager_steps(kind, integ, form):
         assert a.step_count == b.step_count
         assert a.f_evaluation_count == b.f_evaluation_count
         assert_bits_equal(b.log["mean trouble"].cpu().numpy(), a.log["mean trouble"].cpu().numpy(), "log")
-        if form == "learned":
+        if form.startswith("learned"):
             assert np.array_equal(a.predictor_stats.cpu().numpy(), b.predictor_stats.cpu().numpy())
 
     a.advance(3)

@@ -183,6 +183,46 @@ def test_predicted_stage_flags_match_oracle_of_the_preblended_candidate():
     assert st[2] == (pre & ~chk.astype(bool)).sum()
 
 
+def _call_predicted(s, u, infer, overlap):
+    k, dt = _lib.empty(s.u.shape), _lib.empty((1,))
+    pa = s._predict_args(infer, overlap=s._overlap if overlap else None)
+    _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(s._cfg), _lib.ptr(u), _lib.ptr(k), _lib.ptr(dt),
+              _lib.ptr(s._trouble), s._workspace(), None, _lib.ctypes.byref(pa), _lib.stream())
+    _lib.torch().cuda.synchronize()
+    return k.cpu().numpy(), dt.cpu().numpy(), s._trouble.cpu().numpy()
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_overlapped_inference(form):
+    """A call that runs the network on the side stream is the plain stage bit for
+    bit; the next call applies the same flags as the non-overlapped sequence."""
+    n = 48
+    u0 = ics.four_quadrants(n, seed=5)
+    kw = dict(riemann_solver="hllc", a_posteriori_slope_limiting=True, bulk_precision=form)
+    m = TroubledCellPredictor(_theta(6), logit_threshold=0.0)
+    plain = solver_from(u0, (n, n, 1), (3, 3, 0), (Ob, Ob, P), **kw)
+    ov = solver_from(u0, (n, n, 1), (3, 3, 0), (Ob, Ob, P), predictor=m, predictor_overlap=True, **kw)
+    seq = solver_from(u0, (n, n, 1), (3, 3, 0), (Ob, Ob, P), predictor=m, **kw)
+    assert ov._overlap is not None
+    k0, dt0, t0 = _stage_outputs(plain)
+    k1, dt1, t1 = _call_predicted(ov, ov.u, True, True)
+    assert_bits_equal(k1, k0, "dudt of the inferring call")
+    assert_bits_equal(dt1, dt0, "dt")
+    assert np.array_equal(t1, t0)
+    st = ov.predictor_stats.cpu().numpy()
+    assert st[0] == 0 and st[1] == int(t0.sum()) and st[2] == 0
+    u1 = ov.u + float(dt0[0]) * 0.5 * _lib.torch().from_numpy(k0).to(ov.u.device)
+    k2, dt2, t2 = _call_predicted(ov, u1, False, True)
+    _call_predicted(seq, seq.u, True, False)
+    k3, dt3, t3 = _call_predicted(seq, u1, False, False)
+    assert_bits_equal(k2, k3, "dudt of the next call")
+    assert_bits_equal(dt2, dt3, "dt of the next call")
+    assert np.array_equal(t2, t3)
+    pre = seq.predictor_stats.cpu().numpy()[0] // 2
+    assert 0 < pre < n * n
+    assert ov.predictor_stats.cpu().numpy()[0] == pre
+
+
 def test_training_reduces_loss_on_solver_flags():
     """A few hundred Adam steps on flags of one quadrant run lower the loss."""
     torch = _lib.torch()
