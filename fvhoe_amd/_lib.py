@@ -20,7 +20,7 @@ OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH = 0, 1, 2, 3
 STENCIL = {"l": 0, "c": 1, "r": 2, "t": 3}
 RIEMANN = {"advection_upwind": 0, "llf": 1, "hllc": 2}
 LIMITER = {None: 0, "minmod": 1, "moncen": 2}
-BC = {"periodic": 0, "outflow": 1, "reflective": 2, "free": 3}
+BC = {"periodic": 0, "outflow": 1, "reflective": 2, "free": 3, "dirichlet": 4, "ic": 4}
 RK_AXPY, RK_SSPRK3_STAGE3, RK_SSPRK3_FINAL, RK_SSPRK2_FINAL, RK_HALF_AXPY, RK_RK4_FINAL = range(6)
 
 
@@ -33,6 +33,7 @@ class FvhoeStageConfig(ctypes.Structure):
         ("nad_vars", c_int), ("pad_var", c_int), ("pad_lo", c_double), ("pad_hi", c_double),
         ("sed", c_int), ("sed_eps", c_double), ("fixed_dt_on", c_int), ("fixed_dt", c_double),
         ("bulk_fp32", c_int),
+        ("slab", c_void_p * 6), ("slab_g", c_int), ("slab_pad", c_int * 3),
     ]
 
 
@@ -66,6 +67,9 @@ _SIGNATURES = {
     "fvhoe_riemann_faces": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_double, c_double, _P]),
     "fvhoe_muscl": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
     "fvhoe_bc_apply": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, POINTER(c_int), _P]),
+    "fvhoe_bc_apply_dirichlet": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, POINTER(c_int),
+                                         POINTER(c_void_p), _P]),
+    "fvhoe_stage_ghost_depth": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_int)]),
     "fvhoe_euler_equation": (c_int, [_P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_double, c_double,
                                      c_double, _P]),
     "fvhoe_minmax": (c_int, [_P, c_int, c_longlong, _P, _P, _P]),

@@ -1,8 +1,13 @@
 """Ghost-cell fill on device arrays (fvhoe/boundary_conditions.py:154-412).
 
 Kinds: periodic, outflow (mirror), reflective (mirror with the normal
-momentum negated), free (nearest interior cell).  Slabs are filled in the
-reference's x, y, z order, so corners compose the per-axis maps.
+momentum negated), free (nearest interior cell), and the two slab kinds,
+"dirichlet" (constant per-variable values, or a callable f(x, y, z, t)) and
+"ic" (the solver's pointwise conservative initial condition f(x, y, z)).
+Slabs are filled in the reference's x, y, z order, so a later axis decides a
+corner.  Slab values are evaluated on the host, at the slab cell centres of
+`slab_coords` (the reference's buffer of ghost-slab coordinates), copied to
+the device once per (ghost width, time), and read there by the fill kernel.
 """
 
 from __future__ import annotations
@@ -10,7 +15,11 @@ from __future__ import annotations
 from dataclasses import dataclass
 from typing import Tuple, Union
 
+import numpy as np
+
 from fvhoe_amd import _lib
+
+SLAB_KINDS = ("dirichlet", "ic")
 
 
 @dataclass
@@ -21,40 +30,141 @@ class BoundaryCondition:
     x_value: object = None
     y_value: object = None
     z_value: object = None
+    slab_coords: dict = None
 
     def __post_init__(self):
         for d in "xyz":
-            v = getattr(self, d)
-            if isinstance(v, list):
-                raise ValueError(f"Invalid type of {d}: list")
-            if not isinstance(v, tuple):
-                setattr(self, d, (v, v))
-            elif len(v) != 2:
-                raise ValueError(f"Invalid length of {d} tuple")
+            for suffix in ("", "_value"):
+                v = getattr(self, d + suffix)
+                if isinstance(v, list):
+                    raise ValueError(f"Invalid type of {d}{suffix}: list")
+                if not isinstance(v, tuple):
+                    setattr(self, d + suffix, (v, v))
+                elif len(v) != 2:
+                    raise ValueError(f"Invalid length of {d}{suffix} tuple")
             for kind in getattr(self, d):
-                if kind in ("dirichlet", "ic"):
-                    raise NotImplementedError(
-                        f"'{kind}' boundaries (host callables) are not on the device path")
-                if kind not in _lib.BC:
+                if kind is not None and kind not in _lib.BC:
                     raise ValueError(f"Unknown bc type '{kind}'")
+        if self.slab_coords is None:
+            self.slab_buffer_size = (0, 0, 0)
+        else:
+            self.slab_buffer_size = (self.slab_coords["xl"][0].shape[0], self.slab_coords["yl"][0].shape[1],
+                                     self.slab_coords["zl"][0].shape[2])
+        self._slab_cache = {}
 
+    # ------------------------------------------------------------- settings
     def codes(self):
-        return [_lib.BC[k] for d in "xyz" for k in getattr(self, d)]
+        """FVHOE_BC_* per side; None (no fill, the reference leaves such ghosts unset) maps to periodic,
+        which only matters where a ghost width is zero."""
+        return [_lib.BC[k] if k is not None else _lib.BC["periodic"] for d in "xyz" for k in getattr(self, d)]
 
     def periodic(self):
         return tuple(getattr(self, d) == ("periodic", "periodic") for d in "xyz")
 
+    def has_slabs(self) -> bool:
+        return any(k in SLAB_KINDS for d in "xyz" for k in getattr(self, d))
+
+    def time_dependent(self) -> bool:
+        """True if a "dirichlet" side takes a callable (evaluated at the solver's time)."""
+        return any(k == "dirichlet" and callable(v) for d in "xyz"
+                   for k, v in zip(getattr(self, d), getattr(self, f"{d}_value")))
+
+    def reset_value(self, dim: str, pos: str, new_value):
+        """fvhoe/boundary_conditions.py:222-238."""
+        lv, rv = getattr(self, f"{dim}_value")
+        if pos == "l":
+            lv = new_value
+        elif pos == "r":
+            rv = new_value
+        else:
+            raise ValueError(f"Invalid pos '{pos}'")
+        setattr(self, f"{dim}_value", (lv, rv))
+        self._slab_cache = {}
+
+    def trim_slabs(self, gw: Tuple[int, int, int], axis: int, pos: str):
+        """Slab-centre coordinates of side (axis, pos) for ghost widths gw, cut from the
+        slab_coords buffer (fvhoe/boundary_conditions.py:240-304)."""
+        if self.slab_coords is None:
+            raise ValueError("slab boundaries need slab_coords (fvhoe_amd.fv.fv_uniform_meshgen)")
+        X, Y, Z = self.slab_coords["xyz"[axis] + pos]
+        excess = [b - g for b, g in zip(self.slab_buffer_size, gw)]
+        for e in range(3):
+            if excess[e] < 0:
+                raise ValueError(f"Cannot apply bc to region of thickness {gw[e]} with a buffer of thickness "
+                                 f"{self.slab_buffer_size[e]}")
+        cut = []
+        for e in range(3):
+            L = X.shape[e]
+            if e == axis:
+                cut.append(slice(L - gw[e], L) if pos == "l" else slice(0, gw[e]))
+            else:
+                cut.append(slice(excess[e], L - excess[e]))
+        cut = tuple(cut)
+        return X[cut], Y[cut], Z[cut]
+
+    def slab_values(self, gw: Tuple[int, int, int], axis: int, pos: str, nv: int, t: float = None) -> np.ndarray:
+        """Host values (nv, slab shape) of a slab side (fvhoe/boundary_conditions.py:18-57)."""
+        d = "xyz"[axis]
+        kind = getattr(self, d)["lr".index(pos)]
+        value = getattr(self, f"{d}_value")["lr".index(pos)]
+        X, Y, Z = self.trim_slabs(gw, axis, pos)
+        if callable(value):
+            kw = dict(x=X, y=Y, z=Z)
+            if kind == "dirichlet":
+                kw["t"] = t
+            out = np.asarray(value(**kw), dtype=np.float64)
+        elif isinstance(value, np.ndarray):
+            out = np.ones((nv, *X.shape)) * value.reshape(-1, 1, 1, 1)
+        else:
+            raise ValueError(f"Invalid boundary of type '{value}'.")
+        if out.shape != (nv, *X.shape):
+            raise ValueError(f"boundary values of shape {out.shape} do not fill a slab of shape {(nv, *X.shape)}")
+        return np.ascontiguousarray(out)
+
+    def device_slabs(self, gw: Tuple[int, int, int], nv: int, t: float = None):
+        """Device slab per side (None for non-slab sides and zero widths), cached per (gw, nv) and, for
+        callables of "dirichlet" sides, per t.  The tensors stay alive in the cache, so kernels that read
+        them asynchronously never see freed memory."""
+        gw = tuple(int(g) for g in gw)
+        key = (gw, nv, t if self.time_dependent() else None)
+        hit = self._slab_cache.get(key)
+        if hit is None:
+            if self.time_dependent():  # keep one time level: drop the slabs of earlier times
+                self._slab_cache = {k: v for k, v in self._slab_cache.items() if k[2] is None}
+            hit = []
+            for axis in range(3):
+                for j, pos in enumerate("lr"):
+                    kind = getattr(self, "xyz"[axis])[j]
+                    if kind in SLAB_KINDS and gw[axis] > 0:
+                        vals = self.slab_values(gw, axis, pos, nv, t)
+                        hit.append(_lib.torch().from_numpy(vals).to(_lib.device()))
+                    else:
+                        hit.append(None)
+            self._slab_cache[key] = hit
+        return hit
+
+    # ----------------------------------------------------------------- fill
     def apply(self, u, gw: Tuple[int, int, int], t: float = None):
         """Pad u by gw per axis and fill the ghosts (fvhoe/boundary_conditions.py:306-402)."""
         x, was_np = _lib.as_device(u)
         nv, n0, n1, n2 = _lib.shape4(x)
-        for g, n in zip(gw, (n0, n1, n2)):
-            if g > n:
+        gw = tuple(int(g) for g in gw)
+        for (d, kinds), g, n in zip(((d, getattr(self, d)) for d in "xyz"), gw, (n0, n1, n2)):
+            if g == 0:
+                continue
+            if None in kinds:
+                raise ValueError(f"ghost cells of a None boundary along {d} are undefined (ghost width {g})")
+            if g > n and any(k not in SLAB_KINDS for k in kinds):
                 raise ValueError(f"ghost width {g} exceeds the interior size {n}")
         out = _lib.empty((nv, n0 + 2 * gw[0], n1 + 2 * gw[1], n2 + 2 * gw[2]))
         codes = (_lib.ctypes.c_int * 6)(*self.codes())
-        _lib.call("fvhoe_bc_apply", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, int(gw[0]), int(gw[1]), int(gw[2]),
-                  codes, _lib.stream())
+        if self.has_slabs():
+            slabs = self.device_slabs(gw, nv, t)
+            ptrs = (_lib.ctypes.c_void_p * 6)(*[s.data_ptr() if s is not None else None for s in slabs])
+            _lib.call("fvhoe_bc_apply_dirichlet", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, ptrs,
+                      _lib.stream())
+        else:
+            _lib.call("fvhoe_bc_apply", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, _lib.stream())
         return _lib.back(out, was_np)
 
     def to_dict(self) -> dict:

@@ -246,6 +246,21 @@ __global__ void k_bc_apply(const double* __restrict__ u, double* __restrict__ ou
   }
 }
 
+__global__ void k_bc_apply_slabs(const double* __restrict__ u, double* __restrict__ out, int nv, BcArgs a,
+                                 Slabs sl) {
+  const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
+  const int bc[6] = {a.lo[0], a.hi[0], a.lo[1], a.hi[1], a.lo[2], a.hi[2]};
+  const long long total = (long long)nv * N0 * N1 * N2;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    int v, i, j, k;
+    split4(idx, N0, N1, N2, &v, &i, &j, &k);
+    out[idx] = bc_fetch3(a.n, bc, sl, v, i - a.g[0], j - a.g[1], k - a.g[2], [&](int vv, int s0, int s1, int s2) {
+      return u[(((long long)vv * a.n[0] + s0) * a.n[1] + s1) * a.n[2] + s2];
+    });
+  }
+}
+
 // dudt = 0.0 + cx*(F[i+1]-F[i]) + cy*(G[j+1]-G[j]) + cz*(H[k+1]-H[k]), c = -(1/h)  (fvhoe/solver.py:744-750)
 __global__ void k_euler_eq(const double* __restrict__ F, const double* __restrict__ G, const double* __restrict__ H,
                            double* __restrict__ dudt, int nv, int n0, int n1, int n2, int active, double cx,
@@ -623,6 +638,32 @@ int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2,
   }
   const long long total = (long long)nv * (n0 + 2 * g0) * (n1 + 2 * g1) * (n2 + 2 * g2);
   k_bc_apply<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, a);
+  return launch_status();
+}
+
+int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                             const int* bc6, const double* const* slabs6, void* stream) {
+  if (!u || !out || !bc6 || !slabs6 || nv < 1) return FVHOE_ERR_ARG;
+  BcArgs a{{n0, n1, n2}, {g0, g1, g2}, {bc6[0], bc6[2], bc6[4]}, {bc6[1], bc6[3], bc6[5]}};
+  Slabs sl{};
+  for (int d = 0; d < 3; ++d) {
+    if (a.g[d] < 0 || a.n[d] < 1) return FVHOE_ERR_ARG;
+    sl.g[d] = sl.pad[d] = a.g[d];  // the slabs are the ghost regions of out, corners included
+  }
+  for (int side = 0; side < 6; ++side) {
+    const int d = side / 2, kind = bc6[side];
+    if (kind < 0 || kind > BC_DIRICHLET) return FVHOE_ERR_UNSUPPORTED;
+    sl.s[side] = nullptr;
+    if (a.g[d] == 0) continue;
+    if (kind == BC_DIRICHLET) {
+      if (!slabs6[side]) return FVHOE_ERR_ARG;
+      sl.s[side] = slabs6[side];
+    } else if (a.g[d] > a.n[d]) {
+      return FVHOE_ERR_ARG;  // slabs of the other kinds come from the interior
+    }
+  }
+  const long long total = (long long)nv * (n0 + 2 * g0) * (n1 + 2 * g1) * (n2 + 2 * g2);
+  k_bc_apply_slabs<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, a, sl);
   return launch_status();
 }
 

@@ -424,6 +424,7 @@ int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan
     return FVHOE_ERR_UNSUPPORTED;
   if (c->fixed_dt_on || c->bulk_fp32 < 0 || c->bulk_fp32 > 1) return FVHOE_ERR_UNSUPPORTED;
   if (c->riemann < 0 || c->riemann > 2 || c->limiter < 0 || c->limiter > 3) return FVHOE_ERR_ARG;
+  if (c->bc[0] < 0 || c->bc[0] > 3 || c->bc[1] < 0 || c->bc[1] > 3) return FVHOE_ERR_UNSUPPORTED;  // no slabs
   P->count = count;
   P->p = c->p[0];
   P->tile = tile;

@@ -358,7 +358,59 @@ __host__ __device__ __forceinline__ double from_ordered_bits(unsigned long long 
 }
 
 // boundary kinds
-enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_FREE = 3 };
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_FREE = 3, BC_DIRICHLET = 4 };
+
+// Ghost values of the Dirichlet sides ("dirichlet" and "ic" boundaries,
+// fvhoe/boundary_conditions.py:18-57,306-348), one slab per side s (x lo, x hi,
+// y lo, y hi, z lo, z hi), null elsewhere.  The slab of an axis-d side has
+// g[d] layers along d and n[e] + 2*pad[e] cells along each other axis e, C
+// order (v, axis 0, axis 1, axis 2); layer k of a lo side is cell k - g[d], of
+// a hi side cell n[d] + k (the slab coordinates of fvhoe/fv.py:53-84).
+struct Slabs {
+  const double* s[6];
+  int g[3];
+  int pad[3];
+};
+
+// Value of variable v at interior-relative cell c (ghosts allowed), composed
+// like the reference's fill (x, y, z slabs in turn, so a later axis decides a
+// corner): a Dirichlet side of the last axis on which c is a ghost gives its
+// slab value directly; the other kinds map c into the interior first.  at(v,
+// c0, c1, c2) reads an interior cell.  Reflective ghosts negate the normal
+// momentum of every axis mapped after the slab that gave the value.
+template <typename At>
+__device__ __forceinline__ double bc_fetch3(const int n[3], const int bc[6], const Slabs& sl, int v, int c0, int c1,
+                                            int c2, At&& at) {
+  int c[3] = {c0, c1, c2};
+  bool flip[3] = {false, false, false};
+  for (int d = 2; d >= 0; --d) {
+    if (c[d] >= 0 && c[d] < n[d]) continue;
+    const int side = 2 * d + (c[d] >= 0 ? 1 : 0);
+    if (bc[side] == BC_DIRICHLET) {
+      const int layer = c[d] < 0 ? sl.g[d] + c[d] : c[d] - n[d];
+      long long idx = v;
+      for (int e = 0; e < 3; ++e) {
+        const int ext = e == d ? sl.g[d] : n[e] + 2 * sl.pad[e];
+        idx = idx * ext + (e == d ? layer : c[e] + sl.pad[e]);
+      }
+      double x = sl.s[side][idx];
+      for (int e = d + 1; e < 3; ++e)
+        if (flip[e] && v == e + 1) x = -x;
+      return x;
+    }
+    const int lo = bc[2 * d], hi = bc[2 * d + 1];
+    if (c[d] < 0) {
+      c[d] = lo == BC_PERIODIC ? n[d] + c[d] : lo == BC_FREE ? 0 : -1 - c[d];
+      flip[d] = lo == BC_REFLECTIVE;
+    } else {
+      c[d] = hi == BC_PERIODIC ? c[d] - n[d] : hi == BC_FREE ? n[d] - 1 : 2 * n[d] - 1 - c[d];
+      flip[d] = hi == BC_REFLECTIVE;
+    }
+  }
+  double x = at(v, c[0], c[1], c[2]);
+  if ((flip[0] && v == 1) || (flip[1] && v == 2) || (flip[2] && v == 3)) x = -x;
+  return x;
+}
 
 // Source interior index and momentum sign for padded index a on an axis of
 // interior size n, pad g, boundary kinds (lo, hi) (fvhoe/boundary_conditions.py:60-151).

@@ -81,7 +81,17 @@ struct Geo {
   int ge0, ge1;      // ghost width of W / WS per axis (0 on an inactive axis)
   int bc[4];         // x lo, x hi, y lo, y hi
   double cx, cy;     // -(1/hx), -(1/hy)
+  int dirichlet;     // 1 if a side is FVHOE_BC_DIRICHLET: ghosts go through bc_fetch3 and the slabs
+  Slabs sl;
 };
+
+// ghost value through the Dirichlet-aware composition (2-D grid, z of extent 1)
+template <typename At>
+__device__ __forceinline__ double geo_fetch(const Geo& g, int v, int cx, int cy, At&& at) {
+  const int n[3] = {g.n0, g.n1, 1};
+  const int bc[6] = {g.bc[0], g.bc[1], g.bc[2], g.bc[3], BC_PERIODIC, BC_PERIODIC};
+  return bc_fetch3(n, bc, g.sl, v, cx, cy, 0, [&](int vv, int a, int b, int) { return at(vv, a, b); });
+}
 
 __device__ __forceinline__ int cell_map(int c, int n, int lo, int hi, bool* flip) {
   *flip = false;
@@ -108,6 +118,7 @@ struct SrcU {
     return u[(v * g.n0 + sx) * g.n1 + sy];
   }
   __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
+    if (g.dirichlet) return geo_fetch(g, v, cx, cy, [&](int vv, int a, int b) { return at(g, vv, a, b); });
     bool fx, fy;
     const int sx = cell_map(cx, g.n0, g.bc[0], g.bc[1], &fx);
     const int sy = cell_map(cy, g.n1, g.bc[2], g.bc[3], &fy);
@@ -135,6 +146,7 @@ struct SrcUstar {
     return u[(v * g.n0 + sx) * g.n1 + sy] + dt[0] * L;
   }
   __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
+    if (g.dirichlet) return geo_fetch(g, v, cx, cy, [&](int vv, int a, int b) { return at(g, vv, a, b); });
     bool fx, fy;
     const int sx = cell_map(cx, g.n0, g.bc[0], g.bc[1], &fx);
     const int sy = cell_map(cy, g.n1, g.bc[2], g.bc[3], &fy);
@@ -179,6 +191,10 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   const int cx0 = oa - g.ge0 - (RCX + RTX), cy0 = ob - g.ge1 - (RCY + RTY);
   const int tid = threadIdx.x;
   const bool inside = cx0 >= 0 && cx0 + HX0 <= g.n0 && cy0 >= 0 && cy0 + HY0 <= g.n1;
+  // the last tiles overhang W; their extra rows / columns are never stored, and their fetches are
+  // clamped to the ghost box W needs so every source index stays in its array
+  const int xlo = -(g.ge0 + RCX + RTX), xhi = g.n0 + g.ge0 + RCX + RTX - 1;
+  const int ylo = -(g.ge1 + RCY + RTY), yhi = g.n1 + g.ge1 + RCY + RTY - 1;
 
   // all five variables' tile elements are fetched up front (one memory latency per block)
   constexpr int NE = (HX0 * HY0 + NT - 1) / NT;
@@ -191,7 +207,8 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
       if (tid + k * NT < HX0 * HY0) {
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-          pre[v][k] = T(inside ? src.at(g, v, cx0 + r, cy0 + c) : src.get(g, v, cx0 + r, cy0 + c));
+          pre[v][k] = T(inside ? src.at(g, v, cx0 + r, cy0 + c)
+                               : src.get(g, v, min(max(cx0 + r, xlo), xhi), min(max(cy0 + c, ylo), yhi)));
       }
       r += DR;
       c += DC;
@@ -1030,13 +1047,17 @@ struct TimedLaunch {
 };
 
 struct Layout {
-  int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred;
+  int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred, dirichlet;
   size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offPre, offPreList, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int TX2 = 16, TY2 = 32, TX1 = 256;
+
+// deepest ghost read from u: W's ghosts plus the prims stencils' reach
+// (the candidate's 3 ghosts and the fallback's MUSCL stencil reach less)
+int ghost_depth(int p) { return (flux_ghosts(p) > 3 ? flux_ghosts(p) : 3) + ereach_c(p) + ereach_t(p); }
 
 int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   if (!c) return FVHOE_ERR_ARG;
@@ -1047,6 +1068,23 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   if (c->p[0] < 1 || c->p[0] > 8) return FVHOE_ERR_UNSUPPORTED;
   if (c->riemann < 0 || c->riemann > 2 || c->limiter < 0 || c->limiter > 3) return FVHOE_ERR_ARG;
   if (c->bulk_fp32 < 0 || c->bulk_fp32 > 1) return FVHOE_ERR_ARG;
+  bool dirichlet = false;
+  for (int k = 0; k < 4; ++k) {
+    if (c->bc[k] < 0 || c->bc[k] > BC_DIRICHLET) return FVHOE_ERR_ARG;
+    if (c->bc[k] == BC_DIRICHLET) {
+      if (k >= 2 && !ay) return FVHOE_ERR_ARG;
+      if (!c->slab[k]) return FVHOE_ERR_ARG;
+      dirichlet = true;
+    }
+  }
+  {  // the slabs must cover the deepest ghost any kernel reads
+    const int need = ghost_depth(c->p[0]);
+    if (dirichlet && (c->slab_g < need || (ay && c->slab_pad[1] < need) || (ay && c->slab_pad[0] < need) ||
+                      c->slab_pad[2] != 0))
+      return FVHOE_ERR_ARG;
+  }
+  if (dirichlet && pred) return FVHOE_ERR_UNSUPPORTED;  // the predictor's features map ghosts into the interior
+  L->dirichlet = dirichlet;
   L->two_d = ay;
   L->p = c->p[0];
   L->fp32 = c->bulk_fp32;
@@ -1115,6 +1153,11 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   g.ge0 = L.GE;
   g.ge1 = TWO_D ? L.GE : 0;
   g.bc[0] = c->bc[0]; g.bc[1] = c->bc[1]; g.bc[2] = c->bc[2]; g.bc[3] = c->bc[3];
+  g.dirichlet = L.dirichlet;
+  for (int k = 0; k < 6; ++k) g.sl.s[k] = k < 4 && c->bc[k] == BC_DIRICHLET ? c->slab[k] : nullptr;
+  g.sl.g[0] = g.sl.g[1] = c->slab_g;
+  g.sl.g[2] = 0;
+  g.sl.pad[0] = c->slab_pad[0]; g.sl.pad[1] = c->slab_pad[1]; g.sl.pad[2] = 0;
   g.cx = -(1.0 / c->h[0]);
   g.cy = -(1.0 / c->h[1]);
 
@@ -1345,6 +1388,13 @@ int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, d
   return stage_entry(cfg, u, dudt, dt_stage, trouble, workspace, rk, nullptr, stream);
 }
 
+int fvhoe_stage_ghost_depth(const FvhoeStageConfig* cfg, int* depth) {
+  if (!cfg || !depth) return FVHOE_ERR_ARG;
+  if (cfg->p[0] < 1 || cfg->p[0] > 8) return FVHOE_ERR_UNSUPPORTED;
+  *depth = ghost_depth(cfg->p[0]);
+  return FVHOE_OK;
+}
+
 int fvhoe_event_create(void** event) {
   if (!event) return FVHOE_ERR_ARG;
   hipEvent_t e;
@@ -1379,7 +1429,7 @@ int fvhoe_pred_features(const FvhoeStageConfig* cfg, const void* workspace, floa
   const int st = make_layout(cfg, &L);
   if (st != FVHOE_OK) return st;
   if (!workspace || !X) return FVHOE_ERR_ARG;
-  if (!L.two_d || !cfg->a_posteriori || cfg->nad_mode != 0) return FVHOE_ERR_UNSUPPORTED;
+  if (!L.two_d || !cfg->a_posteriori || cfg->nad_mode != 0 || L.dirichlet) return FVHOE_ERR_UNSUPPORTED;
   const char* ws = (const char*)workspace;
   Geo g{};
   g.n0 = L.n0;

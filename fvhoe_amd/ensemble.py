@@ -46,6 +46,8 @@ class Ensemble:
         self.x_domain = x
         self.h = [(x[1] - x[0]) / n for n in self.n]
         self.bc = BoundaryCondition(x=bc.x) if bc is not None else BoundaryCondition(x="outflow")
+        if self.bc.has_slabs():
+            raise NotImplementedError("the ensemble takes periodic / mirror / free boundaries")
         c = _lib.FvhoeStageConfig()
         c.n[:] = [1, 1, 1]
         c.p[:] = [p, 0, 0]

@@ -7,8 +7,40 @@ NumPy -- and runs one HIP kernel per sweep.
 
 from __future__ import annotations
 
+from itertools import product
+
 from fvhoe_amd import _lib
+from fvhoe_amd.initial_conditions import uniform_mesh
 from fvhoe_amd.stencils import get_stencil_size
+
+
+def fv_uniform_meshgen(n, x=(0, 1), y=(0, 1), z=(0, 1), slab_thickness=(0, 0, 0)):
+    """Cell-centre meshes (host NumPy, set-up only; fvhoe/fv.py:12-86).
+
+    With slab_thickness (0, 0, 0): X, Y, Z of shape n.  Otherwise
+    ((X, Y, Z), slabs) where slabs["xl"] etc. hold the centres of the ghost
+    slabs: slab "xl" has slab_thickness[0] cells left of x[0] along x and the
+    interior padded by the other axes' thicknesses along y and z."""
+    if tuple(slab_thickness) == (0, 0, 0) and not isinstance(slab_thickness, list):
+        return tuple(uniform_mesh(n, x, y, z))
+    inner = tuple(uniform_mesh(n, x, y, z))
+    h = ((x[1] - x[0]) / n[0], (y[1] - y[0]) / n[1], (z[1] - z[0]) / n[2])
+    g = tuple(slab_thickness)
+    dom = (x, y, z)
+    slabs = {}
+    for dim, pos in product("xyz", "lr"):
+        d = "xyz".index(dim)
+        cells, ext = [], []
+        for e in range(3):
+            lo, hi = dom[e]
+            if e == d:
+                cells.append(g[e])
+                ext.append((lo - h[e] * g[e], lo) if pos == "l" else (hi, hi + h[e] * g[e]))
+            else:
+                cells.append(n[e] + 2 * g[e])
+                ext.append((lo - h[e] * g[e], hi + h[e] * g[e]))
+        slabs[dim + pos] = tuple(uniform_mesh(tuple(cells), *ext))
+    return inner, slabs
 
 
 def _as4(u):

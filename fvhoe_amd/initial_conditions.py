@@ -98,6 +98,26 @@ class Sinus(InitialCondition):
         return f
 
 
+class ShuOsher1D(InitialCondition):
+    """Shu-Osher shock / entropy-wave problem on [0, 10] along dim
+    (fvhoe/initial_conditions.py:299-327): a Mach 3 shock at 1 running into
+    rho = 1 + 0.2 sin(5 (x - 5))."""
+
+    def base_ic(self, dim="x"):
+        def f(x, y=None, z=None):
+            r = {"x": x, "y": y, "z": z}[dim] - 5
+            v = np.where(r < -4, 2.629369, 0)
+            out = np.empty((5,) + r.shape)
+            out[0] = np.where(r < -4, 3.857143, 1 + 0.2 * np.sin(5 * r))
+            out[1] = v if dim == "x" else 0
+            out[2] = v if dim == "y" else 0
+            out[3] = v if dim == "z" else 0
+            out[4] = np.where(r < -4, 10.33333, 1)
+            return out
+
+        return f
+
+
 def uniform_mesh(n, x=(0, 1), y=(0, 1), z=(0, 1)):
     """Cell centres (fvhoe/fv.py:12-51, slab_thickness 0)."""
     mids = []

@@ -62,6 +62,12 @@ class ODE(ABC):
     def step_helper_function(self):
         pass
 
+    def _before_step(self) -> None:
+        """Host work before a step that needs the host clock (time-dependent boundaries)."""
+
+    def _needs_host_clock(self) -> bool:
+        return False
+
     def read_snapshots(self) -> bool:
         return False
 
@@ -141,6 +147,7 @@ class ODE(ABC):
         """fvhoe/ode.py:183-193: one step, dt read back to advance the host clock."""
         t_dev = self._scal[5:6]
         _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
+        self._before_step()
         dt = float(self._device_step(target_time, t_dev).item())
         self.t += dt
         self.timestamps.append(self.t)
@@ -152,6 +159,9 @@ class ODE(ABC):
         step after the first is captured once in a HIP graph (per integrator) and replayed, which
         removes the host launch work between kernels; the kernels and their arguments are the
         same as in the eager steps, so the results are identical."""
+        if self._needs_host_clock():
+            raise NotImplementedError("time-dependent boundary values need the host clock at every step: "
+                                      "use take_step / integrate")
         t_dev = self._scal[5:6]
         _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
         # the capture needs the steady state of one completed step (workspace allocated, counters reset)

@@ -26,7 +26,7 @@ import numpy as np
 
 from fvhoe_amd import _lib
 from fvhoe_amd.boundary_conditions import BoundaryCondition
-from fvhoe_amd.fv import conservative_interpolation, interpolate_cell_centers, interpolate_fv_averages
+from fvhoe_amd.fv import conservative_interpolation, fv_uniform_meshgen, interpolate_cell_centers, interpolate_fv_averages
 from fvhoe_amd.fv import transverse_reconstruction
 from fvhoe_amd.hydro import HydroState, compute_primitives, hydro_dt_device
 from fvhoe_amd.initial_conditions import InitialCondition, fv_average, primitives_to_conservatives, uniform_mesh
@@ -113,17 +113,35 @@ class EulerSolver(ODE):
         self.hydro_state = HydroState(passive_scalars=passives)
         self.nvars = self.hydro_state.nvars
         ic = w0.build_ic(passives, w0_passives)
+        # self.u0: pointwise conservative IC f(x, y, z) where the IC defines one (fvhoe/solver.py:278-299)
         if conservative_ic and fv_ic:
+            self.u0 = NotImplemented
             u0 = ic(self.X, self.Y, self.Z)
         elif fv_ic:
             raise ValueError("Can't have finite volume averages without conservative IC.")
         else:
-            point = ic if conservative_ic else (lambda xx, yy, zz: primitives_to_conservatives(ic(xx, yy, zz), gamma))
+            point = ic if conservative_ic else (lambda x, y, z: primitives_to_conservatives(ic(x, y, z), gamma))
+            self.u0 = point
             u0 = fv_average(point, self.X, self.Y, self.Z, self.h, self.p)
         super().__init__(u0, progress_bar=progress_bar)
         self.f_evaluation_count = 0
 
-        self.bc = BoundaryCondition() if bc is None else BoundaryCondition(x=bc.x, y=bc.y, z=bc.z)
+        # boundaries (fvhoe/solver.py:314-362): slab kinds read their ghost values at the centres of a
+        # slab_buffer_size-thick ghost buffer; "ic" sides take the pointwise initial condition
+        self.slab_buffer_size = slab_buffer_size
+        if bc is None:
+            self.bc = BoundaryCondition()
+        else:
+            slab_coords = None
+            if bc.has_slabs():
+                sizes = tuple(slab_buffer_size if a else 0 for a in self.active)
+                _, slab_coords = fv_uniform_meshgen(self.n, x, y, z, slab_thickness=sizes)
+            self.bc = BoundaryCondition(x=bc.x, y=bc.y, z=bc.z, x_value=bc.x_value, y_value=bc.y_value,
+                                        z_value=bc.z_value, slab_coords=slab_coords)
+            for d in "xyz":
+                for j, pos in enumerate("lr"):
+                    if getattr(self.bc, d)[j] == "ic":
+                        self.bc.reset_value(d, pos, self.u0)
         self.a_posteriori_slope_limiting = a_posteriori_slope_limiting
         self.slope_limiter = slope_limiter
         self.force_trouble = force_trouble
@@ -160,6 +178,8 @@ class EulerSolver(ODE):
         if predictor is not None:
             if not (self.fused and self.ndim == 2 and self.xdim and self.ydim and a_posteriori_slope_limiting):
                 raise NotImplementedError("the predictor runs in the fused 2-D a-posteriori stage")
+            if self.bc.has_slabs():
+                raise NotImplementedError("the predictor's features take periodic / mirror / free boundaries only")
             self.predictor_stats = _lib.zeros((3,), dtype=_lib.torch().int64)
             if self.predictor_overlap:
                 self._overlap = _lib.OverlapStreams()
@@ -190,7 +210,31 @@ class EulerSolver(ODE):
         c.fixed_dt_on = int(self.fixed_dt is not None)
         c.fixed_dt = float(self.fixed_dt or 0.0)
         c.bulk_fp32 = int(self.bulk_precision == "fp32")
+        self._set_stage_slabs(c)
         return c
+
+    def _set_stage_slabs(self, c) -> None:
+        """Ghost slabs of the fused stage's Dirichlet / "ic" sides, as deep as the stage reads, evaluated at
+        the solver's time (the reference fills every stage's ghosts at self.t, fvhoe/solver.py:784)."""
+        if not self.bc.has_slabs() or self.zdim or not (1 <= self.p[0] <= 8):
+            return
+        depth = _lib.ctypes.c_int(0)
+        _lib.call("fvhoe_stage_ghost_depth", _lib.ctypes.byref(c), _lib.ctypes.byref(depth))
+        gw = (depth.value if self.xdim else 0, depth.value if self.ydim else 0, 0)
+        slabs = self.bc.device_slabs(gw, 5, self.t)
+        for k in range(6):
+            c.slab[k] = slabs[k].data_ptr() if slabs[k] is not None else None
+        c.slab_g = depth.value
+        c.slab_pad[:] = [gw[0], gw[1], 0]
+        self._slab_t = self.t
+
+    def _needs_host_clock(self) -> bool:
+        return self.bc.time_dependent()
+
+    def _before_step(self) -> None:
+        """Refresh time-dependent boundary slabs of the fused stage to the step's start time."""
+        if self.fused and self.bc.time_dependent() and self._slab_t != self.t:
+            self._set_stage_slabs(self._cfg)
 
     # ------------------------------------------------------------------- stage
     def stage(self, u, k_out, dt_out) -> None:

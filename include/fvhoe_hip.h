@@ -35,7 +35,8 @@ enum {
 enum { FVHOE_STENCIL_L = 0, FVHOE_STENCIL_C = 1, FVHOE_STENCIL_R = 2, FVHOE_STENCIL_T = 3 };
 enum { FVHOE_RS_UPWIND = 0, FVHOE_RS_LLF = 1, FVHOE_RS_HLLC = 2 };
 enum { FVHOE_LIM_NONE = 0, FVHOE_LIM_MINMOD = 1, FVHOE_LIM_MONCEN = 2 };
-enum { FVHOE_BC_PERIODIC = 0, FVHOE_BC_OUTFLOW = 1, FVHOE_BC_REFLECTIVE = 2, FVHOE_BC_FREE = 3 };
+enum { FVHOE_BC_PERIODIC = 0, FVHOE_BC_OUTFLOW = 1, FVHOE_BC_REFLECTIVE = 2, FVHOE_BC_FREE = 3,
+       FVHOE_BC_DIRICHLET = 4 /* "dirichlet" and "ic": ghost values from a slab */ };
 
 /* Runge-Kutta combinations (fvhoe/ode.py:245-298), dt read from device */
 enum {
@@ -95,6 +96,15 @@ int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, in
  * bc6 = kinds of (x lo, x hi, y lo, y hi, z lo, z hi). */
 int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
                    const int* bc6, void* stream);
+
+/* The same with FVHOE_BC_DIRICHLET sides ("dirichlet" / "ic",
+ * fvhoe/boundary_conditions.py:18-57,247-304,334-348).  slabs6[s] (device,
+ * NULL for the other sides) holds side s's ghost values, shaped as that
+ * side's ghost region of out: for an axis-d side nv x g_d along d x
+ * (n_e + 2*g_e) along each other axis e, C order (the reference's slab view).
+ * Later axes decide corners, as the reference's x, y, z slab order does. */
+int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                             const int* bc6, const double* const* slabs6, void* stream);
 
 /* EulerSolver.euler_equation (fvhoe/solver.py:734-751); c* = -(1/h*); active bit d = axis d+1 */
 int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
@@ -173,11 +183,23 @@ typedef struct FvhoeStageConfig {
   double fixed_dt;
   int bulk_fp32;        /* 1: FP32 sweeps, Riemann solves and W/F/G storage; FP64 pressure
                            recovery, trouble tests, divergence and RK accumulation */
+  /* ghost slabs of the FVHOE_BC_DIRICHLET sides, laid out as for
+   * fvhoe_bc_apply_dirichlet (nv = 5, slab_pad[2] = 0); slab_g and the pads
+   * of the active axes must reach the stage's deepest ghost
+   * (fvhoe_stage_ghost_depth).  Contents are read at launch time, so a
+   * time-dependent boundary refreshes them between steps. */
+  const double* slab[6];
+  int slab_g;
+  int slab_pad[3];
 } FvhoeStageConfig;
 
 /* 1 if fvhoe_stage handles cfg (1-D along x, or 2-D in x-y, nvars = 5,
  * simple face broadcast), else 0. */
 int fvhoe_stage_supported(const FvhoeStageConfig* cfg);
+
+/* Deepest ghost cell (cells beyond the interior) the fused stage reads from
+ * u for cfg: the slab depth its Dirichlet sides need. */
+int fvhoe_stage_ghost_depth(const FvhoeStageConfig* cfg, int* depth);
 
 /* Bytes of device workspace fvhoe_stage needs for cfg. */
 int fvhoe_stage_workspace_bytes(const FvhoeStageConfig* cfg, unsigned long long* bytes);

@@ -587,12 +587,67 @@ def _convex_2d(t: np.ndarray):
 # ---------------------------------------------------------------------------
 
 
-def apply_bc(u: np.ndarray, gw, bc) -> np.ndarray:
+def fv_uniform_meshgen(n, x=(0, 1), y=(0, 1), z=(0, 1), slab_thickness=(0, 0, 0)):
+    """Cell centres, and with a slab thickness the ghost-slab centres per side
+    (fvhoe/fv.py:12-86): slab "xl" spans slab_thickness[0] cells left of x[0]
+    and the interior padded by the other axes' thicknesses."""
+    def mesh(m, xx, yy, zz):
+        mids = []
+        for (a, b), k in zip((xx, yy, zz), m):
+            bnd = np.linspace(a, b, k + 1)
+            mids.append(0.5 * (bnd[1:] + bnd[:-1]))
+        return tuple(np.meshgrid(*mids, indexing="ij"))
+
+    inner = mesh(n, x, y, z)
+    if tuple(slab_thickness) == (0, 0, 0):
+        return inner
+    h = ((x[1] - x[0]) / n[0], (y[1] - y[0]) / n[1], (z[1] - z[0]) / n[2])
+    g = tuple(slab_thickness)
+    dom = (x, y, z)
+    slabs = {}
+    for d, dim in enumerate("xyz"):
+        for pos in "lr":
+            cells, ext = [], []
+            for e in range(3):
+                lo, hi = dom[e]
+                if e == d:
+                    cells.append(g[e])
+                    ext.append((lo - h[e] * g[e], lo) if pos == "l" else (hi, hi + h[e] * g[e]))
+                else:
+                    cells.append(n[e] + 2 * g[e])
+                    ext.append((lo - h[e] * g[e], hi + h[e] * g[e]))
+            slabs[dim + pos] = mesh(cells, *ext)
+    return inner, slabs
+
+
+def trim_slab(slab_coords, gw, axis: int, pos: str):
+    """The slab-centre coordinates of one side for ghost widths gw, cut from a
+    thicker buffer (fvhoe/boundary_conditions.py:240-304)."""
+    buf = (slab_coords["xl"][0].shape[0], slab_coords["yl"][0].shape[1], slab_coords["zl"][0].shape[2])
+    X, Y, Z = slab_coords["xyz"[axis] + pos]
+    cut = []
+    for e in range(3):
+        ex = buf[e] - gw[e]
+        if ex < 0:
+            raise ValueError(f"Cannot apply bc to region of thickness {gw[e]} with a buffer of thickness {buf[e]}")
+        if e == axis:
+            cut.append(slice(-gw[e], None) if pos == "l" else slice(None, gw[e]))
+        else:
+            cut.append(slice(ex or None, -ex or None))
+    cut = tuple(cut)
+    return X[cut], Y[cut], Z[cut]
+
+
+def apply_bc(u: np.ndarray, gw, bc, values=None, slab_coords=None, t=None) -> np.ndarray:
     """Pad u by gw per axis; slabs filled in x(l,r), y(l,r), z(l,r) order.
 
     bc: ((xl, xr), (yl, yr), (zl, zr)) with types periodic | outflow |
-    reflective | free.  outflow = mirror, reflective = mirror with the normal
-    momentum negated, free = nearest interior cell (fvhoe/boundary_conditions.py:357-365)."""
+    reflective | free | dirichlet | ic.  outflow = mirror, reflective =
+    mirror with the normal momentum negated, free = nearest interior cell
+    (fvhoe/boundary_conditions.py:357-365); dirichlet / ic = the side's value
+    in values (same nesting as bc): a per-variable array broadcast over the
+    slab, or a callable of the slab centres (x, y, z, plus t for dirichlet)
+    (fvhoe/boundary_conditions.py:18-57,334-348)."""
     out = np.pad(u, [(0, 0)] + [(g, g) for g in gw], mode="constant", constant_values=np.nan)
     for i in range(3):
         g = gw[i]
@@ -628,6 +683,19 @@ def apply_bc(u: np.ndarray, gw, bc) -> np.ndarray:
                     out[sl(0, g)] = out[sl(g, g + 1)]
                 else:
                     out[sl(L - g, L)] = out[sl(L - g - 1, L - g)]
+            elif kind in ("dirichlet", "ic"):
+                view = out[sl(0, g)] if pos == "l" else out[sl(L - g, L)]
+                X, Y, Z = trim_slab(slab_coords, gw, i, pos)
+                val = values[i][j]
+                if callable(val):
+                    kw = dict(x=X, y=Y, z=Z)
+                    if kind == "dirichlet":
+                        kw["t"] = t
+                    view[...] = val(**kw)
+                elif isinstance(val, np.ndarray):
+                    view[...] = np.ones_like(view) * val.reshape(-1, 1, 1, 1)
+                else:
+                    raise ValueError(f"Invalid boundary of type '{val}'.")
             elif kind is None:
                 pass
             else:
@@ -665,6 +733,9 @@ class Scheme:
     SED_tolerance: float = 1e-10
     convex: bool = False
     csq_floor: float = 1e-10
+    bc_values: tuple = None  # per side, for dirichlet / ic sides (same nesting as bc)
+    slab_buffer_size: int = 30
+    t: float = 0.0  # the step's start time: every stage fills ghosts at it (fvhoe/solver.py:784)
 
     def __post_init__(self):
         self.h = tuple((b - a) / n for (a, b), n in zip(self.domain, self.n))
@@ -674,13 +745,17 @@ class Scheme:
         if self.PAD is None:
             self.PAD = {"rho": (0.0, np.inf), "P": (0.0, np.inf)}  # fvhoe/solver.py:392-396
         self.bc = tuple(tuple(b) if isinstance(b, (tuple, list)) else (b, b) for b in self.bc)
+        self.slab_coords = None
+        if any(k in ("dirichlet", "ic") for b in self.bc for k in b):
+            sizes = tuple(self.slab_buffer_size if a else 0 for a in self.active)
+            _, self.slab_coords = fv_uniform_meshgen(self.n, *self.domain, slab_thickness=sizes)
 
 
 def interpolate_primitives_from_conservatives(s: Scheme, u, p, gw=(0, 0, 0), fv_average=True):
     """fvhoe/solver.py:753-792."""
     cost = [(2 if fv_average else 1) * stencil_reach(pi) for pi in p]
     g = tuple(gw[i] + cost[i] for i in range(3))
-    u_bc = apply_bc(u, g, s.bc)
+    u_bc = apply_bc(u, g, s.bc, s.bc_values, s.slab_coords, s.t)
     w = compute_primitives(interpolate_cell_centers(u_bc, p), s.gamma)
     return interpolate_fv_averages(w, p) if fv_average else w
 
@@ -797,6 +872,7 @@ def hit_target(t: float, dt: float, target: Optional[float]) -> float:
 def step(s: Scheme, u, t: float, integrator: str = "ssprk3", target: Optional[float] = None,
          logs: Optional[list] = None):
     """One step of the reference's k-form steppers; returns (dt, unext)."""
+    s.t = t
 
     def f(uu):
         lg = StageLog() if logs is not None else None
