@@ -1,10 +1,17 @@
-"""Initial conditions: the reference's IC interface plus the workloads the
-benchmarks and parity tests run on.
+"""Initial conditions: the reference's IC interface and IC classes, plus the
+workloads the benchmarks and parity tests run on.
 
 `InitialCondition` keeps the reference's abstract interface
 (fvhoe/initial_conditions.py:11-81): `base_ic(*args, **kwargs)` returns
 f(x, y, z) -> (5, nx, ny, nz) primitive point values, and `build_ic` appends
-passive scalars.  ICs are host-side set-up that runs once; they are NumPy.
+passive scalars.  The IC functions, `fv_average` and
+`primitives_to_conservatives` take NumPy arrays or float64 torch tensors on
+the GPU alike (`xp_of` picks the array namespace), so a 256^3 or 2048^2
+set-up can be built on the device (`EulerSolver(..., ic_device=True)`).  On
+the host they are the reference's arithmetic bit for bit (golden vectors in
+tests/golden/ics.npz); on the device every +, -, *, /, comparison and sqrt
+is the same IEEE operation, and sin / exp are the device library's (within a
+few ulp of NumPy's).
 
 The workload builders at the bottom return conservative cell averages
 directly (the reference's `conservative_ic=True, fv_ic=True` form,
@@ -14,21 +21,135 @@ fvhoe/solver.py:284-287), so seeded per-cell noise is well defined.
 from __future__ import annotations
 
 from abc import ABC, abstractmethod
+from numbers import Number
 from typing import Callable, Dict, Tuple
 
 import numpy as np
 
 
-def primitives_to_conservatives(w: np.ndarray, gamma: float) -> np.ndarray:
+# ---------------------------------------------------------------------------
+# array namespaces
+# ---------------------------------------------------------------------------
+
+
+class _NumpyNS:
+    where = staticmethod(np.where)
+    sin = staticmethod(np.sin)
+    exp = staticmethod(np.exp)
+    sqrt = staticmethod(np.sqrt)
+    square = staticmethod(np.square)
+    abs = staticmethod(np.abs)
+    logical_and = staticmethod(np.logical_and)
+    logical_not = staticmethod(np.logical_not)
+    zeros_like = staticmethod(np.zeros_like)
+    div = staticmethod(np.divide)
+
+    @staticmethod
+    def empty(shape, like):
+        return np.empty(shape)
+
+    @staticmethod
+    def ones_bool(like):
+        return np.ones_like(like, dtype=bool)
+
+    @staticmethod
+    def mean(a) -> float:
+        return np.mean(a)
+
+
+class _TorchNS:
+    """The same calls on float64 GPU tensors; scalars become float64 (torch.where would make Python
+    floats float32), division by a scalar divides by a device tensor (torch multiplies by the
+    reciprocal of a host scalar, which is not IEEE division), and means are taken on the host with
+    NumPy's summation."""
+
+    def __init__(self, torch, device):
+        self.t, self.dev = torch, device
+
+    def _f64(self, a):
+        if isinstance(a, self.t.Tensor):
+            return a if a.dtype == self.t.bool else a.to(self.t.float64)
+        return self.t.tensor(float(a), dtype=self.t.float64, device=self.dev)
+
+    def where(self, c, a, b):
+        return self.t.where(c, self._f64(a), self._f64(b))
+
+    def sin(self, a):
+        return self.t.sin(a)
+
+    def exp(self, a):
+        return self.t.exp(a)
+
+    def sqrt(self, a):
+        return self.t.sqrt(a)
+
+    def square(self, a):
+        return self.t.square(a)
+
+    def abs(self, a):
+        return self.t.abs(a)
+
+    def logical_and(self, a, b):
+        return self.t.logical_and(self._bool(a), self._bool(b))
+
+    def logical_not(self, a):
+        return self.t.logical_not(a)
+
+    def _bool(self, a):
+        return a if isinstance(a, self.t.Tensor) else self.t.tensor(bool(a), device=self.dev)
+
+    def zeros_like(self, a):
+        return self.t.zeros_like(a)
+
+    def div(self, a, b):
+        return a / self._f64(b)
+
+    def empty(self, shape, like):
+        return self.t.empty(tuple(shape), dtype=self.t.float64, device=self.dev)
+
+    def ones_bool(self, like):
+        return self.t.ones_like(like, dtype=self.t.bool)
+
+    def mean(self, a) -> float:
+        return np.mean(a.cpu().numpy())
+
+
+def xp_of(a):
+    """The array namespace of a (NumPy for anything that is not a torch tensor)."""
+    if type(a).__module__.startswith("torch"):
+        import torch
+
+        return _TorchNS(torch, a.device)
+    return _NumpyNS
+
+
+def primitives_to_conservatives(w, gamma: float):
     """m = rho v, E = P/(gamma-1) + 0.5 sum(m v) in the reference's operation
-    order (fvhoe/hydro.py:173-196); host-side, used only to build ICs."""
+    order (fvhoe/hydro.py:173-196; the sum over the three components runs
+    left to right as NumPy's reduction over the leading axis does); set-up
+    only.  Device tensors go through the library's compute_conservatives
+    kernel (the same operations, IEEE division)."""
+    if type(w).__module__.startswith("torch"):
+        from fvhoe_amd import _lib
+
+        wc = w.contiguous()
+        u = _lib.torch().empty_like(wc)
+        _lib.call("fvhoe_compute_conservatives", _lib.ptr(wc), _lib.ptr(u), int(wc.shape[0]), int(wc[0].numel()),
+                  float(gamma), _lib.stream())
+        return u
     u = np.empty_like(w)
     u[0] = w[0]
     u[1:4] = w[0] * w[1:4]
-    u[4] = w[4] / (gamma - 1) + 0.5 * np.sum(u[1:4] * w[1:4], axis=0)
+    mv = u[1:4] * w[1:4]
+    u[4] = w[4] / (gamma - 1) + 0.5 * ((mv[0] + mv[1]) + mv[2])
     if w.shape[0] > 5:
         u[5:] = w[0] * w[5:]
     return u
+
+
+# ---------------------------------------------------------------------------
+# the IC interface and classes (fvhoe/initial_conditions.py)
+# ---------------------------------------------------------------------------
 
 
 class InitialCondition(ABC):
@@ -49,7 +170,7 @@ class InitialCondition(ABC):
             raise ValueError(f"Missing IC definitions for passive scalars: {missing}")
 
         def ic(x, y, z):
-            out = np.empty((5 + len(passive_scalars),) + x.shape)
+            out = xp_of(x).empty((5 + len(passive_scalars),) + tuple(x.shape), x)
             out[:5] = core(x, y, z)
             for i, name in enumerate(passive_scalars):
                 out[5 + i] = passive_ic_funcs[name](x, y, z)
@@ -58,42 +179,91 @@ class InitialCondition(ABC):
         return ic
 
 
+def _fill(xp, shape, like, rho, vx, vy, vz, P):
+    out = xp.empty((5,) + tuple(shape), like)
+    out[0], out[1], out[2], out[3], out[4] = rho, vx, vy, vz, P
+    return out
+
+
+class Sinus(InitialCondition):
+    """Sinusoidal density wave along the sum of the dims coordinates
+    (fvhoe/initial_conditions.py:84-125)."""
+
+    def base_ic(self, dims="x", rho_min_max=(1, 2), P=1, vx=1, vy=0, vz=0):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            lo, rng = rho_min_max[0], rho_min_max[1] - rho_min_max[0]
+            r = xp.zeros_like(x)
+            for d, c in zip("xyz", (x, y, z)):
+                r += c if d in dims else 0
+            return _fill(xp, r.shape, x, rng * (0.5 * xp.sin(2 * np.pi * r) + 0.5) + lo, vx, vy, vz, P)
+
+        return f
+
+
+class Square(InitialCondition):
+    """Square density wave: rho_max inside (0.25, 0.75) on every dims axis
+    (fvhoe/initial_conditions.py:128-170)."""
+
+    def base_ic(self, dims="x", rho_min_max=(1, 2), P=1, vx=1, vy=0, vz=0):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            inside = xp.ones_bool(x)
+            for d, r in zip("xyz", (x, y, z)):
+                inside &= xp.logical_and(r > 0.25, r < 0.75) if d in dims else True
+            return _fill(xp, x.shape, x, xp.where(inside, rho_min_max[1], rho_min_max[0]), vx, vy, vz, P)
+
+        return f
+
+
+class SlottedDisk(InitialCondition):
+    """Zalesak's slotted disk in solid rotation about (0.5, 0.5)
+    (fvhoe/initial_conditions.py:173-206)."""
+
+    def base_ic(self, rho_min_max=(1, 2), P=1):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            xc, yc = x - 0.5, y - 0.5
+            rsq = xp.square(xc) + xp.square(y - 0.75)
+            inside = rsq < 0.15 ** 2
+            inside &= xp.logical_not(xp.logical_and(xp.abs(xc) < 0.025, y < 0.85))
+            return _fill(xp, rsq.shape, x, xp.where(inside, rho_min_max[1], rho_min_max[0]), -yc, xc, 0, P)
+
+        return f
+
+
 class Shock1D(InitialCondition):
-    """Sod-type shock tube (fvhoe/initial_conditions.py:209-250)."""
+    """Sod-type shock tube along dim (fvhoe/initial_conditions.py:209-250)."""
 
     def base_ic(self, dim="x", position=0.5, rho_left_right=(1, 0.125), P_left_right=(1, 0.1),
                 v_left_right=(0, 0)):
         def f(x, y=None, z=None):
             r = {"x": x, "y": y, "z": z}[dim]
+            xp = xp_of(r)
             left = r < position
-            v = np.where(left, v_left_right[0], v_left_right[1])
-            out = np.empty((5,) + r.shape)
-            out[0] = np.where(left, rho_left_right[0], rho_left_right[1])
-            out[1] = v if dim == "x" else 0
-            out[2] = v if dim == "y" else 0
-            out[3] = v if dim == "z" else 0
-            out[4] = np.where(left, P_left_right[0], P_left_right[1])
-            return out
+            v = xp.where(left, v_left_right[0], v_left_right[1])
+            return _fill(xp, r.shape, r, xp.where(left, rho_left_right[0], rho_left_right[1]),
+                         v if dim == "x" else 0, v if dim == "y" else 0, v if dim == "z" else 0,
+                         xp.where(left, P_left_right[0], P_left_right[1]))
 
         return f
 
 
-class Sinus(InitialCondition):
-    """Sinusoidal density wave (fvhoe/initial_conditions.py:84-130)."""
+class DoubleShock1D(InitialCondition):
+    """Two interfaces along dim, three constant states
+    (fvhoe/initial_conditions.py:253-296)."""
 
-    def base_ic(self, dims="x", rho_min_max=(1, 2), P=1, vx=1, vy=0, vz=0):
+    def base_ic(self, dim="x", positions=(0.1, 0.9), rhos=(1, 1, 1), vs=(0, 0, 0), Ps=(10 ** 3, 10 ** -2, 10 ** 2)):
         def f(x, y=None, z=None):
-            rmin, rrange = rho_min_max[0], rho_min_max[1] - rho_min_max[0]
-            r = np.zeros_like(x)
-            for d, c in zip("xyz", (x, y, z)):
-                r += c if d in dims else 0
-            out = np.empty((5,) + r.shape)
-            out[0] = rrange * (0.5 * np.sin(2 * np.pi * r) + 0.5) + rmin
-            out[1] = vx
-            out[2] = vy
-            out[3] = vz
-            out[4] = P
-            return out
+            r = {"x": x, "y": y, "z": z}[dim]
+            xp = xp_of(r)
+
+            def three(vals):
+                return xp.where(r < positions[0], vals[0], xp.where(r < positions[1], vals[1], vals[2]))
+
+            v = three(vs)
+            return _fill(xp, r.shape, r, three(rhos), v if dim == "x" else 0, v if dim == "y" else 0,
+                         v if dim == "z" else 0, three(Ps))
 
         return f
 
@@ -106,16 +276,121 @@ class ShuOsher1D(InitialCondition):
     def base_ic(self, dim="x"):
         def f(x, y=None, z=None):
             r = {"x": x, "y": y, "z": z}[dim] - 5
-            v = np.where(r < -4, 2.629369, 0)
-            out = np.empty((5,) + r.shape)
-            out[0] = np.where(r < -4, 3.857143, 1 + 0.2 * np.sin(5 * r))
-            out[1] = v if dim == "x" else 0
-            out[2] = v if dim == "y" else 0
-            out[3] = v if dim == "z" else 0
-            out[4] = np.where(r < -4, 10.33333, 1)
+            xp = xp_of(r)
+            v = xp.where(r < -4, 2.629369, 0)
+            return _fill(xp, r.shape, r, xp.where(r < -4, 3.857143, 1 + 0.2 * xp.sin(5 * r)),
+                         v if dim == "x" else 0, v if dim == "y" else 0, v if dim == "z" else 0,
+                         xp.where(r < -4, 10.33333, 1))
+
+        return f
+
+
+class KelvinHelmholtz2D(InitialCondition):
+    """Shear layers at y = 0.25, 0.75 with a Gaussian-localized vy mode
+    (fvhoe/initial_conditions.py:331-366)."""
+
+    def base_ic(self, sigma=0.05 * np.sqrt(2), w0=0.1):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            inner = xp.logical_and(0.25 < y, y < 0.75)
+            vy = w0 * xp.sin(4 * np.pi * x) * (xp.exp(xp.div(-xp.square(y - 0.25), 2 * sigma * sigma))
+                                               + xp.exp(xp.div(-xp.square(y - 0.75), 2 * sigma * sigma)))
+            return _fill(xp, inner.shape, x, xp.where(inner, 2, 1), xp.where(inner, 0.5, -0.5), vy, 0, 2.5)
+
+        return f
+
+
+class DoubleMachReflection2D(InitialCondition):
+    """Mach 10 shock at 60 degrees through x = 1/6 at y = 0
+    (fvhoe/initial_conditions.py:369-399)."""
+
+    def base_ic(self, gamma=1.4):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            xs = x - xp.div(y, np.tan(np.pi / 3))
+            post = xs < 1 / 6
+            return _fill(xp, x.shape, x, xp.where(post, 8, gamma), xp.where(post, 7.145, 0),
+                         xp.where(post, -8.25 / 2, 0), 0, xp.where(post, 116.5, 1))
+
+        return f
+
+
+class ShockTube(InitialCondition):
+    """A ball (mode "tube") or box (mode "cube") of one state in another
+    (fvhoe/initial_conditions.py:402-481)."""
+
+    def base_ic(self, dims="x", center=0.0, radius=0.5, rho_in_out=(1, 0.125), P_in_out=(1, 0.1),
+                vx_in_out=(0, 0), vy_in_out=(0, 0), vz_in_out=(0, 0), mode="tube", x_cube=None, y_cube=None,
+                z_cube=None):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            if mode == "tube":
+                c = (center,) if isinstance(center, Number) else center
+                if len(c) != len(dims):
+                    raise ValueError("center must have the same length as dims")
+                c3 = [np.nan, np.nan, np.nan]
+                for i, d in enumerate(dims):
+                    c3["xyz".index(d)] = c[i]
+                xc = x - c3[0] if "x" in dims else 0
+                yc = y - c3[1] if "y" in dims else 0
+                zc = z - c3[2] if "z" in dims else 0
+                inside = xc ** 2 + yc ** 2 + zc ** 2 <= radius ** 2
+            elif mode == "cube":
+                if x_cube is None and y_cube is None and z_cube is None:
+                    raise ValueError("at least one cube dimension must be specified")
+                inside = xp.ones_bool(x)
+                for r, cube in ((x, x_cube), (y, y_cube), (z, z_cube)):
+                    if cube is not None:
+                        inside &= xp.logical_and(r >= cube[0], r <= cube[1])
+            return _fill(xp, x.shape, x, xp.where(inside, *rho_in_out), xp.where(inside, *vx_in_out),
+                         xp.where(inside, *vy_in_out), xp.where(inside, *vz_in_out), xp.where(inside, *P_in_out))
+
+        return f
+
+
+class SedovBlast(InitialCondition):
+    """Energy E1 deposited in the corner cell or the central 2^d cells of the
+    mesh the function is called on (fvhoe/initial_conditions.py:484-538)."""
+
+    def base_ic(self, mode="corner", dims="xy", rho0=1.0, E0=1e-5, E1=1.0):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            Nx, Ny, Nz = x.shape
+            hx = xp.mean(x[1:, :, :] - x[:-1, :, :])
+            hy = xp.mean(y[:, 1:, :] - y[:, :-1, :])
+            hz = xp.mean(z[:, :, 1:] - z[:, :, :-1])
+            Emax = E1
+            Emax /= 2 * hx if "x" in dims else 1
+            Emax /= 2 * hy if "y" in dims else 1
+            Emax /= 2 * hz if "z" in dims else 1
+            out = _fill(xp, x.shape, x, rho0, 0, 0, 0, E0)
+            if mode == "corner":
+                peak = (0, 0, 0)
+            elif mode == "center":
+                peak = tuple(slice(None) if d not in dims else slice(N // 2 - 1, N // 2 + 1)
+                             for d, N in zip("xyz", (Nx, Ny, Nz)))
+            out[4][peak] = Emax
             return out
 
         return f
+
+
+class AthenaBlast(InitialCondition):
+    """Pressure 10 inside r < 0.1 of the origin, 0.1 outside
+    (fvhoe/initial_conditions.py:541-570)."""
+
+    def base_ic(self, mode="corner", dims="xy", rho0=1.0, E0=1e-5, E1=1.0):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            r = xp.sqrt(xp.square(x) + xp.square(y))
+            return _fill(xp, r.shape, x, 1, 0, 0, 0, xp.where(r < 0.1, 10, 0.1))
+
+        return f
+
+
+# ---------------------------------------------------------------------------
+# meshes and cell averages
+# ---------------------------------------------------------------------------
 
 
 def uniform_mesh(n, x=(0, 1), y=(0, 1), z=(0, 1)):
@@ -127,28 +402,37 @@ def uniform_mesh(n, x=(0, 1), y=(0, 1), z=(0, 1)):
     return np.meshgrid(*mids, indexing="ij")
 
 
-def fv_average(f: Callable, X, Y, Z, h, p=(0, 0, 0)) -> np.ndarray:
+def fv_average(f: Callable, X, Y, Z, h, p=(0, 0, 0)):
     """Gauss-Legendre cell averages of f with ceil((p+1)/2) points per axis
-    (fvhoe/fv.py:227-277)."""
+    (fvhoe/fv.py:227-277); X, Y, Z NumPy or device tensors."""
     pw = []
     for pi in p:
         pts, wts = np.polynomial.legendre.leggauss(-(-(pi + 1) // 2))
         pw.append((pts / 2, wts / 2))
-    out = np.zeros_like(f(X, Y, Z))
+    out = xp_of(X).zeros_like(f(x=X, y=Y, z=Z))
     for xp, xw in zip(*pw[0]):
         for yp, yw in zip(*pw[1]):
             for zp, zw in zip(*pw[2]):
-                out += (xw * yw * zw) * f(X + xp * h[0], Y + yp * h[1], Z + zp * h[2])
+                out += (xw * yw * zw) * f(x=X + xp * h[0], y=Y + yp * h[1], z=Z + zp * h[2])
     return out
 
 
-def conservative_fv_ic(ic: InitialCondition, n, p, gamma, x=(0, 1), y=(0, 1), z=(0, 1)) -> np.ndarray:
+def device_mesh(n, x=(0, 1), y=(0, 1), z=(0, 1)):
+    """uniform_mesh as float64 device tensors (the same doubles)."""
+    from fvhoe_amd import _lib
+
+    torch = _lib.torch()
+    return tuple(torch.from_numpy(np.ascontiguousarray(a)).to(_lib.device()) for a in uniform_mesh(n, x, y, z))
+
+
+def conservative_fv_ic(ic: InitialCondition, n, p, gamma, x=(0, 1), y=(0, 1), z=(0, 1), device: bool = False):
     """Conservative cell averages the reference's EulerSolver builds from a
-    primitive IC (fvhoe/solver.py:292-306)."""
-    X, Y, Z = uniform_mesh(n, x, y, z)
+    primitive IC (fvhoe/solver.py:292-306); device=True builds them on the
+    GPU and returns a device tensor."""
+    X, Y, Z = device_mesh(n, x, y, z) if device else uniform_mesh(n, x, y, z)
     h = tuple((b - a) / k for (a, b), k in zip((x, y, z), n))
     w0 = ic.build_ic()
-    return fv_average(lambda xx, yy, zz: primitives_to_conservatives(w0(xx, yy, zz), gamma), X, Y, Z, h, p)
+    return fv_average(lambda x, y, z: primitives_to_conservatives(w0(x, y, z), gamma), X, Y, Z, h, p)
 
 
 # ---------------------------------------------------------------------------

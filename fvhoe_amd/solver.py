@@ -29,7 +29,8 @@ from fvhoe_amd.boundary_conditions import BoundaryCondition
 from fvhoe_amd.fv import conservative_interpolation, fv_uniform_meshgen, interpolate_cell_centers, interpolate_fv_averages
 from fvhoe_amd.fv import transverse_reconstruction
 from fvhoe_amd.hydro import HydroState, compute_primitives, hydro_dt_device
-from fvhoe_amd.initial_conditions import InitialCondition, fv_average, primitives_to_conservatives, uniform_mesh
+from fvhoe_amd.initial_conditions import (InitialCondition, device_mesh, fv_average, primitives_to_conservatives,
+                                          uniform_mesh)
 from fvhoe_amd.ode import ODE
 from fvhoe_amd.slope_limiting import (MUSCL_interpolations, broadcast_to_troubled_interfaces, detect_troubled_cells,
                                       nad_mask, pad_spec)
@@ -76,6 +77,7 @@ class EulerSolver(ODE):
         predictor=None,
         predictor_every_stage: bool = False,
         predictor_overlap: bool = False,
+        ic_device: bool = False,
     ):
         if w0 is None:
             raise ValueError("w0 (an InitialCondition) is required")
@@ -113,16 +115,19 @@ class EulerSolver(ODE):
         self.hydro_state = HydroState(passive_scalars=passives)
         self.nvars = self.hydro_state.nvars
         ic = w0.build_ic(passives, w0_passives)
+        # the cell averages are built on the GPU with ic_device (IC functions written for NumPy or torch,
+        # as every class of fvhoe_amd.initial_conditions is); the mesh holds the same doubles either way
+        Xi, Yi, Zi = device_mesh(self.n, x, y, z) if ic_device else (self.X, self.Y, self.Z)
         # self.u0: pointwise conservative IC f(x, y, z) where the IC defines one (fvhoe/solver.py:278-299)
         if conservative_ic and fv_ic:
             self.u0 = NotImplemented
-            u0 = ic(self.X, self.Y, self.Z)
+            u0 = ic(Xi, Yi, Zi)
         elif fv_ic:
             raise ValueError("Can't have finite volume averages without conservative IC.")
         else:
             point = ic if conservative_ic else (lambda x, y, z: primitives_to_conservatives(ic(x, y, z), gamma))
             self.u0 = point
-            u0 = fv_average(point, self.X, self.Y, self.Z, self.h, self.p)
+            u0 = fv_average(point, Xi, Yi, Zi, self.h, self.p)
         super().__init__(u0, progress_bar=progress_bar)
         self.f_evaluation_count = 0
 

@@ -62,3 +62,24 @@ def build_stage_cases():
                u0=ics.density_wave_3d(16, 7))
 
 
+# initial conditions: (key, class name, ctor kwargs, n, p, domain); small meshes, every IC class
+IC_CASES = [
+    ("sinus_3d", "Sinus", dict(dims="xyz", rho_min_max=(0.5, 1.5), vx=1, vy=1, vz=1), (8, 6, 5), (7, 2, 1),
+     ((0, 1), (0, 1), (0, 1))),
+    ("square_xy", "Square", dict(dims="xy"), (16, 12, 1), (2, 2, 0), ((0, 1), (0, 1), (0, 1))),
+    ("slotted_disk", "SlottedDisk", {}, (20, 20, 1), (3, 3, 0), ((0, 1), (0, 1), (0, 1))),
+    ("shock1d_y", "Shock1D", dict(dim="y"), (1, 32, 1), (0, 3, 0), ((0, 1), (0, 1), (0, 1))),
+    ("double_shock", "DoubleShock1D", {}, (40, 1, 1), (4, 0, 0), ((0, 1), (0, 1), (0, 1))),
+    ("shu_osher_z", "ShuOsher1D", dict(dim="z"), (1, 1, 32), (0, 0, 3), ((0, 1), (0, 1), (0, 10))),
+    ("kelvin_helmholtz", "KelvinHelmholtz2D", {}, (24, 24, 1), (3, 3, 0), ((0, 1), (0, 1), (0, 1))),
+    ("double_mach", "DoubleMachReflection2D", dict(gamma=1.4), (32, 8, 1), (1, 1, 0), ((0, 4), (0, 1), (0, 1))),
+    ("shock_tube_ball", "ShockTube", dict(dims="xy", center=(0.5, 0.5), radius=0.3), (16, 16, 1), (2, 2, 0),
+     ((0, 1), (0, 1), (0, 1))),
+    ("shock_tube_cube", "ShockTube", dict(mode="cube", x_cube=(0.2, 0.6), y_cube=(0.3, 0.7)), (16, 16, 1),
+     (2, 2, 0), ((0, 1), (0, 1), (0, 1))),
+    ("sedov_center", "SedovBlast", dict(mode="center", dims="xy"), (16, 16, 1), (0, 0, 0),
+     ((0, 1), (0, 1), (0, 1))),
+    ("sedov_corner", "SedovBlast", dict(mode="corner", dims="xy"), (12, 12, 1), (2, 2, 0),
+     ((0, 1), (0, 1), (0, 1))),
+    ("athena_blast", "AthenaBlast", {}, (16, 24, 1), (2, 2, 0), ((-0.5, 0.5), (-0.75, 0.75), (0, 1))),
+]

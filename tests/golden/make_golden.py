@@ -261,8 +261,31 @@ def run_detection():
     print(f"detection: {i} combos", flush=True)
 
 
+def run_initial_conditions():
+    """The reference solver's initial conservative averages (fvhoe/solver.py:255-306) and its
+    pointwise primitive IC at the cell centres, per IC class."""
+    import fvhoe.initial_conditions as ric
+    from fvhoe.hydro import HydroState
+
+    from cases import IC_CASES
+
+    rec = {}
+    for key, cls, kw, n, p, dom in IC_CASES:
+        ic = getattr(ric, cls)(**kw)
+        s = EulerSolver(w0=ic, nx=n[0], ny=n[1], nz=n[2], px=p[0], py=p[1], pz=p[2], x=dom[0], y=dom[1], z=dom[2],
+                        gamma=1.4, progress_bar=False)
+        rec[f"{key}/u0"] = np.asarray(s.am("u"))
+        rec[f"{key}/w"] = ic.build_ic(HydroState())(s.X, s.Y, s.Z)
+    np.savez_compressed(os.path.join(OUT, "ics.npz"), **rec)
+    print(f"ics: {len(IC_CASES)} cases", flush=True)
+
+
 if __name__ == "__main__":
+    if sys.argv[1:] == ["ics"]:
+        run_initial_conditions()
+        sys.exit(0)
     build_stage_cases()
     run_stage_cases()
     run_detection()
     run_trajectories()
+    run_initial_conditions()
