@@ -9,9 +9,10 @@ path (bit-identical to the reference), the FP32 bulk form (FP32 sweeps and
 Riemann solves; FP64 pressure recovery, trouble tests and RK accumulation),
 and the learned form (FP32 bulk plus the troubled-cell predictor, run once
 per step, whose pre-flagged cells get MUSCL fluxes before the candidate; the
-plain check still runs on every cell).
+plain check still runs on every cell), also timed with the network on a side
+stream beside each step's first stage ("learned-overlap", the headline).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32,learned] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32,learned,learned-overlap] [--no-cpu]
 
 Prints one JSON line.  `value` is Mcell-updates/s per RK stage (cells x
 stages / s) of the last form listed.  `roofline` is for the kernel that takes
@@ -74,8 +75,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--forms", default="fp64,fp32,learned",
-                    help="comma list of forms to time (fp64, fp32, learned); the last one is the headline")
+    ap.add_argument("--forms", default="fp64,fp32,learned,learned-overlap",
+                    help="comma list of forms to time (fp64, fp32, learned, learned-overlap = the network on a "
+                         "side stream beside each step's first stage); the last one is the headline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-stages", type=int, default=2)
@@ -203,7 +205,7 @@ def run_ensemble(a):
 
     forms = {}
     for form in a.forms.split(","):
-        if form == "learned":
+        if form.startswith("learned"):
             continue  # the predictor network is two-dimensional
         e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision=form)
         tune = e.autotune(a.ensemble_t, steps=10)
@@ -248,8 +250,9 @@ def cpu_baseline(n: int, stages: int):
 
 
 def run_form(n, form, steps, warmup, u0):
-    """Time `steps` SSP-RK3 steps of one form ("fp64", "fp32", or "learned" = FP32 bulk with the
-    troubled-cell predictor); returns (result dict, final state)."""
+    """Time `steps` SSP-RK3 steps of one form ("fp64", "fp32", "learned" = FP32 bulk with the
+    troubled-cell predictor, "learned-overlap" = the same with the network beside each step's first
+    stage); returns (result dict, final state)."""
     import torch
 
     from fvhoe_amd import _lib

@@ -85,6 +85,8 @@ _SIGNATURES = {
     "fvhoe_set_scalar": (c_int, [_P, c_double, _P]),
     "fvhoe_hit_target": (c_int, [_P, _P, _P, c_double, c_int, _P]),
     "fvhoe_advance_time": (c_int, [_P, _P, _P]),
+    "fvhoe_log_troubles_gated": (c_int, [_P, _P, c_longlong, c_int, c_double, _P, _P]),
+    "fvhoe_advance_time_counted": (c_int, [_P, _P, _P, _P, c_longlong, _P]),
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
     "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),

@@ -492,6 +492,10 @@ __global__ void k_rk(int kind, double* out, const double* u, const double* __res
   const double dt = dtp[0];
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     double r;
+    if (dt == 0.0) {  // a step at its target time leaves the state exactly as it is
+      out[i] = u[i];
+      continue;
+    }
     switch (kind) {
       case FVHOE_RK_AXPY: r = u[i] + dt * k0[i]; break;
       case FVHOE_RK_SSPRK3_STAGE3: r = u[i] + 0.25 * dt * k0[i] + 0.25 * dt * k1[i]; break;
@@ -505,8 +509,11 @@ __global__ void k_rk(int kind, double* out, const double* u, const double* __res
   }
 }
 
-// trouble log (fvhoe/solver.py:794-833): 0: dst = t, 1: dst = dst + t, 2: dst = dst / d
-__global__ void k_log(double* __restrict__ dst, const signed char* __restrict__ t, long long n, int mode, double d) {
+// trouble log (fvhoe/solver.py:794-833): 0: dst = t, 1: dst = dst + t, 2: dst = dst / d; nothing when
+// the gate (a step's dt) is zero
+__global__ void k_log(double* __restrict__ dst, const signed char* __restrict__ t, long long n, int mode, double d,
+                      const double* __restrict__ gate) {
+  if (gate && gate[0] == 0.0) return;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     if (mode == 0) dst[i] = (double)t[i];
     else if (mode == 1) dst[i] = dst[i] + (double)t[i];
@@ -522,6 +529,15 @@ __global__ void k_hit_target(double* dt_step, const double* dt_stage, const doub
 }
 
 __global__ void k_advance_time(double* t, const double* dt) { t[0] = t[0] + dt[0]; }
+
+// t += dt for a step that moved (dt != 0): count it and record its end time while there is room
+__global__ void k_advance_time_counted(double* t, const double* dt, long long* steps, double* times, long long cap) {
+  if (dt[0] == 0.0) return;
+  t[0] = t[0] + dt[0];
+  const long long k = steps[0];
+  if (k < cap) times[k] = t[0];
+  steps[0] = k + 1;
+}
 
 }  // namespace
 
@@ -763,7 +779,15 @@ int fvhoe_log_troubles(double* mean, const signed char* trouble, long long n, in
                        void* stream) {
   if (!mean || n < 0 || mode < 0 || mode > 2 || (mode < 2 && !trouble)) return FVHOE_ERR_ARG;
   if (n == 0) return FVHOE_OK;
-  k_log<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, trouble, n, mode, divisor);
+  k_log<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, trouble, n, mode, divisor, nullptr);
+  return launch_status();
+}
+
+int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long n, int mode, double divisor,
+                             const double* gate, void* stream) {
+  if (!mean || !gate || n < 0 || mode < 0 || mode > 2 || (mode < 2 && !trouble)) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_log<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, trouble, n, mode, divisor, gate);
   return launch_status();
 }
 
@@ -783,6 +807,13 @@ int fvhoe_hit_target(double* dt_step, const double* dt_stage, const double* t, d
 int fvhoe_advance_time(double* t, const double* dt, void* stream) {
   if (!t || !dt) return FVHOE_ERR_ARG;
   k_advance_time<<<1, 1, 0, (hipStream_t)stream>>>(t, dt);
+  return launch_status();
+}
+
+int fvhoe_advance_time_counted(double* t, const double* dt, long long* steps, double* times, long long cap,
+                               void* stream) {
+  if (!t || !dt || !steps || (cap > 0 && !times) || cap < 0) return FVHOE_ERR_ARG;
+  k_advance_time_counted<<<1, 1, 0, (hipStream_t)stream>>>(t, dt, steps, times, cap);
   return launch_status();
 }
 

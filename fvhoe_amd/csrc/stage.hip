@@ -863,6 +863,7 @@ struct RkArgs {
 
 __device__ __forceinline__ double rk_combine(const RkArgs& r, int i, double k_this) {
   const double dt = r.dt[0];
+  if (dt == 0.0) return r.u0[i];  // a step at its target time leaves the state exactly as it is
   switch (r.kind) {
     case FVHOE_RK_AXPY: return r.u0[i] + dt * k_this;
     case FVHOE_RK_SSPRK3_STAGE3: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
@@ -905,8 +906,9 @@ __global__ void k_stage_init(int* count, int reset_pre) {
   }
 }
 
-__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats, int with_pre) {
-  if (threadIdx.x == 0) {
+__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats, int with_pre,
+                             const double* __restrict__ gate) {
+  if (threadIdx.x == 0 && !(gate && gate[0] == 0.0)) {  // a step at its target time counts nothing
     if (with_pre) {
       stats[0] += count[1];
       stats[2] += count[2];
@@ -1278,7 +1280,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       TimedLaunch tl(TK_FALLBACK, s);
       k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count, list, SrcU{u}, g, fp, F, G);
     }
-    if (use_pred && pred->stats) k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats, pre_now ? 1 : 0);
+    if (use_pred && pred->stats)
+      k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats, pre_now ? 1 : 0, rkf && rkf->kind >= 0 ? rkf->dt_step : nullptr);
   }
   {
     TimedLaunch tl(TK_FINAL, s);

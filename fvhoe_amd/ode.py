@@ -27,6 +27,8 @@ def _hit_target(t: float, dt: float, target: float = None) -> float:
 class ODE(ABC):
     """u' = f(t, u) with u a device tensor (fvhoe/ode.py:25-298)."""
 
+    TIMES_CAP = 256  # most steps replayed between two host synchronisations in integrate
+
     def __init__(self, u0, progress_bar: bool = False):
         self.t = 0.0
         self.timestamps = [0.0]
@@ -41,7 +43,10 @@ class ODE(ABC):
         self._s = _lib.empty(self.u.shape)
         self._scal = _lib.zeros((8,))  # dt_stage x4, dt_step, t, spare
         self._n = n
-        self._graphs = {}  # integrator -> (captured step, RHS evaluations per step)
+        self._graphs = {}  # (integrator, t pointer, target) -> (captured step, RHS evaluations per step)
+        # steps that moved since the last reset, and their end times (the first TIMES_CAP of them)
+        self._steps_dev = _lib.zeros((1,), dtype=_lib.torch().int64)
+        self._times_dev = _lib.zeros((self.TIMES_CAP,))
 
     # ------------------------------------------------------------ to override
     @abstractmethod
@@ -110,7 +115,7 @@ class ODE(ABC):
                               dt_step=dt_step)
             else:
                 raise ValueError(name)
-            _lib.call("fvhoe_advance_time", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.stream())
+            self._advance_clock(t_dev, dt_step)
             return dt_step
         self.stage(u, k[0], dts)
         _lib.call("fvhoe_hit_target", _lib.ptr(dt_step), _lib.ptr(dts), _lib.ptr(t_dev),
@@ -137,8 +142,12 @@ class ODE(ABC):
             self._combine(_lib.RK_RK4_FINAL, u, u, k[0], k[1], k[2], k[3], dt=dt_step)
         else:
             raise ValueError(name)
-        _lib.call("fvhoe_advance_time", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.stream())
+        self._advance_clock(t_dev, dt_step)
         return dt_step
+
+    def _advance_clock(self, t_dev, dt_step):
+        _lib.call("fvhoe_advance_time_counted", _lib.ptr(t_dev), _lib.ptr(dt_step), _lib.ptr(self._steps_dev),
+                  _lib.ptr(self._times_dev), self.TIMES_CAP, _lib.stream())
 
     def _fused_rk(self) -> bool:
         return False
@@ -165,7 +174,7 @@ class ODE(ABC):
         t_dev = self._scal[5:6]
         _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
         # the capture needs the steady state of one completed step (workspace allocated, counters reset)
-        have = (self.integrator, t_dev.data_ptr()) in self._graphs
+        have = (self.integrator, t_dev.data_ptr(), None) in self._graphs
         eager = n if not graph else (0 if have else min(n, 1))
         for _ in range(eager):
             self._device_step(None, t_dev)
@@ -181,13 +190,15 @@ class ODE(ABC):
         self.t = float(t_dev.item())
         self.timestamps.append(self.t)
 
-    def _step_graph(self, t_dev):
-        """One step of self.integrator (with its bookkeeping kernels) captured in a graph.  The
-        capture records the launches without running them; host-side counters touched while
-        recording are restored, and they return to the same values after every step, so the
+    def _step_graph(self, t_dev, target: Optional[float] = None):
+        """One step of self.integrator toward target (with its bookkeeping kernels) captured in a
+        graph.  The capture records the launches without running them; host-side counters touched
+        while recording are restored, and they return to the same values after every step, so the
         captured launch arguments hold for every replay."""
-        key = (self.integrator, t_dev.data_ptr())
+        key = (self.integrator, t_dev.data_ptr(), None if target is None else float(target))
         if key not in self._graphs:
+            if target is not None:  # one target graph at a time: drop those of earlier targets
+                self._graphs = {k: v for k, v in self._graphs.items() if k[2] is None}
             torch = _lib.torch()
             fe = getattr(self, "f_evaluation_count", None)
             saved = {k: v for k, v in vars(self).items() if k in ("trouble_counter",)}
@@ -196,7 +207,7 @@ class ODE(ABC):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
-                    self._device_step(None, t_dev)
+                    self._device_step(target, t_dev)
                     self.step_helper_function()
             torch.cuda.current_stream().wait_stream(s)
             per_step = 0
@@ -208,14 +219,64 @@ class ODE(ABC):
             self._graphs[key] = (g, per_step)
         return self._graphs[key]
 
+    def _chunkable(self) -> bool:
+        """Steps can run back to back on the device: fused RK stages, no host clock needed."""
+        return self._fused_rk() and not self._needs_host_clock()
+
+    def _run_steps(self, target: Optional[float], limit: int) -> int:
+        """Up to `limit` (<= TIMES_CAP) steps toward target as graph replays with one host
+        synchronisation at the end; once a step lands on the target the remaining ones move nothing
+        (dt = 0: state, time, trouble log and counters untouched).  Updates t, timestamps, step_count
+        and f_evaluation_count as take_step would have; returns the number of steps that moved."""
+        limit = max(1, min(int(limit), self.TIMES_CAP))
+        t_dev = self._scal[5:6]
+        _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
+        self._steps_dev.zero_()
+        fe = getattr(self, "f_evaluation_count", None)
+        eager = 0
+        if self.step_count == 0:  # the capture needs one completed step (workspace allocated)
+            self._device_step(target, t_dev)
+            self.step_helper_function()
+            eager = 1
+        replays = limit - eager
+        per_step = 0
+        if replays > 0:
+            g, per_step = self._step_graph(t_dev, target)
+            for _ in range(replays):
+                g.replay()
+        moved = int(self._steps_dev.item())
+        times = self._times_dev[:moved].cpu().numpy()
+        self.t = float(t_dev.item())
+        self.timestamps.extend(float(x) for x in times)
+        self.step_count += moved
+        if fe is not None:
+            self.f_evaluation_count = self.f_evaluation_count + per_step * (moved - eager)
+        return moved
+
+    def _chunk_length(self, target: Optional[float]) -> int:
+        """Steps to replay before the next synchronisation: enough to reach target at the last step's
+        dt (so few replays are wasted past it), at most TIMES_CAP."""
+        if target is None:
+            return self.TIMES_CAP
+        if len(self.timestamps) < 2 or self.timestamps[-1] <= self.timestamps[-2]:
+            return 1
+        dt = self.timestamps[-1] - self.timestamps[-2]
+        return int(min(self.TIMES_CAP, max(1, np.ceil((target - self.t) / dt))))
+
     def integrate(self, T: float = None, n: int = None, exact: bool = True, downbeats: Any = None,
                   log_every_step: bool = False, snapshot_dir: str = None, overwrite: bool = False) -> None:
-        """fvhoe/ode.py:94-174."""
+        """fvhoe/ode.py:94-174.  On the fused path the steps between snapshots run as graph replays in
+        chunks, with the host clock read once per chunk instead of once per step."""
         if T is None:
             self.snapshot()
             t0 = time.time()
-            for _ in range(n):
-                self.take_step()
+            if self._chunkable():
+                left = n
+                while left > 0:
+                    left -= self._run_steps(None, min(left, self.TIMES_CAP))
+            else:
+                for _ in range(n):
+                    self.take_step()
             _lib.torch().cuda.synchronize()
             self.timer["TOTAL"] += time.time() - t0
             self.snapshot()
@@ -233,10 +294,14 @@ class ODE(ABC):
             downbeats = downbeats.tolist()
         downbeats = sorted(set([0] + list(downbeats) + [T]))[1:]
         target = downbeats.pop(0) if exact else None
+        chunked = target is not None and not log_every_step and self._chunkable()
         self.snapshot()
         t0 = time.time()
         while self.t < T:
-            self.take_step(target_time=target)
+            if chunked:
+                self._run_steps(target, self._chunk_length(target))
+            else:
+                self.take_step(target_time=target)
             if self.t == target or self.t >= T or log_every_step:
                 ts = time.time()
                 self.snapshot()

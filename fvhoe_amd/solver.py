@@ -291,7 +291,7 @@ class EulerSolver(ODE):
                       _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), _lib.ctypes.byref(rk),
                       _lib.stream())
         if self._trouble is not None:
-            self._log_troubles_device(self._trouble)
+            self._log_troubles_device(self._trouble, gate=dt_step)
         return True
 
     def _predict_args(self, infer: bool, overlap=None):
@@ -452,23 +452,30 @@ class EulerSolver(ODE):
 
     # ------------------------------------------------------------ bookkeeping
     def log_troubles(self, trouble=None, NAD_violation_magnitude=None, PAD_violation_magnitude=None,
-                     reset: bool = False):
+                     reset: bool = False, gate=None):
         """Per-step means of trouble and violation magnitudes (fvhoe/solver.py:794-833);
-        the counter starts at 1 (fvhoe/solver.py:400)."""
+        the counter starts at 1 (fvhoe/solver.py:400).  gate (device scalar, the step's dt): the
+        device work does nothing when it is 0, a step that stays at its target time."""
         lg = self.log
         n = lg["mean trouble"].numel()
+
+        def log(dst, src, mode, d):
+            if gate is None:
+                _lib.call("fvhoe_log_troubles", _lib.ptr(dst), _lib.ptr(src), n, mode, d, _lib.stream())
+            else:
+                _lib.call("fvhoe_log_troubles_gated", _lib.ptr(dst), _lib.ptr(src), n, mode, d, _lib.ptr(gate),
+                          _lib.stream())
+
         if reset:
             if self.trouble_counter > 0:
                 for k in lg:
                     if k != "mean trouble" and not self._log_written.get(k, False):
                         continue  # never written since construction: dividing zeros changes nothing
-                    _lib.call("fvhoe_log_troubles", _lib.ptr(lg[k]), None, n, 2, float(self.trouble_counter),
-                              _lib.stream())
+                    log(lg[k], None, 2, float(self.trouble_counter))
                 self.trouble_counter = 0
             return
         t8 = trouble if trouble.dtype == _lib.torch().int8 else trouble.to(_lib.torch().int8).contiguous()
-        _lib.call("fvhoe_log_troubles", _lib.ptr(lg["mean trouble"]), _lib.ptr(t8), n,
-                  0 if self.trouble_counter == 0 else 1, 1.0, _lib.stream())
+        log(lg["mean trouble"], t8, 0 if self.trouble_counter == 0 else 1, 1.0)
         for k, v in (("mean NAD mag", NAD_violation_magnitude), ("mean PAD mag", PAD_violation_magnitude)):
             if v is None:
                 continue
@@ -478,12 +485,13 @@ class EulerSolver(ODE):
                       None, None, None, _lib.ptr(self._one), n, _lib.stream())
         self.trouble_counter += 1
 
-    def _log_troubles_device(self, flags):
-        self.log_troubles(flags, None, None)
+    def _log_troubles_device(self, flags, gate=None):
+        self.log_troubles(flags, None, None, gate=gate)
 
     def step_helper_function(self):
         if self.a_posteriori_slope_limiting:
-            self.log_troubles(reset=True)
+            # gated by the step's dt: a step that stays at its target time leaves the log as it is
+            self.log_troubles(reset=True, gate=self._scal[4:5] if self._fused_rk() else None)
 
     def snapshot(self):
         """fvhoe/solver.py:840-874."""

@@ -142,7 +142,9 @@ int fvhoe_shift_max(double* out, const long long* out_strides, long long out_off
 /* F = (1 - theta)*F + theta*Fl, theta broadcast over variables (fvhoe/solver.py:729-731) */
 int fvhoe_blend(double* F, const double* Fl, const double* theta, int nv, long long nface, void* stream);
 
-/* Runge-Kutta combinations, FVHOE_RK_* kinds; dt is a device scalar */
+/* Runge-Kutta combinations, FVHOE_RK_* kinds; dt is a device scalar.  dt = 0
+ * (a step already at its target time) writes out = u exactly, here and in the
+ * fused stage's combine. */
 int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, const double* k1, const double* k2,
                      const double* k3, const double* dt, long long n, void* stream);
 
@@ -150,12 +152,20 @@ int fvhoe_rk_combine(int kind, double* out, const double* u, const double* k0, c
  * mean = trouble, 1 mean += trouble, 2 mean /= divisor (trouble unused). */
 int fvhoe_log_troubles(double* mean, const signed char* trouble, long long n, int mode, double divisor,
                        void* stream);
+/* The same, doing nothing when *gate (device: the step's dt) is 0. */
+int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long n, int mode, double divisor,
+                             const double* gate, void* stream);
 
 /* device scalars: *p = v; *dt_step = _hit_target(t, dt_stage, target) (fvhoe/ode.py:11-22); *t += *dt */
 int fvhoe_set_scalar(double* p, double v, void* stream);
 int fvhoe_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target,
                      void* stream);
 int fvhoe_advance_time(double* t, const double* dt, void* stream);
+/* For a step that moved (*dt != 0): *t += *dt, times[*steps] = *t while
+ * *steps < cap, ++*steps (device scalars; steps after the target time move
+ * nothing and count nothing). */
+int fvhoe_advance_time_counted(double* t, const double* dt, long long* steps, double* times, long long cap,
+                               void* stream);
 
 /* ------------------------------------------------------------ fused stage */
 

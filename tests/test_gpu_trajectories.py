@@ -19,13 +19,15 @@ P, O = ("periodic", "periodic"), ("outflow", "outflow")
 
 
 def _record_flags(s, sink):
+    """Copy every stage's flags to the host (so the steps run one by one, not as graph replays)."""
     orig = s._log_troubles_device
 
-    def rec(flags):
+    def rec(flags, gate=None):
         sink.append(flags.cpu().numpy().astype(bool))
-        orig(flags)
+        orig(flags, gate=gate)
 
     s._log_troubles_device = rec
+    s._chunkable = lambda: False
 
 
 @pytest.mark.parametrize("fused", [True, False])
