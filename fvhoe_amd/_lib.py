@@ -41,7 +41,7 @@ class FvhoeRkFuse(ctypes.Structure):
     _fields_ = [
         ("kind", c_int), ("out", c_void_p), ("u0", c_void_p), ("k0", c_void_p), ("k1", c_void_p),
         ("k2", c_void_p), ("dt_step", c_void_p), ("set_dt_step", c_int), ("t", c_void_p),
-        ("target", c_double), ("has_target", c_int),
+        ("target", c_double), ("has_target", c_int), ("target_ptr", c_void_p),
     ]
 
 

@@ -22,6 +22,8 @@
 // In the FP32 form the sweeps, Riemann solves and W/F/G/WS storage are float;
 // the conservative-to-primitive pressure recovery, the NAD/PAD/SED tests, the
 // divergence, the candidate and the RK accumulation stay FP64.
+#include <type_traits>
+
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
 #include "predictor.h"
@@ -32,6 +34,32 @@ using namespace fvhoe;
 namespace {
 
 constexpr int TPB = 256;
+
+// Launch shapes (measured choices; each can be overridden at build time for a sweep):
+// W rows per lane of the column-strip prims kernel, LDS tile of the tiled prims kernel
+// (candidate state and p > 3), face tile of the flux kernels (faces x rows), rows per
+// thread of the 2-D check.
+#ifndef FVHOE_STRIP_ROWS_F64
+#define FVHOE_STRIP_ROWS_F64 4
+#endif
+#ifndef FVHOE_STRIP_ROWS_F32
+#define FVHOE_STRIP_ROWS_F32 8
+#endif
+#ifndef FVHOE_TILE_X
+#define FVHOE_TILE_X 16
+#endif
+#ifndef FVHOE_TILE_Y
+#define FVHOE_TILE_Y 32
+#endif
+#ifndef FVHOE_FLUX_A
+#define FVHOE_FLUX_A 16
+#endif
+#ifndef FVHOE_FLUX_B
+#define FVHOE_FLUX_B 32
+#endif
+#ifndef FVHOE_DETECT_ROWS
+#define FVHOE_DETECT_ROWS 8
+#endif
 
 // effective reach of the c and t stencils (outermost non-zero tap)
 __host__ __device__ constexpr int ereach_c(int p) { return p == 1 ? 0 : (p % 2 ? reach(p) - 1 : reach(p)); }
@@ -81,16 +109,14 @@ struct Geo {
   int ge0, ge1;      // ghost width of W / WS per axis (0 on an inactive axis)
   int bc[4];         // x lo, x hi, y lo, y hi
   double cx, cy;     // -(1/hx), -(1/hy)
-  int dirichlet;     // 1 if a side is FVHOE_BC_DIRICHLET: ghosts go through bc_fetch3 and the slabs
-  Slabs sl;
 };
 
 // ghost value through the Dirichlet-aware composition (2-D grid, z of extent 1)
 template <typename At>
-__device__ __forceinline__ double geo_fetch(const Geo& g, int v, int cx, int cy, At&& at) {
+__device__ __forceinline__ double geo_fetch(const Geo& g, const Slabs& sl, int v, int cx, int cy, At&& at) {
   const int n[3] = {g.n0, g.n1, 1};
   const int bc[6] = {g.bc[0], g.bc[1], g.bc[2], g.bc[3], BC_PERIODIC, BC_PERIODIC};
-  return bc_fetch3(n, bc, g.sl, v, cx, cy, 0, [&](int vv, int a, int b, int) { return at(vv, a, b); });
+  return bc_fetch3(n, bc, sl, v, cx, cy, 0, [&](int vv, int a, int b, int) { return at(vv, a, b); });
 }
 
 __device__ __forceinline__ int cell_map(int c, int n, int lo, int hi, bool* flip) {
@@ -111,14 +137,15 @@ __device__ __forceinline__ int cell_map(int c, int n, int lo, int hi, bool* flip
 }
 
 // conservative state at a (possibly ghost) cell: the BC fill of
-// fvhoe/boundary_conditions.py:306-402 evaluated on demand
+// fvhoe/boundary_conditions.py:306-402 evaluated on demand.  The slab-aware
+// variants (SrcUD, SrcUstarD) are separate types, so kernels of grids without
+// Dirichlet sides carry none of the slab code.
 struct SrcU {
   const double* __restrict__ u;
   __device__ __forceinline__ double at(const Geo& g, int v, int sx, int sy) const {
     return u[(v * g.n0 + sx) * g.n1 + sy];
   }
   __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
-    if (g.dirichlet) return geo_fetch(g, v, cx, cy, [&](int vv, int a, int b) { return at(g, vv, a, b); });
     bool fx, fy;
     const int sx = cell_map(cx, g.n0, g.bc[0], g.bc[1], &fx);
     const int sy = cell_map(cy, g.n1, g.bc[2], g.bc[3], &fy);
@@ -146,7 +173,6 @@ struct SrcUstar {
     return u[(v * g.n0 + sx) * g.n1 + sy] + dt[0] * L;
   }
   __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
-    if (g.dirichlet) return geo_fetch(g, v, cx, cy, [&](int vv, int a, int b) { return at(g, vv, a, b); });
     bool fx, fy;
     const int sx = cell_map(cx, g.n0, g.bc[0], g.bc[1], &fx);
     const int sy = cell_map(cy, g.n1, g.bc[2], g.bc[3], &fy);
@@ -154,6 +180,40 @@ struct SrcUstar {
     return ((fx && v == 1) || (fy && v == 2)) ? -x : x;
   }
 };
+
+// the same sources with Dirichlet / "ic" sides read from the slabs (which
+// travel with these types only, so the other kernels' arguments stay small)
+struct SrcUD : SrcU {
+  Slabs sl;
+  __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
+    return geo_fetch(g, sl, v, cx, cy, [&](int vv, int a, int b) { return at(g, vv, a, b); });
+  }
+};
+template <bool TWO_D, typename T>
+struct SrcUstarD : SrcUstar<TWO_D, T> {
+  Slabs sl;
+  __device__ __forceinline__ double get(const Geo& g, int v, int cx, int cy) const {
+    return geo_fetch(g, sl, v, cx, cy, [&](int vv, int a, int b) { return this->at(g, vv, a, b); });
+  }
+};
+
+template <typename S>
+S make_src(const double* u, const Slabs& sl) {
+  S s{};
+  s.u = u;
+  if constexpr (std::is_same_v<S, SrcUD>) s.sl = sl;
+  return s;
+}
+template <typename S, typename T>
+S make_srcstar(const double* u, const T* F, const T* G, const double* dt, const Slabs& sl) {
+  S s{};
+  s.u = u;
+  s.F = F;
+  s.G = G;
+  s.dt = dt;
+  if constexpr (!std::is_same_v<S, SrcUstar<true, T>> && !std::is_same_v<S, SrcUstar<false, T>>) s.sl = sl;
+  return s;
+}
 
 // Per-block partial reductions, 11 words per block: [0] max of
 // |vx|+|vy|+|vz|+ndim*c (raw bits of a non-negative double), [1 + 2v] / [2 + 2v]
@@ -192,9 +252,9 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   const int tid = threadIdx.x;
   const bool inside = cx0 >= 0 && cx0 + HX0 <= g.n0 && cy0 >= 0 && cy0 + HY0 <= g.n1;
   // the last tiles overhang W; their extra rows / columns are never stored, and their fetches are
-  // clamped to the ghost box W needs so every source index stays in its array
-  const int xlo = -(g.ge0 + RCX + RTX), xhi = g.n0 + g.ge0 + RCX + RTX - 1;
-  const int ylo = -(g.ge1 + RCY + RTY), yhi = g.n1 + g.ge1 + RCY + RTY - 1;
+  // clamped to the ghost box W needs so every source index stays in its array (the box's low
+  // corner is never crossed: cx0 and cy0 start there for the first tiles)
+  const int xhi = g.n0 + g.ge0 + RCX + RTX - 1, yhi = g.n1 + g.ge1 + RCY + RTY - 1;
 
   // all five variables' tile elements are fetched up front (one memory latency per block)
   constexpr int NE = (HX0 * HY0 + NT - 1) / NT;
@@ -208,7 +268,7 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
 #pragma unroll
         for (int v = 0; v < 5; ++v)
           pre[v][k] = T(inside ? src.at(g, v, cx0 + r, cy0 + c)
-                               : src.get(g, v, min(max(cx0 + r, xlo), xhi), min(max(cy0 + c, ylo), yhi)));
+                               : src.get(g, v, min(cx0 + r, xhi), min(cy0 + c, yhi)));
       }
       r += DR;
       c += DC;
@@ -348,7 +408,7 @@ __device__ __forceinline__ T sweepL(Get&& x) {
 __host__ __device__ constexpr int strip_halo(int p) { return ereach_c(p) + ereach_t(p); }
 // W rows per lane in k_prims_strip (register budget: 5 variables x (rows + 2) centres per lane)
 template <typename T>
-constexpr int strip_rows() { return sizeof(T) == 8 ? 4 : 8; }
+constexpr int strip_rows() { return sizeof(T) == 8 ? FVHOE_STRIP_ROWS_F64 : FVHOE_STRIP_ROWS_F32; }
 constexpr int STRIP_MAX_P = 3;  // degrees that use k_prims_strip (higher degrees: the LDS-tiled k_prims)
 
 template <typename T, int P, typename Src>
@@ -789,8 +849,8 @@ struct FinalP {
 
 // primitive fv average of the p=1 fallback at a (ghost) cell: the p=1 c/t
 // sweeps and the degree-0 sweeps are identities apart from 0.0 + x
-template <typename T>
-__device__ __forceinline__ void p1_prims(const SrcU& src, const Geo& g, int cx, int cy, T w[5], double gamma) {
+template <typename T, typename Src>
+__device__ __forceinline__ void p1_prims(const Src& src, const Geo& g, int cx, int cy, T w[5], double gamma) {
   double u[5], wd[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) u[q] = 0.0 + src.get(g, q, cx, cy);
@@ -799,8 +859,8 @@ __device__ __forceinline__ void p1_prims(const SrcU& src, const Geo& g, int cx, 
   for (int q = 0; q < 5; ++q) w[q] = T(0.0 + wd[q]);
 }
 
-template <int D, typename T>
-__device__ void fallback_flux(const SrcU& src, const Geo& g, int i, int j, const FinalP& p, T F[5]) {
+template <int D, typename T, typename Src>
+__device__ void fallback_flux(const Src& src, const Geo& g, int i, int j, const FinalP& p, T F[5]) {
   // face between cells (i-1, j) and (i, j) for D=0, (i, j-1) and (i, j) for D=1
   const int di = D == 0 ? 1 : 0, dj = D == 0 ? 0 : 1;
   T w0[5], w1[5], w2[5], w3[5], wl[5], wr[5];
@@ -827,9 +887,9 @@ __device__ void fallback_flux(const SrcU& src, const Geo& g, int i, int j, const
 // MUSCL fluxes at the faces of every flagged cell, written over the
 // high-order fluxes (theta = 1 faces of the simple broadcast); a face shared by
 // two flagged cells is written twice with the same value.
-template <bool TWO_D, typename T>
+template <bool TWO_D, typename T, typename Src>
 __global__ void __launch_bounds__(TPB) k_fallback(const int* __restrict__ count, const int* __restrict__ list,
-                                                 SrcU src, Geo g, FinalP p, T* __restrict__ F, T* __restrict__ G) {
+                                                 Src src, Geo g, FinalP p, T* __restrict__ F, T* __restrict__ G) {
   const int n = count[0];
   constexpr int NF = TWO_D ? 4 : 2;
   const int nF = (g.n0 + 1) * g.n1, nG = g.n0 * (g.n1 + 1);
@@ -984,7 +1044,7 @@ struct FeatW {
 __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long* __restrict__ partials, int nblocks,
                                                        double* range, double cfl_h, int fixed_on, double fixed_dt,
                                                        double* dt, double* dt_step, const double* t, double target,
-                                                       int has_target) {
+                                                       int has_target, const double* target_ptr) {
   __shared__ unsigned long long part[16][NPART];
   const int tid = threadIdx.x;
   unsigned long long val[NPART];
@@ -1008,7 +1068,8 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
     if (tid == 0) {
       const double d = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)x);
       dt[0] = d;
-      if (dt_step) dt_step[0] = (has_target && target - t[0] < d) ? target - t[0] : d;
+      const double tg = target_ptr ? target_ptr[0] : target;  // a device target serves every replay
+      if (dt_step) dt_step[0] = (has_target && tg - t[0] < d) ? tg - t[0] : d;
     } else {
       range[tid - 1] = from_ordered_bits(x);
     }
@@ -1055,7 +1116,7 @@ struct Layout {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-constexpr int TX2 = 16, TY2 = 32, TX1 = 256;
+constexpr int TX2 = FVHOE_TILE_X, TY2 = FVHOE_TILE_Y, TX1 = 256;
 
 // deepest ghost read from u: W's ghosts plus the prims stencils' reach
 // (the candidate's 3 ghosts and the fallback's MUSCL stencil reach less)
@@ -1132,9 +1193,12 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   return FVHOE_OK;
 }
 
-template <typename T, int P, bool TWO_D>
+template <typename T, int P, bool TWO_D, bool DIR>
 int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePredict* pred, const Layout& L,
               const double* u, double* dudt, double* dt_stage, signed char* trouble_out, char* ws, hipStream_t s) {
+  // sources of the conservative state and of the candidate, slab-aware only where a side needs it
+  using SU = std::conditional_t<DIR, SrcUD, SrcU>;
+  using SUS = std::conditional_t<DIR, SrcUstarD<TWO_D, T>, SrcUstar<TWO_D, T>>;
   constexpr int PY = TWO_D ? P : 0;
   constexpr int TX = TWO_D ? TX2 : TX1, TY = TWO_D ? TY2 : 1;
   constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
@@ -1155,11 +1219,15 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   g.ge0 = L.GE;
   g.ge1 = TWO_D ? L.GE : 0;
   g.bc[0] = c->bc[0]; g.bc[1] = c->bc[1]; g.bc[2] = c->bc[2]; g.bc[3] = c->bc[3];
-  g.dirichlet = L.dirichlet;
-  for (int k = 0; k < 6; ++k) g.sl.s[k] = k < 4 && c->bc[k] == BC_DIRICHLET ? c->slab[k] : nullptr;
-  g.sl.g[0] = g.sl.g[1] = c->slab_g;
-  g.sl.g[2] = 0;
-  g.sl.pad[0] = c->slab_pad[0]; g.sl.pad[1] = c->slab_pad[1]; g.sl.pad[2] = 0;
+  Slabs sl{};
+  if (DIR) {
+    for (int k = 0; k < 6; ++k) sl.s[k] = k < 4 && c->bc[k] == BC_DIRICHLET ? c->slab[k] : nullptr;
+    sl.g[0] = sl.g[1] = c->slab_g;
+    sl.g[2] = 0;
+    sl.pad[0] = c->slab_pad[0];
+    sl.pad[1] = c->slab_pad[1];
+    sl.pad[2] = 0;
+  }
   g.cx = -(1.0 / c->h[0]);
   g.cy = -(1.0 / c->h[1]);
 
@@ -1191,11 +1259,11 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   {
     TimedLaunch tl(TK_PRIMS, s);
     if constexpr (TWO_D && P <= STRIP_MAX_P) {
-      k_prims_strip<T, P, SrcU><<<strip_blocks(P, strip_rows<T>(), L.NW0, L.NW1), 256, 0, s>>>(
-          SrcU{u}, g, W, L.NW0, L.NW1, strip_cwins(P, L.NW1), red, c->gamma);
+      k_prims_strip<T, P, SU><<<strip_blocks(P, strip_rows<T>(), L.NW0, L.NW1), 256, 0, s>>>(
+          make_src<SU>(u, sl), g, W, L.NW0, L.NW1, strip_cwins(P, L.NW1), red, c->gamma);
     } else {
       const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
-      k_prims<T, P, PY, TX, TY, SrcU, NTILE><<<t0 * t1, NTILE, 0, s>>>(SrcU{u}, g, W, L.NW0, L.NW1, t1, red, c->gamma);
+      k_prims<T, P, PY, TX, TY, SU, NTILE><<<t0 * t1, NTILE, 0, s>>>(make_src<SU>(u, sl), g, W, L.NW0, L.NW1, t1, red, c->gamma);
     }
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
@@ -1204,7 +1272,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   if (rkf && rkf->kind >= 0) rk = RkArgs{rkf->kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
   k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
-                                    set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0);
+                                    set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0,
+                                    set_step ? rkf->target_ptr : nullptr);
   if (use_pred && pred->infer) {  // the network on this stage's W, features formed on load
     hipStream_t ps = s;
     if (overlap) {  // fork: the network runs beside the rest of this stage
@@ -1223,7 +1292,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   }
 
   if (TWO_D) {
-    constexpr int TA0 = 16, TB0 = 32, TA1 = 32, TB1 = 16;
+    constexpr int TA0 = FVHOE_FLUX_A, TB0 = FVHOE_FLUX_B, TA1 = FVHOE_FLUX_B, TB1 = FVHOE_FLUX_A;
     const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
     {
       TimedLaunch tl(TK_FLUX_X, s);
@@ -1247,7 +1316,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   const int nc = L.n0 * L.n1;
   if (pre_now) {  // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate
     TimedLaunch tl(TK_FALLBACK, s);
-    k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count + 1, pre_list, SrcU{u}, g, fp, F, G);
+    k_fallback<TWO_D, T, SU><<<1024, TPB, 0, s>>>(count + 1, pre_list, make_src<SU>(u, sl), g, fp, F, G);
   }
   const int gridc = (nc + TPB - 1) / TPB < 65536 ? (nc + TPB - 1) / TPB : 65536;
   if (apost) {
@@ -1259,13 +1328,13 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     {
       TimedLaunch tl(TK_PRIMS_CAND, s);
       // the LDS tiles form each candidate value once per tile (the strips would form it (R + 2H) / R times)
-      k_prims<T, P, PY, TX, TY, SrcUstar<TWO_D, T>, NTILE><<<t0 * t1, NTILE, 0, s>>>(
-          SrcUstar<TWO_D, T>{u, F, G, dt_stage}, gs, WS, L.NS0, L.NS1, t1, none, c->gamma);
+      k_prims<T, P, PY, TX, TY, SUS, NTILE><<<t0 * t1, NTILE, 0, s>>>(
+          make_srcstar<SUS>(u, F, G, dt_stage, sl), gs, WS, L.NS0, L.NS1, t1, none, c->gamma);
     }
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
     const int gridd = (nc + TPB - 1) / TPB;
-    constexpr int DR = 8;  // rows per thread of the 2-D detection
+    constexpr int DR = FVHOE_DETECT_ROWS;  // rows per thread of the 2-D detection
     const int jblocks = (L.n1 + TPB - 1) / TPB, gridd2 = jblocks * ((L.n0 + DR - 1) / DR);
     {
       TimedLaunch tl(TK_DETECT, s);
@@ -1278,7 +1347,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     }
     {
       TimedLaunch tl(TK_FALLBACK, s);
-      k_fallback<TWO_D, T><<<1024, TPB, 0, s>>>(count, list, SrcU{u}, g, fp, F, G);
+      k_fallback<TWO_D, T, SU><<<1024, TPB, 0, s>>>(count, list, make_src<SU>(u, sl), g, fp, F, G);
     }
     if (use_pred && pred->stats)
       k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats, pre_now ? 1 : 0, rkf && rkf->kind >= 0 ? rkf->dt_step : nullptr);
@@ -1293,20 +1362,27 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
+template <typename T, bool TWO_D, bool DIR>
+int dispatch_p(int p, const FvhoeStageConfig* c, const FvhoeRkFuse* rk, const FvhoePredict* pre, const Layout& L,
+               const double* u, double* dudt, double* dt, signed char* t, char* ws, hipStream_t s) {
+  switch (p) {
+    case 1: return run_stage<T, 1, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 2: return run_stage<T, 2, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 3: return run_stage<T, 3, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 4: return run_stage<T, 4, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 5: return run_stage<T, 5, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 6: return run_stage<T, 6, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 7: return run_stage<T, 7, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+    case 8: return run_stage<T, 8, TWO_D, DIR>(c, rk, pre, L, u, dudt, dt, t, ws, s);
+  }
+  return FVHOE_ERR_UNSUPPORTED;
+}
+
 template <typename T, bool TWO_D>
 int dispatch(int p, const FvhoeStageConfig* c, const FvhoeRkFuse* rk, const FvhoePredict* pre, const Layout& L,
              const double* u, double* dudt, double* dt, signed char* t, char* ws, hipStream_t s) {
-  switch (p) {
-    case 1: return run_stage<T, 1, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 2: return run_stage<T, 2, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 3: return run_stage<T, 3, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 4: return run_stage<T, 4, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 5: return run_stage<T, 5, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 6: return run_stage<T, 6, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 7: return run_stage<T, 7, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-    case 8: return run_stage<T, 8, TWO_D>(c, rk, pre, L, u, dudt, dt, t, ws, s);
-  }
-  return FVHOE_ERR_UNSUPPORTED;
+  if (L.dirichlet) return dispatch_p<T, TWO_D, true>(p, c, rk, pre, L, u, dudt, dt, t, ws, s);
+  return dispatch_p<T, TWO_D, false>(p, c, rk, pre, L, u, dudt, dt, t, ws, s);
 }
 
 }  // namespace

@@ -41,7 +41,7 @@ class ODE(ABC):
         n = self.u.numel()
         self._k = [_lib.empty(self.u.shape) for _ in range(4)]
         self._s = _lib.empty(self.u.shape)
-        self._scal = _lib.zeros((8,))  # dt_stage x4, dt_step, t, spare
+        self._scal = _lib.zeros((8,))  # dt_stage x4, dt_step, t, target, spare
         self._n = n
         self._graphs = {}  # (integrator, t pointer, target) -> (captured step, RHS evaluations per step)
         # steps that moved since the last reset, and their end times (the first TIMES_CAP of them)
@@ -85,18 +85,19 @@ class ODE(ABC):
                   _lib.ptr(k3), _lib.ptr(dt), self._n, _lib.stream())
 
     def stage_rk(self, u, k_out, dt_out, kind, out, u0, k0=None, k1=None, k2=None, dt_step=None,
-                 set_dt_step=False, t_dev=None, target=None) -> bool:
+                 set_dt_step=False, t_dev=None, target=None, target_dev=None) -> bool:
         """Optional: a stage that also writes the RK combination (fused path).
         Returns False when not available."""
         return False
 
-    def _device_step(self, target: Optional[float], t_dev):
-        """One step of self.integrator on self.u; returns the dt_step scalar view."""
+    def _device_step(self, target: Optional[float], t_dev, target_dev=None):
+        """One step of self.integrator on self.u; returns the dt_step scalar view.  With target_dev
+        (fused path) the target time is read from that device scalar when the step runs."""
         u, k, s = self.u, self._k, self._s
         sc = self._scal
         dts, dt_step = sc[0:1], sc[4:5]
         name = self.integrator
-        first = dict(dt_step=dt_step, set_dt_step=True, t_dev=t_dev, target=target)
+        first = dict(dt_step=dt_step, set_dt_step=True, t_dev=t_dev, target=target, target_dev=target_dev)
         if self._fused_rk():
             if name == "euler":
                 self.stage_rk(u, k[0], dts, _lib.RK_AXPY, u, u, **first)
@@ -190,15 +191,14 @@ class ODE(ABC):
         self.t = float(t_dev.item())
         self.timestamps.append(self.t)
 
-    def _step_graph(self, t_dev, target: Optional[float] = None):
-        """One step of self.integrator toward target (with its bookkeeping kernels) captured in a
-        graph.  The capture records the launches without running them; host-side counters touched
-        while recording are restored, and they return to the same values after every step, so the
-        captured launch arguments hold for every replay."""
-        key = (self.integrator, t_dev.data_ptr(), None if target is None else float(target))
+    def _step_graph(self, t_dev, target_dev=None):
+        """One step of self.integrator (with its bookkeeping kernels) captured in a graph, toward the
+        target time held in the device scalar target_dev if given.  The capture records the launches
+        without running them; host-side counters touched while recording are restored, and they
+        return to the same values after every step, so the captured launch arguments hold for every
+        replay."""
+        key = (self.integrator, t_dev.data_ptr(), None if target_dev is None else target_dev.data_ptr())
         if key not in self._graphs:
-            if target is not None:  # one target graph at a time: drop those of earlier targets
-                self._graphs = {k: v for k, v in self._graphs.items() if k[2] is None}
             torch = _lib.torch()
             fe = getattr(self, "f_evaluation_count", None)
             saved = {k: v for k, v in vars(self).items() if k in ("trouble_counter",)}
@@ -207,7 +207,7 @@ class ODE(ABC):
             g = torch.cuda.CUDAGraph()
             with torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
-                    self._device_step(target, t_dev)
+                    self._device_step(None if target_dev is None else 0.0, t_dev, target_dev)
                     self.step_helper_function()
             torch.cuda.current_stream().wait_stream(s)
             per_step = 0
@@ -223,14 +223,25 @@ class ODE(ABC):
         """Steps can run back to back on the device: fused RK stages, no host clock needed."""
         return self._fused_rk() and not self._needs_host_clock()
 
-    def _run_steps(self, target: Optional[float], limit: int) -> int:
-        """Up to `limit` (<= TIMES_CAP) steps toward target as graph replays with one host
-        synchronisation at the end; once a step lands on the target the remaining ones move nothing
-        (dt = 0: state, time, trouble log and counters untouched).  Updates t, timestamps, step_count
-        and f_evaluation_count as take_step would have; returns the number of steps that moved."""
+    # steps between synchronisations as graph replays instead of eager launches: measured no faster
+    # on MI355X (Sod 256: 186 vs 160 us/step, KH 256^2: 378-394 vs 360, KH 1024^2 equal within noise)
+    CHUNK_GRAPHS = False
+
+    def _run_steps(self, target: Optional[float], limit: int, graph: Optional[bool] = None) -> int:
+        """Up to `limit` (<= TIMES_CAP) steps toward target, launched back to back (graph replays
+        or eager launches) with one host synchronisation at the end; once a step lands on the target
+        the remaining ones move nothing (dt = 0: state, time, trouble log and counters untouched).
+        Updates t, timestamps, step_count and f_evaluation_count as take_step would have; returns
+        the number of steps that moved."""
         limit = max(1, min(int(limit), self.TIMES_CAP))
+        if graph is None:
+            graph = self.CHUNK_GRAPHS
         t_dev = self._scal[5:6]
         _lib.call("fvhoe_set_scalar", _lib.ptr(t_dev), float(self.t), _lib.stream())
+        target_dev = None
+        if target is not None:  # one captured graph serves every target: it reads this scalar
+            target_dev = self._scal[6:7]
+            _lib.call("fvhoe_set_scalar", _lib.ptr(target_dev), float(target), _lib.stream())
         self._steps_dev.zero_()
         fe = getattr(self, "f_evaluation_count", None)
         eager = 0
@@ -240,13 +251,23 @@ class ODE(ABC):
             eager = 1
         replays = limit - eager
         per_step = 0
-        if replays > 0:
-            g, per_step = self._step_graph(t_dev, target)
+        if replays > 0 and graph:
+            g, per_step = self._step_graph(t_dev, target_dev)
             for _ in range(replays):
                 g.replay()
+        elif replays > 0:
+            fe1 = getattr(self, "f_evaluation_count", None)
+            for _ in range(replays):
+                self._device_step(0.0 if target_dev is not None else None, t_dev, target_dev)
+                self.step_helper_function()
+            if fe1 is not None:  # counted per launched step on the host; keep the moving ones only
+                per_step = (self.f_evaluation_count - fe1) // replays
+                self.f_evaluation_count = fe1
         moved = int(self._steps_dev.item())
         times = self._times_dev[:moved].cpu().numpy()
-        self.t = float(t_dev.item())
+        scal = self._scal.cpu().numpy()
+        self.t = float(scal[5])
+        self._dt_cfl = float(scal[0])  # the last step's CFL dt before any clipping to a target
         self.timestamps.extend(float(x) for x in times)
         self.step_count += moved
         if fe is not None:
@@ -254,13 +275,13 @@ class ODE(ABC):
         return moved
 
     def _chunk_length(self, target: Optional[float]) -> int:
-        """Steps to replay before the next synchronisation: enough to reach target at the last step's
-        dt (so few replays are wasted past it), at most TIMES_CAP."""
+        """Steps to launch before the next synchronisation: enough to reach target at the last
+        step's CFL dt (so few launches are wasted past it), at most TIMES_CAP."""
         if target is None:
             return self.TIMES_CAP
-        if len(self.timestamps) < 2 or self.timestamps[-1] <= self.timestamps[-2]:
+        dt = getattr(self, "_dt_cfl", 0.0)
+        if not dt > 0.0:
             return 1
-        dt = self.timestamps[-1] - self.timestamps[-2]
         return int(min(self.TIMES_CAP, max(1, np.ceil((target - self.t) / dt))))
 
     def integrate(self, T: float = None, n: int = None, exact: bool = True, downbeats: Any = None,

@@ -266,7 +266,7 @@ class EulerSolver(ODE):
         return self.fused
 
     def stage_rk(self, u, k_out, dt_out, kind, out, u0, k0=None, k1=None, k2=None, dt_step=None,
-                 set_dt_step=False, t_dev=None, target=None) -> bool:
+                 set_dt_step=False, t_dev=None, target=None, target_dev=None) -> bool:
         """A fused stage whose last kernel also writes the RK combination `kind`
         into `out` (fvhoe/ode.py:245-298), so the state is touched once per stage."""
         self.f_evaluation_count += 1
@@ -281,6 +281,7 @@ class EulerSolver(ODE):
         rk.t = t_dev.data_ptr() if t_dev is not None else None
         rk.target = float(target) if target is not None else 0.0
         rk.has_target = int(target is not None)
+        rk.target_ptr = target_dev.data_ptr() if target_dev is not None else None
         if self.predictor is not None:
             pa = self._predict_args(bool(set_dt_step) or self.predictor_every_stage, overlap=self._overlap)
             _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),

@@ -240,6 +240,7 @@ typedef struct FvhoeRkFuse {
   const double* t;
   double target;
   int has_target;
+  const double* target_ptr;  /* device target time read at run time instead of target (may be NULL) */
 } FvhoeRkFuse;
 
 int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,

@@ -60,13 +60,15 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("launch", ["graph", "eager"])
 @pytest.mark.parametrize("overshoot", [False, True], ids=["estimated", "overshoot"])
 @pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
-def test_chunked_integrate_equals_per_step(case, overshoot):
+def test_chunked_integrate_equals_per_step(case, overshoot, launch):
     name, mk, integ, kw = case
     a, b = mk(), mk()
     a._chunkable = lambda: False  # host-synchronised steps, as take_step does them
-    if overshoot:  # every chunk replays 40 steps: the ones past each target must move nothing
+    b.CHUNK_GRAPHS = launch == "graph"
+    if overshoot:  # every chunk launches 40 steps: the ones past each target must move nothing
         b._chunk_length = lambda target: 40
     getattr(a, integ)(**kw)
     getattr(b, integ)(**kw)
