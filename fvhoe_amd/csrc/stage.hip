@@ -35,15 +35,15 @@ namespace {
 
 constexpr int TPB = 256;
 
-// Launch shapes (measured choices; each can be overridden at build time for a sweep):
-// W rows per lane of the column-strip prims kernel, LDS tile of the tiled prims kernel
-// (candidate state and p > 3), face tile of the flux kernels (faces x rows), rows per
-// thread of the 2-D check.
+// Launch shapes, chosen by timing KH 2048^2 p=3 on MI355X (each can be overridden at build
+// time for a sweep): W rows per lane of the column-strip prims kernel, LDS tile of the tiled
+// prims kernel (candidate state and p > 3) per arithmetic type, face tile of the flux kernels
+// (faces x rows), rows per thread of the 2-D check.
 #ifndef FVHOE_STRIP_ROWS_F64
 #define FVHOE_STRIP_ROWS_F64 4
 #endif
 #ifndef FVHOE_STRIP_ROWS_F32
-#define FVHOE_STRIP_ROWS_F32 8
+#define FVHOE_STRIP_ROWS_F32 4
 #endif
 #ifndef FVHOE_TILE_X
 #define FVHOE_TILE_X 16
@@ -51,14 +51,20 @@ constexpr int TPB = 256;
 #ifndef FVHOE_TILE_Y
 #define FVHOE_TILE_Y 32
 #endif
+#ifndef FVHOE_TILE32_X
+#define FVHOE_TILE32_X 16
+#endif
+#ifndef FVHOE_TILE32_Y
+#define FVHOE_TILE32_Y 64
+#endif
 #ifndef FVHOE_FLUX_A
-#define FVHOE_FLUX_A 16
+#define FVHOE_FLUX_A 32
 #endif
 #ifndef FVHOE_FLUX_B
 #define FVHOE_FLUX_B 32
 #endif
 #ifndef FVHOE_DETECT_ROWS
-#define FVHOE_DETECT_ROWS 8
+#define FVHOE_DETECT_ROWS 2
 #endif
 
 // effective reach of the c and t stencils (outermost non-zero tap)
@@ -1116,7 +1122,11 @@ struct Layout {
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-constexpr int TX2 = FVHOE_TILE_X, TY2 = FVHOE_TILE_Y, TX1 = 256;
+constexpr int TX1 = 256;
+template <typename T>
+constexpr int tile_x() { return sizeof(T) == 8 ? FVHOE_TILE_X : FVHOE_TILE32_X; }
+template <typename T>
+constexpr int tile_y() { return sizeof(T) == 8 ? FVHOE_TILE_Y : FVHOE_TILE32_Y; }
 
 // deepest ghost read from u: W's ghosts plus the prims stencils' reach
 // (the candidate's 3 ghosts and the fallback's MUSCL stencil reach less)
@@ -1176,7 +1186,8 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   if (L->two_d && L->p <= STRIP_MAX_P) {
     L->prims_blocks = strip_blocks(L->p, L->fp32 ? strip_rows<float>() : strip_rows<double>(), L->NW0, L->NW1);
   } else if (L->two_d) {
-    L->prims_blocks = ((L->NW0 + TX2 - 1) / TX2) * ((L->NW1 + TY2 - 1) / TY2);
+    const int tx = L->fp32 ? tile_x<float>() : tile_x<double>(), ty = L->fp32 ? tile_y<float>() : tile_y<double>();
+    L->prims_blocks = ((L->NW0 + tx - 1) / tx) * ((L->NW1 + ty - 1) / ty);
   } else {
     L->prims_blocks = (L->NW0 + TX1 - 1) / TX1;
   }
@@ -1200,7 +1211,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   using SU = std::conditional_t<DIR, SrcUD, SrcU>;
   using SUS = std::conditional_t<DIR, SrcUstarD<TWO_D, T>, SrcUstar<TWO_D, T>>;
   constexpr int PY = TWO_D ? P : 0;
-  constexpr int TX = TWO_D ? TX2 : TX1, TY = TWO_D ? TY2 : 1;
+  constexpr int TX = TWO_D ? tile_x<T>() : TX1, TY = TWO_D ? tile_y<T>() : 1;
   constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
   T* W = (T*)(ws + L.offW);
   T* F = (T*)(ws + L.offF);
