@@ -22,6 +22,7 @@ RIEMANN = {"advection_upwind": 0, "llf": 1, "hllc": 2}
 LIMITER = {None: 0, "minmod": 1, "moncen": 2}
 BC = {"periodic": 0, "outflow": 1, "reflective": 2, "free": 3, "dirichlet": 4, "ic": 4}
 RK_AXPY, RK_SSPRK3_STAGE3, RK_SSPRK3_FINAL, RK_SSPRK2_FINAL, RK_HALF_AXPY, RK_RK4_FINAL = range(6)
+RK_SSPRK3_STAGE3_SUM, RK_SSPRK3_FINAL_SUM = 6, 7  # fused stage only
 
 
 class FvhoeStageConfig(ctypes.Structure):
@@ -87,6 +88,8 @@ _SIGNATURES = {
     "fvhoe_advance_time": (c_int, [_P, _P, _P]),
     "fvhoe_log_troubles_gated": (c_int, [_P, _P, c_longlong, c_int, c_double, _P, _P]),
     "fvhoe_advance_time_counted": (c_int, [_P, _P, _P, _P, c_longlong, _P]),
+    "fvhoe_log_count": (c_int, [_P, _P, c_longlong, c_int, _P, _P]),
+    "fvhoe_log_mean": (c_int, [_P, _P, c_longlong, c_double, _P, _P]),
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
     "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),

@@ -521,6 +521,20 @@ __global__ void k_log(double* __restrict__ dst, const signed char* __restrict__ 
   }
 }
 
+__global__ void k_log_count(int* __restrict__ cnt, const signed char* __restrict__ t, long long n, int mode,
+                            const double* __restrict__ gate) {
+  if (gate && gate[0] == 0.0) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    cnt[i] = mode == 0 ? (int)t[i] : cnt[i] + (int)t[i];
+}
+
+__global__ void k_log_mean(double* __restrict__ dst, const int* __restrict__ cnt, long long n, double d,
+                           const double* __restrict__ gate) {
+  if (gate && gate[0] == 0.0) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (double)cnt[i] / d;
+}
+
 __global__ void k_set_scalar(double* p, double v) { p[0] = v; }
 
 __global__ void k_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target) {
@@ -788,6 +802,20 @@ int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long
   if (!mean || !gate || n < 0 || mode < 0 || mode > 2 || (mode < 2 && !trouble)) return FVHOE_ERR_ARG;
   if (n == 0) return FVHOE_OK;
   k_log<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, trouble, n, mode, divisor, gate);
+  return launch_status();
+}
+
+int fvhoe_log_count(int* count, const signed char* trouble, long long n, int mode, const double* gate, void* stream) {
+  if (!count || !trouble || n < 0 || mode < 0 || mode > 1) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_log_count<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(count, trouble, n, mode, gate);
+  return launch_status();
+}
+
+int fvhoe_log_mean(double* mean, const int* count, long long n, double divisor, const double* gate, void* stream) {
+  if (!mean || !count || n < 0) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_log_mean<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, count, n, divisor, gate);
   return launch_status();
 }
 

@@ -937,6 +937,8 @@ __device__ __forceinline__ double rk_combine(const RkArgs& r, int i, double k_th
     case FVHOE_RK_SSPRK2_FINAL: return 0.5 * r.u0[i] + 0.5 * (r.k0[i] + dt * k_this);
     case FVHOE_RK_HALF_AXPY: return r.u0[i] + 0.5 * dt * k_this;
     case FVHOE_RK_RK4_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 2.0 * r.k1[i] + 2.0 * r.k2[i] + k_this);
+    case FVHOE_RK_SSPRK3_STAGE3_SUM: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
+    case FVHOE_RK_SSPRK3_FINAL_SUM: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 4.0 * k_this);
   }
   return 0.0;
 }
@@ -957,7 +959,7 @@ __global__ void __launch_bounds__(TPB) k_final(const T* __restrict__ F, const T*
     for (int q = 0; q < 5; ++q) {
       double acc = 0.0 + g.cx * ((double)F[q * nF + fi + g.n1] - (double)F[q * nF + fi]);
       if (TWO_D) acc = acc + g.cy * ((double)G[q * nG + gi + 1] - (double)G[q * nG + gi]);
-      if (dudt) dudt[q * nc + c] = acc;
+      if (dudt) dudt[q * nc + c] = rk.kind == FVHOE_RK_SSPRK3_STAGE3_SUM ? rk.k0[q * nc + c] + acc : acc;
       if (rk.kind >= 0) rk.out[q * nc + c] = rk_combine(rk, q * nc + c, acc);
     }
   }
@@ -1454,8 +1456,11 @@ int stage_entry(const FvhoeStageConfig* cfg, const double* u, double* dudt, doub
   if (!u || !dt_stage || !workspace) return FVHOE_ERR_ARG;
   if (!dudt && !(rk && rk->kind >= 0)) return FVHOE_ERR_ARG;
   if (rk && rk->kind >= 0) {
-    if (rk->kind > FVHOE_RK_RK4_FINAL || !rk->out || !rk->u0 || !rk->dt_step) return FVHOE_ERR_ARG;
-    if ((rk->kind == FVHOE_RK_SSPRK3_STAGE3 || rk->kind == FVHOE_RK_SSPRK2_FINAL) && !rk->k0) return FVHOE_ERR_ARG;
+    if (rk->kind > FVHOE_RK_SSPRK3_FINAL_SUM || !rk->out || !rk->u0 || !rk->dt_step) return FVHOE_ERR_ARG;
+    if ((rk->kind == FVHOE_RK_SSPRK3_STAGE3 || rk->kind == FVHOE_RK_SSPRK2_FINAL ||
+         rk->kind == FVHOE_RK_SSPRK3_FINAL_SUM) && !rk->k0)
+      return FVHOE_ERR_ARG;
+    if (rk->kind == FVHOE_RK_SSPRK3_STAGE3_SUM && (!rk->k0 || !dudt)) return FVHOE_ERR_ARG;
     if (rk->kind == FVHOE_RK_SSPRK3_FINAL && (!rk->k0 || !rk->k1)) return FVHOE_ERR_ARG;
     if (rk->kind == FVHOE_RK_RK4_FINAL && (!rk->k0 || !rk->k1 || !rk->k2)) return FVHOE_ERR_ARG;
     if (rk->out == dudt) return FVHOE_ERR_ARG;

@@ -105,9 +105,10 @@ class ODE(ABC):
                 self.stage_rk(u, k[0], dts, _lib.RK_AXPY, s, u, **first)
                 self.stage_rk(s, k[1], sc[1:2], _lib.RK_SSPRK2_FINAL, u, u, k0=s, dt_step=dt_step)
             elif name == "ssprk3":
+                # k[1] receives k0 + k1, the sum the final combine starts with (same rounding)
                 self.stage_rk(u, k[0], dts, _lib.RK_AXPY, s, u, **first)
-                self.stage_rk(s, k[1], sc[1:2], _lib.RK_SSPRK3_STAGE3, s, u, k0=k[0], dt_step=dt_step)
-                self.stage_rk(s, None, sc[2:3], _lib.RK_SSPRK3_FINAL, u, u, k0=k[0], k1=k[1], dt_step=dt_step)
+                self.stage_rk(s, k[1], sc[1:2], _lib.RK_SSPRK3_STAGE3_SUM, s, u, k0=k[0], dt_step=dt_step)
+                self.stage_rk(s, None, sc[2:3], _lib.RK_SSPRK3_FINAL_SUM, u, u, k0=k[1], dt_step=dt_step)
             elif name == "rk4":
                 self.stage_rk(u, k[0], dts, _lib.RK_HALF_AXPY, s, u, **first)
                 self.stage_rk(s, k[1], sc[1:2], _lib.RK_HALF_AXPY, s, u, dt_step=dt_step)

@@ -37,6 +37,24 @@ from fvhoe_amd.slope_limiting import (MUSCL_interpolations, broadcast_to_trouble
 from fvhoe_amd.stencils import get_stencil_size
 
 
+class _TroubleLog(dict):
+    """The solver's log arrays.  Trouble flags are counted in integers between resets and turned into
+    the mean at the step's reset; reading "mean trouble" before a reset writes the running sum the
+    reference holds at that point (counts are exact integer sums, so the doubles are equal)."""
+
+    def __init__(self, solver, arrays):
+        super().__init__(arrays)
+        self._solver = solver
+
+    def __getitem__(self, key):
+        out = dict.__getitem__(self, key)
+        s = self._solver
+        if key == "mean trouble" and s._count_pending:
+            _lib.call("fvhoe_log_mean", _lib.ptr(out), _lib.ptr(s._trouble_count), out.numel(), 1.0, None,
+                      _lib.stream())
+        return out
+
+
 class EulerSolver(ODE):
     def __init__(
         self,
@@ -156,9 +174,12 @@ class EulerSolver(ODE):
         self.SED, self.SED_tolerance, self.convex = SED, SED_tolerance, convex
         self.trouble_counter = 1  # fvhoe/solver.py:400
         nx_, ny_, nz_ = self.n
-        self.log = {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag", "mean PAD mag")}
+        self.log = _TroubleLog(self, {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag",
+                                                                               "mean PAD mag")})
         self._zero_like_log = _lib.zeros((nx_, ny_, nz_))
         self._log_written = {}  # log arrays that have received magnitudes (the fused stage logs flags only)
+        self._trouble_count = _lib.zeros((nx_, ny_, nz_), dtype=_lib.torch().int32)  # flags counted since a reset
+        self._count_pending = False
         self._one = _lib.zeros((1,))
         self._one.fill_(1.0)
 
@@ -458,7 +479,7 @@ class EulerSolver(ODE):
         the counter starts at 1 (fvhoe/solver.py:400).  gate (device scalar, the step's dt): the
         device work does nothing when it is 0, a step that stays at its target time."""
         lg = self.log
-        n = lg["mean trouble"].numel()
+        n = self._trouble_count.numel()
 
         def log(dst, src, mode, d):
             if gate is None:
@@ -467,16 +488,24 @@ class EulerSolver(ODE):
                 _lib.call("fvhoe_log_troubles_gated", _lib.ptr(dst), _lib.ptr(src), n, mode, d, _lib.ptr(gate),
                           _lib.stream())
 
+        gp = _lib.ptr(gate) if gate is not None else None
         if reset:
             if self.trouble_counter > 0:
+                # the trouble flags were counted since the last reset (exact integer sums); the mean is
+                # written from the counts here
+                _lib.call("fvhoe_log_mean", _lib.ptr(dict.__getitem__(lg, "mean trouble")),
+                          _lib.ptr(self._trouble_count), n, float(self.trouble_counter), gp, _lib.stream())
+                self._count_pending = False
                 for k in lg:
-                    if k != "mean trouble" and not self._log_written.get(k, False):
+                    if k == "mean trouble" or not self._log_written.get(k, False):
                         continue  # never written since construction: dividing zeros changes nothing
                     log(lg[k], None, 2, float(self.trouble_counter))
                 self.trouble_counter = 0
             return
         t8 = trouble if trouble.dtype == _lib.torch().int8 else trouble.to(_lib.torch().int8).contiguous()
-        log(lg["mean trouble"], t8, 0 if self.trouble_counter == 0 else 1, 1.0)
+        _lib.call("fvhoe_log_count", _lib.ptr(self._trouble_count), _lib.ptr(t8), n,
+                  0 if self.trouble_counter == 0 else 1, gp, _lib.stream())
+        self._count_pending = True
         for k, v in (("mean NAD mag", NAD_violation_magnitude), ("mean PAD mag", PAD_violation_magnitude)):
             if v is None:
                 continue

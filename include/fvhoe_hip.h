@@ -45,7 +45,12 @@ enum {
   FVHOE_RK_SSPRK3_FINAL = 2,   /* u + (1/6)*dt*(k0 + k1 + 4*k2)               (ode.py:279) */
   FVHOE_RK_SSPRK2_FINAL = 3,   /* 0.5*u + 0.5*(k0 + dt*k1), k0 holding u1     (ode.py:265) */
   FVHOE_RK_HALF_AXPY = 4,      /* u + 0.5*dt*k0                               (ode.py:291-292) */
-  FVHOE_RK_RK4_FINAL = 5       /* u + (1/6)*dt*(k0 + 2*k1 + 2*k2 + k3)        (ode.py:294) */
+  FVHOE_RK_RK4_FINAL = 5,      /* u + (1/6)*dt*(k0 + 2*k1 + 2*k2 + k3)        (ode.py:294) */
+  /* fused stage only: SSP-RK3 with k0 + k1 formed once (the sum the final
+   * combine starts with, the same rounding) so the last stage reads one
+   * array less */
+  FVHOE_RK_SSPRK3_STAGE3_SUM = 6,  /* as SSPRK3_STAGE3; dudt receives k0 + this stage's k */
+  FVHOE_RK_SSPRK3_FINAL_SUM = 7    /* u + (1/6)*dt*(k0 + 4*k2), k0 holding k0 + k1 */
 };
 
 /* ---------------------------------------------------------------- host-only */
@@ -155,6 +160,12 @@ int fvhoe_log_troubles(double* mean, const signed char* trouble, long long n, in
 /* The same, doing nothing when *gate (device: the step's dt) is 0. */
 int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long n, int mode, double divisor,
                              const double* gate, void* stream);
+/* The trouble log kept as counts between resets: mode 0 count = trouble,
+ * 1 count += trouble; fvhoe_log_mean then writes mean = count / divisor,
+ * which equals the reference's running double sum divided at the reset (sums
+ * of 0/1 values are exact).  gate (device, may be NULL) as above. */
+int fvhoe_log_count(int* count, const signed char* trouble, long long n, int mode, const double* gate, void* stream);
+int fvhoe_log_mean(double* mean, const int* count, long long n, double divisor, const double* gate, void* stream);
 
 /* device scalars: *p = v; *dt_step = _hit_target(t, dt_stage, target) (fvhoe/ode.py:11-22); *t += *dt */
 int fvhoe_set_scalar(double* p, double v, void* stream);
