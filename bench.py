@@ -46,11 +46,11 @@ BYTES_PER_CELL = {
     "detect": 80 + 1,             # read W, WS, write flags
     "fallback": 0,                # work and traffic scale with the flagged cells only
     "final": 80 + 40 + 80,        # read F, G, write dudt; RK combine reads u0 (+ k) and writes the next state
-    "predict": 40 + 16 + 16 + 1,  # features: read W (rho, vx, vy, P + range), write X (4 floats); network: read X, write flags
+    "predict": 32 + 8 + 8 + 1,    # features: read W (rho, vx, vy, P), write 4 bf16; network: read them, write flags
 }
 # bytes of the arrays stored as the bulk type (W, F, G, WS) per cell for each kernel, FP64
 BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 120, "detect": 80, "fallback": 0,
-              "final": 80, "predict": 40}
+              "final": 80, "predict": 32}
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
 BF16_PEAK_TFLOPS = 2500.0
 # useful multiply-adds of the predictor network per cell: 2 * 25 * (4*16 + 16*16 + 16*1)

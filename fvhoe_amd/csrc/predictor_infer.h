@@ -74,6 +74,26 @@ struct FeatPlanar {
   }
 };
 
+// The four features of a pixel as bfloat16, the form the network stages them in.
+template <typename Feat>
+__device__ __forceinline__ bf16x4v load_bf16(const Feat& feat, int gy, int gx, const double* sh) {
+  float f[4];
+  feat.load(gy, gx, f, sh);
+  return bf16x4v{(__bf16)f[0], (__bf16)f[1], (__bf16)f[2], (__bf16)f[3]};
+}
+
+// Features already rounded to bfloat16, channel last: XB[(gy * WX + gx) * 4 + c].
+struct FeatBF16 {
+  const bf16x4v* __restrict__ XB;
+  int HX, WX;
+  __device__ __forceinline__ void init(double*) const {}
+};
+template <>
+__device__ __forceinline__ bf16x4v load_bf16<FeatBF16>(const FeatBF16& feat, int gy, int gx, const double*) {
+  if (gy < feat.HX && gx < feat.WX) return feat.XB[(long long)gy * feat.WX + gx];
+  return bf16x4v{(__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f, (__bf16)0.0f};
+}
+
 constexpr int INFER_THREADS = 256;
 
 // Activations are stored 16 channels (32 bytes) per pixel.  Element index of
@@ -128,10 +148,7 @@ __global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, i
     __syncthreads();  // the previous tile is done with xin / a2 / wbase
     for (int p = tid; p < L0 * L0; p += INFER_THREADS) {
       const int y = p / L0, x = p % L0;
-      float f[4];
-      feat.load(oy0 + y, ox0 + x, f, fconst);
-      const bf16x4v v = {(__bf16)f[0], (__bf16)f[1], (__bf16)f[2], (__bf16)f[3]};
-      *(bf16x4v*)&xin[p * C0] = v;
+      *(bf16x4v*)&xin[p * C0] = load_bf16(feat, oy0 + y, ox0 + x, fconst);
     }
     __syncthreads();
     // layer 1: 24x24 pixels = 36 groups of 16; lane slice = taps 8s+2kb and 8s+2kb+1, 4 channels each

@@ -1046,6 +1046,22 @@ struct FeatW {
   }
 };
 
+// The features of the extended grid rounded to bfloat16 once per cell (channel
+// last), so the network's tiles load 8 bytes per pixel instead of forming the
+// features of their halos again; the values are FeatW's, converted as the
+// network converts them.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_feat_pack(FeatW<T> feat, pred::bf16x4v* __restrict__ XB, int HX, int WX) {
+  __shared__ double sh[8];
+  feat.init(sh);
+  __syncthreads();
+  const int total = HX * WX;
+  for (int e = blockIdx.x * TPB + threadIdx.x; e < total; e += gridDim.x * TPB) {
+    const int a = e / WX, b = e - (e / WX) * WX;
+    XB[e] = pred::load_bf16(feat, a, b, sh);
+  }
+}
+
 // final reduction of the prims partials: dt (fvhoe/hydro.py:290-293, and
 // _hit_target, fvhoe/ode.py:11-22, when asked) and the per-variable global
 // (min, max) (fvhoe/slope_limiting.py:168-170)
@@ -1119,7 +1135,7 @@ struct TimedLaunch {
 
 struct Layout {
   int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred, dirichlet;
-  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offPre, offPreList, total;
+  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offPre, offPreList, offFeat, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1196,11 +1212,12 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offPart = o; o = align256(o + sizeof(unsigned long long) * NPART * (size_t)L->prims_blocks);
   L->offScal = o; o = align256(o + 256);
   L->pred = pred;
-  L->offPre = L->offPreList = o;
-  if (pred) {  // pre-flags and their list
+  L->offPre = L->offPreList = L->offFeat = o;
+  if (pred) {  // pre-flags, their list and the network's bfloat16 features of the extended grid
     if (!L->two_d || !c->a_posteriori) return FVHOE_ERR_UNSUPPORTED;
     L->offPre = o; o = align256(o + (size_t)L->n0 * L->n1);
     L->offPreList = o; o = align256(o + sizeof(int) * (size_t)L->n0 * L->n1);
+    L->offFeat = o; o = align256(o + 8 * (size_t)(L->n0 + 12) * (L->n1 + 12));
   }
   L->total = o;
   return FVHOE_OK;
@@ -1297,7 +1314,11 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     }
     {
       TimedLaunch tl(TK_PREDICT, ps);
-      const int st = pred::launch_infer(FeatW<T>{W, L.NW0, L.NW1, g, range}, L.n0, L.n1, pred->packed,
+      const int HX = L.n0 + 12, WX = L.n1 + 12;
+      pred::bf16x4v* XB = (pred::bf16x4v*)(ws + L.offFeat);
+      const int fb = (HX * WX + TPB - 1) / TPB;
+      k_feat_pack<T><<<fb < 4096 ? fb : 4096, TPB, 0, ps>>>(FeatW<T>{W, L.NW0, L.NW1, g, range}, XB, HX, WX);
+      const int st = pred::launch_infer(pred::FeatBF16{XB, HX, WX}, L.n0, L.n1, pred->packed,
                                         pred->logit_threshold, preflags_out, nullptr, count + 1, pre_list, ps);
       if (st != FVHOE_OK) return st;
     }
