@@ -24,6 +24,8 @@ FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(REPO, "include"), "-I", HERE,
 ]
+# launch-shape overrides for a sweep, e.g. FVHOE_DEFINES="-DFVHOE_DETECT_ROWS=4" (stage.hip documents them)
+FLAGS += [d for d in os.environ.get("FVHOE_DEFINES", "").split() if d.startswith("-DFVHOE_")]
 
 
 def build(verbose: bool = True) -> str:
