@@ -83,6 +83,99 @@ __global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, do
   out[(long long)vi * (m1 * m2) + jk] = acc;
 }
 
+// The same sweep with compile-time weights, each thread sliding a register
+// window over R consecutive outputs along the sweep axis: every input is
+// loaded (R + K - 1) / R times instead of K times.  Strided axes (x, y):
+// threads run along the contiguous z axis, so every load is coalesced; the
+// contiguous axis (z): each thread walks its own run of z.  Zero taps are
+// skipped and the accumulation order is the reference's (bit-identical).
+template <int KIND, int P, int R, bool ZAX>
+__global__ void __launch_bounds__(256) k_sweep_slide(const double* __restrict__ u, double* __restrict__ out, int n0,
+                                                     int n1, int n2, int axis, int nblk_fast, int nb_axis) {
+  constexpr int K = ntaps(P), NW = R + K - 1;
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
+  const int m0 = axis == 1 ? ma : n0, m1 = axis == 2 ? ma : n1, m2 = axis == 3 ? ma : n2;
+  int v, i, j, k, a0;
+  if (!ZAX) {  // fast index: k over n2; slow: (v, other axis, block of R along the sweep axis)
+    k = (int)(blockIdx.x % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+    if (k >= n2) return;
+    unsigned rest = blockIdx.x / (unsigned)nblk_fast;
+    const int ab = (int)(rest % (unsigned)nb_axis);
+    rest /= (unsigned)nb_axis;
+    const int no = axis == 1 ? n1 : n0;
+    const int o = (int)(rest % (unsigned)no);
+    v = (int)(rest / (unsigned)no);
+    a0 = ab * R;
+    i = axis == 1 ? a0 : o;
+    j = axis == 1 ? o : a0;
+  } else {  // fast index: block of R along z; slow: (v, i, j)
+    const unsigned g = blockIdx.x * 256u + threadIdx.x;
+    const int kb = (int)(g % (unsigned)nb_axis);
+    unsigned rest = g / (unsigned)nb_axis;
+    j = (int)(rest % (unsigned)n1);
+    rest /= (unsigned)n1;
+    i = (int)(rest % (unsigned)n0);
+    v = (int)(rest / (unsigned)n0);
+    if (v >= nblk_fast) return;  // nblk_fast carries nv here
+    a0 = kb * R;
+    k = a0;
+  }
+  const int s = axis == 1 ? n1 * n2 : axis == 2 ? n2 : 1;
+  const int in0 = ((v * n0 + i) * n1 + j) * n2 + k;
+  double win[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) win[r] = a0 + r < na ? u[in0 + r * s] : 0.0;
+  const int os = axis == 1 ? m1 * m2 : axis == 2 ? m2 : 1;
+  const int o0 = ((v * m0 + (axis == 1 ? a0 : i)) * m1 + (axis == 2 ? a0 : j)) * m2 + (axis == 3 ? a0 : k);
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (a0 + q < ma) {
+      double acc = 0.0;
+      static_for<K>([&](auto I) {
+        constexpr double wt = weight(KIND, P, decltype(I)::value);
+        if constexpr (wt != 0.0) acc = acc + wt * win[q + decltype(I)::value];
+      });
+      out[o0 + q * os] = acc;
+    }
+  }
+}
+
+template <int KIND, int P>
+int launch_slide(const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+  constexpr int K = ntaps(P);
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
+  if (axis != 3) {
+    constexpr int R = 8;
+    const int nbk = (n2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? n1 : n0;
+    const long long blocks = (long long)nbk * nb * no * nv;
+    if (blocks >= (1ll << 31)) return -1;
+    k_sweep_slide<KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbk, nb);
+  } else {
+    constexpr int R = 4;
+    const int nb = (ma + R - 1) / R;
+    const long long threads = (long long)nb * n1 * n0 * nv;
+    if (threads >= (1ll << 31)) return -1;
+    k_sweep_slide<KIND, P, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(u, out, n0, n1, n2, axis, nv, nb);
+  }
+  return 0;
+}
+
+template <int KIND>
+int launch_slide_p(int p, const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+  switch (p) {
+    case 0: return launch_slide<KIND, 0>(u, out, nv, n0, n1, n2, axis, st);
+    case 1: return launch_slide<KIND, 1>(u, out, nv, n0, n1, n2, axis, st);
+    case 2: return launch_slide<KIND, 2>(u, out, nv, n0, n1, n2, axis, st);
+    case 3: return launch_slide<KIND, 3>(u, out, nv, n0, n1, n2, axis, st);
+    case 4: return launch_slide<KIND, 4>(u, out, nv, n0, n1, n2, axis, st);
+    case 5: return launch_slide<KIND, 5>(u, out, nv, n0, n1, n2, axis, st);
+    case 6: return launch_slide<KIND, 6>(u, out, nv, n0, n1, n2, axis, st);
+    case 7: return launch_slide<KIND, 7>(u, out, nv, n0, n1, n2, axis, st);
+    case 8: return launch_slide<KIND, 8>(u, out, nv, n0, n1, n2, axis, st);
+  }
+  return -1;
+}
+
 __global__ void k_prims(const double* __restrict__ u, double* __restrict__ w, int nv, long long nc, double gamma) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
     double uu[5], ww[5];
@@ -580,6 +673,18 @@ int fvhoe_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, in
   const long long total = (long long)nv * n0 * n1 * n2;
   const int m0 = axis == 1 ? n0 - (w.k - 1) : n0, m1 = axis == 2 ? n1 - (w.k - 1) : n1,
             m2 = axis == 3 ? n2 - (w.k - 1) : n2;
+  // large grids: the sliding-window kernel (32-bit element offsets); small ones keep one output per thread
+  if (total >= (1ll << 16) && total < (1ll << 31)) {
+    const hipStream_t st = (hipStream_t)stream;
+    int r = -1;
+    switch (kind) {
+      case 0: r = launch_slide_p<0>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 1: r = launch_slide_p<1>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 2: r = launch_slide_p<2>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 3: r = launch_slide_p<3>(p, u, out, nv, n0, n1, n2, axis, st); break;
+    }
+    if (r == 0) return launch_status();
+  }
   if ((long long)m1 * m2 >= 64 && (long long)m1 * m2 < (1ll << 30) && (long long)nv * m0 <= 65535) {
     const dim3 grid((unsigned)((m1 * m2 + TPB - 1) / TPB), (unsigned)(nv * m0));
     k_sweep2<<<grid, TPB, 0, (hipStream_t)stream>>>(u, out, n0, n1, n2, axis, w, m0, m1, m2);
