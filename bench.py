@@ -83,10 +83,12 @@ def parse():
     ap.add_argument("--cpu-stages", type=int, default=2)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run 3 device copies of 167,772,160 bytes (scale for HBM byte counters)")
-    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "kh-accuracy", "predictor"],
+    ap.add_argument("--workload", default="kh", choices=["kh", "ensemble", "kh-accuracy", "predictor", "wave3d"],
                     help="kh: the headline Kelvin-Helmholtz stage; ensemble: 512 uneven 1-D Shu-Osher runs to t=1.8; "
                          "kh-accuracy: every form to --t-end against a 2x finer FP64 reference; "
-                         "predictor: the inference kernel alone on random features")
+                         "predictor: the inference kernel alone on random features; "
+                         "wave3d: the 3-D density wave at --n3^3, p=7, LLF, no limiter")
+    ap.add_argument("--n3", type=int, default=256)
     ap.add_argument("--t-end", type=float, default=0.8)
     ap.add_argument("--ensemble-t", type=float, default=1.8)
     ap.add_argument("--ensemble-count", type=int, default=512)
@@ -194,6 +196,52 @@ def run_predictor(a):
     tf = PREDICT_FLOP_PER_CELL * n * n / (ms / 1e3) / 1e12
     print(json.dumps({"metric": "predictor inference", "value": round(tf, 1), "unit": "TFLOP/s", "ms": round(ms, 4),
                       "cells": n * n, "frac_bf16_peak": round(tf / BF16_PEAK_TFLOPS, 4)}), flush=True)
+
+
+def run_wave3d(a):
+    """The 3-D density wave rho = 1 + 0.5 sin 2 pi (x+y+z), v = (1,1,1), P = 1 on the periodic unit cube at
+    n^3, p=7, LLF, SSP-RK3, no limiter (composed path), initial state built on the device; `steps` timed
+    host-synchronised steps after one warm-up step, and the L1 density error against the exact
+    translated wave."""
+    import numpy as np
+    import torch
+
+    from fvhoe_amd import initial_conditions as ics
+    from fvhoe_amd.solver import EulerSolver
+
+    n = a.n3
+    wave = ics.Sinus(dims="xyz", rho_min_max=(0.5, 1.5), P=1, vx=1, vy=1, vz=1)
+    t0 = time.perf_counter()
+    s = EulerSolver(w0=wave, nx=n, ny=n, nz=n, px=7, py=7, pz=7, gamma=1.4, riemann_solver="llf",
+                    a_posteriori_slope_limiting=False, progress_bar=False, ic_device=True)
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    s.integrator = "ssprk3"
+    for _ in range(max(a.warmup, 1)):
+        s.take_step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s.take_step()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    ms = sec / a.steps * 1e3
+    # exact solution: the initial averages translated by t (1, 1, 1)
+    X, Y, Z = ics.device_mesh((n, n, n))
+    h = (1.0 / n,) * 3
+    exact = ics.fv_average(lambda x, y, z: ics.primitives_to_conservatives(wave.build_ic()(x - s.t, y - s.t, z - s.t), 1.4),
+                           X, Y, Z, h, (7, 7, 7))
+    l1 = float((s.u[0] - exact[0]).abs().mean().item())
+    value = n ** 3 * 3 / (ms / 1e3) / 1e6
+    bytes_alg = 40 * (5 / 3 + 1)  # FP64, no limiter: 106.7 B per cell-stage
+    gbs = bytes_alg * value * 1e6 / 1e9
+    print(json.dumps({"metric": "Mcell-updates/s per RK stage", "value": round(value, 1), "unit": "Mcell-updates/s",
+                      "ms_per_step": round(ms, 2), "steps": a.steps, "dtype": "f64",
+                      "config": {"workload": f"density_wave_{n}^3_p7_llf_ssprk3_nolimiter", "path": "composed"},
+                      "setup_s": round(setup, 2), "t": s.t, "l1_density_vs_exact": l1,
+                      "stage_roofline": {"bytes_per_cell_stage": round(bytes_alg, 1), "achieved": round(gbs, 1),
+                                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}),
+          flush=True)
 
 
 def run_ensemble(a):
@@ -323,6 +371,9 @@ def main():
         return
     if a.workload == "predictor":
         run_predictor(a)
+        return
+    if a.workload == "wave3d":
+        run_wave3d(a)
         return
     import torch
 

@@ -85,7 +85,8 @@ __global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, do
 
 // The same sweep with compile-time weights, each thread sliding a register
 // window over R consecutive outputs along the sweep axis: every input is
-// loaded (R + K - 1) / R times instead of K times.  Strided axes (x, y):
+// loaded (R + K - 1) / R times instead of K times (R = 16 along x and y, 2
+// along z).  Strided axes (x, y):
 // threads run along the contiguous z axis, so every load is coalesced; the
 // contiguous axis (z): each thread walks its own run of z.  Zero taps are
 // skipped and the accumulation order is the reference's (bit-identical).
@@ -140,18 +141,26 @@ __global__ void __launch_bounds__(256) k_sweep_slide(const double* __restrict__ 
   }
 }
 
+// outputs per thread along x / y and along z (measured on the 3-D p=7 wave; overridable for a sweep)
+#ifndef FVHOE_SLIDE_R
+#define FVHOE_SLIDE_R 16
+#endif
+#ifndef FVHOE_SLIDE_RZ
+#define FVHOE_SLIDE_RZ 2
+#endif
+
 template <int KIND, int P>
 int launch_slide(const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
   constexpr int K = ntaps(P);
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
   if (axis != 3) {
-    constexpr int R = 8;
+    constexpr int R = FVHOE_SLIDE_R;
     const int nbk = (n2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? n1 : n0;
     const long long blocks = (long long)nbk * nb * no * nv;
     if (blocks >= (1ll << 31)) return -1;
     k_sweep_slide<KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbk, nb);
   } else {
-    constexpr int R = 4;
+    constexpr int R = FVHOE_SLIDE_RZ;
     const int nb = (ma + R - 1) / R;
     const long long threads = (long long)nb * n1 * n0 * nv;
     if (threads >= (1ll << 31)) return -1;
