@@ -310,6 +310,117 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
   }
 }
 
+// The face states and the Riemann solve in one pass: for face f between cells
+// f and f+1 along the axis, left = r-sweep of w at cell f, right = l-sweep at
+// cell f+1 (conservative_interpolation "r" / "l", fvhoe/fv.py:143-159), then
+// F = RS(left, right) (fvhoe/solver.py:573-606).  Each thread slides a
+// register window of the five variables over R consecutive faces, so w is
+// read once and only the fluxes are written; the sums and the solver are the
+// separate kernels' operations in the same order (bit-identical).
+template <int P, int D, int R, bool ZAX>
+__global__ void __launch_bounds__(256) k_face_flux(const double* __restrict__ w, double* __restrict__ out, int n0,
+                                                   int n1, int n2, int nblk_fast, int nb_axis, int solver,
+                                                   double gamma, double csq_floor) {
+  constexpr int K = ntaps(P), NW = R + K;
+  const int axis = D + 1;
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
+  const int m0 = axis == 1 ? mf : n0, m1 = axis == 2 ? mf : n1, m2 = axis == 3 ? mf : n2;
+  int i, j, k, a0;
+  if (!ZAX) {
+    k = (int)(blockIdx.x % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+    if (k >= n2) return;
+    const unsigned rest = blockIdx.x / (unsigned)nblk_fast;
+    const int ab = (int)(rest % (unsigned)nb_axis), o = (int)(rest / (unsigned)nb_axis);
+    if (o >= (axis == 1 ? n1 : n0)) return;
+    a0 = ab * R;
+    i = axis == 1 ? a0 : o;
+    j = axis == 1 ? o : a0;
+  } else {
+    const unsigned g = blockIdx.x * 256u + threadIdx.x;
+    const int kb = (int)(g % (unsigned)nb_axis);
+    const unsigned rest = g / (unsigned)nb_axis;
+    j = (int)(rest % (unsigned)n1);
+    i = (int)(rest / (unsigned)n1);
+    if (i >= n0) return;
+    a0 = kb * R;
+    k = a0;
+  }
+  const int s = axis == 1 ? n1 * n2 : axis == 2 ? n2 : 1;
+  const int nc = n0 * n1 * n2, no = m0 * m1 * m2;
+  const int in0 = (i * n1 + j) * n2 + k;
+  double win[5][NW];
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+#pragma unroll
+    for (int r = 0; r < NW; ++r) win[q][r] = a0 + r < na ? w[q * nc + in0 + r * s] : 0.0;
+  const int os = axis == 1 ? m1 * m2 : axis == 2 ? m2 : 1;
+  const int o0 = ((axis == 1 ? a0 : i) * m1 + (axis == 2 ? a0 : j)) * m2 + (axis == 3 ? a0 : k);
+#pragma unroll
+  for (int f = 0; f < R; ++f) {
+    if (a0 + f < mf) {
+      double left[5], right[5], F[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        double a = 0.0, b = 0.0;
+        static_for<K>([&](auto I) {
+          constexpr int t = decltype(I)::value;
+          constexpr double wr = weight(KIND_R, P, t), wl = weight(KIND_L, P, t);
+          if constexpr (wr != 0.0) a = a + wr * win[q][f + t];
+          if constexpr (wl != 0.0) b = b + wl * win[q][f + 1 + t];
+        });
+        left[q] = a;
+        right[q] = b;
+      }
+      riemann<D>(solver, left, right, F, gamma, csq_floor);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) out[q * no + o0 + f * os] = F[q];
+    }
+  }
+}
+
+// faces per thread of the fused face-flux kernel
+#ifndef FVHOE_FACE_R
+#define FVHOE_FACE_R 4
+#endif
+
+template <int P, int D>
+int launch_face_flux(const double* w, double* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
+                     hipStream_t st) {
+  constexpr int K = ntaps(P), R = FVHOE_FACE_R;
+  const int axis = D + 1, na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
+  const int nb = (mf + R - 1) / R;
+  if (axis != 3) {
+    const int nbk = (n2 + 255) / 256, no = axis == 1 ? n1 : n0;
+    const long long blocks = (long long)nbk * nb * no;
+    if (blocks >= (1ll << 31)) return -1;
+    k_face_flux<P, D, R, false><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, solver, gamma,
+                                                                   csq_floor);
+  } else {
+    const long long threads = (long long)nb * n1 * n0;
+    if (threads >= (1ll << 31)) return -1;
+    k_face_flux<P, D, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(w, out, n0, n1, n2, 0, nb, solver,
+                                                                                  gamma, csq_floor);
+  }
+  return 0;
+}
+
+template <int D>
+int launch_face_flux_p(int p, const double* w, double* out, int n0, int n1, int n2, int solver, double gamma,
+                       double csq_floor, hipStream_t st) {
+  switch (p) {
+    case 0: return launch_face_flux<0, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 1: return launch_face_flux<1, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 2: return launch_face_flux<2, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 3: return launch_face_flux<3, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 4: return launch_face_flux<4, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 5: return launch_face_flux<5, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 6: return launch_face_flux<6, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 7: return launch_face_flux<7, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 8: return launch_face_flux<8, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  }
+  return -1;
+}
+
 __global__ void k_muscl(const double* __restrict__ u, double* __restrict__ left, double* __restrict__ right, int nv,
                         int n0, int n1, int n2, int axis, int limiter) {
   const int m0 = axis == 1 ? n0 - 2 : n0, m1 = axis == 2 ? n1 - 2 : n1, m2 = axis == 3 ? n2 - 2 : n2;
@@ -758,6 +869,23 @@ int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* ou
   if (axis == 1) k_riemann_faces<0><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
   else if (axis == 2) k_riemann_faces<1><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
   else k_riemann_faces<2><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
+  return launch_status();
+}
+
+int fvhoe_face_fluxes(const double* w, double* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
+                      double csq_floor, void* stream) {
+  if (!w || !out || axis < 1 || axis > 3 || solver < 0 || solver > 2 || n0 < 1 || n1 < 1 || n2 < 1)
+    return FVHOE_ERR_ARG;
+  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
+  if (na < ntaps(p) + 1) return FVHOE_ERR_ARG;
+  if ((long long)5 * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
+  const hipStream_t st = (hipStream_t)stream;
+  int r = -1;
+  if (axis == 1) r = launch_face_flux_p<0>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  else if (axis == 2) r = launch_face_flux_p<1>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  else r = launch_face_flux_p<2>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
   return launch_status();
 }
 

@@ -391,10 +391,18 @@ class EulerSolver(ODE):
                 wt = conservative_interpolation(wt, p[ax - 1], ax, "c")
             if limit and p[d] == 1:
                 wl_face, wr_face = MUSCL_interpolations(wt, a, slope_limiter)
+                f = self._riemann_faces(wr_face, wl_face, a, rs)
+            elif wt.shape[0] == 5 and wt.numel() < (1 << 31):
+                # face states and Riemann solve in one pass (the same operations as the three kernels)
+                shape = list(wt.shape)
+                shape[a] -= 2 * get_stencil_size(p[d]) + 1
+                f = _lib.empty(shape)
+                _lib.call("fvhoe_face_fluxes", _lib.ptr(wt), _lib.ptr(f), *wt.shape[1:], a, p[d], rs,
+                          float(self.gamma), float(self.csq_floor), _lib.stream())
             else:
                 wl_face = conservative_interpolation(wt, p[d], a, "l")
                 wr_face = conservative_interpolation(wt, p[d], a, "r")
-            f = self._riemann_faces(wr_face, wl_face, a, rs)
+                f = self._riemann_faces(wr_face, wl_face, a, rs)
             for ax in tr_order[d]:
                 f = transverse_reconstruction(f, p[ax - 1], ax)
             excess = [0, 0, 0, 0]

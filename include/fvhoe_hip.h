@@ -92,6 +92,14 @@ int fvhoe_riemann(const double* wl, const double* wr, double* out, int nv, long 
 int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* out, int n0, int n1, int n2, int axis,
                         int solver, double gamma, double csq_floor, void* stream);
 
+/* The l / r conservative interpolations along axis and the Riemann call of
+ * hydrofluxes in one pass (fvhoe/fv.py:143-159, fvhoe/solver.py:538-606):
+ * out[f] = RS(r-sweep(w)[f], l-sweep(w)[f+1]) for the 5-variable primitive
+ * array w (5, n0, n1, n2); out has axis length n_axis - (2 ceil(p/2) + 1).
+ * Bit-identical to fvhoe_stencil_sweep (kinds r, l) + fvhoe_riemann_faces. */
+int fvhoe_face_fluxes(const double* w, double* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
+                      double csq_floor, void* stream);
+
 /* MUSCL_interpolations with minmod / moncen / no limiter (fvhoe/slope_limiting.py:24-84) */
 int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
                 void* stream);
