@@ -68,6 +68,7 @@ _SIGNATURES = {
     "fvhoe_riemann_faces": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_double, c_double, _P]),
     "fvhoe_muscl": (c_int, [_P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
     "fvhoe_face_fluxes": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_double, c_double, _P]),
+    "fvhoe_stencil_sweep_zpair": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
     "fvhoe_bc_apply": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, POINTER(c_int), _P]),
     "fvhoe_bc_apply_dirichlet": (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, POINTER(c_int),
                                          POINTER(c_void_p), _P]),

@@ -378,6 +378,95 @@ __global__ void __launch_bounds__(256) k_face_flux(const double* __restrict__ w,
   }
 }
 
+// Two sweeps, along a strided axis S (x or y) and along z, in one pass: the
+// lanes of a wave hold 64 consecutive z (the z taps come from neighbouring
+// lanes through DPP shifts; 64 - (Kz - 1) lanes produce outputs) and each
+// thread slides a register window over R outputs along S.  ZFIRST: the z-sweep
+// is applied first (to every window row), else the S-sweep first.  Every
+// intermediate value is the double the separate sweeps would store, summed in
+// the same order (bit-identical to two fvhoe_stencil_sweep calls).
+template <int P, int KIND>
+__device__ __forceinline__ double zsweep(double x) {
+  constexpr int K = ntaps(P);
+  double acc = 0.0, sh = x;
+  static_for<K>([&](auto I) {
+    constexpr int t = decltype(I)::value;
+    if constexpr (t > 0) sh = lane_next(sh);  // sh = x of lane l + t
+    constexpr double wt = weight(KIND, P, t);
+    if constexpr (wt != 0.0) acc = acc + wt * sh;
+  });
+  return acc;
+}
+
+template <int KIND, int PS, int PZ, bool ZFIRST, int R>
+__global__ void __launch_bounds__(256) k_sweep_zpair(const double* __restrict__ u, double* __restrict__ out, int n0,
+                                                     int n1, int n2, int axis, int nzw, int nsb, int nother, int nv) {
+  constexpr int KS = ntaps(PS), KZ = ntaps(PZ), NW = R + KS - 1, VC = 64 - (KZ - 1);
+  const int lane = (int)threadIdx.x & 63;
+  const unsigned gw = blockIdx.x * 4u + (threadIdx.x >> 6);  // one wave: one z window, one run along S
+  const int zw = (int)(gw % (unsigned)nzw);
+  unsigned rest = gw / (unsigned)nzw;
+  const int sb = (int)(rest % (unsigned)nsb);
+  rest /= (unsigned)nsb;
+  const int o = (int)(rest % (unsigned)nother);
+  const int v = (int)(rest / (unsigned)nother);
+  if (v >= nv) return;  // whole waves only: the DPP shifts below need every lane
+  const int ns = axis == 1 ? n0 : n1, ms = ns - (KS - 1), m2 = n2 - (KZ - 1);
+  const int z = zw * VC + lane, s0 = sb * R;
+  const int i = axis == 1 ? s0 : o, j = axis == 1 ? o : s0;
+  const int ss = axis == 1 ? n1 * n2 : n2;
+  const int in0 = ((v * n0 + i) * n1 + j) * n2 + z;
+  const bool zin = z < n2;
+  double win[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) win[r] = (zin && s0 + r < ns) ? u[in0 + r * ss] : 0.0;
+  const int m0 = axis == 1 ? ms : n0, m1 = axis == 1 ? n1 : ms;
+  const int os = axis == 1 ? m1 * m2 : m2;
+  const int o0 = ((v * m0 + (axis == 1 ? s0 : i)) * m1 + (axis == 1 ? j : s0)) * m2 + z;
+  const bool zok = lane < VC && z < m2;
+  if constexpr (ZFIRST) {
+#pragma unroll
+    for (int r = 0; r < NW; ++r) win[r] = zsweep<PZ, KIND>(win[r]);
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    double acc = 0.0;
+    static_for<KS>([&](auto I) {
+      constexpr double wt = weight(KIND, PS, decltype(I)::value);
+      if constexpr (wt != 0.0) acc = acc + wt * win[q + decltype(I)::value];
+    });
+    if constexpr (!ZFIRST) acc = zsweep<PZ, KIND>(acc);
+    if (zok && s0 + q < ms) out[o0 + q * os] = acc;
+  }
+}
+
+template <int KIND, int P, bool ZFIRST>
+int launch_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+  constexpr int K = ntaps(P), R = 8, VC = 64 - (K - 1);
+  const int ns = axis == 1 ? n0 : n1, ms = ns - (K - 1), m2 = n2 - (K - 1);
+  const int nzw = (m2 + VC - 1) / VC, nsb = (ms + R - 1) / R, nother = axis == 1 ? n1 : n0;
+  const long long waves = (long long)nzw * nsb * nother * nv;
+  if (waves * 64 >= (1ll << 31)) return -1;
+  k_sweep_zpair<KIND, P, P, ZFIRST, R><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(u, out, n0, n1, n2, axis, nzw, nsb,
+                                                                                     nother, nv);
+  return 0;
+}
+
+template <int KIND, bool ZFIRST>
+int launch_zpair_p(int p, const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+  switch (p) {
+    case 1: return launch_zpair<KIND, 1, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 2: return launch_zpair<KIND, 2, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 3: return launch_zpair<KIND, 3, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 4: return launch_zpair<KIND, 4, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 5: return launch_zpair<KIND, 5, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 6: return launch_zpair<KIND, 6, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 7: return launch_zpair<KIND, 7, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 8: return launch_zpair<KIND, 8, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+  }
+  return -1;
+}
+
 // faces per thread of the fused face-flux kernel
 #ifndef FVHOE_FACE_R
 #define FVHOE_FACE_R 4
@@ -869,6 +958,26 @@ int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* ou
   if (axis == 1) k_riemann_faces<0><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
   else if (axis == 2) k_riemann_faces<1><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
   else k_riemann_faces<2><<<g, TPB, 0, s>>>(wr_face, wl_face, out, n0, n1, n2, axis, solver, gamma, csq_floor);
+  return launch_status();
+}
+
+int fvhoe_stencil_sweep_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                              int zfirst, void* stream) {
+  if (!u || !out || nv < 1 || (axis != 1 && axis != 2) || (kind != FVHOE_STENCIL_C && kind != FVHOE_STENCIL_T))
+    return FVHOE_ERR_ARG;
+  if (p < 1 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const int k = ntaps(p), ns = axis == 1 ? n0 : n1;
+  if (ns < k || n2 < 64) return FVHOE_ERR_UNSUPPORTED;  // a z window spans a whole wave
+  if ((long long)nv * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
+  const hipStream_t st = (hipStream_t)stream;
+  int r;
+  if (kind == FVHOE_STENCIL_C)
+    r = zfirst ? launch_zpair_p<KIND_C, true>(p, u, out, nv, n0, n1, n2, axis, st)
+               : launch_zpair_p<KIND_C, false>(p, u, out, nv, n0, n1, n2, axis, st);
+  else
+    r = zfirst ? launch_zpair_p<KIND_T, true>(p, u, out, nv, n0, n1, n2, axis, st)
+               : launch_zpair_p<KIND_T, false>(p, u, out, nv, n0, n1, n2, axis, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
   return launch_status();
 }
 

@@ -357,6 +357,31 @@ __host__ __device__ __forceinline__ double from_ordered_bits(unsigned long long 
 #endif
 }
 
+// Values of neighbouring lanes of a 64-lane wave through DPP wavefront shifts
+// (0x130: lane l receives lane l + 1; 0x138: lane l - 1; lanes shifted in
+// from outside the wave read 0).
+__device__ __forceinline__ int dpp_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
+__device__ __forceinline__ int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float x) { return __int_as_float(dpp_next(__float_as_int(x))); }
+__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(dpp_prev(__float_as_int(x))); }
+__device__ __forceinline__ double lane_next(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_next((int)(b & 0xffffffffll)), hi = dpp_next((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_prev(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = dpp_prev((int)(b & 0xffffffffll)), hi = dpp_prev((int)(b >> 32));
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// x of lane l + O
+template <int O, typename T>
+__device__ __forceinline__ T lane_at(T x) {
+  if constexpr (O == 0) return x;
+  else if constexpr (O > 0) return lane_at<O - 1>(lane_next(x));
+  else return lane_at<O + 1>(lane_prev(x));
+}
+
 // boundary kinds
 enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2, BC_FREE = 3, BC_DIRICHLET = 4 };
 

@@ -379,27 +379,6 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
 // wave yields 64 - 2H valid columns (H = c and t reaches).  Same operations in
 // the same order as k_prims, so W is identical.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int dpp_next(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xf, 0xf, false); }
-__device__ __forceinline__ int dpp_prev(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xf, 0xf, false); }
-__device__ __forceinline__ float lane_next(float x) { return __int_as_float(dpp_next(__float_as_int(x))); }
-__device__ __forceinline__ float lane_prev(float x) { return __int_as_float(dpp_prev(__float_as_int(x))); }
-__device__ __forceinline__ double lane_next(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_next((int)(b & 0xffffffffll)), hi = dpp_next((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double lane_prev(double x) {
-  const long long b = __double_as_longlong(x);
-  const int lo = dpp_prev((int)(b & 0xffffffffll)), hi = dpp_prev((int)(b >> 32));
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-// x of lane l + O (|O| <= 4)
-template <int O, typename T>
-__device__ __forceinline__ T lane_at(T x) {
-  if constexpr (O == 0) return x;
-  else if constexpr (O > 0) return lane_at<O - 1>(lane_next(x));
-  else return lane_at<O + 1>(lane_prev(x));
-}
 // stencil sweep whose taps are compile-time offsets (for lane_at)
 template <int KIND, int P, typename T, typename Get>
 __device__ __forceinline__ T sweepL(Get&& x) {

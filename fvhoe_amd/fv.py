@@ -87,15 +87,54 @@ def transverse_reconstruction(u, p: int, axis: int):
     return stencil_sweep(u, "reconstruction", p, axis)
 
 
+def _zpair(x, kind: int, p: int, axis: int, zfirst: bool):
+    """Sweeps along axis (1 or 2) and along z of one kind and degree in one pass, or None where the
+    pair kernel does not apply (p = 0, fewer than 64 cells along z)."""
+    k = 2 * get_stencil_size(p) + 1
+    oshape = list(x.shape)
+    oshape[axis] -= k - 1
+    oshape[3] -= k - 1
+    if min(oshape) < 1:
+        return None
+    out = _lib.empty(oshape)
+    st = _lib.lib().fvhoe_stencil_sweep_zpair(_lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, axis, int(zfirst),
+                                                _lib.stream())
+    if st == _lib.ERR_UNSUPPORTED:
+        return None
+    _lib.check(st, "fvhoe_stencil_sweep_zpair")
+    return out
+
+
+def sweep_chain(u, pos: str, steps):
+    """Sweeps of one kind ("c" conservative interpolation, "t" transverse reconstruction) applied in
+    order, steps = ((p, axis), ...) on a 4-D array; an x or y sweep next to a z sweep of the same degree
+    runs as one pass (the same values as the separate sweeps)."""
+    x, was_np = _lib.as_device(u)
+    kind = _lib.STENCIL[pos]
+    mode = "reconstruction" if pos == "t" else "interpolation"
+    steps = list(steps)
+    i = 0
+    while i < len(steps):
+        p, a = steps[i]
+        if x.dim() == 4 and i + 1 < len(steps) and steps[i + 1][0] == p and p > 0:
+            a2 = steps[i + 1][1]
+            pair = (a, False) if a in (1, 2) and a2 == 3 else (a2, True) if a == 3 and a2 in (1, 2) else None
+            if pair is not None:
+                y = _zpair(x, kind, p, *pair)
+                if y is not None:
+                    x = y
+                    i += 2
+                    continue
+        x = stencil_sweep(x, mode, p, a, None if pos == "t" else pos)
+        i += 1
+    return _lib.back(x, was_np)
+
+
 def interpolate_cell_centers(u, p=(0, 0, 0)):
     """c-sweeps x, then y, then z (fvhoe/fv.py:175-197)."""
-    out = conservative_interpolation(u, p[0], 1, "c")
-    out = conservative_interpolation(out, p[1], 2, "c")
-    return conservative_interpolation(out, p[2], 3, "c")
+    return sweep_chain(u, "c", ((p[0], 1), (p[1], 2), (p[2], 3)))
 
 
 def interpolate_fv_averages(u, p=(0, 0, 0)):
     """transverse sweeps x, then y, then z (fvhoe/fv.py:200-224)."""
-    out = transverse_reconstruction(u, p[0], 1)
-    out = transverse_reconstruction(out, p[1], 2)
-    return transverse_reconstruction(out, p[2], 3)
+    return sweep_chain(u, "t", ((p[0], 1), (p[1], 2), (p[2], 3)))

@@ -92,6 +92,13 @@ int fvhoe_riemann(const double* wl, const double* wr, double* out, int nv, long 
 int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* out, int n0, int n1, int n2, int axis,
                         int solver, double gamma, double csq_floor, void* stream);
 
+/* Two sweeps of one kind (c or t) and degree p in one pass, along axis (1 = x
+ * or 2 = y) and along z (3), z first if zfirst: the same result as two
+ * fvhoe_stencil_sweep calls (fvhoe/fv.py:106-224), bit for bit.  Needs
+ * n2 >= 64 (else FVHOE_ERR_UNSUPPORTED: sweep separately). */
+int fvhoe_stencil_sweep_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                              int zfirst, void* stream);
+
 /* The l / r conservative interpolations along axis and the Riemann call of
  * hydrofluxes in one pass (fvhoe/fv.py:143-159, fvhoe/solver.py:538-606):
  * out[f] = RS(r-sweep(w)[f], l-sweep(w)[f+1]) for the 5-variable primitive
