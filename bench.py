@@ -89,6 +89,8 @@ def parse():
                          "predictor: the inference kernel alone on random features; "
                          "wave3d: the 3-D density wave at --n3^3, p=7, LLF, no limiter")
     ap.add_argument("--n3", type=int, default=256)
+    ap.add_argument("--wave-t", type=float, default=None,
+                    help="wave3d: after the timed steps run each form on to this time and compare the final states")
     ap.add_argument("--t-end", type=float, default=0.8)
     ap.add_argument("--ensemble-t", type=float, default=1.8)
     ap.add_argument("--ensemble-count", type=int, default=512)
@@ -200,10 +202,11 @@ def run_predictor(a):
 
 def run_wave3d(a):
     """The 3-D density wave rho = 1 + 0.5 sin 2 pi (x+y+z), v = (1,1,1), P = 1 on the periodic unit cube at
-    n^3, p=7, LLF, SSP-RK3, no limiter (composed path), initial state built on the device; `steps` timed
-    host-synchronised steps after one warm-up step, and the L1 density error against the exact
-    translated wave."""
-    import numpy as np
+    n^3, p=7, LLF, SSP-RK3, no limiter (composed path), initial state built on the device, for each of the
+    forms fp64 (bit-identical to the reference) and fp32 (float32 intermediates): `steps` timed
+    host-synchronised steps after one warm-up step, and the L1 density error against the exact translated
+    wave.  With --wave-t T each form then runs on to t = T (the last step clipped to land on T) and the
+    L1 density errors at T against the exact wave and between the forms are reported."""
     import torch
 
     from fvhoe_amd import initial_conditions as ics
@@ -211,37 +214,64 @@ def run_wave3d(a):
 
     n = a.n3
     wave = ics.Sinus(dims="xyz", rho_min_max=(0.5, 1.5), P=1, vx=1, vy=1, vz=1)
-    t0 = time.perf_counter()
-    s = EulerSolver(w0=wave, nx=n, ny=n, nz=n, px=7, py=7, pz=7, gamma=1.4, riemann_solver="llf",
-                    a_posteriori_slope_limiting=False, progress_bar=False, ic_device=True)
-    torch.cuda.synchronize()
-    setup = time.perf_counter() - t0
-    s.integrator = "ssprk3"
-    for _ in range(max(a.warmup, 1)):
-        s.take_step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        s.take_step()
-    torch.cuda.synchronize()
-    sec = time.perf_counter() - t0
-    ms = sec / a.steps * 1e3
-    # exact solution: the initial averages translated by t (1, 1, 1)
     X, Y, Z = ics.device_mesh((n, n, n))
     h = (1.0 / n,) * 3
-    exact = ics.fv_average(lambda x, y, z: ics.primitives_to_conservatives(wave.build_ic()(x - s.t, y - s.t, z - s.t), 1.4),
-                           X, Y, Z, h, (7, 7, 7))
-    l1 = float((s.u[0] - exact[0]).abs().mean().item())
-    value = n ** 3 * 3 / (ms / 1e3) / 1e6
-    bytes_alg = 40 * (5 / 3 + 1)  # FP64, no limiter: 106.7 B per cell-stage
-    gbs = bytes_alg * value * 1e6 / 1e9
-    print(json.dumps({"metric": "Mcell-updates/s per RK stage", "value": round(value, 1), "unit": "Mcell-updates/s",
-                      "ms_per_step": round(ms, 2), "steps": a.steps, "dtype": "f64",
-                      "config": {"workload": f"density_wave_{n}^3_p7_llf_ssprk3_nolimiter", "path": "composed"},
-                      "setup_s": round(setup, 2), "t": s.t, "l1_density_vs_exact": l1,
-                      "stage_roofline": {"bytes_per_cell_stage": round(bytes_alg, 1), "achieved": round(gbs, 1),
-                                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}),
-          flush=True)
+
+    def exact_rho(t):
+        ic = wave.build_ic()
+        return ics.fv_average(lambda x, y, z: ics.primitives_to_conservatives(ic(x - t, y - t, z - t), 1.4),
+                              X, Y, Z, h, (7, 7, 7))[0]
+
+    forms = [f for f in a.forms.split(",") if f in ("fp64", "fp32")] or ["fp64"]
+    finals = {}
+    for form in forms:
+        t0 = time.perf_counter()
+        s = EulerSolver(w0=wave, nx=n, ny=n, nz=n, px=7, py=7, pz=7, gamma=1.4, riemann_solver="llf",
+                        a_posteriori_slope_limiting=False, progress_bar=False, ic_device=True, bulk_precision=form)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        s.integrator = "ssprk3"
+        for _ in range(max(a.warmup, 1)):
+            s.take_step()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            s.take_step()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        ms = sec / a.steps * 1e3
+        l1 = float((s.u[0] - exact_rho(s.t)).abs().mean().item())
+        value = n ** 3 * 3 / (ms / 1e3) / 1e6
+        # compulsory state traffic per cell-stage, SSP-RK3 minimum (5/3 state reads, 1 write): 106.7 B; the
+        # state and the RK accumulation stay FP64 in both forms
+        bytes_alg = 40 * (5 / 3 + 1)
+        gbs = bytes_alg * value * 1e6 / 1e9
+        line = {"metric": "Mcell-updates/s per RK stage", "value": round(value, 1), "unit": "Mcell-updates/s",
+                "ms_per_step": round(ms, 2), "steps": a.steps, "dtype": "f64" if form == "fp64" else "f32",
+                "config": {"workload": f"density_wave_{n}^3_p7_llf_ssprk3_nolimiter", "path": "composed",
+                           "form": form},
+                "setup_s": round(setup, 2), "t": s.t, "l1_density_vs_exact": l1,
+                "stage_roofline": {"bytes_per_cell_stage": round(bytes_alg, 1), "achieved": round(gbs, 1),
+                                   "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+        if a.wave_t:
+            t0 = time.perf_counter()
+            steps = 0
+            while s.t < a.wave_t:
+                s.take_step(target_time=a.wave_t)
+                steps += 1
+                if steps % 200 == 0:
+                    print(f"# wave3d {form}: t = {s.t:.4f} after {steps} steps", file=sys.stderr, flush=True)
+            torch.cuda.synchronize()
+            line["run_to_t"] = {"t": s.t, "steps": steps, "seconds": round(time.perf_counter() - t0, 1),
+                                "l1_density_vs_exact": float((s.u[0] - exact_rho(s.t)).abs().mean().item())}
+            finals[form] = s.u[0].clone()
+        print(json.dumps(line), flush=True)
+        del s
+        torch.cuda.empty_cache()
+    if len(finals) == 2:
+        d = float((finals["fp32"] - finals["fp64"]).abs().mean().item())
+        print(json.dumps({"metric": "wave3d L1 density difference fp32 vs fp64", "value": d, "t": a.wave_t,
+                          "n": n}), flush=True)
 
 
 def run_ensemble(a):

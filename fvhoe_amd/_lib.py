@@ -118,6 +118,11 @@ _SIGNATURES = {
                                       POINTER(FvhoePredict), _P]),
 }
 
+# float32 twins of the composed-stage entries (the FP32 bulk form): same arguments
+for _name in ("fvhoe_stencil_sweep", "fvhoe_stencil_sweep_zpair", "fvhoe_compute_primitives", "fvhoe_hydro_dt",
+              "fvhoe_face_fluxes", "fvhoe_bc_apply", "fvhoe_euler_equation"):
+    _SIGNATURES[_name + "_f32"] = _SIGNATURES[_name]
+
 _lib = None
 
 
@@ -217,6 +222,19 @@ def as_device(x, dtype=None):
     if x.device.type != "cuda":
         return x.to(device=device(), dtype=dtype).contiguous(), True
     return x.to(dtype=dtype).contiguous(), False
+
+
+def as_device_bulk(x):
+    """as_device, except that a float32 device tensor (an intermediate of the FP32 bulk form) stays float32."""
+    t = torch()
+    if isinstance(x, t.Tensor) and x.dtype == t.float32 and x.device.type == "cuda":
+        return x.contiguous(), False
+    return as_device(x)
+
+
+def sfx(x) -> str:
+    """Entry-name suffix for an array of the bulk type: "_f32" for float32, "" for float64."""
+    return "_f32" if x.dtype == torch().float32 else ""
 
 
 def empty(shape, dtype=None):

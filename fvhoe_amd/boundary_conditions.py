@@ -144,9 +144,12 @@ class BoundaryCondition:
         return hit
 
     # ----------------------------------------------------------------- fill
-    def apply(self, u, gw: Tuple[int, int, int], t: float = None):
-        """Pad u by gw per axis and fill the ghosts (fvhoe/boundary_conditions.py:306-402)."""
+    def apply(self, u, gw: Tuple[int, int, int], t: float = None, dtype=None):
+        """Pad u by gw per axis and fill the ghosts (fvhoe/boundary_conditions.py:306-402).  dtype
+        torch.float32 writes the float32 rounding of the filled state (the FP32 bulk form's first array;
+        periodic / outflow / reflective / free sides only)."""
         x, was_np = _lib.as_device(u)
+        f32 = dtype is not None and dtype == _lib.torch().float32
         nv, n0, n1, n2 = _lib.shape4(x)
         gw = tuple(int(g) for g in gw)
         for (d, kinds), g, n in zip(((d, getattr(self, d)) for d in "xyz"), gw, (n0, n1, n2)):
@@ -156,9 +159,13 @@ class BoundaryCondition:
                 raise ValueError(f"ghost cells of a None boundary along {d} are undefined (ghost width {g})")
             if g > n and any(k not in SLAB_KINDS for k in kinds):
                 raise ValueError(f"ghost width {g} exceeds the interior size {n}")
-        out = _lib.empty((nv, n0 + 2 * gw[0], n1 + 2 * gw[1], n2 + 2 * gw[2]))
+        out = _lib.empty((nv, n0 + 2 * gw[0], n1 + 2 * gw[1], n2 + 2 * gw[2]), dtype=dtype)
         codes = (_lib.ctypes.c_int * 6)(*self.codes())
-        if self.has_slabs():
+        if f32:
+            if self.has_slabs():
+                raise NotImplementedError("the float32 ghost fill takes periodic / outflow / reflective / free sides")
+            _lib.call("fvhoe_bc_apply_f32", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, _lib.stream())
+        elif self.has_slabs():
             slabs = self.device_slabs(gw, nv, t)
             ptrs = (_lib.ctypes.c_void_p * 6)(*[s.data_ptr() if s is not None else None for s in slabs])
             _lib.call("fvhoe_bc_apply_dirichlet", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, ptrs,

@@ -1,6 +1,10 @@
 // Elementary kernels: one HIP kernel per reference array function, behind
 // the C-ABI declared in include/fvhoe_hip.h.  Arrays are contiguous
-// (nvars, n0, n1, n2) float64 in the reference's layout.
+// (nvars, n0, n1, n2) float64 in the reference's layout.  The sweep, primitive,
+// CFL, face-flux, ghost-fill and divergence kernels are also instantiated for
+// float32 intermediates (the FP32 bulk form of the composed stage, entries
+// ending in _f32): their sums and Riemann solves run in float, the pressure
+// recovery, the CFL maximum and the divergence in double.
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
 
@@ -46,8 +50,9 @@ __device__ __forceinline__ void split4(long long idx, int m0, int m1, int m2, in
   *i = a - *v * m0;
 }
 
-__global__ void k_sweep(const double* __restrict__ u, double* __restrict__ out, int nv, int n0, int n1, int n2,
-                        int axis, Weights9 w) {
+template <typename T>
+__global__ void k_sweep(const T* __restrict__ u, T* __restrict__ out, int nv, int n0, int n1, int n2, int axis,
+                        Weights9 w) {
   const int m0 = axis == 1 ? n0 - (w.k - 1) : n0;
   const int m1 = axis == 2 ? n1 - (w.k - 1) : n1;
   const int m2 = axis == 3 ? n2 - (w.k - 1) : n2;
@@ -58,17 +63,18 @@ __global__ void k_sweep(const double* __restrict__ u, double* __restrict__ out, 
     int v, i, j, k;
     split4(idx, m0, m1, m2, &v, &i, &j, &k);
     const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
-    double acc = 0.0;
+    T acc = T(0);
     for (int t = 0; t < w.k; ++t)
-      if (w.w[t] != 0.0) acc = acc + w.w[t] * u[base + t * s];
+      if (w.w[t] != 0.0) acc = acc + T(w.w[t]) * u[base + t * s];
     out[idx] = acc;
   }
 }
 
 // The same sweep over a 2-D launch: blockIdx.y = v * m0 + i, x = j * m2 + k
 // (32-bit index math, taps unrolled; zero taps skipped as above).
-__global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, double* __restrict__ out, int n0, int n1,
-                                               int n2, int axis, Weights9 w, int m0, int m1, int m2) {
+template <typename T>
+__global__ void __launch_bounds__(256) k_sweep2(const T* __restrict__ u, T* __restrict__ out, int n0, int n1, int n2,
+                                               int axis, Weights9 w, int m0, int m1, int m2) {
   const int jk = blockIdx.x * blockDim.x + threadIdx.x;
   if (jk >= m1 * m2) return;
   const int vi = blockIdx.y;
@@ -76,10 +82,10 @@ __global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, do
   const int j = m2 == 1 ? jk : jk / m2, k = m2 == 1 ? 0 : jk - (jk / m2) * m2;
   const long long base = (((long long)v * n0 + i) * n1 + j) * n2 + k;
   const long long s = axis == 1 ? (long long)n1 * n2 : axis == 2 ? n2 : 1;
-  double acc = 0.0;
+  T acc = T(0);
 #pragma unroll
   for (int t = 0; t < 9; ++t)
-    if (t < w.k && w.w[t] != 0.0) acc = acc + w.w[t] * u[base + t * s];
+    if (t < w.k && w.w[t] != 0.0) acc = acc + T(w.w[t]) * u[base + t * s];
   out[(long long)vi * (m1 * m2) + jk] = acc;
 }
 
@@ -90,9 +96,9 @@ __global__ void __launch_bounds__(256) k_sweep2(const double* __restrict__ u, do
 // threads run along the contiguous z axis, so every load is coalesced; the
 // contiguous axis (z): each thread walks its own run of z.  Zero taps are
 // skipped and the accumulation order is the reference's (bit-identical).
-template <int KIND, int P, int R, bool ZAX>
-__global__ void __launch_bounds__(256) k_sweep_slide(const double* __restrict__ u, double* __restrict__ out, int n0,
-                                                     int n1, int n2, int axis, int nblk_fast, int nb_axis) {
+template <typename T, int KIND, int P, int R, bool ZAX>
+__global__ void __launch_bounds__(256) k_sweep_slide(const T* __restrict__ u, T* __restrict__ out, int n0, int n1,
+                                                     int n2, int axis, int nblk_fast, int nb_axis) {
   constexpr int K = ntaps(P), NW = R + K - 1;
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
   const int m0 = axis == 1 ? ma : n0, m1 = axis == 2 ? ma : n1, m2 = axis == 3 ? ma : n2;
@@ -123,18 +129,18 @@ __global__ void __launch_bounds__(256) k_sweep_slide(const double* __restrict__ 
   }
   const int s = axis == 1 ? n1 * n2 : axis == 2 ? n2 : 1;
   const int in0 = ((v * n0 + i) * n1 + j) * n2 + k;
-  double win[NW];
+  T win[NW];
 #pragma unroll
-  for (int r = 0; r < NW; ++r) win[r] = a0 + r < na ? u[in0 + r * s] : 0.0;
+  for (int r = 0; r < NW; ++r) win[r] = a0 + r < na ? u[in0 + r * s] : T(0);
   const int os = axis == 1 ? m1 * m2 : axis == 2 ? m2 : 1;
   const int o0 = ((v * m0 + (axis == 1 ? a0 : i)) * m1 + (axis == 2 ? a0 : j)) * m2 + (axis == 3 ? a0 : k);
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     if (a0 + q < ma) {
-      double acc = 0.0;
+      T acc = T(0);
       static_for<K>([&](auto I) {
         constexpr double wt = weight(KIND, P, decltype(I)::value);
-        if constexpr (wt != 0.0) acc = acc + wt * win[q + decltype(I)::value];
+        if constexpr (wt != 0.0) acc = acc + T(wt) * win[q + decltype(I)::value];
       });
       out[o0 + q * os] = acc;
     }
@@ -149,8 +155,8 @@ __global__ void __launch_bounds__(256) k_sweep_slide(const double* __restrict__ 
 #define FVHOE_SLIDE_RZ 2
 #endif
 
-template <int KIND, int P>
-int launch_slide(const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+template <typename T, int KIND, int P>
+int launch_slide(const T* u, T* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
   constexpr int K = ntaps(P);
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
   if (axis != 3) {
@@ -158,42 +164,43 @@ int launch_slide(const double* u, double* out, int nv, int n0, int n1, int n2, i
     const int nbk = (n2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? n1 : n0;
     const long long blocks = (long long)nbk * nb * no * nv;
     if (blocks >= (1ll << 31)) return -1;
-    k_sweep_slide<KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbk, nb);
+    k_sweep_slide<T, KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbk, nb);
   } else {
     constexpr int R = FVHOE_SLIDE_RZ;
     const int nb = (ma + R - 1) / R;
     const long long threads = (long long)nb * n1 * n0 * nv;
     if (threads >= (1ll << 31)) return -1;
-    k_sweep_slide<KIND, P, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(u, out, n0, n1, n2, axis, nv, nb);
+    k_sweep_slide<T, KIND, P, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(u, out, n0, n1, n2, axis, nv, nb);
   }
   return 0;
 }
 
-template <int KIND>
-int launch_slide_p(int p, const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+template <typename T, int KIND>
+int launch_slide_p(int p, const T* u, T* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
   switch (p) {
-    case 0: return launch_slide<KIND, 0>(u, out, nv, n0, n1, n2, axis, st);
-    case 1: return launch_slide<KIND, 1>(u, out, nv, n0, n1, n2, axis, st);
-    case 2: return launch_slide<KIND, 2>(u, out, nv, n0, n1, n2, axis, st);
-    case 3: return launch_slide<KIND, 3>(u, out, nv, n0, n1, n2, axis, st);
-    case 4: return launch_slide<KIND, 4>(u, out, nv, n0, n1, n2, axis, st);
-    case 5: return launch_slide<KIND, 5>(u, out, nv, n0, n1, n2, axis, st);
-    case 6: return launch_slide<KIND, 6>(u, out, nv, n0, n1, n2, axis, st);
-    case 7: return launch_slide<KIND, 7>(u, out, nv, n0, n1, n2, axis, st);
-    case 8: return launch_slide<KIND, 8>(u, out, nv, n0, n1, n2, axis, st);
+    case 0: return launch_slide<T, KIND, 0>(u, out, nv, n0, n1, n2, axis, st);
+    case 1: return launch_slide<T, KIND, 1>(u, out, nv, n0, n1, n2, axis, st);
+    case 2: return launch_slide<T, KIND, 2>(u, out, nv, n0, n1, n2, axis, st);
+    case 3: return launch_slide<T, KIND, 3>(u, out, nv, n0, n1, n2, axis, st);
+    case 4: return launch_slide<T, KIND, 4>(u, out, nv, n0, n1, n2, axis, st);
+    case 5: return launch_slide<T, KIND, 5>(u, out, nv, n0, n1, n2, axis, st);
+    case 6: return launch_slide<T, KIND, 6>(u, out, nv, n0, n1, n2, axis, st);
+    case 7: return launch_slide<T, KIND, 7>(u, out, nv, n0, n1, n2, axis, st);
+    case 8: return launch_slide<T, KIND, 8>(u, out, nv, n0, n1, n2, axis, st);
   }
   return -1;
 }
 
-__global__ void k_prims(const double* __restrict__ u, double* __restrict__ w, int nv, long long nc, double gamma) {
+template <typename T>
+__global__ void k_prims(const T* __restrict__ u, T* __restrict__ w, int nv, long long nc, double gamma) {
   for (long long c = blockIdx.x * (long long)blockDim.x + threadIdx.x; c < nc; c += (long long)gridDim.x * blockDim.x) {
     double uu[5], ww[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) uu[q] = u[q * nc + c];
+    for (int q = 0; q < 5; ++q) uu[q] = (double)u[q * nc + c];
     prims_from_cons(uu, ww, gamma);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) w[q * nc + c] = ww[q];
-    for (int q = 5; q < nv; ++q) w[q * nc + c] = u[q * nc + c] / uu[0];
+    for (int q = 0; q < 5; ++q) w[q * nc + c] = T(ww[q]);
+    for (int q = 5; q < nv; ++q) w[q * nc + c] = T((double)u[q * nc + c] / uu[0]);
   }
 }
 
@@ -235,12 +242,13 @@ __device__ __forceinline__ void block_atomic_max_pos(double x, unsigned long lon
   }
 }
 
-__global__ void k_dt_partial(const double* __restrict__ w, long long nc, int ndim, double gamma, double csq_floor,
+template <typename T>
+__global__ void k_dt_partial(const T* __restrict__ w, long long nc, int ndim, double gamma, double csq_floor,
                              unsigned long long* acc) {
   double m = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
-    const double c = sound_speed(w[i], w[4 * nc + i], gamma, csq_floor);
-    const double s = (fabs(w[nc + i]) + fabs(w[2 * nc + i])) + fabs(w[3 * nc + i]);
+    const double c = sound_speed((double)w[i], (double)w[4 * nc + i], gamma, csq_floor);
+    const double s = (fabs((double)w[nc + i]) + fabs((double)w[2 * nc + i])) + fabs((double)w[3 * nc + i]);
     m = np_max(m, s + (double)ndim * c);
   }
   block_atomic_max_pos(m, acc);
@@ -317,10 +325,11 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
 // register window of the five variables over R consecutive faces, so w is
 // read once and only the fluxes are written; the sums and the solver are the
 // separate kernels' operations in the same order (bit-identical).
-template <int P, int D, int R, bool ZAX>
-__global__ void __launch_bounds__(256) k_face_flux(const double* __restrict__ w, double* __restrict__ out, int n0,
-                                                   int n1, int n2, int nblk_fast, int nb_axis, int solver,
-                                                   double gamma, double csq_floor) {
+template <typename T, int P, int D, int R, bool ZAX>
+__global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* __restrict__ out, int n0, int n1,
+                                                   int n2, int nblk_fast, int nb_axis, int solver, double gamma_d,
+                                                   double csq_floor_d) {
+  const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
   constexpr int K = ntaps(P), NW = R + K;
   const int axis = D + 1;
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
@@ -348,25 +357,25 @@ __global__ void __launch_bounds__(256) k_face_flux(const double* __restrict__ w,
   const int s = axis == 1 ? n1 * n2 : axis == 2 ? n2 : 1;
   const int nc = n0 * n1 * n2, no = m0 * m1 * m2;
   const int in0 = (i * n1 + j) * n2 + k;
-  double win[5][NW];
+  T win[5][NW];
 #pragma unroll
   for (int q = 0; q < 5; ++q)
 #pragma unroll
-    for (int r = 0; r < NW; ++r) win[q][r] = a0 + r < na ? w[q * nc + in0 + r * s] : 0.0;
+    for (int r = 0; r < NW; ++r) win[q][r] = a0 + r < na ? w[q * nc + in0 + r * s] : T(0);
   const int os = axis == 1 ? m1 * m2 : axis == 2 ? m2 : 1;
   const int o0 = ((axis == 1 ? a0 : i) * m1 + (axis == 2 ? a0 : j)) * m2 + (axis == 3 ? a0 : k);
 #pragma unroll
   for (int f = 0; f < R; ++f) {
     if (a0 + f < mf) {
-      double left[5], right[5], F[5];
+      T left[5], right[5], F[5];
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        double a = 0.0, b = 0.0;
+        T a = T(0), b = T(0);
         static_for<K>([&](auto I) {
           constexpr int t = decltype(I)::value;
           constexpr double wr = weight(KIND_R, P, t), wl = weight(KIND_L, P, t);
-          if constexpr (wr != 0.0) a = a + wr * win[q][f + t];
-          if constexpr (wl != 0.0) b = b + wl * win[q][f + 1 + t];
+          if constexpr (wr != 0.0) a = a + T(wr) * win[q][f + t];
+          if constexpr (wl != 0.0) b = b + T(wl) * win[q][f + 1 + t];
         });
         left[q] = a;
         right[q] = b;
@@ -385,22 +394,22 @@ __global__ void __launch_bounds__(256) k_face_flux(const double* __restrict__ w,
 // is applied first (to every window row), else the S-sweep first.  Every
 // intermediate value is the double the separate sweeps would store, summed in
 // the same order (bit-identical to two fvhoe_stencil_sweep calls).
-template <int P, int KIND>
-__device__ __forceinline__ double zsweep(double x) {
+template <int P, int KIND, typename T>
+__device__ __forceinline__ T zsweep(T x) {
   constexpr int K = ntaps(P);
-  double acc = 0.0, sh = x;
+  T acc = T(0), sh = x;
   static_for<K>([&](auto I) {
     constexpr int t = decltype(I)::value;
     if constexpr (t > 0) sh = lane_next(sh);  // sh = x of lane l + t
     constexpr double wt = weight(KIND, P, t);
-    if constexpr (wt != 0.0) acc = acc + wt * sh;
+    if constexpr (wt != 0.0) acc = acc + T(wt) * sh;
   });
   return acc;
 }
 
-template <int KIND, int PS, int PZ, bool ZFIRST, int R>
-__global__ void __launch_bounds__(256) k_sweep_zpair(const double* __restrict__ u, double* __restrict__ out, int n0,
-                                                     int n1, int n2, int axis, int nzw, int nsb, int nother, int nv) {
+template <typename T, int KIND, int PS, int PZ, bool ZFIRST, int R>
+__global__ void __launch_bounds__(256) k_sweep_zpair(const T* __restrict__ u, T* __restrict__ out, int n0, int n1,
+                                                     int n2, int axis, int nzw, int nsb, int nother, int nv) {
   constexpr int KS = ntaps(PS), KZ = ntaps(PZ), NW = R + KS - 1, VC = 64 - (KZ - 1);
   const int lane = (int)threadIdx.x & 63;
   const unsigned gw = blockIdx.x * 4u + (threadIdx.x >> 6);  // one wave: one z window, one run along S
@@ -417,9 +426,9 @@ __global__ void __launch_bounds__(256) k_sweep_zpair(const double* __restrict__ 
   const int ss = axis == 1 ? n1 * n2 : n2;
   const int in0 = ((v * n0 + i) * n1 + j) * n2 + z;
   const bool zin = z < n2;
-  double win[NW];
+  T win[NW];
 #pragma unroll
-  for (int r = 0; r < NW; ++r) win[r] = (zin && s0 + r < ns) ? u[in0 + r * ss] : 0.0;
+  for (int r = 0; r < NW; ++r) win[r] = (zin && s0 + r < ns) ? u[in0 + r * ss] : T(0);
   const int m0 = axis == 1 ? ms : n0, m1 = axis == 1 ? n1 : ms;
   const int os = axis == 1 ? m1 * m2 : m2;
   const int o0 = ((v * m0 + (axis == 1 ? s0 : i)) * m1 + (axis == 1 ? j : s0)) * m2 + z;
@@ -430,39 +439,39 @@ __global__ void __launch_bounds__(256) k_sweep_zpair(const double* __restrict__ 
   }
 #pragma unroll
   for (int q = 0; q < R; ++q) {
-    double acc = 0.0;
+    T acc = T(0);
     static_for<KS>([&](auto I) {
       constexpr double wt = weight(KIND, PS, decltype(I)::value);
-      if constexpr (wt != 0.0) acc = acc + wt * win[q + decltype(I)::value];
+      if constexpr (wt != 0.0) acc = acc + T(wt) * win[q + decltype(I)::value];
     });
     if constexpr (!ZFIRST) acc = zsweep<PZ, KIND>(acc);
     if (zok && s0 + q < ms) out[o0 + q * os] = acc;
   }
 }
 
-template <int KIND, int P, bool ZFIRST>
-int launch_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+template <typename T, int KIND, int P, bool ZFIRST>
+int launch_zpair(const T* u, T* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
   constexpr int K = ntaps(P), R = 8, VC = 64 - (K - 1);
   const int ns = axis == 1 ? n0 : n1, ms = ns - (K - 1), m2 = n2 - (K - 1);
   const int nzw = (m2 + VC - 1) / VC, nsb = (ms + R - 1) / R, nother = axis == 1 ? n1 : n0;
   const long long waves = (long long)nzw * nsb * nother * nv;
   if (waves * 64 >= (1ll << 31)) return -1;
-  k_sweep_zpair<KIND, P, P, ZFIRST, R><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(u, out, n0, n1, n2, axis, nzw, nsb,
+  k_sweep_zpair<T, KIND, P, P, ZFIRST, R><<<(unsigned)((waves + 3) / 4), 256, 0, st>>>(u, out, n0, n1, n2, axis, nzw, nsb,
                                                                                      nother, nv);
   return 0;
 }
 
-template <int KIND, bool ZFIRST>
-int launch_zpair_p(int p, const double* u, double* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
+template <typename T, int KIND, bool ZFIRST>
+int launch_zpair_p(int p, const T* u, T* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
   switch (p) {
-    case 1: return launch_zpair<KIND, 1, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 2: return launch_zpair<KIND, 2, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 3: return launch_zpair<KIND, 3, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 4: return launch_zpair<KIND, 4, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 5: return launch_zpair<KIND, 5, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 6: return launch_zpair<KIND, 6, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 7: return launch_zpair<KIND, 7, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
-    case 8: return launch_zpair<KIND, 8, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 1: return launch_zpair<T, KIND, 1, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 2: return launch_zpair<T, KIND, 2, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 3: return launch_zpair<T, KIND, 3, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 4: return launch_zpair<T, KIND, 4, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 5: return launch_zpair<T, KIND, 5, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 6: return launch_zpair<T, KIND, 6, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 7: return launch_zpair<T, KIND, 7, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
+    case 8: return launch_zpair<T, KIND, 8, ZFIRST>(u, out, nv, n0, n1, n2, axis, st);
   }
   return -1;
 }
@@ -472,8 +481,8 @@ int launch_zpair_p(int p, const double* u, double* out, int nv, int n0, int n1, 
 #define FVHOE_FACE_R 4
 #endif
 
-template <int P, int D>
-int launch_face_flux(const double* w, double* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
+template <typename T, int P, int D>
+int launch_face_flux(const T* w, T* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
                      hipStream_t st) {
   constexpr int K = ntaps(P), R = FVHOE_FACE_R;
   const int axis = D + 1, na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
@@ -482,30 +491,30 @@ int launch_face_flux(const double* w, double* out, int n0, int n1, int n2, int s
     const int nbk = (n2 + 255) / 256, no = axis == 1 ? n1 : n0;
     const long long blocks = (long long)nbk * nb * no;
     if (blocks >= (1ll << 31)) return -1;
-    k_face_flux<P, D, R, false><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, solver, gamma,
+    k_face_flux<T, P, D, R, false><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, solver, gamma,
                                                                    csq_floor);
   } else {
     const long long threads = (long long)nb * n1 * n0;
     if (threads >= (1ll << 31)) return -1;
-    k_face_flux<P, D, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(w, out, n0, n1, n2, 0, nb, solver,
+    k_face_flux<T, P, D, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(w, out, n0, n1, n2, 0, nb, solver,
                                                                                   gamma, csq_floor);
   }
   return 0;
 }
 
-template <int D>
-int launch_face_flux_p(int p, const double* w, double* out, int n0, int n1, int n2, int solver, double gamma,
-                       double csq_floor, hipStream_t st) {
+template <typename T, int D>
+int launch_face_flux_p(int p, const T* w, T* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
+                       hipStream_t st) {
   switch (p) {
-    case 0: return launch_face_flux<0, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 1: return launch_face_flux<1, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 2: return launch_face_flux<2, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 3: return launch_face_flux<3, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 4: return launch_face_flux<4, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 5: return launch_face_flux<5, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 6: return launch_face_flux<6, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 7: return launch_face_flux<7, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-    case 8: return launch_face_flux<8, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 0: return launch_face_flux<T, 0, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 1: return launch_face_flux<T, 1, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 2: return launch_face_flux<T, 2, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 3: return launch_face_flux<T, 3, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 4: return launch_face_flux<T, 4, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 5: return launch_face_flux<T, 5, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 6: return launch_face_flux<T, 6, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 7: return launch_face_flux<T, 7, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+    case 8: return launch_face_flux<T, 8, D>(w, out, n0, n1, n2, solver, gamma, csq_floor, st);
   }
   return -1;
 }
@@ -531,7 +540,8 @@ struct BcArgs {
   int n[3], g[3], lo[3], hi[3];
 };
 
-__global__ void k_bc_apply(const double* __restrict__ u, double* __restrict__ out, int nv, BcArgs a) {
+template <typename TO>
+__global__ void k_bc_apply(const double* __restrict__ u, TO* __restrict__ out, int nv, BcArgs a) {
   const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
   const long long total = (long long)nv * N0 * N1 * N2;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
@@ -544,7 +554,7 @@ __global__ void k_bc_apply(const double* __restrict__ u, double* __restrict__ ou
     const int sk = bc_source(k, a.n[2], a.g[2], a.lo[2], a.hi[2], &f2);
     double x = u[(((long long)v * a.n[0] + si) * a.n[1] + sj) * a.n[2] + sk];
     if ((f0 && v == 1) || (f1 && v == 2) || (f2 && v == 3)) x = -x;
-    out[idx] = x;
+    out[idx] = TO(x);
   }
 }
 
@@ -564,7 +574,8 @@ __global__ void k_bc_apply_slabs(const double* __restrict__ u, double* __restric
 }
 
 // dudt = 0.0 + cx*(F[i+1]-F[i]) + cy*(G[j+1]-G[j]) + cz*(H[k+1]-H[k]), c = -(1/h)  (fvhoe/solver.py:744-750)
-__global__ void k_euler_eq(const double* __restrict__ F, const double* __restrict__ G, const double* __restrict__ H,
+template <typename T>
+__global__ void k_euler_eq(const T* __restrict__ F, const T* __restrict__ G, const T* __restrict__ H,
                            double* __restrict__ dudt, int nv, int n0, int n1, int n2, int active, double cx,
                            double cy, double cz) {
   const long long total = (long long)nv * n0 * n1 * n2;
@@ -575,15 +586,15 @@ __global__ void k_euler_eq(const double* __restrict__ F, const double* __restric
     double acc = 0.0;
     if (active & 1) {
       const long long b = (((long long)v * (n0 + 1) + i) * n1 + j) * n2 + k;
-      acc = acc + cx * (F[b + (long long)n1 * n2] - F[b]);
+      acc = acc + cx * ((double)F[b + (long long)n1 * n2] - (double)F[b]);
     }
     if (active & 2) {
       const long long b = (((long long)v * n0 + i) * (n1 + 1) + j) * n2 + k;
-      acc = acc + cy * (G[b + n2] - G[b]);
+      acc = acc + cy * ((double)G[b + n2] - (double)G[b]);
     }
     if (active & 4) {
       const long long b = (((long long)v * n0 + i) * n1 + j) * (n2 + 1) + k;
-      acc = acc + cz * (H[b + 1] - H[b]);
+      acc = acc + cz * ((double)H[b + 1] - (double)H[b]);
     }
     dudt[idx] = acc;
   }
@@ -855,6 +866,123 @@ __global__ void k_advance_time_counted(double* t, const double* dt, long long* s
   steps[0] = k + 1;
 }
 
+// entry bodies shared by the float64 entries and their float32 (_f32) twins
+template <typename T>
+int stencil_sweep_impl(const T* u, T* out, int nv, int n0, int n1, int n2, int kind, int p, int axis, void* stream) {
+  if (!u || !out || nv < 1 || axis < 1 || axis > 3 || kind < 0 || kind > 3) return FVHOE_ERR_ARG;
+  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  Weights9 w{};
+  w.k = ntaps(p);
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
+  if (na < w.k) return FVHOE_ERR_ARG;
+  for (int i = 0; i < w.k; ++i) w.w[i] = weight(kind, p, i);
+  const long long total = (long long)nv * n0 * n1 * n2;
+  const int m0 = axis == 1 ? n0 - (w.k - 1) : n0, m1 = axis == 2 ? n1 - (w.k - 1) : n1,
+            m2 = axis == 3 ? n2 - (w.k - 1) : n2;
+  // large grids: the sliding-window kernel (32-bit element offsets); small ones keep one output per thread
+  if (total >= (1ll << 16) && total < (1ll << 31)) {
+    const hipStream_t st = (hipStream_t)stream;
+    int r = -1;
+    switch (kind) {
+      case 0: r = launch_slide_p<T, 0>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 1: r = launch_slide_p<T, 1>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 2: r = launch_slide_p<T, 2>(p, u, out, nv, n0, n1, n2, axis, st); break;
+      case 3: r = launch_slide_p<T, 3>(p, u, out, nv, n0, n1, n2, axis, st); break;
+    }
+    if (r == 0) return launch_status();
+  }
+  if ((long long)m1 * m2 >= 64 && (long long)m1 * m2 < (1ll << 30) && (long long)nv * m0 <= 65535) {
+    const dim3 grid((unsigned)((m1 * m2 + TPB - 1) / TPB), (unsigned)(nv * m0));
+    k_sweep2<T><<<grid, TPB, 0, (hipStream_t)stream>>>(u, out, n0, n1, n2, axis, w, m0, m1, m2);
+  } else {
+    k_sweep<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, n0, n1, n2, axis, w);
+  }
+  return launch_status();
+}
+
+template <typename T>
+int compute_primitives_impl(const T* u, T* w, int nv, long long ncell, double gamma, void* stream) {
+  if (!u || !w || nv < 5 || ncell < 0) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  k_prims<T><<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(u, w, nv, ncell, gamma);
+  return launch_status();
+}
+
+template <typename T>
+int hydro_dt_impl(const T* w, long long ncell, double h, int ndim, double CFL, double gamma, double csq_floor,
+                  double* dt_out, unsigned long long* scratch, void* stream) {
+  if (!w || !dt_out || !scratch || ncell <= 0) return FVHOE_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  k_fill_u64<<<1, 64, 0, s>>>(scratch, 1, 0ull);
+  k_dt_partial<T><<<grid_for(ncell) < 2048 ? grid_for(ncell) : 2048, TPB, 0, s>>>(w, ncell, ndim, gamma, csq_floor,
+                                                                                 scratch);
+  k_dt_final<<<1, 1, 0, s>>>(scratch, CFL * h, dt_out);
+  return launch_status();
+}
+
+template <typename T>
+int zpair_impl(const T* u, T* out, int nv, int n0, int n1, int n2, int kind, int p, int axis, int zfirst,
+               void* stream) {
+  if (!u || !out || nv < 1 || (axis != 1 && axis != 2) || (kind != FVHOE_STENCIL_C && kind != FVHOE_STENCIL_T))
+    return FVHOE_ERR_ARG;
+  if (p < 1 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const int k = ntaps(p), ns = axis == 1 ? n0 : n1;
+  if (ns < k || n2 < 64) return FVHOE_ERR_UNSUPPORTED;  // a z window spans a whole wave
+  if ((long long)nv * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
+  const hipStream_t st = (hipStream_t)stream;
+  int r;
+  if (kind == FVHOE_STENCIL_C)
+    r = zfirst ? launch_zpair_p<T, KIND_C, true>(p, u, out, nv, n0, n1, n2, axis, st)
+               : launch_zpair_p<T, KIND_C, false>(p, u, out, nv, n0, n1, n2, axis, st);
+  else
+    r = zfirst ? launch_zpair_p<T, KIND_T, true>(p, u, out, nv, n0, n1, n2, axis, st)
+               : launch_zpair_p<T, KIND_T, false>(p, u, out, nv, n0, n1, n2, axis, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
+  return launch_status();
+}
+
+template <typename T>
+int face_fluxes_impl(const T* w, T* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
+                     double csq_floor, void* stream) {
+  if (!w || !out || axis < 1 || axis > 3 || solver < 0 || solver > 2 || n0 < 1 || n1 < 1 || n2 < 1)
+    return FVHOE_ERR_ARG;
+  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
+  if (na < ntaps(p) + 1) return FVHOE_ERR_ARG;
+  if ((long long)5 * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
+  const hipStream_t st = (hipStream_t)stream;
+  int r = -1;
+  if (axis == 1) r = launch_face_flux_p<T, 0>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  else if (axis == 2) r = launch_face_flux_p<T, 1>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  else r = launch_face_flux_p<T, 2>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
+  return launch_status();
+}
+
+template <typename TO>
+int bc_apply_impl(const double* u, TO* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2, const int* bc6,
+                  void* stream) {
+  if (!u || !out || !bc6 || nv < 1) return FVHOE_ERR_ARG;
+  BcArgs a{{n0, n1, n2}, {g0, g1, g2}, {bc6[0], bc6[2], bc6[4]}, {bc6[1], bc6[3], bc6[5]}};
+  for (int d = 0; d < 3; ++d) {
+    if (a.g[d] < 0 || a.n[d] < 1) return FVHOE_ERR_ARG;
+    if (a.g[d] > 0 && a.g[d] > a.n[d]) return FVHOE_ERR_ARG;  // slabs must come from the interior
+    if (a.lo[d] < 0 || a.lo[d] > 3 || a.hi[d] < 0 || a.hi[d] > 3) return FVHOE_ERR_UNSUPPORTED;
+  }
+  const long long total = (long long)nv * (n0 + 2 * g0) * (n1 + 2 * g1) * (n2 + 2 * g2);
+  k_bc_apply<TO><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, a);
+  return launch_status();
+}
+
+template <typename T>
+int euler_equation_impl(const T* F, const T* G, const T* H, double* dudt, int nv, int n0, int n1, int n2, int active,
+                        double cx, double cy, double cz, void* stream) {
+  if (!dudt || nv < 1 || ((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * n0 * n1 * n2;
+  k_euler_eq<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz);
+  return launch_status();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C-ABI
@@ -872,42 +1000,20 @@ int fvhoe_stencil_weights(int p, int kind, double* out, int* n) {
 
 int fvhoe_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
                         void* stream) {
-  if (!u || !out || nv < 1 || axis < 1 || axis > 3 || kind < 0 || kind > 3) return FVHOE_ERR_ARG;
-  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
-  Weights9 w{};
-  w.k = ntaps(p);
-  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
-  if (na < w.k) return FVHOE_ERR_ARG;
-  for (int i = 0; i < w.k; ++i) w.w[i] = weight(kind, p, i);
-  const long long total = (long long)nv * n0 * n1 * n2;
-  const int m0 = axis == 1 ? n0 - (w.k - 1) : n0, m1 = axis == 2 ? n1 - (w.k - 1) : n1,
-            m2 = axis == 3 ? n2 - (w.k - 1) : n2;
-  // large grids: the sliding-window kernel (32-bit element offsets); small ones keep one output per thread
-  if (total >= (1ll << 16) && total < (1ll << 31)) {
-    const hipStream_t st = (hipStream_t)stream;
-    int r = -1;
-    switch (kind) {
-      case 0: r = launch_slide_p<0>(p, u, out, nv, n0, n1, n2, axis, st); break;
-      case 1: r = launch_slide_p<1>(p, u, out, nv, n0, n1, n2, axis, st); break;
-      case 2: r = launch_slide_p<2>(p, u, out, nv, n0, n1, n2, axis, st); break;
-      case 3: r = launch_slide_p<3>(p, u, out, nv, n0, n1, n2, axis, st); break;
-    }
-    if (r == 0) return launch_status();
-  }
-  if ((long long)m1 * m2 >= 64 && (long long)m1 * m2 < (1ll << 30) && (long long)nv * m0 <= 65535) {
-    const dim3 grid((unsigned)((m1 * m2 + TPB - 1) / TPB), (unsigned)(nv * m0));
-    k_sweep2<<<grid, TPB, 0, (hipStream_t)stream>>>(u, out, n0, n1, n2, axis, w, m0, m1, m2);
-  } else {
-    k_sweep<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, n0, n1, n2, axis, w);
-  }
-  return launch_status();
+  return stencil_sweep_impl<double>(u, out, nv, n0, n1, n2, kind, p, axis, stream);
+}
+
+int fvhoe_stencil_sweep_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                            void* stream) {
+  return stencil_sweep_impl<float>(u, out, nv, n0, n1, n2, kind, p, axis, stream);
 }
 
 int fvhoe_compute_primitives(const double* u, double* w, int nv, long long ncell, double gamma, void* stream) {
-  if (!u || !w || nv < 5 || ncell < 0) return FVHOE_ERR_ARG;
-  if (ncell == 0) return FVHOE_OK;
-  k_prims<<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(u, w, nv, ncell, gamma);
-  return launch_status();
+  return compute_primitives_impl<double>(u, w, nv, ncell, gamma, stream);
+}
+
+int fvhoe_compute_primitives_f32(const float* u, float* w, int nv, long long ncell, double gamma, void* stream) {
+  return compute_primitives_impl<float>(u, w, nv, ncell, gamma, stream);
 }
 
 int fvhoe_compute_conservatives(const double* w, double* u, int nv, long long ncell, double gamma, void* stream) {
@@ -927,13 +1033,12 @@ int fvhoe_compute_sound_speed(const double* w, double* c, long long ncell, doubl
 
 int fvhoe_hydro_dt(const double* w, long long ncell, double h, int ndim, double CFL, double gamma, double csq_floor,
                    double* dt_out, unsigned long long* scratch, void* stream) {
-  if (!w || !dt_out || !scratch || ncell <= 0) return FVHOE_ERR_ARG;
-  hipStream_t s = (hipStream_t)stream;
-  k_fill_u64<<<1, 64, 0, s>>>(scratch, 1, 0ull);
-  k_dt_partial<<<grid_for(ncell) < 2048 ? grid_for(ncell) : 2048, TPB, 0, s>>>(w, ncell, ndim, gamma, csq_floor,
-                                                                              scratch);
-  k_dt_final<<<1, 1, 0, s>>>(scratch, CFL * h, dt_out);
-  return launch_status();
+  return hydro_dt_impl<double>(w, ncell, h, ndim, CFL, gamma, csq_floor, dt_out, scratch, stream);
+}
+
+int fvhoe_hydro_dt_f32(const float* w, long long ncell, double h, int ndim, double CFL, double gamma,
+                       double csq_floor, double* dt_out, unsigned long long* scratch, void* stream) {
+  return hydro_dt_impl<float>(w, ncell, h, ndim, CFL, gamma, csq_floor, dt_out, scratch, stream);
 }
 
 int fvhoe_riemann(const double* wl, const double* wr, double* out, int nv, long long ncell, int solver, int dim,
@@ -963,39 +1068,22 @@ int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* ou
 
 int fvhoe_stencil_sweep_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
                               int zfirst, void* stream) {
-  if (!u || !out || nv < 1 || (axis != 1 && axis != 2) || (kind != FVHOE_STENCIL_C && kind != FVHOE_STENCIL_T))
-    return FVHOE_ERR_ARG;
-  if (p < 1 || p > 8) return FVHOE_ERR_UNSUPPORTED;
-  const int k = ntaps(p), ns = axis == 1 ? n0 : n1;
-  if (ns < k || n2 < 64) return FVHOE_ERR_UNSUPPORTED;  // a z window spans a whole wave
-  if ((long long)nv * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
-  const hipStream_t st = (hipStream_t)stream;
-  int r;
-  if (kind == FVHOE_STENCIL_C)
-    r = zfirst ? launch_zpair_p<KIND_C, true>(p, u, out, nv, n0, n1, n2, axis, st)
-               : launch_zpair_p<KIND_C, false>(p, u, out, nv, n0, n1, n2, axis, st);
-  else
-    r = zfirst ? launch_zpair_p<KIND_T, true>(p, u, out, nv, n0, n1, n2, axis, st)
-               : launch_zpair_p<KIND_T, false>(p, u, out, nv, n0, n1, n2, axis, st);
-  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
-  return launch_status();
+  return zpair_impl<double>(u, out, nv, n0, n1, n2, kind, p, axis, zfirst, stream);
+}
+
+int fvhoe_stencil_sweep_zpair_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p,
+                                  int axis, int zfirst, void* stream) {
+  return zpair_impl<float>(u, out, nv, n0, n1, n2, kind, p, axis, zfirst, stream);
 }
 
 int fvhoe_face_fluxes(const double* w, double* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
                       double csq_floor, void* stream) {
-  if (!w || !out || axis < 1 || axis > 3 || solver < 0 || solver > 2 || n0 < 1 || n1 < 1 || n2 < 1)
-    return FVHOE_ERR_ARG;
-  if (p < 0 || p > 8) return FVHOE_ERR_UNSUPPORTED;
-  const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2;
-  if (na < ntaps(p) + 1) return FVHOE_ERR_ARG;
-  if ((long long)5 * n0 * n1 * n2 >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;
-  const hipStream_t st = (hipStream_t)stream;
-  int r = -1;
-  if (axis == 1) r = launch_face_flux_p<0>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-  else if (axis == 2) r = launch_face_flux_p<1>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-  else r = launch_face_flux_p<2>(p, w, out, n0, n1, n2, solver, gamma, csq_floor, st);
-  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
-  return launch_status();
+  return face_fluxes_impl<double>(w, out, n0, n1, n2, axis, p, solver, gamma, csq_floor, stream);
+}
+
+int fvhoe_face_fluxes_f32(const float* w, float* out, int n0, int n1, int n2, int axis, int p, int solver,
+                          double gamma, double csq_floor, void* stream) {
+  return face_fluxes_impl<float>(w, out, n0, n1, n2, axis, p, solver, gamma, csq_floor, stream);
 }
 
 int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
@@ -1010,16 +1098,12 @@ int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, in
 
 int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
                    const int* bc6, void* stream) {
-  if (!u || !out || !bc6 || nv < 1) return FVHOE_ERR_ARG;
-  BcArgs a{{n0, n1, n2}, {g0, g1, g2}, {bc6[0], bc6[2], bc6[4]}, {bc6[1], bc6[3], bc6[5]}};
-  for (int d = 0; d < 3; ++d) {
-    if (a.g[d] < 0 || a.n[d] < 1) return FVHOE_ERR_ARG;
-    if (a.g[d] > 0 && a.g[d] > a.n[d]) return FVHOE_ERR_ARG;  // slabs must come from the interior
-    if (a.lo[d] < 0 || a.lo[d] > 3 || a.hi[d] < 0 || a.hi[d] > 3) return FVHOE_ERR_UNSUPPORTED;
-  }
-  const long long total = (long long)nv * (n0 + 2 * g0) * (n1 + 2 * g1) * (n2 + 2 * g2);
-  k_bc_apply<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(u, out, nv, a);
-  return launch_status();
+  return bc_apply_impl<double>(u, out, nv, n0, n1, n2, g0, g1, g2, bc6, stream);
+}
+
+int fvhoe_bc_apply_f32(const double* u, float* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                       const int* bc6, void* stream) {
+  return bc_apply_impl<float>(u, out, nv, n0, n1, n2, g0, g1, g2, bc6, stream);
 }
 
 int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
@@ -1050,10 +1134,12 @@ int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n
 
 int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
                          int n2, int active, double cx, double cy, double cz, void* stream) {
-  if (!dudt || nv < 1 || ((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
-  const long long total = (long long)nv * n0 * n1 * n2;
-  k_euler_eq<<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz);
-  return launch_status();
+  return euler_equation_impl<double>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
+}
+
+int fvhoe_euler_equation_f32(const float* F, const float* G, const float* H, double* dudt, int nv, int n0, int n1,
+                             int n2, int active, double cx, double cy, double cz, void* stream) {
+  return euler_equation_impl<float>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
 }
 
 int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream) {

@@ -2,7 +2,9 @@
 
 Each function takes a 4-D (nvars, nx, ny, nz) float64 array -- a torch
 tensor on the GPU, or a NumPy array, in which case the result comes back as
-NumPy -- and runs one HIP kernel per sweep.
+NumPy -- and runs one HIP kernel per sweep.  A float32 device tensor (an
+intermediate of the solver's FP32 bulk form) is swept in float32 and stays
+float32.
 """
 
 from __future__ import annotations
@@ -45,7 +47,7 @@ def fv_uniform_meshgen(n, x=(0, 1), y=(0, 1), z=(0, 1), slab_thickness=(0, 0, 0)
 
 def _as4(u):
     """Device tensor viewed as 4-D plus the axis shift for arrays of lower rank."""
-    x, was_np = _lib.as_device(u)
+    x, was_np = _lib.as_device_bulk(u)
     shift = 4 - x.dim()
     if shift < 0:
         raise ValueError(f"arrays of rank {x.dim()} > 4 are not supported")
@@ -71,8 +73,8 @@ def stencil_sweep(u, mode: str, p: int, axis: int, pos: str = None):
     oshape[a4] -= k - 1
     if oshape[a4] < 1:
         raise ValueError(f"axis {axis} of length {x.shape[a4]} is shorter than the {k}-point stencil")
-    out = _lib.empty(oshape)
-    _lib.call("fvhoe_stencil_sweep", _lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, a4, _lib.stream())
+    out = _lib.empty(oshape, dtype=x.dtype)
+    _lib.call("fvhoe_stencil_sweep" + _lib.sfx(x), _lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, a4, _lib.stream())
     out = out.reshape(tuple(oshape[shift:]))
     return _lib.back(out, was_np)
 
@@ -96,12 +98,12 @@ def _zpair(x, kind: int, p: int, axis: int, zfirst: bool):
     oshape[3] -= k - 1
     if min(oshape) < 1:
         return None
-    out = _lib.empty(oshape)
-    st = _lib.lib().fvhoe_stencil_sweep_zpair(_lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, axis, int(zfirst),
-                                                _lib.stream())
+    out = _lib.empty(oshape, dtype=x.dtype)
+    name = "fvhoe_stencil_sweep_zpair" + _lib.sfx(x)
+    st = getattr(_lib.lib(), name)(_lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, axis, int(zfirst), _lib.stream())
     if st == _lib.ERR_UNSUPPORTED:
         return None
-    _lib.check(st, "fvhoe_stencil_sweep_zpair")
+    _lib.check(st, name)
     return out
 
 
@@ -109,7 +111,7 @@ def sweep_chain(u, pos: str, steps):
     """Sweeps of one kind ("c" conservative interpolation, "t" transverse reconstruction) applied in
     order, steps = ((p, axis), ...) on a 4-D array; an x or y sweep next to a z sweep of the same degree
     runs as one pass (the same values as the separate sweeps)."""
-    x, was_np = _lib.as_device(u)
+    x, was_np = _lib.as_device_bulk(u)
     kind = _lib.STENCIL[pos]
     mode = "reconstruction" if pos == "t" else "interpolation"
     steps = list(steps)

@@ -69,10 +69,10 @@ def _ncell(x):
 
 def compute_primitives(hs: HydroState, u, gamma: float):
     """v = m/rho, P = (gamma-1)(E - 0.5 sum(m v)) (fvhoe/hydro.py:146-170)."""
-    x, was_np = _lib.as_device(u)
-    out = _lib.empty(x.shape)
-    _lib.call("fvhoe_compute_primitives", _lib.ptr(x), _lib.ptr(out), x.shape[0], _ncell(x), float(gamma),
-              _lib.stream())
+    x, was_np = _lib.as_device_bulk(u)
+    out = _lib.empty(x.shape, dtype=x.dtype)
+    _lib.call("fvhoe_compute_primitives" + _lib.sfx(x), _lib.ptr(x), _lib.ptr(out), x.shape[0], _ncell(x),
+              float(gamma), _lib.stream())
     return _lib.back(out, was_np)
 
 
@@ -96,11 +96,11 @@ def compute_sound_speed(w, gamma: float, csq_floor: float):
 
 def hydro_dt_device(w, h: float, ndim: int, CFL: float, gamma: float, csq_floor: float):
     """hydro_dt with the result left on the device (1-element tensor)."""
-    x, _ = _lib.as_device(w)
+    x, _ = _lib.as_device_bulk(w)
     t = _lib.torch()
     dt = _lib.empty((1,))
     scratch = _lib.empty((1,), dtype=t.int64)
-    _lib.call("fvhoe_hydro_dt", _lib.ptr(x), _ncell(x), float(h), int(ndim), float(CFL), float(gamma),
+    _lib.call("fvhoe_hydro_dt" + _lib.sfx(x), _lib.ptr(x), _ncell(x), float(h), int(ndim), float(CFL), float(gamma),
               float(csq_floor), _lib.ptr(dt), _lib.ptr(scratch), _lib.stream())
     return dt
 

@@ -189,8 +189,12 @@ class EulerSolver(ODE):
         self.fused = supported if fused is None else (bool(fused) and supported)
         if fused and not supported:
             raise NotImplementedError("the fused stage does not cover this configuration")
-        if bulk_precision == "fp32" and not self.fused:
-            raise NotImplementedError("the FP32 bulk form runs on the fused stage (1-D x or 2-D x-y, 5 variables)")
+        # FP32 bulk form: the fused stage (1-D x or 2-D x-y), or the composed stage with float32 intermediates
+        # (5 variables, no a-posteriori limiting, no slab boundaries: the 3-D form)
+        self._bulk32 = bulk_precision == "fp32" and not self.fused
+        if self._bulk32 and (a_posteriori_slope_limiting or self.nvars != 5 or self.bc.has_slabs()):
+            raise NotImplementedError("the FP32 bulk form runs on the fused stage (1-D x or 2-D x-y, 5 variables) or, "
+                                      "without a-posteriori limiting or slab boundaries, on the composed stage")
         self._ws = None
         self._trouble = _lib.empty(self.n, dtype=_lib.torch().int8) if a_posteriori_slope_limiting else None
         # learned troubled-cell predictor (fvhoe_amd.predictor): pre-flagged cells get MUSCL fluxes before
@@ -357,7 +361,7 @@ class EulerSolver(ODE):
         """fvhoe/solver.py:753-792."""
         cost = [(2 if fv_average else 1) * get_stencil_size(pi) for pi in p]
         g = tuple(int(gw[i]) + cost[i] for i in range(3))
-        u_bc = self.bc.apply(u, g, t=self.t)
+        u_bc = self.bc.apply(u, g, t=self.t, dtype=_lib.torch().float32 if self._bulk32 else None)
         w = compute_primitives(self.hydro_state, interpolate_cell_centers(u_bc, p), self.gamma)
         return interpolate_fv_averages(w, p) if fv_average else w
 
@@ -395,9 +399,11 @@ class EulerSolver(ODE):
                 # face states and Riemann solve in one pass (the same operations as the three kernels)
                 shape = list(wt.shape)
                 shape[a] -= 2 * get_stencil_size(p[d]) + 1
-                f = _lib.empty(shape)
-                _lib.call("fvhoe_face_fluxes", _lib.ptr(wt), _lib.ptr(f), *wt.shape[1:], a, p[d], rs,
+                f = _lib.empty(shape, dtype=wt.dtype)
+                _lib.call("fvhoe_face_fluxes" + _lib.sfx(wt), _lib.ptr(wt), _lib.ptr(f), *wt.shape[1:], a, p[d], rs,
                           float(self.gamma), float(self.csq_floor), _lib.stream())
+            elif self._bulk32:
+                raise NotImplementedError("the FP32 composed stage needs 5 variables and < 2^31 elements per array")
             else:
                 wl_face = conservative_interpolation(wt, p[d], a, "l")
                 wr_face = conservative_interpolation(wt, p[d], a, "r")
@@ -469,7 +475,8 @@ class EulerSolver(ODE):
     def _euler_equation_into(self, F, G, H, out):
         nv, n0, n1, n2 = out.shape
         active = sum(1 << d for d in range(3) if self.active[d])
-        _lib.call("fvhoe_euler_equation", _lib.ptr(F), _lib.ptr(G), _lib.ptr(H), _lib.ptr(out), nv, n0, n1, n2,
+        first = next(f for f in (F, G, H) if f is not None)
+        _lib.call("fvhoe_euler_equation" + _lib.sfx(first), _lib.ptr(F), _lib.ptr(G), _lib.ptr(H), _lib.ptr(out), nv, n0, n1, n2,
                   active, -(1 / self.h[0]), -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
 
     def euler_equation(self, F, G, H):

@@ -130,6 +130,29 @@ int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n
 int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
                          int n2, int active, double cx, double cy, double cz, void* stream);
 
+/* FP32 bulk form of the composed stage: the same operations with float32
+ * intermediates (no reference counterpart; the reference computes in float64
+ * only).  Sums, face states and Riemann solves run in float with the weights
+ * rounded to float once; the pressure recovery (fvhoe/hydro.py:146-170), the
+ * CFL maximum (fvhoe/hydro.py:270-294) and the divergence
+ * (fvhoe/solver.py:734-751) run in double on the float values.  Shapes,
+ * argument meaning and errors as the float64 entries above.
+ * fvhoe_bc_apply_f32 reads the float64 state and writes its float32 rounding
+ * (non-slab kinds); fvhoe_euler_equation_f32 writes a float64 dudt. */
+int fvhoe_stencil_sweep_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
+                            void* stream);
+int fvhoe_stencil_sweep_zpair_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p,
+                                  int axis, int zfirst, void* stream);
+int fvhoe_compute_primitives_f32(const float* u, float* w, int nv, long long ncell, double gamma, void* stream);
+int fvhoe_hydro_dt_f32(const float* w, long long ncell, double h, int ndim, double CFL, double gamma,
+                       double csq_floor, double* dt_out, unsigned long long* scratch, void* stream);
+int fvhoe_face_fluxes_f32(const float* w, float* out, int n0, int n1, int n2, int axis, int p, int solver,
+                          double gamma, double csq_floor, void* stream);
+int fvhoe_bc_apply_f32(const double* u, float* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                       const int* bc6, void* stream);
+int fvhoe_euler_equation_f32(const float* F, const float* G, const float* H, double* dudt, int nv, int n0, int n1,
+                             int n2, int active, double cx, double cy, double cz, void* stream);
+
 /* per-variable (min, max) over ncell values: out[2v] = min, out[2v+1] = max
  * (the global range of fvhoe/slope_limiting.py:168-170,184-185); scratch 2*nv x u64 */
 int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream);

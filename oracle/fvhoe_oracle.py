@@ -5,6 +5,12 @@ that FP64 results can be compared bit for bit.  Arrays are 4-D
 ``(nvars, nx, ny, nz)`` C-order float64 as in the reference.  Variable order:
 rho, vx|mx, vy|my, vz|mz, P|E, passives (fvhoe/hydro.py:36-63).
 
+The FP32 bulk form (``Scheme(bulk="fp32")``, no reference counterpart) runs
+the same functions on float32 intermediates: the sweeps and Riemann solves in
+float32 (weights and gamma rounded to float32 once), the pressure recovery,
+CFL maximum and divergence in float64 on the float32 values, the state and
+the RK combine in float64.
+
 Only tests, smoke() and bench.py's cpu_baseline leg use this module.
 """
 
@@ -157,6 +163,8 @@ def stencil_sweep(u: np.ndarray, weights: np.ndarray, axis: int) -> np.ndarray:
     zeros (fvhoe/fv.py:106-140)."""
     k = len(weights)
     out = np.zeros_like(_shift(u, axis, 0, k - 1))
+    if u.dtype == np.float32:  # float32 sums: each product and partial sum rounded to float32
+        weights = np.asarray(weights, dtype=np.float64).astype(np.float32)
     for i, w in enumerate(weights):
         np.add(out, w * _shift(u, axis, i, k - 1 - i), out=out)
     return out
@@ -197,7 +205,10 @@ def second_order_central_difference(u: np.ndarray, axis: int) -> np.ndarray:
 
 
 def compute_primitives(u: np.ndarray, gamma: float) -> np.ndarray:
-    """fvhoe/hydro.py:146-170: v = m/rho, P = (gamma-1)(E - 0.5 sum(m v))."""
+    """fvhoe/hydro.py:146-170: v = m/rho, P = (gamma-1)(E - 0.5 sum(m v)).  A float32 u is recovered in
+    float64 and rounded back to float32."""
+    if u.dtype == np.float32:
+        return compute_primitives(u.astype(np.float64), gamma).astype(np.float32)
     w = np.empty_like(u)
     rho = u[0]
     w[0] = rho
@@ -245,6 +256,7 @@ def compute_fluxes(w: np.ndarray, u: np.ndarray, dim: str, include_pressure: boo
 
 def hydro_dt(w: np.ndarray, h: float, ndim: int, CFL: float, gamma: float, csq_floor: float) -> float:
     """CFL * h / max(|vx|+|vy|+|vz| + ndim*c) over every cell given (fvhoe/hydro.py:270-294)."""
+    w = w.astype(np.float64, copy=False)
     c = compute_sound_speed(w, gamma, csq_floor)
     va = np.abs(w[1:4])
     return float(CFL * h / np.max(np.sum(va, axis=0) + ndim * c))
@@ -736,6 +748,7 @@ class Scheme:
     bc_values: tuple = None  # per side, for dirichlet / ic sides (same nesting as bc)
     slab_buffer_size: int = 30
     t: float = 0.0  # the step's start time: every stage fills ghosts at it (fvhoe/solver.py:784)
+    bulk: str = "fp64"  # "fp32": float32 intermediates (the FP32 bulk form of the composed stage)
 
     def __post_init__(self):
         self.h = tuple((b - a) / n for (a, b), n in zip(self.domain, self.n))
@@ -756,6 +769,8 @@ def interpolate_primitives_from_conservatives(s: Scheme, u, p, gw=(0, 0, 0), fv_
     cost = [(2 if fv_average else 1) * stencil_reach(pi) for pi in p]
     g = tuple(gw[i] + cost[i] for i in range(3))
     u_bc = apply_bc(u, g, s.bc, s.bc_values, s.slab_coords, s.t)
+    if s.bulk == "fp32":
+        u_bc = u_bc.astype(np.float32)
     w = compute_primitives(interpolate_cell_centers(u_bc, p), s.gamma)
     return interpolate_fv_averages(w, p) if fv_average else w
 
@@ -793,7 +808,10 @@ def hydrofluxes(s: Scheme, u, p, slope_limiter=None):
         else:
             wl_face = conservative_interpolation(wt, p[d], a, "l")
             wr_face = conservative_interpolation(wt, p[d], a, "r")
-        f = rs(_shift(wr_face, a, 0, 1), _shift(wl_face, a, 1, 0), s.gamma, "xyz"[d], s.csq_floor)
+        gamma, csq_floor = s.gamma, s.csq_floor
+        if wt.dtype == np.float32:  # the float32 solve takes gamma and the floor rounded to float32
+            gamma, csq_floor = np.float32(gamma), np.float32(csq_floor)
+        f = rs(_shift(wr_face, a, 0, 1), _shift(wl_face, a, 1, 0), gamma, "xyz"[d], csq_floor)
         for ax in tr_order[d]:
             f = transverse_reconstruction(f, p[ax - 1], ax)
         excess = [0, 0, 0, 0]
@@ -808,6 +826,7 @@ def hydrofluxes(s: Scheme, u, p, slope_limiter=None):
 def euler_equation(s: Scheme, F, G, H):
     """dudt = 0.0 + -(1/hx)(F[1:]-F[:-1]) + ... in x, y, z order (fvhoe/solver.py:734-751)."""
     dudt = 0.0
+    F, G, H = (None if f is None else f.astype(np.float64, copy=False) for f in (F, G, H))
     if s.active[0]:
         dudt = dudt + -(1 / s.h[0]) * (F[:, 1:] - F[:, :-1])
     if s.active[1]:
