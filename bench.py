@@ -89,6 +89,7 @@ def parse():
                          "predictor: the inference kernel alone on random features; "
                          "wave3d: the 3-D density wave at --n3^3, p=7, LLF, no limiter")
     ap.add_argument("--n3", type=int, default=256)
+    ap.add_argument("--cpu-n3", type=int, default=176, help="wave3d: grid of the one-stage CPU baseline sample")
     ap.add_argument("--wave-t", type=float, default=None,
                     help="wave3d: after the timed steps run each form on to this time and compare the final states")
     ap.add_argument("--t-end", type=float, default=0.8)
@@ -253,6 +254,8 @@ def run_wave3d(a):
                 "setup_s": round(setup, 2), "t": s.t, "l1_density_vs_exact": l1,
                 "stage_roofline": {"bytes_per_cell_stage": round(bytes_alg, 1), "achieved": round(gbs, 1),
                                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+        if form == forms[0] and not a.no_cpu:
+            line["cpu_baseline"] = cpu_baseline_wave3d(a.cpu_n3)
         if a.wave_t:
             t0 = time.perf_counter()
             steps = 0
@@ -324,6 +327,25 @@ def cpu_baseline(n: int, stages: int):
     return {"value": round(n * n * stages / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
             "sample": f"{stages} a-posteriori FP64 stages of Kelvin-Helmholtz {n}^2 p=3 HLLC (NumPy oracle), "
                       f"{dt:.1f} s; {os.cpu_count()} logical cores visible, 1 used",
+            "numpy": np.__version__}
+
+
+def cpu_baseline_wave3d(n: int):
+    """NumPy oracle, one FP64 stage of the 3-D density wave at n^3 (p=7, LLF, no limiter) on one core."""
+    import numpy as np
+
+    import oracle as O
+
+    X, Y, Z = O.fv_uniform_meshgen((n, n, n))
+    one = np.ones_like(X)
+    u = O.compute_conservatives(np.stack([1 + 0.5 * np.sin(2 * np.pi * (X + Y + Z)), one, one, one, one]), 1.4)
+    s = O.Scheme(n=(n, n, n), p=(7, 7, 7), gamma=1.4, riemann_solver="llf")
+    t0 = time.perf_counter()
+    O.hydrodynamics(s, u)
+    dt = time.perf_counter() - t0
+    return {"value": round(n ** 3 / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"1 FP64 stage of the density wave at {n}^3 p=7 LLF (NumPy oracle; the cell-centre "
+                      f"sample of the point-value IC), {dt:.1f} s; {os.cpu_count()} logical cores visible, 1 used",
             "numpy": np.__version__}
 
 

@@ -122,6 +122,10 @@ _SIGNATURES = {
 for _name in ("fvhoe_stencil_sweep", "fvhoe_stencil_sweep_zpair", "fvhoe_compute_primitives", "fvhoe_hydro_dt",
               "fvhoe_face_fluxes", "fvhoe_bc_apply", "fvhoe_euler_equation"):
     _SIGNATURES[_name + "_f32"] = _SIGNATURES[_name]
+_SIGNATURES["fvhoe_euler_equation_strided"] = (c_int, [_P, POINTER(c_longlong), _P, POINTER(c_longlong), _P,
+                                                       POINTER(c_longlong), _P, c_int, c_int, c_int, c_int, c_int,
+                                                       c_double, c_double, c_double, _P])
+_SIGNATURES["fvhoe_euler_equation_strided_f32"] = _SIGNATURES["fvhoe_euler_equation_strided"]
 
 _lib = None
 

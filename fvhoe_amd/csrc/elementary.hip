@@ -574,10 +574,15 @@ __global__ void k_bc_apply_slabs(const double* __restrict__ u, double* __restric
 }
 
 // dudt = 0.0 + cx*(F[i+1]-F[i]) + cy*(G[j+1]-G[j]) + cz*(H[k+1]-H[k]), c = -(1/h)  (fvhoe/solver.py:744-750)
+// element strides (variable, axis 0, axis 1) of a face-flux array whose last axis is contiguous: the
+// fluxes may be views into larger arrays (the trimmed fluxes of hydrofluxes)
+struct Strides3 {
+  long long v, a, b;
+};
 template <typename T>
 __global__ void k_euler_eq(const T* __restrict__ F, const T* __restrict__ G, const T* __restrict__ H,
                            double* __restrict__ dudt, int nv, int n0, int n1, int n2, int active, double cx,
-                           double cy, double cz) {
+                           double cy, double cz, Strides3 fs, Strides3 gs, Strides3 hs) {
   const long long total = (long long)nv * n0 * n1 * n2;
   for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
        idx += (long long)gridDim.x * blockDim.x) {
@@ -585,15 +590,15 @@ __global__ void k_euler_eq(const T* __restrict__ F, const T* __restrict__ G, con
     split4(idx, n0, n1, n2, &v, &i, &j, &k);
     double acc = 0.0;
     if (active & 1) {
-      const long long b = (((long long)v * (n0 + 1) + i) * n1 + j) * n2 + k;
-      acc = acc + cx * ((double)F[b + (long long)n1 * n2] - (double)F[b]);
+      const long long b = v * fs.v + i * fs.a + j * fs.b + k;
+      acc = acc + cx * ((double)F[b + fs.a] - (double)F[b]);
     }
     if (active & 2) {
-      const long long b = (((long long)v * n0 + i) * (n1 + 1) + j) * n2 + k;
-      acc = acc + cy * ((double)G[b + n2] - (double)G[b]);
+      const long long b = v * gs.v + i * gs.a + j * gs.b + k;
+      acc = acc + cy * ((double)G[b + gs.b] - (double)G[b]);
     }
     if (active & 4) {
-      const long long b = (((long long)v * n0 + i) * n1 + j) * (n2 + 1) + k;
+      const long long b = v * hs.v + i * hs.a + j * hs.b + k;
       acc = acc + cz * ((double)H[b + 1] - (double)H[b]);
     }
     dudt[idx] = acc;
@@ -975,11 +980,21 @@ int bc_apply_impl(const double* u, TO* out, int nv, int n0, int n1, int n2, int 
 }
 
 template <typename T>
-int euler_equation_impl(const T* F, const T* G, const T* H, double* dudt, int nv, int n0, int n1, int n2, int active,
-                        double cx, double cy, double cz, void* stream) {
-  if (!dudt || nv < 1 || ((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
+int euler_equation_impl(const T* F, const long long* fst, const T* G, const long long* gst, const T* H,
+                        const long long* hst, double* dudt, int nv, int n0, int n1, int n2, int active, double cx,
+                        double cy, double cz, void* stream) {
+  if (!dudt || nv < 1 || n0 < 1 || n1 < 1 || n2 < 1) return FVHOE_ERR_ARG;
+  if (((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
+  const long long N0 = n0, N1 = n1, N2 = n2;
+  // contiguous (nv, n0+1, n1, n2), (nv, n0, n1+1, n2), (nv, n0, n1, n2+1) unless strides are given
+  Strides3 fs{(N0 + 1) * N1 * N2, N1 * N2, N2}, gs{N0 * (N1 + 1) * N2, (N1 + 1) * N2, N2},
+      hs{N0 * N1 * (N2 + 1), N1 * (N2 + 1), N2 + 1};
+  if (fst) fs = Strides3{fst[0], fst[1], fst[2]};
+  if (gst) gs = Strides3{gst[0], gst[1], gst[2]};
+  if (hst) hs = Strides3{hst[0], hst[1], hst[2]};
   const long long total = (long long)nv * n0 * n1 * n2;
-  k_euler_eq<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz);
+  k_euler_eq<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz,
+                                                                  fs, gs, hs);
   return launch_status();
 }
 
@@ -1134,12 +1149,26 @@ int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n
 
 int fvhoe_euler_equation(const double* F, const double* G, const double* H, double* dudt, int nv, int n0, int n1,
                          int n2, int active, double cx, double cy, double cz, void* stream) {
-  return euler_equation_impl<double>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
+  return euler_equation_impl<double>(F, nullptr, G, nullptr, H, nullptr, dudt, nv, n0, n1, n2, active, cx, cy, cz,
+                                     stream);
+}
+
+int fvhoe_euler_equation_strided(const double* F, const long long* fst, const double* G, const long long* gst,
+                                 const double* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                                 int active, double cx, double cy, double cz, void* stream) {
+  return euler_equation_impl<double>(F, fst, G, gst, H, hst, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
+}
+
+int fvhoe_euler_equation_strided_f32(const float* F, const long long* fst, const float* G, const long long* gst,
+                                     const float* H, const long long* hst, double* dudt, int nv, int n0, int n1,
+                                     int n2, int active, double cx, double cy, double cz, void* stream) {
+  return euler_equation_impl<float>(F, fst, G, gst, H, hst, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
 }
 
 int fvhoe_euler_equation_f32(const float* F, const float* G, const float* H, double* dudt, int nv, int n0, int n1,
                              int n2, int active, double cx, double cy, double cz, void* stream) {
-  return euler_equation_impl<float>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
+  return euler_equation_impl<float>(F, nullptr, G, nullptr, H, nullptr, dudt, nv, n0, n1, n2, active, cx, cy, cz,
+                                    stream);
 }
 
 int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream) {

@@ -282,7 +282,7 @@ class EulerSolver(ODE):
             if self._trouble is not None:
                 self._log_troubles_device(self._trouble)
         else:
-            dt, (F, G, H) = self.hydrofluxes(u, self.p)
+            dt, (F, G, H) = self.hydrofluxes(u, self.p, views=not self.a_posteriori_slope_limiting)
             if self.a_posteriori_slope_limiting:
                 F, G, H = self.revise_fluxes(u, F, G, H, dt, self.force_trouble)
             self._euler_equation_into(F, G, H, k_out)
@@ -365,8 +365,9 @@ class EulerSolver(ODE):
         w = compute_primitives(self.hydro_state, interpolate_cell_centers(u_bc, p), self.gamma)
         return interpolate_fv_averages(w, p) if fv_average else w
 
-    def hydrofluxes(self, u, p, slope_limiter: str = None):
-        """dt (device), (F, G, H) (fvhoe/solver.py:456-651)."""
+    def hydrofluxes(self, u, p, slope_limiter: str = None, views: bool = False):
+        """dt (device), (F, G, H) (fvhoe/solver.py:456-651).  views=True returns the trimmed fluxes as views
+        into the untrimmed arrays (for the strided divergence), else contiguous copies."""
         if slope_limiter is None:
             limit = False
         elif max(p) <= 1:
@@ -413,8 +414,8 @@ class EulerSolver(ODE):
             for e in range(3):
                 if self.active[e]:
                     excess[e + 1] = gw - get_stencil_size(p[e]) - 1 if e == d else gw - 2 * get_stencil_size(p[e])
-            f = f[tuple(slice(x or None, -x or None) for x in excess)].contiguous()
-            out.append(f)
+            f = f[tuple(slice(x or None, -x or None) for x in excess)]
+            out.append(f if views else f.contiguous())
         return dt, tuple(out)
 
     def _riemann_faces(self, wr_face, wl_face, axis, rs):
@@ -473,11 +474,27 @@ class EulerSolver(ODE):
         return out
 
     def _euler_equation_into(self, F, G, H, out):
+        """dudt into out from face fluxes that may be views into larger arrays (last axis contiguous)."""
         nv, n0, n1, n2 = out.shape
         active = sum(1 << d for d in range(3) if self.active[d])
         first = next(f for f in (F, G, H) if f is not None)
-        _lib.call("fvhoe_euler_equation" + _lib.sfx(first), _lib.ptr(F), _lib.ptr(G), _lib.ptr(H), _lib.ptr(out), nv, n0, n1, n2,
-                  active, -(1 / self.h[0]), -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
+        arrs, strides = [], []
+        for d, f in enumerate((F, G, H)):
+            if f is None:
+                arrs.append(None)
+                strides.append(None)
+                continue
+            want = [nv, n0, n1, n2]
+            want[d + 1] += 1
+            if list(f.shape) != want or f.dtype != first.dtype:
+                raise ValueError(f"face fluxes of shape {tuple(f.shape)} / {f.dtype} for cells {tuple(out.shape)}")
+            if f.stride(3) != 1:
+                f = f.contiguous()
+            arrs.append(f)
+            strides.append((_lib.ctypes.c_longlong * 3)(*f.stride()[:3]))
+        _lib.call("fvhoe_euler_equation_strided" + _lib.sfx(first), _lib.ptr(arrs[0]), strides[0], _lib.ptr(arrs[1]),
+                  strides[1], _lib.ptr(arrs[2]), strides[2], _lib.ptr(out), nv, n0, n1, n2, active, -(1 / self.h[0]),
+                  -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
 
     def euler_equation(self, F, G, H):
         """dudt from face fluxes (fvhoe/solver.py:734-751)."""

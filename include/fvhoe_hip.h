@@ -153,6 +153,19 @@ int fvhoe_bc_apply_f32(const double* u, float* out, int nv, int n0, int n1, int 
 int fvhoe_euler_equation_f32(const float* F, const float* G, const float* H, double* dudt, int nv, int n0, int n1,
                              int n2, int active, double cx, double cy, double cz, void* stream);
 
+/* EulerSolver.euler_equation (fvhoe/solver.py:734-751) on face-flux arrays
+ * given by element strides: fst / gst / hst = (variable, axis-0, axis-1)
+ * strides of F / G / H, whose last axis is contiguous (NULL: the contiguous
+ * shapes of fvhoe_euler_equation).  The fluxes may be views into larger
+ * arrays, as the trimmed fluxes of hydrofluxes (fvhoe/solver.py:609-649), so
+ * no trimmed copies are made.  Bit-identical to fvhoe_euler_equation. */
+int fvhoe_euler_equation_strided(const double* F, const long long* fst, const double* G, const long long* gst,
+                                 const double* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                                 int active, double cx, double cy, double cz, void* stream);
+int fvhoe_euler_equation_strided_f32(const float* F, const long long* fst, const float* G, const long long* gst,
+                                     const float* H, const long long* hst, double* dudt, int nv, int n0, int n1,
+                                     int n2, int active, double cx, double cy, double cz, void* stream);
+
 /* per-variable (min, max) over ncell values: out[2v] = min, out[2v+1] = max
  * (the global range of fvhoe/slope_limiting.py:168-170,184-185); scratch 2*nv x u64 */
 int fvhoe_minmax(const double* u, int nv, long long ncell, double* out, unsigned long long* scratch, void* stream);
