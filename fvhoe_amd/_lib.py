@@ -210,6 +210,28 @@ def _destroy_events(library, *events):
         library.fvhoe_event_destroy(c_void_p(e))
 
 
+class capture_guard:
+    """Context for a HIP graph capture: dead objects that hold device resources (solvers in reference
+    cycles with their own graphs and tensors) must not be finalised while a capture is open -- destroying
+    a graph or releasing memory is not allowed during a capture and aborts the process.  Collects them
+    on entry and keeps Python's cyclic collector off until the capture is closed."""
+
+    def __enter__(self):
+        import gc
+
+        gc.collect()
+        self._was_enabled = gc.isenabled()
+        gc.disable()
+        return self
+
+    def __exit__(self, *exc):
+        import gc
+
+        if self._was_enabled:
+            gc.enable()
+        return False
+
+
 def ptr(t) -> c_void_p:
     return c_void_p(t.data_ptr()) if t is not None else c_void_p(0)
 

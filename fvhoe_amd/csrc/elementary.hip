@@ -104,9 +104,11 @@ __global__ void __launch_bounds__(256) k_sweep_slide(const T* __restrict__ u, T*
   const int m0 = axis == 1 ? ma : n0, m1 = axis == 2 ? ma : n1, m2 = axis == 3 ? ma : n2;
   int v, i, j, k, a0;
   if (!ZAX) {  // fast index: k over n2; slow: (v, other axis, block of R along the sweep axis)
-    k = (int)(blockIdx.x % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+    // blocks of neighbouring windows (which share K - 1 input rows) on one XCD
+    const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
     if (k >= n2) return;
-    unsigned rest = blockIdx.x / (unsigned)nblk_fast;
+    unsigned rest = blk / (unsigned)nblk_fast;
     const int ab = (int)(rest % (unsigned)nb_axis);
     rest /= (unsigned)nb_axis;
     const int no = axis == 1 ? n1 : n0;
@@ -336,9 +338,10 @@ __global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* _
   const int m0 = axis == 1 ? mf : n0, m1 = axis == 2 ? mf : n1, m2 = axis == 3 ? mf : n2;
   int i, j, k, a0;
   if (!ZAX) {
-    k = (int)(blockIdx.x % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+    const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour windows on one XCD
+    k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
     if (k >= n2) return;
-    const unsigned rest = blockIdx.x / (unsigned)nblk_fast;
+    const unsigned rest = blk / (unsigned)nblk_fast;
     const int ab = (int)(rest % (unsigned)nb_axis), o = (int)(rest / (unsigned)nb_axis);
     if (o >= (axis == 1 ? n1 : n0)) return;
     a0 = ab * R;
@@ -412,7 +415,8 @@ __global__ void __launch_bounds__(256) k_sweep_zpair(const T* __restrict__ u, T*
                                                      int n2, int axis, int nzw, int nsb, int nother, int nv) {
   constexpr int KS = ntaps(PS), KZ = ntaps(PZ), NW = R + KS - 1, VC = 64 - (KZ - 1);
   const int lane = (int)threadIdx.x & 63;
-  const unsigned gw = blockIdx.x * 4u + (threadIdx.x >> 6);  // one wave: one z window, one run along S
+  // one wave: one z window, one run along S; neighbouring runs (sharing KS - 1 rows) on one XCD
+  const unsigned gw = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x) * 4u + (threadIdx.x >> 6);
   const int zw = (int)(gw % (unsigned)nzw);
   unsigned rest = gw / (unsigned)nzw;
   const int sb = (int)(rest % (unsigned)nsb);

@@ -462,4 +462,12 @@ __host__ __device__ __forceinline__ int bc_map(int c, int n, int g, int lo, int 
   return bc_source(c + g, n, g, lo, hi, flip);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
+// Give each XCD a contiguous run of tiles so neighbouring tiles (which share
+// halo data) are served by the same L2: a bijection of [0, nb).
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 }  // namespace fvhoe

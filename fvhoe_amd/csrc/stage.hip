@@ -63,6 +63,21 @@ constexpr int TPB = 256;
 #ifndef FVHOE_FLUX_B
 #define FVHOE_FLUX_B 32
 #endif
+// FP64 flux tiles (faces x rows) and threads of the FP64 / FP32 flux tiles.  FP64: 24 x 16 faces on
+// 256 threads (27 KB of LDS, 5 tiles per CU) measured 105 / 108 us per launch at KH 2048^2 against
+// 114 / 113 for 32 x 32 on 512 threads (61 KB, 2 tiles per CU)
+#ifndef FVHOE_FLUX64_A
+#define FVHOE_FLUX64_A 24
+#endif
+#ifndef FVHOE_FLUX64_B
+#define FVHOE_FLUX64_B 16
+#endif
+#ifndef FVHOE_FLUX_NT_F64
+#define FVHOE_FLUX_NT_F64 256
+#endif
+#ifndef FVHOE_FLUX_NT_F32
+#define FVHOE_FLUX_NT_F32 512
+#endif
 #ifndef FVHOE_DETECT_ROWS
 #define FVHOE_DETECT_ROWS 2
 #endif
@@ -102,13 +117,6 @@ __device__ __forceinline__ void tile_for(F&& f) {
   }
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.
-// Give each XCD a contiguous run of tiles so neighbouring tiles (which share
-// halo data) are served by the same L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int per = nb / 8, rem = nb % 8, x = b % 8, k = b / 8;
-  return x * per + (x < rem ? x : rem) + k;
-}
 
 struct Geo {
   int n0, n1;        // interior cells (n1 = 1 in 1-D)
@@ -498,9 +506,9 @@ inline int strip_blocks(int p, int rows, int NW0, int NW1) {
 // fa), rows b in [b0, b0+TB).  D = 1: y faces, rows along x.  "normal" is the
 // face-normal axis with degree PN, "trans" the other with degree PT.
 // ---------------------------------------------------------------------------
-template <typename T, int D, int PN, int PT, int TA, int TB, int NT>
+template <typename T, int D, int PN, int PT, int TA, int TB, int NT, int RS>
 __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, int NW1, Geo g, T* __restrict__ Fout,
-                                            int tilesB, int solver, double gamma_d, double csq_floor_d) {
+                                            int tilesB, double gamma_d, double csq_floor_d) {
   constexpr int EL = reach(PN);
   constexpr int RC = ereach_c(PT), RT = ereach_t(PT);
   constexpr int HN = TA + 1 + 2 * EL;
@@ -573,7 +581,7 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
         wl[q] = sweepT<KIND_R, PN, T>([&](int t) { return X[q][(fr + t) * HT1 + c]; });
         wr[q] = sweepT<KIND_L, PN, T>([&](int t) { return X[q][(fr + 1 + t) * HT1 + c]; });
       }
-      riemann<D>(solver, wl, wr, res[k], gamma, csq_floor);
+      riemann_t<RS, D>(wl, wr, res[k], gamma, csq_floor);  // the solver is a compile-time choice
     }
   }
   __syncthreads();
@@ -1304,26 +1312,35 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     if (overlap && hipEventRecord((hipEvent_t)pred->ev_join, ps) != hipSuccess) return FVHOE_ERR_LAUNCH;
   }
 
-  if (TWO_D) {
-    constexpr int TA0 = FVHOE_FLUX_A, TB0 = FVHOE_FLUX_B, TA1 = FVHOE_FLUX_B, TB1 = FVHOE_FLUX_A;
-    const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
-    {
+  // the flux tiles with the Riemann solver as a template argument (no code or registers of the others)
+  auto flux = [&](auto rs) {
+    constexpr int RS = decltype(rs)::value;
+    if (TWO_D) {
+      constexpr bool F64 = sizeof(T) == 8;
+      constexpr int FA = F64 ? FVHOE_FLUX64_A : FVHOE_FLUX_A, FB = F64 ? FVHOE_FLUX64_B : FVHOE_FLUX_B;
+      constexpr int TA0 = FA, TB0 = FB, TA1 = FB, TB1 = FA;
+      constexpr int NF = F64 ? FVHOE_FLUX_NT_F64 : FVHOE_FLUX_NT_F32;
+      const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
+      {
+        TimedLaunch tl(TK_FLUX_X, s);
+        k_flux<T, 0, P, PY, TA0, TB0, NF, RS><<<tx * ty, NF, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->gamma,
+                                                                      c->csq_floor);
+      }
+      const int fy = (L.n1 + 1 + TA1 - 1) / TA1, rx = (L.n0 + TB1 - 1) / TB1;
+      {
+        TimedLaunch tl(TK_FLUX_Y, s);
+        k_flux<T, 1, PY, P, TA1, TB1, NF, RS><<<fy * rx, NF, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->gamma,
+                                                                      c->csq_floor);
+      }
+    } else {
+      const int tx = (L.n0 + 1 + 255) / 256;
       TimedLaunch tl(TK_FLUX_X, s);
-      k_flux<T, 0, P, PY, TA0, TB0, NTILE><<<tx * ty, NTILE, 0, s>>>(W, L.NW0, L.NW1, g, F, ty, c->riemann,
-                                                                      c->gamma, c->csq_floor);
+      k_flux<T, 0, P, 0, 256, 1, 256, RS><<<tx, 256, 0, s>>>(W, L.NW0, L.NW1, g, F, 1, c->gamma, c->csq_floor);
     }
-    const int fy = (L.n1 + 1 + TA1 - 1) / TA1, rx = (L.n0 + TB1 - 1) / TB1;
-    {
-      TimedLaunch tl(TK_FLUX_Y, s);
-      k_flux<T, 1, PY, P, TA1, TB1, NTILE><<<fy * rx, NTILE, 0, s>>>(W, L.NW0, L.NW1, g, G, rx, c->riemann,
-                                                                      c->gamma, c->csq_floor);
-    }
-  } else {
-    const int tx = (L.n0 + 1 + 255) / 256;
-    TimedLaunch tl(TK_FLUX_X, s);
-    k_flux<T, 0, P, 0, 256, 1, 256><<<tx, 256, 0, s>>>(W, L.NW0, L.NW1, g, F, 1, c->riemann, c->gamma,
-                                                        c->csq_floor);
-  }
+  };
+  if (c->riemann == RS_HLLC) flux(std::integral_constant<int, RS_HLLC>{});
+  else if (c->riemann == RS_LLF) flux(std::integral_constant<int, RS_LLF>{});
+  else flux(std::integral_constant<int, RS_UPWIND>{});
 
   FinalP fp{c->riemann, c->limiter, c->gamma, c->csq_floor};
   const int nc = L.n0 * L.n1;

@@ -112,7 +112,7 @@ class Ensemble:
         s = torch.cuda.Stream()
         s.wait_stream(torch.cuda.current_stream())
         g = torch.cuda.CUDAGraph()
-        with torch.cuda.stream(s):
+        with _lib.capture_guard(), torch.cuda.stream(s):
             # recorded, not run: the state is unchanged by the capture
             with torch.cuda.graph(g, stream=s):
                 _lib.call("fvhoe_ensemble_step", _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,

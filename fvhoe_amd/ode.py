@@ -206,7 +206,7 @@ class ODE(ABC):
             s = torch.cuda.Stream()
             s.wait_stream(torch.cuda.current_stream())
             g = torch.cuda.CUDAGraph()
-            with torch.cuda.stream(s):
+            with _lib.capture_guard(), torch.cuda.stream(s):
                 with torch.cuda.graph(g, stream=s):
                     self._device_step(None if target_dev is None else 0.0, t_dev, target_dev)
                     self.step_helper_function()

@@ -60,3 +60,21 @@ def test_graph_replays_equal_eager_steps(kind, integ, form):
     a.advance(4)
     b.advance(4, graph=True)
     check("4 more replays")
+
+
+def test_capture_keeps_the_cyclic_collector_off():
+    """A dead solver in a reference cycle (with its own captured graph) must not be finalised while
+    another capture is open; the capture guard collects before the capture and keeps the collector
+    off during it."""
+    import gc
+
+    dead, _ = _pair("kh", "ssprk3")
+    dead.advance(2, graph=True)  # holds a captured graph
+    del dead
+    a, _ = _pair("kh", "ssprk3")
+    seen = []
+    helper = a.step_helper_function
+    a.step_helper_function = lambda: (seen.append(gc.isenabled()), helper())
+    a.advance(3, graph=True)  # 1 eager step, capture, 2 replays
+    assert seen == [True, False]  # the eager step with the collector on, the captured one with it off
+    assert gc.isenabled()
