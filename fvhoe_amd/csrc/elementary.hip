@@ -327,9 +327,9 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
 // register window of the five variables over R consecutive faces, so w is
 // read once and only the fluxes are written; the sums and the solver are the
 // separate kernels' operations in the same order (bit-identical).
-template <typename T, int P, int D, int R, bool ZAX>
+template <typename T, int P, int D, int R, bool ZAX, int RS>
 __global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* __restrict__ out, int n0, int n1,
-                                                   int n2, int nblk_fast, int nb_axis, int solver, double gamma_d,
+                                                   int n2, int nblk_fast, int nb_axis, double gamma_d,
                                                    double csq_floor_d) {
   const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
   constexpr int K = ntaps(P), NW = R + K;
@@ -383,7 +383,7 @@ __global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* _
         left[q] = a;
         right[q] = b;
       }
-      riemann<D>(solver, left, right, F, gamma, csq_floor);
+      riemann_t<RS, D>(left, right, F, gamma, csq_floor);  // the solver is a compile-time choice
 #pragma unroll
       for (int q = 0; q < 5; ++q) out[q * no + o0 + f * os] = F[q];
     }
@@ -485,9 +485,8 @@ int launch_zpair_p(int p, const T* u, T* out, int nv, int n0, int n1, int n2, in
 #define FVHOE_FACE_R 4
 #endif
 
-template <typename T, int P, int D>
-int launch_face_flux(const T* w, T* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
-                     hipStream_t st) {
+template <typename T, int P, int D, int RS>
+int launch_face_flux_rs(const T* w, T* out, int n0, int n1, int n2, double gamma, double csq_floor, hipStream_t st) {
   constexpr int K = ntaps(P), R = FVHOE_FACE_R;
   const int axis = D + 1, na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
   const int nb = (mf + R - 1) / R;
@@ -495,15 +494,23 @@ int launch_face_flux(const T* w, T* out, int n0, int n1, int n2, int solver, dou
     const int nbk = (n2 + 255) / 256, no = axis == 1 ? n1 : n0;
     const long long blocks = (long long)nbk * nb * no;
     if (blocks >= (1ll << 31)) return -1;
-    k_face_flux<T, P, D, R, false><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, solver, gamma,
-                                                                   csq_floor);
+    k_face_flux<T, P, D, R, false, RS><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, gamma,
+                                                                       csq_floor);
   } else {
     const long long threads = (long long)nb * n1 * n0;
     if (threads >= (1ll << 31)) return -1;
-    k_face_flux<T, P, D, R, true><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(w, out, n0, n1, n2, 0, nb, solver,
-                                                                                  gamma, csq_floor);
+    k_face_flux<T, P, D, R, true, RS><<<(unsigned)((threads + 255) / 256), 256, 0, st>>>(w, out, n0, n1, n2, 0, nb,
+                                                                                      gamma, csq_floor);
   }
   return 0;
+}
+
+template <typename T, int P, int D>
+int launch_face_flux(const T* w, T* out, int n0, int n1, int n2, int solver, double gamma, double csq_floor,
+                     hipStream_t st) {
+  if (solver == RS_HLLC) return launch_face_flux_rs<T, P, D, RS_HLLC>(w, out, n0, n1, n2, gamma, csq_floor, st);
+  if (solver == RS_LLF) return launch_face_flux_rs<T, P, D, RS_LLF>(w, out, n0, n1, n2, gamma, csq_floor, st);
+  return launch_face_flux_rs<T, P, D, RS_UPWIND>(w, out, n0, n1, n2, gamma, csq_floor, st);
 }
 
 template <typename T, int D>
