@@ -52,7 +52,7 @@ class FvhoePredict(ctypes.Structure):
 
 
 PRED_NPARAMS = 8433
-PRED_PACKED_BYTES = 20480
+PRED_PACKED_BYTES = 24576
 PRED_HALO = 6
 
 _P = c_void_p

@@ -35,17 +35,17 @@ __global__ void k_pack(const float* __restrict__ th, unsigned char* __restrict__
   float* b1 = (float*)(out + PB1);
   float* b2 = (float*)(out + PB2);
   float* b3 = (float*)(out + PB3);
-  constexpr int N1 = K1STEPS * 16 * 32, N2 = K2STEPS * 16 * 32, N3 = C2 * KK;
+  constexpr int N1 = KS * 16 * 32, N2 = KS * 3 * 16 * 32, N3 = C2 * KK;
   for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < N1 + N2 + N3 + 33; i += blockDim.x * gridDim.x) {
-    if (i < N1) {  // [step][n][k]: tap 8 step + k/4, channel k%4
-      const int s = i / 512, n = (i / 32) % 16, k = i % 32;
-      const int t = 8 * s + k / 4, c = k % 4;
-      w1s[i] = t < KK ? f2bf(th[OW1 + (n * C0 + c) * KK + t]) : (unsigned short)0;
-    } else if (i < N1 + N2) {  // [step][n][k]: tap 2 step + k/16, channel k%16
+    if (i < N1) {  // [dy][n][k]: tap (dy, k/4) for k < 20, channel k%4
+      const int dy = i / 512, n = (i / 32) % 16, k = i % 32;
+      const int dx = k / 4, c = k % 4;
+      w1s[i] = dx < KS ? f2bf(th[OW1 + (n * C0 + c) * KK + dy * KS + dx]) : (unsigned short)0;
+    } else if (i < N1 + N2) {  // [dy][pair][n][k]: tap (dy, 2 pair + k/16), channel k%16
       const int j = i - N1;
-      const int s = j / 512, n = (j / 32) % 16, k = j % 32;
-      const int t = 2 * s + k / 16, c = k % 16;
-      w2s[j] = t < KK ? f2bf(th[OW2 + (n * C1 + c) * KK + t]) : (unsigned short)0;
+      const int dy = j / 1536, pair = (j / 512) % 3, n = (j / 32) % 16, k = j % 32;
+      const int dx = 2 * pair + k / 16, c = k % 16;
+      w2s[j] = dx < KS ? f2bf(th[OW2 + (n * C1 + c) * KK + dy * KS + dx]) : (unsigned short)0;
     } else if (i < N1 + N2 + N3) {
       const int j = i - N1 - N2;
       w3s[j] = bf2f(f2bf(th[OW3 + j]));

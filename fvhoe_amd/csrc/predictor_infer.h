@@ -1,18 +1,25 @@
 // Fused inference kernel of the troubled-cell predictor (included by the
 // library sources that launch it).
 //
-// Persistent blocks of 256 threads (4 waves) walk over 16x16 output tiles.
-// Per tile: the 28x28x4 feature patch is staged in LDS as bfloat16 (channel
-// last), layer 1 (24x24 pixels) and layer 2 (20x20 pixels) run on
-// v_mfma_f32_16x16x32_bf16 with FP32 accumulation (A = 16 pixels x 32 of K,
-// B = weights held in registers for the whole kernel, D = 16 pixels x 16
-// channels), activations go back to LDS as bfloat16.  Layer 3 (16 -> 1
-// channel) is split as G[q][t] = sum_c a2[q][c] w3[c][t] for every layer-2
-// pixel q and tap t (v_mfma_f32_16x16x16_bf16, taps as the N dimension, in two
-// halves of 13 and 12 taps kept in FP32 in LDS) and z[p] = b3 + sum_t
-// G[p + offset(t)][t], one pixel per thread.  Lane l of a wave holds A row
-// l%16 and B column l%16 for the K slice 8*(l/16)..+7, and D rows
-// 4*(l/16)..+3 of column l%16.
+// Persistent blocks of 256 threads (4 waves) walk over output tiles of 16
+// rows x 28 columns (x = the contiguous axis).  Per tile the 28x40x4 feature
+// patch is staged in LDS as bfloat16 (channel last); layer 1 (24x36 pixels)
+// and layer 2 (20x32 pixels) run on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation, A = weights (16 output channels x 32 of K, held in registers
+// for the whole kernel), B = 16 consecutive pixels of one row x 32 of K,
+// D = 16 channels x 16 pixels.  Each wave owns a column group of 16 pixels
+// and a run of output rows, and slides down the input rows: an input row's
+// B operand (taps dx of one row dy offset) is loaded from LDS once and used
+// by the 5 output rows it feeds (one accumulator per output row), so each
+// activation is read from LDS 5 (rows + 4) / rows times (once per tap dx)
+// instead of 25 times (once per tap).  K of layer 1: taps dx = 0..4 of one row x 4 channels (20 of 32, the
+// rest zero); K of layer 2: a pair of taps dx x 16 channels (dx = 5 zero).
+// Layer 3 (16 -> 1 channel) is split as G[q][t] = sum_c a2[q][c] w3[c][t]
+// for every layer-2 pixel q and tap t (v_mfma_f32_16x16x16_bf16, taps as the
+// M dimension, in two halves of 13 and 12 taps kept in FP32 in LDS) and
+// z[p] = b3 + sum_t G[p + offset(t)][t].  Lane l of a wave holds A row l%16
+// and B column l%16 for the K slice 8*(l/16)..+7, and D rows 4*(l/16)..+3 of
+// column l%16.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,22 +31,24 @@ namespace pred {
 constexpr int KS = 5, KK = 25;
 constexpr int C0 = 4, C1 = 16, C2 = 16;
 constexpr int HALO = 6;
-constexpr int PT = 16;  // output tile edge
-constexpr int L0 = PT + 12, L1 = PT + 8, L2 = PT + 4;
-constexpr int K1STEPS = 4;   // layer 1: K = 25 taps x 4 channels in 4 steps of 32 (tap = 8 step + k/4, channel k%4)
-constexpr int K2STEPS = 13;  // layer 2: K = 25 taps x 16 channels in 13 steps of 32 (tap = 2 step + k/16, channel k%16)
+constexpr int PTY = 16, PTX = 28;                 // output tile: rows x columns
+constexpr int L2Y = PTY + 4, L2X = PTX + 4;       // layer-2 pixels (32 columns: 2 groups of 16)
+constexpr int L1Y = PTY + 8, L1X = PTX + 8;       // layer-1 pixels (36 columns: groups at 0, 16, 20)
+constexpr int L0Y = PTY + 12, L0X = PTX + 12;     // feature pixels
+constexpr int R1 = 6, R2 = 10;                    // output rows per wave unit in layers 1 and 2
+static_assert(L2X == 32 && L1Y % R1 == 0 && L2Y == 2 * R2, "tile geometry");
 
-// packed inference weights (bytes); taps beyond 24 are zero
-constexpr int PW1 = 0;                            // bf16 [4][16][32]  (step, n, k)
-constexpr int PW2 = PW1 + K1STEPS * 16 * 32 * 2;  // bf16 [13][16][32] (step, n, k)
-constexpr int PW3 = PW2 + K2STEPS * 16 * 32 * 2;  // f32 [16][25]      (bf16-rounded values)
+// packed inference weights (bytes)
+constexpr int PW1 = 0;                        // bf16 [5 dy][16 n][32 k]: dx = k/4 (k < 20), channel k%4
+constexpr int PW2 = PW1 + 5 * 16 * 32 * 2;    // bf16 [5 dy][3 pair][16 n][32 k]: dx = 2 pair + k/16, channel k%16
+constexpr int PW3 = PW2 + 5 * 3 * 16 * 32 * 2;  // f32 [16][25] (bf16-rounded values)
 constexpr int PB1 = PW3 + C2 * KK * 4, PB2 = PB1 + 64, PB3 = PB2 + 64, PACKED = PB3 + 64;
 
 // LDS carve-up of the inference kernel; the layer-3 tap sums G alias xin and a1
 constexpr int GT = 13;  // taps per layer-3 half
-constexpr int SM_XIN = 0, SM_A1 = SM_XIN + L0 * L0 * C0 * 2, SM_A2 = SM_A1 + L1 * L1 * C1 * 2,
-              SM_TOTAL = SM_A2 + L2 * L2 * C2 * 2;
-static_assert(L2 * L2 * GT * 4 <= SM_A2, "layer-3 tap sums fit over xin and a1");
+constexpr int SM_XIN = 0, SM_A1 = SM_XIN + L0Y * L0X * C0 * 2, SM_A2 = SM_A1 + L1Y * L1X * C1 * 2,
+              SM_TOTAL = SM_A2 + L2Y * L2X * C2 * 2;
+static_assert(L2Y * L2X * GT * 4 <= SM_A2, "layer-3 tap sums fit over xin and a1");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x4v __attribute__((ext_vector_type(4)));
@@ -101,7 +110,7 @@ constexpr int INFER_THREADS = 256;
 __device__ __forceinline__ int swz(int q, int k) { return q * 16 + 4 * k; }
 
 template <typename Feat>
-__global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, int n1,
+__global__ void __launch_bounds__(INFER_THREADS, 2) k_infer(Feat feat, int n0, int n1,
                                                         const unsigned char* __restrict__ pw, float thr,
                                                         signed char* __restrict__ flags,
                                                         float* __restrict__ logits, int* __restrict__ count,
@@ -118,12 +127,15 @@ __global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, i
   const int mrow = lane & 15, kb = lane >> 4;
   __shared__ double fconst[8];
   feat.init(fconst);  // per-block constants of the feature map (visible after the first barrier)
-  // B fragments for the whole kernel: lane (n = mrow, kb) holds B[8kb..8kb+7][n]
-  bf16x8 wb1[K1STEPS], wb2[K2STEPS];
+  // A fragments for the whole kernel: lane (n = mrow, kb) holds A[n][8kb..8kb+7]
+  bf16x8 wa1[KS], wa2[KS][3];
 #pragma unroll
-  for (int s = 0; s < K1STEPS; ++s) wb1[s] = *(const bf16x8*)(pw + PW1 + 2 * ((s * 16 + mrow) * 32 + kb * 8));
+  for (int dy = 0; dy < KS; ++dy) {
+    wa1[dy] = *(const bf16x8*)(pw + PW1 + 2 * ((dy * 16 + mrow) * 32 + kb * 8));
 #pragma unroll
-  for (int s = 0; s < K2STEPS; ++s) wb2[s] = *(const bf16x8*)(pw + PW2 + 2 * ((s * 16 + mrow) * 32 + kb * 8));
+    for (int k = 0; k < 3; ++k)
+      wa2[dy][k] = *(const bf16x8*)(pw + PW2 + 2 * (((dy * 3 + k) * 16 + mrow) * 32 + kb * 8));
+  }
   // the products are formed as D = W x pixels, so lane (mrow, kb) receives channels 4kb..4kb+3 of pixel mrow
   float bias1[4], bias2[4];
 #pragma unroll
@@ -132,7 +144,7 @@ __global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, i
     bias2[r] = ((const float*)(pw + PB2))[4 * kb + r];
   }
   const float bias3 = ((const float*)(pw + PB3))[0];
-  // layer-3 B fragments: lane (n = mrow, kb) holds w3[4kb..4kb+3][tap 13h + n] for half h
+  // layer-3 A fragments: lane (n = mrow, kb) holds w3[4kb..4kb+3][tap 13h + n] for half h
   bf16x4 wb3[2];
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
@@ -142,113 +154,158 @@ __global__ void __launch_bounds__(INFER_THREADS, 3) k_infer(Feat feat, int n0, i
     for (int i = 0; i < 4; ++i)
       wb3[h][i] = ok ? (short)f2bf(((const float*)(pw + PW3))[(4 * kb + i) * KK + t]) : (short)0;
   }
+  const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
 
   for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    const int oy0 = (tile / tiles1) * PT, ox0 = (tile % tiles1) * PT;
-    __syncthreads();  // the previous tile is done with xin / a2 / wbase
-    for (int p = tid; p < L0 * L0; p += INFER_THREADS) {
-      const int y = p / L0, x = p % L0;
+    const int oy0 = (tile / tiles1) * PTY, ox0 = (tile % tiles1) * PTX;
+    __syncthreads();  // the previous tile is done with xin / a2 / G / wbase
+    for (int p = tid; p < L0Y * L0X; p += INFER_THREADS) {
+      const int y = p / L0X, x = p - (p / L0X) * L0X;
       *(bf16x4v*)&xin[p * C0] = load_bf16(feat, oy0 + y, ox0 + x, fconst);
     }
     __syncthreads();
-    // layer 1: 24x24 pixels = 36 groups of 16; lane slice = taps 8s+2kb and 8s+2kb+1, 4 channels each
-    for (int grp = wave; grp < (L1 * L1) / 16; grp += 4) {
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      const int p = grp * 16 + mrow, py = p / L1, px = p % L1;
-      const unsigned short* base = &xin[(py * L0 + px) * C0];
+    // layer 1: 12 units = 3 column groups (x0 = 0, 16, 20; 20..31 are computed twice, identically) x 4 runs
+    // of R1 rows; lane slice kb: pixels x0 + mrow + 2kb, +1 (taps dx = 2kb, 2kb+1), 4 channels each
+    for (int u = wave; u < 12; u += 4) {
+      const int g = u % 3, y0 = (u / 3) * R1, x0 = g == 0 ? 0 : (g == 1 ? 16 : 20);
+      floatx4 acc[R1];
 #pragma unroll
-      for (int s = 0; s < K1STEPS; ++s) {
-        const int t0 = 8 * s + 2 * kb, t1 = t0 + 1;
+      for (int j = 0; j < R1; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < R1 + 4; ++r) {
+        const unsigned short* px = &xin[((y0 + r) * L0X + x0 + mrow + 2 * kb) * C0];
+        // two pixels (dx = 2kb, 2kb + 1), channels 0..3 each, as two 8-byte reads (a pixel is 8-byte
+        // aligned); for kb = 2 only dx = 4 (the K slots of dx = 5 stay zero), nothing for kb = 3
         u16x4 lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
-        if (t0 < KK) lo = *(const u16x4*)(base + ((t0 / KS) * L0 + t0 % KS) * C0);
-        if (t1 < KK) hi = *(const u16x4*)(base + ((t1 / KS) * L0 + t1 % KS) * C0);
-        const u16x8 a = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb1[s], __builtin_bit_cast(bf16x8, a), acc, 0, 0, 0);
-      }
-      // acc[j] = D[4kb + j][mrow]: channel 4kb + j of pixel grp*16 + mrow
+        if (kb < 3) lo = *(const u16x4*)px;
+        if (kb < 2) hi = *(const u16x4*)(px + C0);
+        const u16x8 b = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+        const bf16x8 bb = __builtin_bit_cast(bf16x8, b);
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] = fmaxf(acc[j] + bias1[j], 0.f);
-      const bf16x4v v = {(__bf16)acc[0], (__bf16)acc[1], (__bf16)acc[2], (__bf16)acc[3]};
-      *(bf16x4v*)&a1[swz(grp * 16 + mrow, kb)] = v;
+        for (int dy = 0; dy < KS; ++dy) {
+          const int j = r - dy;  // output row fed by input row r through tap row dy
+          if (j >= 0 && j < R1) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1[dy], bb, acc[j], 0, 0, 0);
+        }
+        const int j = r - (KS - 1);  // row j is complete: at most 5 accumulators are live at a time
+        if (j >= 0) {  // acc[j][i] = channel 4kb + i of pixel (y0 + j, x0 + mrow)
+          const bf16x4v v = {(__bf16)fmaxf(acc[j][0] + bias1[0], 0.f), (__bf16)fmaxf(acc[j][1] + bias1[1], 0.f),
+                             (__bf16)fmaxf(acc[j][2] + bias1[2], 0.f), (__bf16)fmaxf(acc[j][3] + bias1[3], 0.f)};
+          *(bf16x4v*)&a1[swz((y0 + j) * L1X + x0 + mrow, kb)] = v;
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep each row's loads with its products (register pressure)
+      }
     }
     __syncthreads();
-    // layer 2: 20x20 pixels = 25 groups of 16; lane slice = tap 2s + kb/2, channels 8(kb%2)..+7
-    for (int grp = wave; grp < (L2 * L2) / 16; grp += 4) {
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
-      const int p = grp * 16 + mrow, py = p / L2, px = p % L2;
-      const int q0 = py * L1 + px, hb = kb >> 1;
+    // layer 2: 4 units = 2 column groups (x0 = 0, 16) x 2 runs of R2 rows, one per wave; K step k: taps
+    // dx = 2k + kb/2 of the row, channels 8(kb%2)..+7 (dx = 5 is zero)
+    {
+      const int x0 = 16 * (wave & 1), y0 = R2 * (wave >> 1);
+      floatx4 acc[R2];
 #pragma unroll
-      for (int s = 0; s < K2STEPS; ++s) {
-        const int ta = 2 * s, tb = 2 * s + 1;
-        const int q = q0 + (hb ? (tb / KS) * L1 + tb % KS : (ta / KS) * L1 + ta % KS);
-        u16x8 a = {0, 0, 0, 0, 0, 0, 0, 0};
-        if (tb < KK || !hb) a = *(const u16x8*)&a1[swz(q, 2 * (kb & 1))];
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wb2[s], __builtin_bit_cast(bf16x8, a), acc, 0, 0, 0);
+      for (int j = 0; j < R2; ++j) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int r = 0; r < R2 + 4; ++r) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const int dx = 2 * k + (kb >> 1);
+          u16x8 b = zero8;
+          if (dx < KS) b = *(const u16x8*)&a1[swz((y0 + r) * L1X + x0 + mrow + dx, 2 * (kb & 1))];
+          const bf16x8 bb = __builtin_bit_cast(bf16x8, b);
+#pragma unroll
+          for (int dy = 0; dy < KS; ++dy) {
+            const int j = r - dy;
+            if (j >= 0 && j < R2) acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2[dy][k], bb, acc[j], 0, 0, 0);
+          }
+        }
+        const int j = r - (KS - 1);  // row j is complete
+        if (j >= 0) {
+          const bf16x4v v = {(__bf16)fmaxf(acc[j][0] + bias2[0], 0.f), (__bf16)fmaxf(acc[j][1] + bias2[1], 0.f),
+                             (__bf16)fmaxf(acc[j][2] + bias2[2], 0.f), (__bf16)fmaxf(acc[j][3] + bias2[3], 0.f)};
+          *(bf16x4v*)&a2[swz((y0 + j) * L2X + x0 + mrow, kb)] = v;
+        }
+        __builtin_amdgcn_sched_barrier(0);
       }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] = fmaxf(acc[j] + bias2[j], 0.f);
-      const bf16x4v v = {(__bf16)acc[0], (__bf16)acc[1], (__bf16)acc[2], (__bf16)acc[3]};
-      *(bf16x4v*)&a2[swz(grp * 16 + mrow, kb)] = v;
     }
     __syncthreads();
-    // layer 3 in two halves of taps: tap sums G on MFMA, then one gather-sum per output pixel
-    const int y = tid / PT, x = tid % PT;
-    float z = bias3;
+    // layer 3 in two halves of taps: tap sums G on MFMA, then one gather-sum per output pixel; each thread
+    // owns output pixels tid and tid + 256 (of 16 x 28 = 448)
+    constexpr int NOUT = PTY * PTX;
+    float z[2] = {bias3, bias3};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
       const int tb = GT * h, nt = h ? KK - GT : GT;
-      for (int grp = wave; grp < (L2 * L2) / 16; grp += 4) {
+      for (int grp = wave; grp < (L2Y * L2X) / 16; grp += 4) {
         const bf16x4 a = *(const bf16x4*)&a2[swz(grp * 16 + mrow, kb)];
-        floatx4 g = {0.f, 0.f, 0.f, 0.f};
-        g = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wb3[h], a, g, 0, 0, 0);
+        floatx4 gsum = {0.f, 0.f, 0.f, 0.f};
+        gsum = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(wb3[h], a, gsum, 0, 0, 0);
 #pragma unroll
         for (int j = 0; j < 4; ++j)  // lane (mrow, kb): taps 4kb+j of pixel grp*16 + mrow
-          if (4 * kb + j < nt) G[(grp * 16 + mrow) * GT + 4 * kb + j] = g[j];
+          if (4 * kb + j < nt) G[(grp * 16 + mrow) * GT + 4 * kb + j] = gsum[j];
       }
       __syncthreads();
 #pragma unroll
-      for (int n = 0; n < nt; ++n) {
-        const int t = tb + n;
-        z += G[((y + t / KS) * L2 + x + t % KS) * GT + n];
-      }
-      __syncthreads();
-    }
-    const int gy = oy0 + y, gx = ox0 + x;
-    const bool inside = gy < n0 && gx < n1;
-    const int hit = (inside && z > thr) ? 1 : 0;
-    if (inside) {
-      const long long o = (long long)gy * n1 + gx;
-      if (flags) flags[o] = (signed char)hit;
-      if (logits) logits[o] = z;
-    }
-    if (list) {  // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
-      const unsigned long long ball = __ballot(hit);
-      if (lane == 0) wbase[wave] = __popcll(ball);
-      __syncthreads();
-      if (tid == 0) {
-        int tot = 0;
-        for (int w = 0; w < 4; ++w) {
-          const int c = wbase[w];
-          wbase[w] = tot;
-          tot += c;
+      for (int s = 0; s < 2; ++s) {
+        const int p = tid + s * INFER_THREADS;
+        if (p < NOUT) {
+          const int y = p / PTX, x = p - (p / PTX) * PTX;
+#pragma unroll
+          for (int n = 0; n < nt; ++n) {
+            const int t = tb + n;
+            z[s] += G[((y + t / KS) * L2X + x + t % KS) * GT + n];
+          }
         }
-        bbase = tot ? atomicAdd(count, tot) : 0;
       }
       __syncthreads();
-      if (hit) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = gy * n1 + gx;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int p = tid + s * INFER_THREADS;
+      const int y = p / PTX, x = p - (p / PTX) * PTX;
+      const int gy = oy0 + y, gx = ox0 + x;
+      const bool inside = p < NOUT && gy < n0 && gx < n1;
+      const int hit = (inside && z[s] > thr) ? 1 : 0;
+      if (inside) {
+        const long long o = (long long)gy * n1 + gx;
+        if (flags) flags[o] = (signed char)hit;
+        if (logits) logits[o] = z[s];
+      }
+      if (list) {  // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
+        const unsigned long long ball = __ballot(hit);
+        if (lane == 0) wbase[wave] = __popcll(ball);
+        __syncthreads();
+        if (tid == 0) {
+          int tot = 0;
+          for (int w = 0; w < 4; ++w) {
+            const int c = wbase[w];
+            wbase[w] = tot;
+            tot += c;
+          }
+          bbase = tot ? atomicAdd(count, tot) : 0;
+        }
+        __syncthreads();
+        if (hit) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = gy * n1 + gx;
+        __syncthreads();  // wbase / bbase are reused by the next slot
+      }
     }
   }
 }
 
-// Grid of persistent blocks: up to 4 per CU on 256 CUs.
-inline int infer_grid(int ntiles) { return ntiles < 1024 ? ntiles : 1024; }
+// Persistent blocks: up to 2 per CU on 256 CUs (57 KB of LDS each) when the network runs alone;
+// fewer when it runs beside a stage's kernels, which need LDS and CUs of their own.
+#ifndef FVHOE_INFER_BLOCKS
+#define FVHOE_INFER_BLOCKS 512
+#endif
+#ifndef FVHOE_INFER_BLOCKS_BESIDE
+#define FVHOE_INFER_BLOCKS_BESIDE 256
+#endif
 
 template <typename Feat>
 inline int launch_infer(const Feat& f, int n0, int n1, const void* packed, float thr, signed char* flags,
-                        float* logits, int* count, int* list, hipStream_t s) {
-  const int t0 = (n0 + PT - 1) / PT, t1 = (n1 + PT - 1) / PT, nt = t0 * t1;
-  k_infer<Feat><<<infer_grid(nt), INFER_THREADS, 0, s>>>(f, n0, n1, (const unsigned char*)packed, thr, flags,
-                                                         logits, count, list, t1, nt);
+                        float* logits, int* count, int* list, hipStream_t s, bool beside = false) {
+  const int t0 = (n0 + PTY - 1) / PTY, t1 = (n1 + PTX - 1) / PTX, nt = t0 * t1;
+  const int cap = beside ? FVHOE_INFER_BLOCKS_BESIDE : FVHOE_INFER_BLOCKS;
+  k_infer<Feat><<<nt < cap ? nt : cap, INFER_THREADS, 0, s>>>(f, n0, n1, (const unsigned char*)packed, thr, flags,
+                                                              logits, count, list, t1, nt);
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 

@@ -1306,7 +1306,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       const int fb = (HX * WX + TPB - 1) / TPB;
       k_feat_pack<T><<<fb < 4096 ? fb : 4096, TPB, 0, ps>>>(FeatW<T>{W, L.NW0, L.NW1, g, range}, XB, HX, WX);
       const int st = pred::launch_infer(pred::FeatBF16{XB, HX, WX}, L.n0, L.n1, pred->packed,
-                                        pred->logit_threshold, preflags_out, nullptr, count + 1, pre_list, ps);
+                                        pred->logit_threshold, preflags_out, nullptr, count + 1, pre_list, ps,
+                                        overlap);
       if (st != FVHOE_OK) return st;
     }
     if (overlap && hipEventRecord((hipEvent_t)pred->ev_join, ps) != hipSuccess) return FVHOE_ERR_LAUNCH;

@@ -332,7 +332,7 @@ int fvhoe_stage_timing_read(double* ms, long long* count);
  * applied to the FP32 master copy).
  * ---------------------------------------------------------------------- */
 #define FVHOE_PRED_NPARAMS 8433
-#define FVHOE_PRED_PACKED_BYTES 20480
+#define FVHOE_PRED_PACKED_BYTES 24576
 #define FVHOE_PRED_HALO 6
 
 /* theta -> packed inference weights (FVHOE_PRED_PACKED_BYTES, device). */
