@@ -857,6 +857,33 @@ __global__ void k_log_count(int* __restrict__ cnt, const signed char* __restrict
     cnt[i] = mode == 0 ? (int)t[i] : cnt[i] + (int)t[i];
 }
 
+// The same over 16 cells per thread (16-byte flag words, n a multiple of 16, aligned arrays): mode 1
+// touches the counts only where a flag is set (the flags are sparse), mode 0 writes them all.
+__device__ __forceinline__ int flag_byte(unsigned w, int b) { return (int)(signed char)((w >> (8 * b)) & 0xffu); }
+__global__ void k_log_count16(int* __restrict__ cnt, const uint4* __restrict__ t16, long long n16, int mode,
+                              const double* __restrict__ gate) {
+  if (gate && gate[0] == 0.0) return;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n16;
+       i += (long long)gridDim.x * blockDim.x) {
+    const uint4 w = t16[i];
+    const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+    int4* c = (int4*)(cnt + 16 * i);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      if (mode == 0) {
+        c[q] = int4{flag_byte(ws[q], 0), flag_byte(ws[q], 1), flag_byte(ws[q], 2), flag_byte(ws[q], 3)};
+      } else if (ws[q]) {
+        int4 v = c[q];
+        v.x += flag_byte(ws[q], 0);
+        v.y += flag_byte(ws[q], 1);
+        v.z += flag_byte(ws[q], 2);
+        v.w += flag_byte(ws[q], 3);
+        c[q] = v;
+      }
+    }
+  }
+}
+
 __global__ void k_log_mean(double* __restrict__ dst, const int* __restrict__ cnt, long long n, double d,
                            const double* __restrict__ gate) {
   if (gate && gate[0] == 0.0) return;
@@ -1285,7 +1312,12 @@ int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long
 int fvhoe_log_count(int* count, const signed char* trouble, long long n, int mode, const double* gate, void* stream) {
   if (!count || !trouble || n < 0 || mode < 0 || mode > 1) return FVHOE_ERR_ARG;
   if (n == 0) return FVHOE_OK;
-  k_log_count<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(count, trouble, n, mode, gate);
+  if (n % 16 == 0 && ((uintptr_t)count % 16) == 0 && ((uintptr_t)trouble % 16) == 0) {
+    k_log_count16<<<grid_for(n / 16), TPB, 0, (hipStream_t)stream>>>(count, (const uint4*)trouble, n / 16, mode,
+                                                                      gate);
+  } else {
+    k_log_count<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(count, trouble, n, mode, gate);
+  }
   return launch_status();
 }
 

@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   if (tid < NPART) {
     unsigned long long x = part[0][tid];
     for (int w = 1; w < NT / 64; ++w) x = red_op(tid, x, part[w][tid]);
-    red.partials[blockIdx.x * NPART + tid] = x;
+    red.partials[(long long)tid * gridDim.x + blockIdx.x] = x;  // [quantity][block]: coalesced for the reduction
   }
 }
 
@@ -492,7 +492,7 @@ __global__ void __launch_bounds__(256) k_prims_strip(Src src, Geo g, T* __restri
   if (threadIdx.x < NPART) {
     unsigned long long x = part[0][threadIdx.x];
     for (int w = 1; w < 4; ++w) x = red_op(threadIdx.x, x, part[w][threadIdx.x]);
-    red.partials[blockIdx.x * NPART + threadIdx.x] = x;
+    red.partials[(long long)threadIdx.x * gridDim.x + blockIdx.x] = x;
   }
 }
 
@@ -953,13 +953,6 @@ __global__ void __launch_bounds__(TPB) k_final(const T* __restrict__ F, const T*
 }
 
 // counters: [0] cells listed by the check, [1] pre-flagged cells, [2] pre-flagged cells the check cleared
-__global__ void k_stage_init(int* count, int reset_pre) {
-  if (threadIdx.x == 0) {
-    count[0] = 0;
-    count[2] = 0;
-    if (reset_pre) count[1] = 0;
-  }
-}
 
 __global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats, int with_pre,
                              const double* __restrict__ gate) {
@@ -1052,37 +1045,37 @@ __global__ void __launch_bounds__(TPB) k_feat_pack(FeatW<T> feat, pred::bf16x4v*
 // final reduction of the prims partials: dt (fvhoe/hydro.py:290-293, and
 // _hit_target, fvhoe/ode.py:11-22, when asked) and the per-variable global
 // (min, max) (fvhoe/slope_limiting.py:168-170)
+// One block per quantity k (0: the CFL denominator, 1 + 2v / 2 + 2v: min / max of variable v) over the
+// per-block partials, stored [quantity][block] so every block reads its row coalesced.  Max / min of
+// the ordered bits are exact in any order.
 __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long* __restrict__ partials, int nblocks,
                                                        double* range, double cfl_h, int fixed_on, double fixed_dt,
                                                        double* dt, double* dt_step, const double* t, double target,
-                                                       int has_target, const double* target_ptr) {
-  __shared__ unsigned long long part[16][NPART];
-  const int tid = threadIdx.x;
-  unsigned long long val[NPART];
-  val[0] = 0ull;
-#pragma unroll
-  for (int k = 1; k < NPART; ++k) val[k] = (k & 1) ? ~0ull : 0ull;
-  for (int b = tid; b < nblocks; b += 1024) {
-#pragma unroll
-    for (int k = 0; k < NPART; ++k) val[k] = red_op(k, val[k], partials[b * NPART + k]);
+                                                       int has_target, const double* target_ptr, int* count,
+                                                       int reset_pre) {
+  __shared__ unsigned long long part[16];
+  const int tid = threadIdx.x, k = blockIdx.x;
+  if (k == 0 && tid == 0) {  // this stage's flag counters (the previous stage's consumers ran before)
+    count[0] = 0;
+    count[2] = 0;
+    if (reset_pre) count[1] = 0;
   }
-#pragma unroll
-  for (int k = 0; k < NPART; ++k) {
-    unsigned long long x = val[k];
-    for (int off = 32; off > 0; off >>= 1) x = red_op(k, x, __shfl_down(x, off, 64));
-    if ((tid & 63) == 0) part[tid >> 6][k] = x;
-  }
+  unsigned long long x = k == 0 ? 0ull : ((k & 1) ? ~0ull : 0ull);
+  const unsigned long long* row = partials + (long long)k * nblocks;
+  for (int b = tid; b < nblocks; b += 1024) x = red_op(k, x, row[b]);
+  for (int off = 32; off > 0; off >>= 1) x = red_op(k, x, __shfl_down(x, off, 64));
+  if ((tid & 63) == 0) part[tid >> 6] = x;
   __syncthreads();
-  if (tid < NPART) {
-    unsigned long long x = part[0][tid];
-    for (int w = 1; w < 16; ++w) x = red_op(tid, x, part[w][tid]);
-    if (tid == 0) {
+  if (tid == 0) {
+    x = part[0];
+    for (int w = 1; w < 16; ++w) x = red_op(k, x, part[w]);
+    if (k == 0) {
       const double d = fixed_on ? fixed_dt : cfl_h / __longlong_as_double((long long)x);
       dt[0] = d;
       const double tg = target_ptr ? target_ptr[0] : target;  // a device target serves every replay
       if (dt_step) dt_step[0] = (has_target && tg - t[0] < d) ? tg - t[0] : d;
     } else {
-      range[tid - 1] = from_ordered_bits(x);
+      range[k - 1] = from_ordered_bits(x);
     }
   }
 }
@@ -1257,7 +1250,6 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   signed char* preflags = pre_now ? (signed char*)(ws + L.offPre) : nullptr;
   signed char* preflags_out = use_pred ? (signed char*)(ws + L.offPre) : nullptr;
   int* pre_list = (int*)(ws + L.offPreList);
-  k_stage_init<<<1, 64, 0, s>>>(count, use_pred && pred->infer);
 
   Reduce red{};
   red.partials = partials;
@@ -1287,10 +1279,10 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   RkArgs rk{-1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   const bool set_step = rkf && rkf->set_dt_step;
   if (rkf && rkf->kind >= 0) rk = RkArgs{rkf->kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
-  k_stage_reduce<<<1, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
+  k_stage_reduce<<<NPART, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
                                     set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0,
-                                    set_step ? rkf->target_ptr : nullptr);
+                                    set_step ? rkf->target_ptr : nullptr, count, use_pred && pred->infer);
   if (use_pred && pred->infer) {  // the network on this stage's W, features formed on load
     hipStream_t ps = s;
     if (overlap) {  // fork: the network runs beside the rest of this stage
