@@ -126,6 +126,9 @@ _SIGNATURES["fvhoe_euler_equation_strided"] = (c_int, [_P, POINTER(c_longlong), 
                                                        POINTER(c_longlong), _P, c_int, c_int, c_int, c_int, c_int,
                                                        c_double, c_double, c_double, _P])
 _SIGNATURES["fvhoe_euler_equation_strided_f32"] = _SIGNATURES["fvhoe_euler_equation_strided"]
+_SIGNATURES["fvhoe_bc_stencil_sweep"] = (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                                 POINTER(c_int), c_int, c_int, c_int, _P])
+_SIGNATURES["fvhoe_bc_stencil_sweep_f32"] = _SIGNATURES["fvhoe_bc_stencil_sweep"]
 
 _lib = None
 

@@ -174,5 +174,35 @@ class BoundaryCondition:
             _lib.call("fvhoe_bc_apply", _lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, _lib.stream())
         return _lib.back(out, was_np)
 
+    def apply_swept(self, u, gw: Tuple[int, int, int], pos: str, p: int, axis: int, dtype=None):
+        """The fill of apply() followed by one stencil sweep of kind pos ("c" or "t") and degree p along
+        axis (1 or 2), in one pass (the padded array is never stored); the same values as apply() and
+        fvhoe_amd.fv.stencil_sweep.  None where the fused kernel does not apply (slab sides, p = 0,
+        small or very large arrays): then fill and sweep separately."""
+        if self.has_slabs() or p < 1 or axis not in (1, 2):
+            return None
+        x, was_np = _lib.as_device(u)
+        if was_np:
+            return None
+        nv, n0, n1, n2 = _lib.shape4(x)
+        gw = tuple(int(g) for g in gw)
+        for (d, kinds), g, n in zip(((d, getattr(self, d)) for d in "xyz"), gw, (n0, n1, n2)):
+            if g > 0 and (None in kinds or g > n):
+                return None
+        shape = [nv, n0 + 2 * gw[0], n1 + 2 * gw[1], n2 + 2 * gw[2]]
+        shape[axis] -= 2 * ((p + 1) // 2)
+        if shape[axis] < 1:
+            return None
+        f32 = dtype is not None and dtype == _lib.torch().float32
+        out = _lib.empty(shape, dtype=dtype)
+        codes = (_lib.ctypes.c_int * 6)(*self.codes())
+        name = "fvhoe_bc_stencil_sweep" + ("_f32" if f32 else "")
+        st = getattr(_lib.lib(), name)(_lib.ptr(x), _lib.ptr(out), nv, n0, n1, n2, *gw, codes, _lib.STENCIL[pos], p,
+                                        axis, _lib.stream())
+        if st == _lib.ERR_UNSUPPORTED:
+            return None
+        _lib.check(st, name)
+        return out
+
     def to_dict(self) -> dict:
         return dict(x=self.x, y=self.y, z=self.z)

@@ -584,6 +584,60 @@ __global__ void k_bc_apply_slabs(const double* __restrict__ u, double* __restric
   }
 }
 
+// The ghost fill and the first sweep of the padded array in one pass: out is the sweep along axis
+// (1 = x or 2 = y) of the array fvhoe_bc_apply would write, each padded element fetched from u through
+// the boundary index maps (the same values as the fill, rounded to T as the float32 fill rounds
+// them), so the padded array is never stored.  Threads along the contiguous z, a register window of R
+// outputs along the sweep axis, as k_sweep_slide (bit-identical to fill + sweep).
+template <typename T, int KIND, int P, int R>
+__global__ void __launch_bounds__(256) k_bc_sweep_slide(const double* __restrict__ u, T* __restrict__ out, BcArgs a,
+                                                        int axis, int nblk_fast, int nb_axis) {
+  constexpr int K = ntaps(P), NW = R + K - 1;
+  const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
+  const int na = axis == 1 ? N0 : N1, ma = na - (K - 1);
+  const int m0 = axis == 1 ? ma : N0, m1 = axis == 2 ? ma : N1;
+  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour windows on one XCD
+  const int k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+  if (k >= N2) return;
+  unsigned rest = blk / (unsigned)nblk_fast;
+  const int ab = (int)(rest % (unsigned)nb_axis);
+  rest /= (unsigned)nb_axis;
+  const int no = axis == 1 ? N1 : N0;
+  const int o = (int)(rest % (unsigned)no), v = (int)(rest / (unsigned)no);
+  const int a0 = ab * R;
+  const int dS = axis == 1 ? 0 : 1, dO = axis == 1 ? 1 : 0;  // sweep axis and the other strided axis
+  bool fk, fo;
+  const int sk = bc_source(k, a.n[2], a.g[2], a.lo[2], a.hi[2], &fk);
+  const int so = bc_source(o, a.n[dO], a.g[dO], a.lo[dO], a.hi[dO], &fo);
+  T win[NW];
+#pragma unroll
+  for (int r = 0; r < NW; ++r) {
+    win[r] = T(0);
+    if (a0 + r < na) {
+      bool fs;
+      const int ss = bc_source(a0 + r, a.n[dS], a.g[dS], a.lo[dS], a.hi[dS], &fs);
+      const int si = axis == 1 ? ss : so, sj = axis == 1 ? so : ss;
+      const bool fi = axis == 1 ? fs : fo, fj = axis == 1 ? fo : fs;
+      double x = u[(((long long)v * a.n[0] + si) * a.n[1] + sj) * a.n[2] + sk];
+      if ((fi && v == 1) || (fj && v == 2) || (fk && v == 3)) x = -x;
+      win[r] = T(x);
+    }
+  }
+  const long long os = axis == 1 ? (long long)m1 * N2 : N2;
+  const long long o0 = (((long long)v * m0 + (axis == 1 ? a0 : o)) * m1 + (axis == 2 ? a0 : o)) * N2 + k;
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    if (a0 + q < ma) {
+      T acc = T(0);
+      static_for<K>([&](auto I) {
+        constexpr double wt = weight(KIND, P, decltype(I)::value);
+        if constexpr (wt != 0.0) acc = acc + T(wt) * win[q + decltype(I)::value];
+      });
+      out[o0 + q * os] = acc;
+    }
+  }
+}
+
 // dudt = 0.0 + cx*(F[i+1]-F[i]) + cy*(G[j+1]-G[j]) + cz*(H[k+1]-H[k]), c = -(1/h)  (fvhoe/solver.py:744-750)
 // element strides (variable, axis 0, axis 1) of a face-flux array whose last axis is contiguous: the
 // fluxes may be views into larger arrays (the trimmed fluxes of hydrofluxes)
@@ -1036,6 +1090,56 @@ int euler_equation_impl(const T* F, const long long* fst, const T* G, const long
   return launch_status();
 }
 
+template <typename T, int KIND, int P>
+int launch_bc_slide(const double* u, T* out, const BcArgs& a, int nv, int axis, hipStream_t st) {
+  constexpr int K = ntaps(P), R = FVHOE_SLIDE_R;
+  const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
+  const int na = axis == 1 ? N0 : N1, ma = na - (K - 1);
+  const int nbk = (N2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? N1 : N0;
+  const long long blocks = (long long)nbk * nb * no * nv;
+  if (blocks >= (1ll << 31)) return -1;
+  k_bc_sweep_slide<T, KIND, P, R><<<(unsigned)blocks, 256, 0, st>>>(u, out, a, axis, nbk, nb);
+  return 0;
+}
+
+template <typename T, int KIND>
+int launch_bc_slide_p(int p, const double* u, T* out, const BcArgs& a, int nv, int axis, hipStream_t st) {
+  switch (p) {
+    case 1: return launch_bc_slide<T, KIND, 1>(u, out, a, nv, axis, st);
+    case 2: return launch_bc_slide<T, KIND, 2>(u, out, a, nv, axis, st);
+    case 3: return launch_bc_slide<T, KIND, 3>(u, out, a, nv, axis, st);
+    case 4: return launch_bc_slide<T, KIND, 4>(u, out, a, nv, axis, st);
+    case 5: return launch_bc_slide<T, KIND, 5>(u, out, a, nv, axis, st);
+    case 6: return launch_bc_slide<T, KIND, 6>(u, out, a, nv, axis, st);
+    case 7: return launch_bc_slide<T, KIND, 7>(u, out, a, nv, axis, st);
+    case 8: return launch_bc_slide<T, KIND, 8>(u, out, a, nv, axis, st);
+  }
+  return -1;
+}
+
+template <typename T>
+int bc_stencil_sweep_impl(const double* u, T* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                          const int* bc6, int kind, int p, int axis, void* stream) {
+  if (!u || !out || !bc6 || nv < 1 || (axis != 1 && axis != 2) || (kind != FVHOE_STENCIL_C && kind != FVHOE_STENCIL_T))
+    return FVHOE_ERR_ARG;
+  BcArgs a{{n0, n1, n2}, {g0, g1, g2}, {bc6[0], bc6[2], bc6[4]}, {bc6[1], bc6[3], bc6[5]}};
+  for (int d = 0; d < 3; ++d) {
+    if (a.g[d] < 0 || a.n[d] < 1) return FVHOE_ERR_ARG;
+    if (a.g[d] > 0 && a.g[d] > a.n[d]) return FVHOE_ERR_ARG;
+    if (a.lo[d] < 0 || a.lo[d] > 3 || a.hi[d] < 0 || a.hi[d] > 3) return FVHOE_ERR_UNSUPPORTED;  // no slab kinds
+  }
+  if (p < 1 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const long long N0 = n0 + 2 * g0, N1 = n1 + 2 * g1, N2 = n2 + 2 * g2;
+  if ((axis == 1 ? N0 : N1) < ntaps(p)) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * N0 * N1 * N2;
+  if (total < (1ll << 16) || total >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;  // small grids: fill + sweep
+  const hipStream_t st = (hipStream_t)stream;
+  const int r = kind == FVHOE_STENCIL_C ? launch_bc_slide_p<T, KIND_C>(p, u, out, a, nv, axis, st)
+                                        : launch_bc_slide_p<T, KIND_T>(p, u, out, a, nv, axis, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
+  return launch_status();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C-ABI
@@ -1157,6 +1261,16 @@ int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2,
 int fvhoe_bc_apply_f32(const double* u, float* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
                        const int* bc6, void* stream) {
   return bc_apply_impl<float>(u, out, nv, n0, n1, n2, g0, g1, g2, bc6, stream);
+}
+
+int fvhoe_bc_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                           const int* bc6, int kind, int p, int axis, void* stream) {
+  return bc_stencil_sweep_impl<double>(u, out, nv, n0, n1, n2, g0, g1, g2, bc6, kind, p, axis, stream);
+}
+
+int fvhoe_bc_stencil_sweep_f32(const double* u, float* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                               const int* bc6, int kind, int p, int axis, void* stream) {
+  return bc_stencil_sweep_impl<float>(u, out, nv, n0, n1, n2, g0, g1, g2, bc6, kind, p, axis, stream);
 }
 
 int fvhoe_bc_apply_dirichlet(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,

@@ -361,8 +361,14 @@ class EulerSolver(ODE):
         """fvhoe/solver.py:753-792."""
         cost = [(2 if fv_average else 1) * get_stencil_size(pi) for pi in p]
         g = tuple(int(gw[i]) + cost[i] for i in range(3))
-        u_bc = self.bc.apply(u, g, t=self.t, dtype=_lib.torch().float32 if self._bulk32 else None)
-        w = compute_primitives(self.hydro_state, interpolate_cell_centers(u_bc, p), self.gamma)
+        dtype = _lib.torch().float32 if self._bulk32 else None
+        # the ghost fill and the x c-sweep in one pass where the fused kernel applies (large grids)
+        first = self.bc.apply_swept(u, g, "c", p[0], 1, dtype=dtype) if p[0] > 0 else None
+        if first is not None:
+            centres = sweep_chain(first, "c", ((p[1], 2), (p[2], 3)))
+        else:
+            centres = interpolate_cell_centers(self.bc.apply(u, g, t=self.t, dtype=dtype), p)
+        w = compute_primitives(self.hydro_state, centres, self.gamma)
         return interpolate_fv_averages(w, p) if fv_average else w
 
     def hydrofluxes(self, u, p, slope_limiter: str = None, views: bool = False):

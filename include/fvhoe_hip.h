@@ -117,6 +117,19 @@ int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, in
 int fvhoe_bc_apply(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
                    const int* bc6, void* stream);
 
+/* BoundaryCondition.apply (periodic / outflow / reflective / free) followed
+ * by the first stencil sweep of interpolate_cell_centers or
+ * interpolate_fv_averages (fvhoe/solver.py:774-789, fvhoe/fv.py:106-224)
+ * along axis (1 = x or 2 = y), in one pass: out has the padded shape
+ * (nv, n0+2g0, n1+2g1, n2+2g2) shortened by 2*ceil(p/2) on axis, and equals
+ * fvhoe_bc_apply + fvhoe_stencil_sweep bit for bit (_f32: the float32 fill +
+ * fvhoe_stencil_sweep_f32).  For padded arrays of 2^16 to 2^31 elements and
+ * p >= 1; FVHOE_ERR_UNSUPPORTED otherwise (fill and sweep separately). */
+int fvhoe_bc_stencil_sweep(const double* u, double* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                           const int* bc6, int kind, int p, int axis, void* stream);
+int fvhoe_bc_stencil_sweep_f32(const double* u, float* out, int nv, int n0, int n1, int n2, int g0, int g1, int g2,
+                               const int* bc6, int kind, int p, int axis, void* stream);
+
 /* The same with FVHOE_BC_DIRICHLET sides ("dirichlet" / "ic",
  * fvhoe/boundary_conditions.py:18-57,247-304,334-348).  slabs6[s] (device,
  * NULL for the other sides) holds side s's ghost values, shaped as that

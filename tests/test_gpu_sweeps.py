@@ -67,3 +67,28 @@ def test_composed_3d_stage_with_long_z(p, rs):
     print(f"[3d p={p} {rs}] dt={dt!r}", flush=True)
     assert dt == dto
     assert_bits_equal(dudt.cpu().numpy(), dudto, "dudt")
+
+
+@pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
+@pytest.mark.parametrize("axis", [1, 2])
+def test_fused_fill_and_sweep(axis, f32):
+    """fvhoe_bc_stencil_sweep == fill + sweep bit for bit, every non-slab kind on every side."""
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+
+    torch = _lib.torch()
+    rng = np.random.default_rng(axis)
+    u = rng.standard_normal((5, 23, 21, 40))
+    x = _lib.as_device(u)[0]
+    dtype = torch.float32 if f32 else None
+    for kinds in ((("periodic", "periodic"),) * 3, (("outflow", "reflective"), ("free", "reflective"), ("reflective", "free"))):
+        bc = BoundaryCondition(x=kinds[0], y=kinds[1], z=kinds[2])
+        for p in (1, 3, 7, 8):
+            g = (8, 7, 6)
+            got = bc.apply_swept(x, g, "c", p, axis, dtype=dtype)
+            assert got is not None  # the fused kernel ran
+            ref = fv.stencil_sweep(bc.apply(x, g, dtype=dtype), "interpolation", p, axis, "c")
+            assert got.dtype == ref.dtype and got.shape == ref.shape
+            if f32:
+                assert torch.equal(got.view(torch.int32), ref.view(torch.int32)), (kinds, p, axis)
+            else:
+                assert_bits_equal(got.cpu().numpy(), ref.cpu().numpy(), f"{kinds} p={p} axis={axis}")
