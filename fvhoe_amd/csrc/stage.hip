@@ -404,8 +404,18 @@ template <typename T>
 constexpr int strip_rows() { return sizeof(T) == 8 ? FVHOE_STRIP_ROWS_F64 : FVHOE_STRIP_ROWS_F32; }
 constexpr int STRIP_MAX_P = 3;  // degrees that use k_prims_strip (higher degrees: the LDS-tiled k_prims)
 
+// minimum waves per SIMD of k_prims_strip (1: the compiler's choice; more caps its registers).  FP32: 4
+// (107 VGPRs, no spills) runs the KH 2048^2 strip in 98 us against 111 at 3 waves (135 VGPRs); FP64 at 3
+// waves spills (168 VGPRs, 172 B of scratch) and runs 192 us against 151 at 2 waves (211 VGPRs)
+#ifndef FVHOE_STRIP_WAVES_F64
+#define FVHOE_STRIP_WAVES_F64 1
+#endif
+#ifndef FVHOE_STRIP_WAVES_F32
+#define FVHOE_STRIP_WAVES_F32 4
+#endif
 template <typename T, int P, typename Src>
-__global__ void __launch_bounds__(256) k_prims_strip(Src src, Geo g, T* __restrict__ W, int NW0, int NW1,
+__global__ void __launch_bounds__(256, sizeof(T) == 8 ? FVHOE_STRIP_WAVES_F64 : FVHOE_STRIP_WAVES_F32)
+    k_prims_strip(Src src, Geo g, T* __restrict__ W, int NW0, int NW1,
                                                        int cwins, Reduce red, double gamma) {
   constexpr int RC = ereach_c(P), RT = ereach_t(P), H = RC + RT, VC = 64 - 2 * H;
   constexpr int R = strip_rows<T>(), NU = R + 2 * H, NC = R + 2 * RT;
