@@ -609,6 +609,11 @@ __global__ void __launch_bounds__(256) k_bc_sweep_slide(const double* __restrict
   bool fk, fo;
   const int sk = bc_source(k, a.n[2], a.g[2], a.lo[2], a.hi[2], &fk);
   const int so = bc_source(o, a.n[dO], a.g[dO], a.lo[dO], a.hi[dO], &fo);
+  // 32-bit offsets (the caller checks the sizes): u[v][i][j][k] = fixed + source index along the sweep
+  // axis x its stride; a reflective side negates its normal momentum (variable 1 + axis)
+  const int plane = a.n[1] * a.n[2], sstride = axis == 1 ? plane : a.n[2];
+  const int fixed = v * a.n[0] * plane + so * (axis == 1 ? a.n[2] : plane) + sk;
+  const bool flip_fixed = (fo && v == 1 + dO) || (fk && v == 3);
   T win[NW];
 #pragma unroll
   for (int r = 0; r < NW; ++r) {
@@ -616,10 +621,8 @@ __global__ void __launch_bounds__(256) k_bc_sweep_slide(const double* __restrict
     if (a0 + r < na) {
       bool fs;
       const int ss = bc_source(a0 + r, a.n[dS], a.g[dS], a.lo[dS], a.hi[dS], &fs);
-      const int si = axis == 1 ? ss : so, sj = axis == 1 ? so : ss;
-      const bool fi = axis == 1 ? fs : fo, fj = axis == 1 ? fo : fs;
-      double x = u[(((long long)v * a.n[0] + si) * a.n[1] + sj) * a.n[2] + sk];
-      if ((fi && v == 1) || (fj && v == 2) || (fk && v == 3)) x = -x;
+      double x = u[fixed + ss * sstride];
+      if (flip_fixed || (fs && v == 1 + dS)) x = -x;
       win[r] = T(x);
     }
   }
