@@ -129,6 +129,10 @@ _SIGNATURES["fvhoe_euler_equation_strided_f32"] = _SIGNATURES["fvhoe_euler_equat
 _SIGNATURES["fvhoe_bc_stencil_sweep"] = (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                                  POINTER(c_int), c_int, c_int, c_int, _P])
 _SIGNATURES["fvhoe_bc_stencil_sweep_f32"] = _SIGNATURES["fvhoe_bc_stencil_sweep"]
+_SIGNATURES["fvhoe_euler_equation_rk"] = (c_int, [_P, POINTER(c_longlong), _P, POINTER(c_longlong), _P,
+                                                  POINTER(c_longlong), _P, c_int, c_int, c_int, c_int, c_int,
+                                                  c_double, c_double, c_double, POINTER(FvhoeRkFuse), _P])
+_SIGNATURES["fvhoe_euler_equation_rk_f32"] = _SIGNATURES["fvhoe_euler_equation_rk"]
 
 _lib = None
 

@@ -673,6 +673,37 @@ __global__ void k_euler_eq(const T* __restrict__ F, const T* __restrict__ G, con
   }
 }
 
+// The same divergence followed by the stage's RK combination in one pass (the composed stage's version
+// of the fused stage's last kernel): dudt (when requested; k0 + dudt for the SSP-RK3 *_SUM stage) and
+// rk.out = the combination with dudt as the stage's k, the value the separate kernels would store and
+// read back.  rk.out may alias rk.u0 (pointwise).
+template <typename T>
+__global__ void k_euler_eq_rk(const T* __restrict__ F, const T* __restrict__ G, const T* __restrict__ H,
+                              double* __restrict__ dudt, int nv, int n0, int n1, int n2, int active, double cx,
+                              double cy, double cz, Strides3 fs, Strides3 gs, Strides3 hs, RkArgs rk) {
+  const long long total = (long long)nv * n0 * n1 * n2;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x; idx < total;
+       idx += (long long)gridDim.x * blockDim.x) {
+    int v, i, j, k;
+    split4(idx, n0, n1, n2, &v, &i, &j, &k);
+    double acc = 0.0;
+    if (active & 1) {
+      const long long b = v * fs.v + i * fs.a + j * fs.b + k;
+      acc = acc + cx * ((double)F[b + fs.a] - (double)F[b]);
+    }
+    if (active & 2) {
+      const long long b = v * gs.v + i * gs.a + j * gs.b + k;
+      acc = acc + cy * ((double)G[b + gs.b] - (double)G[b]);
+    }
+    if (active & 4) {
+      const long long b = v * hs.v + i * hs.a + j * hs.b + k;
+      acc = acc + cz * ((double)H[b + 1] - (double)H[b]);
+    }
+    if (dudt) dudt[idx] = rk.kind == FVHOE_RK_SSPRK3_STAGE3_SUM ? rk.k0[idx] + acc : acc;
+    rk.out[idx] = rk_combine(rk, (int)idx, acc);
+  }
+}
+
 __global__ void k_init_minmax(unsigned long long* acc, int nv) {
   const int i = threadIdx.x;
   if (i < 2 * nv) acc[i] = (i & 1) ? 0ull : ~0ull;  // (min, max) pairs
@@ -1143,6 +1174,35 @@ int bc_stencil_sweep_impl(const double* u, T* out, int nv, int n0, int n1, int n
   return launch_status();
 }
 
+template <typename T>
+int euler_equation_rk_impl(const T* F, const long long* fst, const T* G, const long long* gst, const T* H,
+                           const long long* hst, double* dudt, int nv, int n0, int n1, int n2, int active, double cx,
+                           double cy, double cz, const FvhoeRkFuse* rkf, void* stream) {
+  if (!rkf || nv < 1 || n0 < 1 || n1 < 1 || n2 < 1) return FVHOE_ERR_ARG;
+  if (((active & 1) && !F) || ((active & 2) && !G) || ((active & 4) && !H)) return FVHOE_ERR_ARG;
+  const int kind = rkf->kind;
+  if (kind < 0 || kind > FVHOE_RK_SSPRK3_FINAL_SUM || !rkf->out || !rkf->u0 || !rkf->dt_step) return FVHOE_ERR_ARG;
+  const bool need_k0 = kind == FVHOE_RK_SSPRK3_STAGE3 || kind == FVHOE_RK_SSPRK3_FINAL ||
+                       kind == FVHOE_RK_SSPRK2_FINAL || kind == FVHOE_RK_RK4_FINAL ||
+                       kind == FVHOE_RK_SSPRK3_STAGE3_SUM || kind == FVHOE_RK_SSPRK3_FINAL_SUM;
+  if (need_k0 && !rkf->k0) return FVHOE_ERR_ARG;
+  if ((kind == FVHOE_RK_SSPRK3_FINAL || kind == FVHOE_RK_RK4_FINAL) && !rkf->k1) return FVHOE_ERR_ARG;
+  if (kind == FVHOE_RK_RK4_FINAL && !rkf->k2) return FVHOE_ERR_ARG;
+  if (kind == FVHOE_RK_SSPRK3_STAGE3_SUM && !dudt) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * n0 * n1 * n2;
+  if (total >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;  // rk_combine takes 32-bit indices
+  const long long N0 = n0, N1 = n1, N2 = n2;
+  Strides3 fs{(N0 + 1) * N1 * N2, N1 * N2, N2}, gs{N0 * (N1 + 1) * N2, (N1 + 1) * N2, N2},
+      hs{N0 * N1 * (N2 + 1), N1 * (N2 + 1), N2 + 1};
+  if (fst) fs = Strides3{fst[0], fst[1], fst[2]};
+  if (gst) gs = Strides3{gst[0], gst[1], gst[2]};
+  if (hst) hs = Strides3{hst[0], hst[1], hst[2]};
+  const RkArgs rk{kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
+  k_euler_eq_rk<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy,
+                                                                     cz, fs, gs, hs, rk);
+  return launch_status();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------- C-ABI
@@ -1312,6 +1372,18 @@ int fvhoe_euler_equation_strided(const double* F, const long long* fst, const do
                                  const double* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
                                  int active, double cx, double cy, double cz, void* stream) {
   return euler_equation_impl<double>(F, fst, G, gst, H, hst, dudt, nv, n0, n1, n2, active, cx, cy, cz, stream);
+}
+
+int fvhoe_euler_equation_rk(const double* F, const long long* fst, const double* G, const long long* gst,
+                            const double* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                            int active, double cx, double cy, double cz, const FvhoeRkFuse* rk, void* stream) {
+  return euler_equation_rk_impl<double>(F, fst, G, gst, H, hst, dudt, nv, n0, n1, n2, active, cx, cy, cz, rk, stream);
+}
+
+int fvhoe_euler_equation_rk_f32(const float* F, const long long* fst, const float* G, const long long* gst,
+                                const float* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                                int active, double cx, double cy, double cz, const FvhoeRkFuse* rk, void* stream) {
+  return euler_equation_rk_impl<float>(F, fst, G, gst, H, hst, dudt, nv, n0, n1, n2, active, cx, cy, cz, rk, stream);
 }
 
 int fvhoe_euler_equation_strided_f32(const float* F, const long long* fst, const float* G, const long long* gst,

@@ -11,6 +11,8 @@
 #include <stdint.h>
 #include <utility>
 
+#include "fvhoe_hip.h"
+
 namespace fvhoe {
 
 // ---------------------------------------------------------------------------
@@ -460,6 +462,31 @@ __host__ __device__ __forceinline__ int bc_source(int a, int n, int g, int lo, i
 // Same map for a signed interior-relative index c in [-g, n+g).
 __host__ __device__ __forceinline__ int bc_map(int c, int n, int g, int lo, int hi, bool* flip) {
   return bc_source(c + g, n, g, lo, hi, flip);
+}
+
+// The RK combination of a stage's derivative k_this (fvhoe/ode.py:245-298), kinds FVHOE_RK_*; the
+// *_SUM kinds read k0 + k1 stored once (the same rounding as the reference's (k0 + k1)).
+struct RkArgs {
+  int kind;                     // FVHOE_RK_*, -1 = none
+  double* out;
+  const double *u0, *k0, *k1, *k2;
+  const double* dt;             // step dt (device)
+};
+
+__device__ __forceinline__ double rk_combine(const RkArgs& r, int i, double k_this) {
+  const double dt = r.dt[0];
+  if (dt == 0.0) return r.u0[i];  // a step at its target time leaves the state exactly as it is
+  switch (r.kind) {
+    case FVHOE_RK_AXPY: return r.u0[i] + dt * k_this;
+    case FVHOE_RK_SSPRK3_STAGE3: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
+    case FVHOE_RK_SSPRK3_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + r.k1[i] + 4.0 * k_this);
+    case FVHOE_RK_SSPRK2_FINAL: return 0.5 * r.u0[i] + 0.5 * (r.k0[i] + dt * k_this);
+    case FVHOE_RK_HALF_AXPY: return r.u0[i] + 0.5 * dt * k_this;
+    case FVHOE_RK_RK4_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 2.0 * r.k1[i] + 2.0 * r.k2[i] + k_this);
+    case FVHOE_RK_SSPRK3_STAGE3_SUM: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
+    case FVHOE_RK_SSPRK3_FINAL_SUM: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 4.0 * k_this);
+  }
+  return 0.0;
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs, each with its own L2.

@@ -917,29 +917,6 @@ __global__ void __launch_bounds__(TPB) k_fallback(const int* __restrict__ count,
 
 // Runge-Kutta combination written by the stage's last kernel; the stage's own
 // dudt is the last k operand of the kind (fvhoe/ode.py:245-298).
-struct RkArgs {
-  int kind;                     // FVHOE_RK_*, -1 = none
-  double* out;
-  const double *u0, *k0, *k1, *k2;
-  const double* dt;             // step dt (device)
-};
-
-__device__ __forceinline__ double rk_combine(const RkArgs& r, int i, double k_this) {
-  const double dt = r.dt[0];
-  if (dt == 0.0) return r.u0[i];  // a step at its target time leaves the state exactly as it is
-  switch (r.kind) {
-    case FVHOE_RK_AXPY: return r.u0[i] + dt * k_this;
-    case FVHOE_RK_SSPRK3_STAGE3: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
-    case FVHOE_RK_SSPRK3_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + r.k1[i] + 4.0 * k_this);
-    case FVHOE_RK_SSPRK2_FINAL: return 0.5 * r.u0[i] + 0.5 * (r.k0[i] + dt * k_this);
-    case FVHOE_RK_HALF_AXPY: return r.u0[i] + 0.5 * dt * k_this;
-    case FVHOE_RK_RK4_FINAL: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 2.0 * r.k1[i] + 2.0 * r.k2[i] + k_this);
-    case FVHOE_RK_SSPRK3_STAGE3_SUM: return r.u0[i] + 0.25 * dt * r.k0[i] + 0.25 * dt * k_this;
-    case FVHOE_RK_SSPRK3_FINAL_SUM: return r.u0[i] + (1.0 / 6.0) * dt * (r.k0[i] + 4.0 * k_this);
-  }
-  return 0.0;
-}
-
 // dudt = 0.0 + cx (F[i+1]-F[i]) + cy (G[j+1]-G[j])  (fvhoe/solver.py:744-750) in
 // FP64, then the optional RK combination.  rk.out may alias the stage input or
 // rk.u0 (pointwise).

@@ -289,13 +289,46 @@ class EulerSolver(ODE):
             dt_out.copy_(dt)
 
     def _fused_rk(self) -> bool:
-        return self.fused
+        # the composed stage without a-posteriori limiting ends in the divergence + RK combination kernel
+        return self.fused or not self.a_posteriori_slope_limiting
+
+    def _chunkable(self) -> bool:
+        """Steps back to back on the device: the fused stage only (the composed stage takes no device
+        target time)."""
+        return self.fused and not self._needs_host_clock()
+
+    def _composed_stage_rk(self, u, k_out, dt_out, kind, out, u0, k0, k1, k2, dt_step, set_dt_step, t_dev, target):
+        """The composed stage (no a-posteriori limiting) with the divergence and the RK combination in one
+        kernel; the step's dt is clipped to the target first, as the separate combine kernels take it."""
+        dt, (F, G, H) = self.hydrofluxes(u, self.p, views=True)
+        dt_out.copy_(dt)
+        if set_dt_step:
+            _lib.call("fvhoe_hit_target", _lib.ptr(dt_step), _lib.ptr(dt_out), _lib.ptr(t_dev),
+                      float(target) if target is not None else 0.0, int(target is not None), _lib.stream())
+        rk = _lib.FvhoeRkFuse()
+        rk.kind = kind
+        rk.out, rk.u0 = out.data_ptr(), u0.data_ptr()
+        rk.k0 = k0.data_ptr() if k0 is not None else None
+        rk.k1 = k1.data_ptr() if k1 is not None else None
+        rk.k2 = k2.data_ptr() if k2 is not None else None
+        rk.dt_step = dt_step.data_ptr()
+        nv, n0, n1, n2 = u.shape
+        active = sum(1 << d for d in range(3) if self.active[d])
+        sfx, args, _keep = self._flux_args(F, G, H, u.shape)
+        _lib.call("fvhoe_euler_equation_rk" + sfx, *args, _lib.ptr(k_out), nv, n0, n1, n2, active, -(1 / self.h[0]),
+                  -(1 / self.h[1]), -(1 / self.h[2]), _lib.ctypes.byref(rk), _lib.stream())
+        return True
 
     def stage_rk(self, u, k_out, dt_out, kind, out, u0, k0=None, k1=None, k2=None, dt_step=None,
                  set_dt_step=False, t_dev=None, target=None, target_dev=None) -> bool:
         """A fused stage whose last kernel also writes the RK combination `kind`
         into `out` (fvhoe/ode.py:245-298), so the state is touched once per stage."""
         self.f_evaluation_count += 1
+        if not self.fused:
+            if target_dev is not None:
+                raise NotImplementedError("the composed stage takes a host target time")
+            return self._composed_stage_rk(u, k_out, dt_out, kind, out, u0, k0, k1, k2, dt_step, set_dt_step,
+                                           t_dev, target)
         rk = _lib.FvhoeRkFuse()
         rk.kind = kind
         rk.out, rk.u0 = out.data_ptr(), u0.data_ptr()
@@ -479,10 +512,9 @@ class EulerSolver(ODE):
         _lib.call("fvhoe_blend", _lib.ptr(out), _lib.ptr(Fl), _lib.ptr(th), F.shape[0], th.numel(), _lib.stream())
         return out
 
-    def _euler_equation_into(self, F, G, H, out):
-        """dudt into out from face fluxes that may be views into larger arrays (last axis contiguous)."""
-        nv, n0, n1, n2 = out.shape
-        active = sum(1 << d for d in range(3) if self.active[d])
+    def _flux_args(self, F, G, H, shape):
+        """(pointer, element strides) per face-flux array for the strided divergence entries."""
+        nv, n0, n1, n2 = shape
         first = next(f for f in (F, G, H) if f is not None)
         arrs, strides = [], []
         for d, f in enumerate((F, G, H)):
@@ -498,9 +530,18 @@ class EulerSolver(ODE):
                 f = f.contiguous()
             arrs.append(f)
             strides.append((_lib.ctypes.c_longlong * 3)(*f.stride()[:3]))
-        _lib.call("fvhoe_euler_equation_strided" + _lib.sfx(first), _lib.ptr(arrs[0]), strides[0], _lib.ptr(arrs[1]),
-                  strides[1], _lib.ptr(arrs[2]), strides[2], _lib.ptr(out), nv, n0, n1, n2, active, -(1 / self.h[0]),
-                  -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
+        args = []
+        for a, st in zip(arrs, strides):
+            args += [_lib.ptr(a), st]
+        return _lib.sfx(first), args, arrs
+
+    def _euler_equation_into(self, F, G, H, out):
+        """dudt into out from face fluxes that may be views into larger arrays (last axis contiguous)."""
+        nv, n0, n1, n2 = out.shape
+        active = sum(1 << d for d in range(3) if self.active[d])
+        sfx, args, _keep = self._flux_args(F, G, H, out.shape)
+        _lib.call("fvhoe_euler_equation_strided" + sfx, *args, _lib.ptr(out), nv, n0, n1, n2, active,
+                  -(1 / self.h[0]), -(1 / self.h[1]), -(1 / self.h[2]), _lib.stream())
 
     def euler_equation(self, F, G, H):
         """dudt from face fluxes (fvhoe/solver.py:734-751)."""

@@ -318,6 +318,20 @@ typedef struct FvhoeRkFuse {
   const double* target_ptr;  /* device target time read at run time instead of target (may be NULL) */
 } FvhoeRkFuse;
 
+/* The composed stage's divergence (fvhoe_euler_equation_strided) and the
+ * step's RK combination (fvhoe_rk_combine, fvhoe/ode.py:245-298) in one
+ * pass: rk->out = combination of kind rk->kind with the stage's dudt as its
+ * last k and rk->dt_step as dt; dudt (may be NULL) receives dudt, or
+ * rk->k0 + dudt for FVHOE_RK_SSPRK3_STAGE3_SUM.  rk->set_dt_step and the
+ * target fields are not used (set dt_step first, e.g. with
+ * fvhoe_hit_target).  Bit-identical to the two separate calls. */
+int fvhoe_euler_equation_rk(const double* F, const long long* fst, const double* G, const long long* gst,
+                            const double* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                            int active, double cx, double cy, double cz, const FvhoeRkFuse* rk, void* stream);
+int fvhoe_euler_equation_rk_f32(const float* F, const long long* fst, const float* G, const long long* gst,
+                                const float* H, const long long* hst, double* dudt, int nv, int n0, int n1, int n2,
+                                int active, double cx, double cy, double cz, const FvhoeRkFuse* rk, void* stream);
+
 int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
                    signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream);
 
