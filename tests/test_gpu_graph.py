@@ -78,3 +78,28 @@ def test_capture_keeps_the_cyclic_collector_off():
     a.advance(3, graph=True)  # 1 eager step, capture, 2 replays
     assert seen == [True, False]  # the eager step with the collector on, the captured one with it off
     assert gc.isenabled()
+
+
+@pytest.mark.parametrize("form", ["fp64", "fp32"])
+def test_composed_graph_replays_equal_eager_steps(form):
+    """The composed 3-D stage (divergence and RK combination in one kernel) captured in a graph."""
+    n = (16, 12, 10)
+    rng = np.random.default_rng(4)
+    x = np.linspace(0, 1, n[0])[:, None, None]
+    w = np.empty((5, *n))
+    w[0] = 1.0 + 0.2 * np.sin(2 * np.pi * x) + 0.05 * rng.random(n)
+    w[1], w[2], w[3] = 0.3, -0.2, 0.1
+    w[4] = 1.0 + 0.1 * rng.random(n)
+    from oracle import fvhoe_oracle as O
+
+    u0 = O.compute_conservatives(w, 1.4)
+    mk = lambda: solver_from(u0, n, (3, 3, 3), (P, P, P), riemann_solver="llf",  # noqa: E731
+                             a_posteriori_slope_limiting=False, bulk_precision=form)
+    a, b = mk(), mk()
+    assert not a.fused
+    a.integrator = b.integrator = "ssprk3"
+    a.advance(4)
+    b.advance(4, graph=True)  # 1 eager step, capture, 3 replays
+    _lib.torch().cuda.synchronize()
+    assert a.t == b.t and a.step_count == b.step_count
+    assert_bits_equal(b.u.cpu().numpy(), a.u.cpu().numpy(), f"composed {form}")
