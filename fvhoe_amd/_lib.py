@@ -133,6 +133,8 @@ _SIGNATURES["fvhoe_euler_equation_rk"] = (c_int, [_P, POINTER(c_longlong), _P, P
                                                   POINTER(c_longlong), _P, c_int, c_int, c_int, c_int, c_int,
                                                   c_double, c_double, c_double, POINTER(FvhoeRkFuse), _P])
 _SIGNATURES["fvhoe_euler_equation_rk_f32"] = _SIGNATURES["fvhoe_euler_equation_rk"]
+_SIGNATURES["fvhoe_stencil_sweep_xypair"] = (c_int, [_P, _P, c_int, c_int, c_int, c_int, c_int, c_int, c_int, _P])
+_SIGNATURES["fvhoe_stencil_sweep_xypair_f32"] = _SIGNATURES["fvhoe_stencil_sweep_xypair"]
 
 _lib = None
 

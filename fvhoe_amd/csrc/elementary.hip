@@ -480,6 +480,103 @@ int launch_zpair_p(int p, const T* u, T* out, int nv, int n0, int n1, int n2, in
   return -1;
 }
 
+// Two sweeps along the strided axes, first along A then along B ({A, B} = {x, y}), in one pass:
+// threads along the contiguous z; each thread owns R consecutive outputs along A and a run of RB along
+// B.  It slides down the input rows along B, sweeps each row along A (R + K - 1 loads, coalesced over
+// z) and keeps, for each of its R outputs along A, a register window of the last K A-swept rows, from
+// which the B-sweep is formed.  Every intermediate is the value the separate sweeps would store, summed
+// in the same order (bit-identical to two fvhoe_stencil_sweep calls), and the intermediate array is
+// never written.
+template <typename T, int KIND, int P, int R, int RB>
+__global__ void __launch_bounds__(256) k_sweep_xypair(const T* __restrict__ u, T* __restrict__ out, int n0, int n1,
+                                                      int n2, int afirst, int nblk_fast, int nbA, int nbB) {
+  constexpr int K = ntaps(P), NA = R + K - 1, NB = RB + K - 1;
+  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour runs on one XCD
+  const int k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+  if (k >= n2) return;
+  unsigned rest = blk / (unsigned)nblk_fast;
+  const int ia = (int)(rest % (unsigned)nbA);
+  rest /= (unsigned)nbA;
+  const int ib = (int)(rest % (unsigned)nbB), v = (int)(rest / (unsigned)nbB);
+  const bool a_is_x = afirst == 1;
+  const int nA = a_is_x ? n0 : n1, nB = a_is_x ? n1 : n0, mA = nA - (K - 1), mB = nB - (K - 1);
+  const int m0 = n0 - (K - 1), m1 = n1 - (K - 1);
+  const int sA = a_is_x ? n1 * n2 : n2, sB = a_is_x ? n2 : n1 * n2;      // input strides
+  const int osA = a_is_x ? m1 * n2 : n2, osB = a_is_x ? n2 : m1 * n2;   // output strides
+  const int a0 = ia * R, b0 = ib * RB;
+  const int in0 = v * n0 * n1 * n2 + a0 * sA + b0 * sB + k;
+  const int o0 = v * m0 * m1 * n2 + a0 * osA + b0 * osB + k;
+  T ring[R][K];
+#pragma unroll
+  for (int rb = 0; rb < NB; ++rb) {
+    if (b0 + rb < nB) {
+      T win[NA];
+#pragma unroll
+      for (int j = 0; j < NA; ++j) win[j] = a0 + j < nA ? u[in0 + j * sA + rb * sB] : T(0);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        T acc = T(0);
+        static_for<K>([&](auto I) {
+          constexpr double wt = weight(KIND, P, decltype(I)::value);
+          if constexpr (wt != 0.0) acc = acc + T(wt) * win[r + decltype(I)::value];
+        });
+        ring[r][rb % K] = acc;  // the A-swept value of row b0 + rb
+      }
+    }
+    if (rb >= K - 1) {
+      const int q = rb - (K - 1);  // output row b0 + q along B: rows q .. q + K - 1 are in the windows
+      if (b0 + q < mB) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (a0 + r < mA) {
+            T acc = T(0);
+            static_for<K>([&](auto I) {
+              constexpr int t = decltype(I)::value;
+              constexpr double wt = weight(KIND, P, t);
+              if constexpr (wt != 0.0) acc = acc + T(wt) * ring[r][(q + t) % K];
+            });
+            out[o0 + r * osA + q * osB] = acc;
+          }
+        }
+      }
+    }
+  }
+}
+
+// outputs per thread along the first and the second axis of the x-y pair kernel
+#ifndef FVHOE_XY_R
+#define FVHOE_XY_R 4
+#endif
+#ifndef FVHOE_XY_RB
+#define FVHOE_XY_RB 16
+#endif
+
+template <typename T, int KIND, int P>
+int launch_xypair(const T* u, T* out, int nv, int n0, int n1, int n2, int afirst, hipStream_t st) {
+  constexpr int K = ntaps(P), R = FVHOE_XY_R, RB = FVHOE_XY_RB;
+  const int nA = afirst == 1 ? n0 : n1, nB = afirst == 1 ? n1 : n0;
+  const int nbk = (n2 + 255) / 256, nbA = (nA - (K - 1) + R - 1) / R, nbB = (nB - (K - 1) + RB - 1) / RB;
+  const long long blocks = (long long)nbk * nbA * nbB * nv;
+  if (blocks >= (1ll << 31)) return -1;
+  k_sweep_xypair<T, KIND, P, R, RB><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, afirst, nbk, nbA, nbB);
+  return 0;
+}
+
+template <typename T, int KIND>
+int launch_xypair_p(int p, const T* u, T* out, int nv, int n0, int n1, int n2, int afirst, hipStream_t st) {
+  switch (p) {
+    case 1: return launch_xypair<T, KIND, 1>(u, out, nv, n0, n1, n2, afirst, st);
+    case 2: return launch_xypair<T, KIND, 2>(u, out, nv, n0, n1, n2, afirst, st);
+    case 3: return launch_xypair<T, KIND, 3>(u, out, nv, n0, n1, n2, afirst, st);
+    case 4: return launch_xypair<T, KIND, 4>(u, out, nv, n0, n1, n2, afirst, st);
+    case 5: return launch_xypair<T, KIND, 5>(u, out, nv, n0, n1, n2, afirst, st);
+    case 6: return launch_xypair<T, KIND, 6>(u, out, nv, n0, n1, n2, afirst, st);
+    case 7: return launch_xypair<T, KIND, 7>(u, out, nv, n0, n1, n2, afirst, st);
+    case 8: return launch_xypair<T, KIND, 8>(u, out, nv, n0, n1, n2, afirst, st);
+  }
+  return -1;
+}
+
 // faces per thread of the fused face-flux kernel
 #ifndef FVHOE_FACE_R
 #define FVHOE_FACE_R 4
@@ -1073,6 +1170,22 @@ int zpair_impl(const T* u, T* out, int nv, int n0, int n1, int n2, int kind, int
 }
 
 template <typename T>
+int xypair_impl(const T* u, T* out, int nv, int n0, int n1, int n2, int kind, int p, int afirst, void* stream) {
+  if (!u || !out || nv < 1 || (afirst != 1 && afirst != 2) || (kind != FVHOE_STENCIL_C && kind != FVHOE_STENCIL_T))
+    return FVHOE_ERR_ARG;
+  if (p < 1 || p > 8) return FVHOE_ERR_UNSUPPORTED;
+  const int k = ntaps(p);
+  if (n0 < k || n1 < k) return FVHOE_ERR_ARG;
+  const long long total = (long long)nv * n0 * n1 * n2;
+  if (total < (1ll << 16) || total >= (1ll << 31)) return FVHOE_ERR_UNSUPPORTED;  // small grids: two sweeps
+  const hipStream_t st = (hipStream_t)stream;
+  const int r = kind == FVHOE_STENCIL_C ? launch_xypair_p<T, KIND_C>(p, u, out, nv, n0, n1, n2, afirst, st)
+                                        : launch_xypair_p<T, KIND_T>(p, u, out, nv, n0, n1, n2, afirst, st);
+  if (r != 0) return FVHOE_ERR_UNSUPPORTED;
+  return launch_status();
+}
+
+template <typename T>
 int face_fluxes_impl(const T* w, T* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
                      double csq_floor, void* stream) {
   if (!w || !out || axis < 1 || axis > 3 || solver < 0 || solver > 2 || n0 < 1 || n1 < 1 || n2 < 1)
@@ -1294,6 +1407,16 @@ int fvhoe_stencil_sweep_zpair(const double* u, double* out, int nv, int n0, int 
 int fvhoe_stencil_sweep_zpair_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p,
                                   int axis, int zfirst, void* stream) {
   return zpair_impl<float>(u, out, nv, n0, n1, n2, kind, p, axis, zfirst, stream);
+}
+
+int fvhoe_stencil_sweep_xypair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p,
+                               int first_axis, void* stream) {
+  return xypair_impl<double>(u, out, nv, n0, n1, n2, kind, p, first_axis, stream);
+}
+
+int fvhoe_stencil_sweep_xypair_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p,
+                                   int first_axis, void* stream) {
+  return xypair_impl<float>(u, out, nv, n0, n1, n2, kind, p, first_axis, stream);
 }
 
 int fvhoe_face_fluxes(const double* w, double* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,

@@ -107,10 +107,28 @@ def _zpair(x, kind: int, p: int, axis: int, zfirst: bool):
     return out
 
 
+def _xypair(x, kind: int, p: int, first: int):
+    """Sweeps along x and y (first along axis `first`) of one kind and degree in one pass, or None where
+    the pair kernel does not apply (p = 0, small or very large arrays)."""
+    k = 2 * get_stencil_size(p) + 1
+    oshape = list(x.shape)
+    oshape[1] -= k - 1
+    oshape[2] -= k - 1
+    if min(oshape) < 1:
+        return None
+    out = _lib.empty(oshape, dtype=x.dtype)
+    name = "fvhoe_stencil_sweep_xypair" + _lib.sfx(x)
+    st = getattr(_lib.lib(), name)(_lib.ptr(x), _lib.ptr(out), *x.shape, kind, p, first, _lib.stream())
+    if st == _lib.ERR_UNSUPPORTED:
+        return None
+    _lib.check(st, name)
+    return out
+
+
 def sweep_chain(u, pos: str, steps):
     """Sweeps of one kind ("c" conservative interpolation, "t" transverse reconstruction) applied in
-    order, steps = ((p, axis), ...) on a 4-D array; an x or y sweep next to a z sweep of the same degree
-    runs as one pass (the same values as the separate sweeps)."""
+    order, steps = ((p, axis), ...) on a 4-D array; an x or y sweep next to a z sweep of the same degree,
+    or an x sweep next to a y sweep, runs as one pass (the same values as the separate sweeps)."""
     x, was_np = _lib.as_device_bulk(u)
     kind = _lib.STENCIL[pos]
     mode = "reconstruction" if pos == "t" else "interpolation"
@@ -123,6 +141,13 @@ def sweep_chain(u, pos: str, steps):
             pair = (a, False) if a in (1, 2) and a2 == 3 else (a2, True) if a == 3 and a2 in (1, 2) else None
             if pair is not None:
                 y = _zpair(x, kind, p, *pair)
+                if y is not None:
+                    x = y
+                    i += 2
+                    continue
+            z_next = i + 2 < len(steps) and steps[i + 2] == (p, 3)
+            if {a, a2} == {1, 2} and not z_next:  # x and y in one pass (a following z pairs with y instead)
+                y = _xypair(x, kind, p, a)
                 if y is not None:
                     x = y
                     i += 2

@@ -99,6 +99,16 @@ int fvhoe_riemann_faces(const double* wr_face, const double* wl_face, double* ou
 int fvhoe_stencil_sweep_zpair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p, int axis,
                               int zfirst, void* stream);
 
+/* Two sweeps of one kind (c or t) and degree p in one pass along the two
+ * strided axes, first_axis (1 = x or 2 = y) first, then the other: the same
+ * result as two fvhoe_stencil_sweep calls (fvhoe/fv.py:106-224), bit for
+ * bit, without storing the intermediate array.  For arrays of 2^16 to 2^31
+ * elements (else FVHOE_ERR_UNSUPPORTED: sweep separately). */
+int fvhoe_stencil_sweep_xypair(const double* u, double* out, int nv, int n0, int n1, int n2, int kind, int p,
+                               int first_axis, void* stream);
+int fvhoe_stencil_sweep_xypair_f32(const float* u, float* out, int nv, int n0, int n1, int n2, int kind, int p,
+                                   int first_axis, void* stream);
+
 /* The l / r conservative interpolations along axis and the Riemann call of
  * hydrofluxes in one pass (fvhoe/fv.py:143-159, fvhoe/solver.py:538-606):
  * out[f] = RS(r-sweep(w)[f], l-sweep(w)[f+1]) for the 5-variable primitive

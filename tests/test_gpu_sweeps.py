@@ -92,3 +92,26 @@ def test_fused_fill_and_sweep(axis, f32):
                 assert torch.equal(got.view(torch.int32), ref.view(torch.int32)), (kinds, p, axis)
             else:
                 assert_bits_equal(got.cpu().numpy(), ref.cpu().numpy(), f"{kinds} p={p} axis={axis}")
+
+
+@pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
+@pytest.mark.parametrize("pos", ["c", "t"])
+@pytest.mark.parametrize("first", [1, 2], ids=["x-y", "y-x"])
+def test_xypair_sweeps(first, pos, f32):
+    """fvhoe_stencil_sweep_xypair == two single sweeps bit for bit."""
+    rng = np.random.default_rng(7 + first)
+    u = rng.standard_normal((5, 29, 27, 33))
+    x = _lib.as_device(u)[0]
+    if f32:
+        x = x.float()
+    mode = "reconstruction" if pos == "t" else "interpolation"
+    for p in range(1, 9):
+        pair = fv._xypair(x, _lib.STENCIL[pos], p, first)
+        assert pair is not None  # the pair kernel ran
+        ref = fv.stencil_sweep(fv.stencil_sweep(x, mode, p, first, None if pos == "t" else pos), mode, p, 3 - first,
+                               None if pos == "t" else pos)
+        assert pair.dtype == ref.dtype and pair.shape == ref.shape
+        if f32:
+            assert _lib.torch().equal(pair.view(_lib.torch().int32), ref.view(_lib.torch().int32)), (pos, p, first)
+        else:
+            assert_bits_equal(pair.cpu().numpy(), ref.cpu().numpy(), f"{pos} p={p} first={first}")
