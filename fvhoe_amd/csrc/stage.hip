@@ -72,6 +72,14 @@ constexpr int TPB = 256;
 #ifndef FVHOE_FLUX64_B
 #define FVHOE_FLUX64_B 16
 #endif
+// FP64 y-flux tiles (faces along y x rows along x): 32 x 8 writes G in 256-byte row runs (16-face runs
+// straddle cache lines), 104 us per launch at KH 2048^2 against 110 for the transposed x-flux tile
+#ifndef FVHOE_FLUX64_YA
+#define FVHOE_FLUX64_YA 32
+#endif
+#ifndef FVHOE_FLUX64_YB
+#define FVHOE_FLUX64_YB 8
+#endif
 #ifndef FVHOE_FLUX_NT_F64
 #define FVHOE_FLUX_NT_F64 256
 #endif
@@ -1298,7 +1306,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     if (TWO_D) {
       constexpr bool F64 = sizeof(T) == 8;
       constexpr int FA = F64 ? FVHOE_FLUX64_A : FVHOE_FLUX_A, FB = F64 ? FVHOE_FLUX64_B : FVHOE_FLUX_B;
-      constexpr int TA0 = FA, TB0 = FB, TA1 = FB, TB1 = FA;
+      constexpr int TA0 = FA, TB0 = FB, TA1 = F64 ? FVHOE_FLUX64_YA : FB, TB1 = F64 ? FVHOE_FLUX64_YB : FA;
       constexpr int NF = F64 ? FVHOE_FLUX_NT_F64 : FVHOE_FLUX_NT_F32;
       const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
       {
