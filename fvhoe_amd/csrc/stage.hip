@@ -80,6 +80,13 @@ constexpr int TPB = 256;
 #ifndef FVHOE_FLUX64_YB
 #define FVHOE_FLUX64_YB 8
 #endif
+// FP32 y-flux tiles (faces along y x rows along x); default: the x-flux tile transposed
+#ifndef FVHOE_FLUX32_YA
+#define FVHOE_FLUX32_YA FVHOE_FLUX_B
+#endif
+#ifndef FVHOE_FLUX32_YB
+#define FVHOE_FLUX32_YB FVHOE_FLUX_A
+#endif
 #ifndef FVHOE_FLUX_NT_F64
 #define FVHOE_FLUX_NT_F64 256
 #endif
@@ -1306,7 +1313,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     if (TWO_D) {
       constexpr bool F64 = sizeof(T) == 8;
       constexpr int FA = F64 ? FVHOE_FLUX64_A : FVHOE_FLUX_A, FB = F64 ? FVHOE_FLUX64_B : FVHOE_FLUX_B;
-      constexpr int TA0 = FA, TB0 = FB, TA1 = F64 ? FVHOE_FLUX64_YA : FB, TB1 = F64 ? FVHOE_FLUX64_YB : FA;
+      constexpr int TA0 = FA, TB0 = FB, TA1 = F64 ? FVHOE_FLUX64_YA : FVHOE_FLUX32_YA,
+                    TB1 = F64 ? FVHOE_FLUX64_YB : FVHOE_FLUX32_YB;
       constexpr int NF = F64 ? FVHOE_FLUX_NT_F64 : FVHOE_FLUX_NT_F32;
       const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
       {
