@@ -57,6 +57,26 @@ constexpr int TPB = 256;
 #ifndef FVHOE_TILE32_Y
 #define FVHOE_TILE32_Y 64
 #endif
+// candidate-state tiles (rows x columns per arithmetic type) and their threads; the same shape as the
+// prims tile unless overridden
+#ifndef FVHOE_CAND_X
+#define FVHOE_CAND_X FVHOE_TILE_X
+#endif
+#ifndef FVHOE_CAND_Y
+#define FVHOE_CAND_Y FVHOE_TILE_Y
+#endif
+#ifndef FVHOE_CAND32_X
+#define FVHOE_CAND32_X FVHOE_TILE32_X
+#endif
+#ifndef FVHOE_CAND32_Y
+#define FVHOE_CAND32_Y FVHOE_TILE32_Y
+#endif
+#ifndef FVHOE_CAND_NT_F64
+#define FVHOE_CAND_NT_F64 512
+#endif
+#ifndef FVHOE_CAND_NT_F32
+#define FVHOE_CAND_NT_F32 512
+#endif
 #ifndef FVHOE_FLUX_A
 #define FVHOE_FLUX_A 32
 #endif
@@ -1214,6 +1234,9 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   constexpr int PY = TWO_D ? P : 0;
   constexpr int TX = TWO_D ? tile_x<T>() : TX1, TY = TWO_D ? tile_y<T>() : 1;
   constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
+  constexpr int CX = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_X : FVHOE_CAND32_X) : TX1;
+  constexpr int CY = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_Y : FVHOE_CAND32_Y) : 1;
+  constexpr int NCAND = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_NT_F64 : FVHOE_CAND_NT_F32) : 256;
   T* W = (T*)(ws + L.offW);
   T* F = (T*)(ws + L.offF);
   T* G = TWO_D ? (T*)(ws + L.offG) : nullptr;
@@ -1349,12 +1372,12 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     Geo gs = g;
     gs.ge0 = 3;
     gs.ge1 = TWO_D ? 3 : 0;
-    const int t0 = (L.NS0 + TX - 1) / TX, t1 = (L.NS1 + TY - 1) / TY;
+    const int t0 = (L.NS0 + CX - 1) / CX, t1 = (L.NS1 + CY - 1) / CY;
     Reduce none{};
     {
       TimedLaunch tl(TK_PRIMS_CAND, s);
       // the LDS tiles form each candidate value once per tile (the strips would form it (R + 2H) / R times)
-      k_prims<T, P, PY, TX, TY, SUS, NTILE><<<t0 * t1, NTILE, 0, s>>>(
+      k_prims<T, P, PY, CX, CY, SUS, NCAND><<<t0 * t1, NCAND, 0, s>>>(
           make_srcstar<SUS>(u, F, G, dt_stage, sl), gs, WS, L.NS0, L.NS1, t1, none, c->gamma);
     }
     DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
