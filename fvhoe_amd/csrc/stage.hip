@@ -57,13 +57,19 @@ constexpr int TPB = 256;
 #ifndef FVHOE_TILE32_Y
 #define FVHOE_TILE32_Y 64
 #endif
-// candidate-state tiles (rows x columns per arithmetic type) and their threads; the same shape as the
-// prims tile unless overridden
+// candidate-state tiles (rows x columns per arithmetic type) and their threads.  FP64 at degrees up to
+// FVHOE_CAND_MAX_P: 32 x 32 on 512 threads (21 KB of LDS at p = 3 with the centre values held in
+// registers) measured 191 us per launch at KH 2048^2 p = 3 against 208 for 16 x 32; higher degrees
+// (whose halo pre-loads need 112-222 VGPRs at 32 x 32) keep the prims tile.  FP32 keeps the prims
+// tile's 16 x 64 (141 us; 165 at 32 x 64)
+#ifndef FVHOE_CAND_MAX_P
+#define FVHOE_CAND_MAX_P 3
+#endif
 #ifndef FVHOE_CAND_X
-#define FVHOE_CAND_X FVHOE_TILE_X
+#define FVHOE_CAND_X 32
 #endif
 #ifndef FVHOE_CAND_Y
-#define FVHOE_CAND_Y FVHOE_TILE_Y
+#define FVHOE_CAND_Y 32
 #endif
 #ifndef FVHOE_CAND32_X
 #define FVHOE_CAND32_X FVHOE_TILE32_X
@@ -133,6 +139,25 @@ __device__ __forceinline__ T sweepT(Get&& x) {
     if constexpr (w != 0.0) acc = acc + T(w) * x(decltype(I)::value);
   });
   return acc;
+}
+
+// The same visit unrolled, with the visit's ordinal k (0 <= k < ceil(H*Wd/NT))
+// passed as well, so per-thread register arrays can be indexed by it.
+template <int H, int Wd, int NT, typename F>
+__device__ __forceinline__ void tile_for_k(F&& f) {
+  constexpr int DR = NT / Wd, DC = NT % Wd, NK = (H * Wd + NT - 1) / NT;
+  int r = (int)threadIdx.x / Wd, c = (int)threadIdx.x % Wd;
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const int e = (int)threadIdx.x + k * NT;
+    if (e < H * Wd) f(k, e, r, c);
+    r += DR;
+    c += DC;
+    if (c >= Wd) {
+      c -= Wd;
+      ++r;
+    }
+  }
 }
 
 // Visit the H x Wd elements of a row-major LDS tile, NT threads, element
@@ -291,9 +316,10 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   constexpr int HX0 = TX + 2 * (RCX + RTX), HY0 = TY + 2 * (RCY + RTY);
   constexpr int HX1 = TX + 2 * RTX, HY1 = TY + 2 * RTY;
   constexpr int NPT = (TX * TY + NT - 1) / NT;
+  constexpr int NC = (HX1 * HY1 + NT - 1) / NT;  // centre values per thread
+  static_assert(HX1 * HY1 <= HX0 * HY0 && TX * HY1 <= HX1 * HY0, "S0 / S1 hold the t-sweep stages");
   __shared__ T S0[HX0 * HY0];
   __shared__ T S1[HX1 * HY0];
-  __shared__ T C[5][HX1 * HY1];
 
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int oa = (tile / tiles1) * TX, ob = (tile % tiles1) * TY;
@@ -327,6 +353,8 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
       }
     }
   }
+  // centre values of the HX1 x HY1 box stay in registers: thread tid holds elements tid + k * NT
+  T cen[5][NC];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
 #pragma unroll
@@ -337,26 +365,34 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
       S1[e] = sweepT<KIND_C, PX, T>([&](int t) { return S0[(i + RCX + t - reach(PX)) * HY0 + j]; });
     });
     __syncthreads();
-    tile_for<HX1, HY1, NT>([&](int e, int i, int j) {
+    tile_for_k<HX1, HY1, NT>([&](int k, int e, int i, int j) {
       const T x = sweepT<KIND_C, PY, T>([&](int t) { return S1[i * HY0 + j + RCY + t - reach(PY)]; });
-      C[v][e] = T(0) + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
+      cen[v][k] = T(0) + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
     });
     __syncthreads();
   }
   // pressure recovery in FP64 (fvhoe/hydro.py:146-170)
-  for (int e = tid; e < HX1 * HY1; e += NT) {
-    double uu[5], ww[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) uu[q] = (double)C[q][e];
-    prims_from_cons(uu, ww, gamma);
+  for (int k = 0; k < NC; ++k) {
+    if (tid + k * NT < HX1 * HY1) {
+      double uu[5], ww[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) C[q][e] = T(ww[q]);
+      for (int q = 0; q < 5; ++q) uu[q] = (double)cen[q][k];
+      prims_from_cons(uu, ww, gamma);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) cen[q][k] = T(ww[q]);
+    }
   }
-  __syncthreads();
   T wr[5][NPT];
+#pragma unroll
   for (int v = 0; v < 5; ++v) {
+    // this variable's centre values through S0 (row stride HY1) for the x sweep
+#pragma unroll
+    for (int k = 0; k < NC; ++k)
+      if (tid + k * NT < HX1 * HY1) S0[tid + k * NT] = cen[v][k];
+    __syncthreads();
     tile_for<TX, HY1, NT>([&](int e, int i, int j) {
-      S1[e] = sweepT<KIND_T, PX, T>([&](int t) { return C[v][(i + RTX + t - reach(PX)) * HY1 + j]; });
+      S1[e] = sweepT<KIND_T, PX, T>([&](int t) { return S0[(i + RTX + t - reach(PX)) * HY1 + j]; });
     });
     __syncthreads();
 #pragma unroll
@@ -370,7 +406,8 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
         if (a < NW0 && b < NW1) W[(v * NW0 + a) * NW1 + b] = wr[v][q];
       }
     }
-    __syncthreads();
+    // no barrier here: the next variable's S0 stores touch no S1 element, and its barrier comes
+    // before S1 is written again
   }
   if (!red.partials) return;
   // partial reductions over this tile's cells of the dt box and the range box
@@ -1234,8 +1271,9 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   constexpr int PY = TWO_D ? P : 0;
   constexpr int TX = TWO_D ? tile_x<T>() : TX1, TY = TWO_D ? tile_y<T>() : 1;
   constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
-  constexpr int CX = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_X : FVHOE_CAND32_X) : TX1;
-  constexpr int CY = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_Y : FVHOE_CAND32_Y) : 1;
+  constexpr bool CBIG = sizeof(T) == 8 && P <= FVHOE_CAND_MAX_P;
+  constexpr int CX = TWO_D ? (CBIG ? FVHOE_CAND_X : sizeof(T) == 8 ? FVHOE_TILE_X : FVHOE_CAND32_X) : TX1;
+  constexpr int CY = TWO_D ? (CBIG ? FVHOE_CAND_Y : sizeof(T) == 8 ? FVHOE_TILE_Y : FVHOE_CAND32_Y) : 1;
   constexpr int NCAND = TWO_D ? (sizeof(T) == 8 ? FVHOE_CAND_NT_F64 : FVHOE_CAND_NT_F32) : 256;
   T* W = (T*)(ws + L.offW);
   T* F = (T*)(ws + L.offF);
