@@ -474,7 +474,19 @@ __host__ __device__ constexpr int strip_halo(int p) { return ereach_c(p) + ereac
 // W rows per lane in k_prims_strip (register budget: 5 variables x (rows + 2) centres per lane)
 template <typename T>
 constexpr int strip_rows() { return sizeof(T) == 8 ? FVHOE_STRIP_ROWS_F64 : FVHOE_STRIP_ROWS_F32; }
-constexpr int STRIP_MAX_P = 3;  // degrees that use k_prims_strip (higher degrees: the LDS-tiled k_prims)
+// degrees that use k_prims_strip per arithmetic type (others: the LDS-tiled k_prims).  FP64 uses the
+// tiles throughout: 32 x 32 tiles measured 133 us per launch at KH 2048^2 p = 3 against 152 for the
+// strips; FP32 strips measured 98 against 118 for 16 x 64 tiles
+#ifndef FVHOE_STRIP_MAX_P_F64
+#define FVHOE_STRIP_MAX_P_F64 0
+#endif
+#ifndef FVHOE_STRIP_MAX_P_F32
+#define FVHOE_STRIP_MAX_P_F32 3
+#endif
+template <typename T>
+constexpr bool use_strip(int p) {
+  return p <= (sizeof(T) == 8 ? FVHOE_STRIP_MAX_P_F64 : FVHOE_STRIP_MAX_P_F32);
+}
 
 // minimum waves per SIMD of k_prims_strip (1: the compiler's choice; more caps its registers).  FP32: 4
 // (107 VGPRs, no spills) runs the KH 2048^2 strip in 98 us against 111 at 3 waves (135 VGPRs); FP64 at 3
@@ -1180,10 +1192,16 @@ struct Layout {
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 constexpr int TX1 = 256;
+// LDS-tiled prims shape per arithmetic type and degree (FP64 at degrees up to FVHOE_CAND_MAX_P: the
+// candidate's 32 x 32)
 template <typename T>
-constexpr int tile_x() { return sizeof(T) == 8 ? FVHOE_TILE_X : FVHOE_TILE32_X; }
+constexpr int tile_x(int p) {
+  return sizeof(T) == 8 ? (p <= FVHOE_CAND_MAX_P ? FVHOE_CAND_X : FVHOE_TILE_X) : FVHOE_TILE32_X;
+}
 template <typename T>
-constexpr int tile_y() { return sizeof(T) == 8 ? FVHOE_TILE_Y : FVHOE_TILE32_Y; }
+constexpr int tile_y(int p) {
+  return sizeof(T) == 8 ? (p <= FVHOE_CAND_MAX_P ? FVHOE_CAND_Y : FVHOE_TILE_Y) : FVHOE_TILE32_Y;
+}
 
 // deepest ghost read from u: W's ghosts plus the prims stencils' reach
 // (the candidate's 3 ghosts and the fallback's MUSCL stencil reach less)
@@ -1240,10 +1258,11 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offWS = o; o = align256(o + (c->a_posteriori ? esz * 5 * (size_t)L->NS0 * L->NS1 : 0));
   L->offFlags = o; o = align256(o + (c->a_posteriori ? (size_t)L->n0 * L->n1 : 0));
   L->offList = o; o = align256(o + (c->a_posteriori ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
-  if (L->two_d && L->p <= STRIP_MAX_P) {
+  if (L->two_d && (L->fp32 ? use_strip<float>(L->p) : use_strip<double>(L->p))) {
     L->prims_blocks = strip_blocks(L->p, L->fp32 ? strip_rows<float>() : strip_rows<double>(), L->NW0, L->NW1);
   } else if (L->two_d) {
-    const int tx = L->fp32 ? tile_x<float>() : tile_x<double>(), ty = L->fp32 ? tile_y<float>() : tile_y<double>();
+    const int tx = L->fp32 ? tile_x<float>(L->p) : tile_x<double>(L->p);
+    const int ty = L->fp32 ? tile_y<float>(L->p) : tile_y<double>(L->p);
     L->prims_blocks = ((L->NW0 + tx - 1) / tx) * ((L->NW1 + ty - 1) / ty);
   } else {
     L->prims_blocks = (L->NW0 + TX1 - 1) / TX1;
@@ -1269,7 +1288,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   using SU = std::conditional_t<DIR, SrcUD, SrcU>;
   using SUS = std::conditional_t<DIR, SrcUstarD<TWO_D, T>, SrcUstar<TWO_D, T>>;
   constexpr int PY = TWO_D ? P : 0;
-  constexpr int TX = TWO_D ? tile_x<T>() : TX1, TY = TWO_D ? tile_y<T>() : 1;
+  constexpr int TX = TWO_D ? tile_x<T>(P) : TX1, TY = TWO_D ? tile_y<T>(P) : 1;
   constexpr int NTILE = TWO_D ? 512 : 256;  // threads of the LDS-tiled kernels
   constexpr bool CBIG = sizeof(T) == 8 && P <= FVHOE_CAND_MAX_P;
   constexpr int CX = TWO_D ? (CBIG ? FVHOE_CAND_X : sizeof(T) == 8 ? FVHOE_TILE_X : FVHOE_CAND32_X) : TX1;
@@ -1330,7 +1349,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
 
   {
     TimedLaunch tl(TK_PRIMS, s);
-    if constexpr (TWO_D && P <= STRIP_MAX_P) {
+    if constexpr (TWO_D && use_strip<T>(P)) {
       k_prims_strip<T, P, SU><<<strip_blocks(P, strip_rows<T>(), L.NW0, L.NW1), 256, 0, s>>>(
           make_src<SU>(u, sl), g, W, L.NW0, L.NW1, strip_cwins(P, L.NW1), red, c->gamma);
     } else {
