@@ -71,14 +71,22 @@ def test_stage_support_matrix(n, p, ok):
     assert _lib.lib().fvhoe_stage_supported(_cfg(n, p)) == ok
 
 
-def test_stage_workspace_size():
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_stage_workspace_size(fp32):
+    c = _cfg((2048, 2048, 1), (3, 3, 0))
+    c.bulk_fp32 = fp32
     b = ctypes.c_ulonglong(0)
-    assert _lib.lib().fvhoe_stage_workspace_bytes(_cfg((2048, 2048, 1), (3, 3, 0)), ctypes.byref(b)) == 0
+    assert _lib.lib().fvhoe_stage_workspace_bytes(ctypes.byref(c), ctypes.byref(b)) == 0
     n = 2048
-    # per-block reduction partials of the column-strip prims kernel: 60 valid columns per wave
-    # (64 lanes - 2 x (c + t reach 1 + 1)), 4 waves of 4 FP64 rows per block
-    blocks = ((n + 8 + 59) // 60) * ((n + 8 + 15) // 16)
-    expect = 8 * 5 * ((n + 8) ** 2 + 2 * (n + 1) * n + (n + 6) ** 2) + 5 * n * n + 88 * blocks
+    if fp32:
+        # per-block reduction partials of the FP32 column-strip prims kernel: 60 valid columns per
+        # wave (64 lanes - 2 x (c + t reach 1 + 1)), 4 waves of 4 rows per block
+        blocks = ((n + 8 + 59) // 60) * ((n + 8 + 15) // 16)
+    else:
+        # FP64 prims: one block per 32 x 32 tile of the (n + 8)^2 W box
+        blocks = ((n + 8 + 31) // 32) ** 2
+    esz = 4 if fp32 else 8
+    expect = esz * 5 * ((n + 8) ** 2 + 2 * (n + 1) * n + (n + 6) ** 2) + 5 * n * n + 88 * blocks
     assert expect <= b.value <= expect + 4096
 
 
