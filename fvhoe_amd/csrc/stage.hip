@@ -65,6 +65,11 @@ constexpr int TPB = 256;
 #ifndef FVHOE_CAND_MAX_P
 #define FVHOE_CAND_MAX_P 3
 #endif
+// LDS-tiled prims / candidate: 1 = two buffers per sweep stage and one barrier per variable, 0 = one
+// buffer and three (two in the t-sweeps) barriers per variable
+#ifndef FVHOE_TILE_PIPE
+#define FVHOE_TILE_PIPE 1
+#endif
 #ifndef FVHOE_CAND_X
 #define FVHOE_CAND_X 32
 #endif
@@ -318,8 +323,9 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   constexpr int NPT = (TX * TY + NT - 1) / NT;
   constexpr int NC = (HX1 * HY1 + NT - 1) / NT;  // centre values per thread
   static_assert(HX1 * HY1 <= HX0 * HY0 && TX * HY1 <= HX1 * HY0, "S0 / S1 hold the t-sweep stages");
-  __shared__ T S0[HX0 * HY0];
-  __shared__ T S1[HX1 * HY0];
+  constexpr int NB = FVHOE_TILE_PIPE ? 2 : 1;  // buffers per sweep stage
+  __shared__ T S0[NB][HX0 * HY0];
+  __shared__ T S1[NB][HX1 * HY0];
 
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int oa = (tile / tiles1) * TX, ob = (tile % tiles1) * TY;
@@ -355,60 +361,104 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
   }
   // centre values of the HX1 x HY1 box stay in registers: thread tid holds elements tid + k * NT
   T cen[5][NC];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
+  T wr[5][NPT];
+  auto put_pre = [&](int v, int b) {
 #pragma unroll
     for (int k = 0; k < NE; ++k)
-      if (tid + k * NT < HX0 * HY0) S0[tid + k * NT] = pre[v][k];
-    __syncthreads();
-    tile_for<HX1, HY0, NT>([&](int e, int i, int j) {
-      S1[e] = sweepT<KIND_C, PX, T>([&](int t) { return S0[(i + RCX + t - reach(PX)) * HY0 + j]; });
-    });
-    __syncthreads();
-    tile_for_k<HX1, HY1, NT>([&](int k, int e, int i, int j) {
-      const T x = sweepT<KIND_C, PY, T>([&](int t) { return S1[i * HY0 + j + RCY + t - reach(PY)]; });
-      cen[v][k] = T(0) + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
-    });
-    __syncthreads();
-  }
-  // pressure recovery in FP64 (fvhoe/hydro.py:146-170)
-#pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    if (tid + k * NT < HX1 * HY1) {
-      double uu[5], ww[5];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) uu[q] = (double)cen[q][k];
-      prims_from_cons(uu, ww, gamma);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) cen[q][k] = T(ww[q]);
-    }
-  }
-  T wr[5][NPT];
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
-    // this variable's centre values through S0 (row stride HY1) for the x sweep
+      if (tid + k * NT < HX0 * HY0) S0[b][tid + k * NT] = pre[v][k];
+  };
+  auto put_cen = [&](int v, int b) {  // row stride HY1 for the t-sweeps
 #pragma unroll
     for (int k = 0; k < NC; ++k)
-      if (tid + k * NT < HX1 * HY1) S0[tid + k * NT] = cen[v][k];
-    __syncthreads();
-    tile_for<TX, HY1, NT>([&](int e, int i, int j) {
-      S1[e] = sweepT<KIND_T, PX, T>([&](int t) { return S0[(i + RTX + t - reach(PX)) * HY1 + j]; });
+      if (tid + k * NT < HX1 * HY1) S0[b][tid + k * NT] = cen[v][k];
+  };
+  auto csweep_x = [&](int b) {
+    tile_for<HX1, HY0, NT>([&](int e, int i, int j) {
+      S1[b][e] = sweepT<KIND_C, PX, T>([&](int t) { return S0[b][(i + RCX + t - reach(PX)) * HY0 + j]; });
     });
-    __syncthreads();
+  };
+  auto csweep_y = [&](int v, int b) {
+    tile_for_k<HX1, HY1, NT>([&](int k, int e, int i, int j) {
+      const T x = sweepT<KIND_C, PY, T>([&](int t) { return S1[b][i * HY0 + j + RCY + t - reach(PY)]; });
+      cen[v][k] = T(0) + x;  // degree-0 z sweep (fvhoe/fv.py:186-196)
+    });
+  };
+  auto tsweep_x = [&](int b) {
+    tile_for<TX, HY1, NT>([&](int e, int i, int j) {
+      S1[b][e] = sweepT<KIND_T, PX, T>([&](int t) { return S0[b][(i + RTX + t - reach(PX)) * HY1 + j]; });
+    });
+  };
+  auto tsweep_y = [&](int v, int b) {
 #pragma unroll
     for (int q = 0; q < NPT; ++q) {
       const int e = tid + q * NT;
       if (e < TX * TY) {
         const int i = e / TY, j = e % TY;
-        const T x = sweepT<KIND_T, PY, T>([&](int t) { return S1[i * HY1 + j + RTY + t - reach(PY)]; });
+        const T x = sweepT<KIND_T, PY, T>([&](int t) { return S1[b][i * HY1 + j + RTY + t - reach(PY)]; });
         wr[v][q] = T(0) + x;  // z-axis sweep of degree 0 (fvhoe/fv.py:221-223)
-        const int a = oa + i, b = ob + j;
-        if (a < NW0 && b < NW1) W[(v * NW0 + a) * NW1 + b] = wr[v][q];
+        const int a = oa + i, bb = ob + j;
+        if (a < NW0 && bb < NW1) W[(v * NW0 + a) * NW1 + bb] = wr[v][q];
       }
     }
+  };
+  auto recover = [&]() {  // pressure recovery in FP64 (fvhoe/hydro.py:146-170)
+#pragma unroll
+    for (int k = 0; k < NC; ++k) {
+      if (tid + k * NT < HX1 * HY1) {
+        double uu[5], ww[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) uu[q] = (double)cen[q][k];
+        prims_from_cons(uu, ww, gamma);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) cen[q][k] = T(ww[q]);
+      }
+    }
+  };
+#if FVHOE_TILE_PIPE
+  // Two buffers per sweep stage: variable v's second sweep and variable v+1's first sweep share a
+  // barrier interval, one barrier per variable.  Each step writes a buffer whose last reader ran
+  // before the previous barrier (buffer v & 1 holds variable v).
+  put_pre(0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    csweep_x(v & 1);
+    if (v + 1 < 5) put_pre(v + 1, (v + 1) & 1);
+    __syncthreads();
+    csweep_y(v, v & 1);
+  }
+  recover();
+  put_cen(0, 0);
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    tsweep_x(v & 1);
+    if (v + 1 < 5) put_cen(v + 1, (v + 1) & 1);
+    __syncthreads();
+    tsweep_y(v, v & 1);
+  }
+#else
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    put_pre(v, 0);
+    __syncthreads();
+    csweep_x(0);
+    __syncthreads();
+    csweep_y(v, 0);
+    __syncthreads();
+  }
+  recover();
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    put_cen(v, 0);
+    __syncthreads();
+    tsweep_x(0);
+    __syncthreads();
+    tsweep_y(v, 0);
     // no barrier here: the next variable's S0 stores touch no S1 element, and its barrier comes
     // before S1 is written again
   }
+#endif
   if (!red.partials) return;
   // partial reductions over this tile's cells of the dt box and the range box
   double m = 0.0;
