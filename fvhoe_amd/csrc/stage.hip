@@ -70,6 +70,15 @@ constexpr int TPB = 256;
 #ifndef FVHOE_TILE_PIPE
 #define FVHOE_TILE_PIPE 1
 #endif
+// flux tiles: 1 = two staging buffers for the transverse sweep (one barrier per variable), 0 = one.
+// Two buffers measured 107 / 108 us per FP64 launch at KH 2048^2 against 104 / 105 for one, and no
+// change in FP32 (60 / 62 us)
+#ifndef FVHOE_FLUX_PIPE_F64
+#define FVHOE_FLUX_PIPE_F64 0
+#endif
+#ifndef FVHOE_FLUX_PIPE_F32
+#define FVHOE_FLUX_PIPE_F32 0
+#endif
 #ifndef FVHOE_CAND_X
 #define FVHOE_CAND_X 32
 #endif
@@ -660,8 +669,9 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
   constexpr int HS = HT0 | 1;  // S0 row stride, odd: the transposed stores of D = 1 hit distinct banks
   constexpr int NR = (TA * HT1 + NT - 1) / NT;   // Riemann problems per thread
   static_assert(TA * HT1 <= HN * HT1, "flux results reuse the X buffer");
+  constexpr int NB = (sizeof(T) == 8 ? FVHOE_FLUX_PIPE_F64 : FVHOE_FLUX_PIPE_F32) ? 2 : 1;  // S0 buffers
   __shared__ T X[5][HN * HT1];
-  __shared__ T S0[HN * HS];
+  __shared__ T S0[NB][HN * HS];
   const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
 
   const int nN = D == 0 ? g.n0 : g.n1, nT = D == 0 ? g.n1 : g.n0;
@@ -699,17 +709,36 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
       }
     }
   }
-#pragma unroll
-  for (int v = 0; v < 5; ++v) {
+  auto put = [&](int v, int bf) {
 #pragma unroll
     for (int k = 0; k < NE; ++k)
-      if (tid + k * NT < HN * HT0) S0[dst[k]] = pre[v][k];
-    __syncthreads();
+      if (tid + k * NT < HN * HT0) S0[bf][dst[k]] = pre[v][k];
+  };
+  auto csweep = [&](int v, int bf) {
     tile_for<HN, HT1, NT>([&](int e, int r, int c) {
-      const T x = sweepT<KIND_C, PT, T>([&](int t) { return S0[r * HS + c + RC + t - reach(PT)]; });
+      const T x = sweepT<KIND_C, PT, T>([&](int t) { return S0[bf][r * HS + c + RC + t - reach(PT)]; });
       X[v][e] = T(0) + x;  // degree-0 z sweep precedes/follows (fvhoe/solver.py:524,541)
     });
+  };
+  if constexpr (NB == 2) {
+    // variable v + 1 is staged in the other buffer while v is swept: one barrier per variable (the
+    // buffer each store writes was last read before the previous barrier)
+    put(0, 0);
     __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      csweep(v, v & 1);
+      if (v + 1 < 5) put(v + 1, (v + 1) & 1);
+      __syncthreads();
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      put(v, 0);
+      __syncthreads();
+      csweep(v, 0);
+      __syncthreads();
+    }
   }
   // Riemann problems at faces f0+fr, transverse rows c in [0, HT1)
   T res[NR][5];
