@@ -412,8 +412,8 @@ def run_form(n, form, steps, warmup, u0):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is described in DESIGN.md "
-                                   "and not built", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; sharding over several GPUs (x-slab domain "
+                                   "decomposition with halo exchanges) is not built", "n_gpus": a.gpus}))
         sys.exit(2)
     if a.workload == "ensemble":
         run_ensemble(a)
