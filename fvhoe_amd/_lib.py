@@ -35,6 +35,8 @@ class FvhoeStageConfig(ctypes.Structure):
         ("sed", c_int), ("sed_eps", c_double), ("fixed_dt_on", c_int), ("fixed_dt", c_double),
         ("bulk_fp32", c_int),
         ("slab", c_void_p * 6), ("slab_g", c_int), ("slab_pad", c_int * 3),
+        ("mag_count", c_void_p), ("mag_idx", c_void_p), ("mag_nad", c_void_p), ("mag_pad", c_void_p),
+        ("mag_slot", c_int),
     ]
 
 
@@ -92,6 +94,11 @@ _SIGNATURES = {
     "fvhoe_advance_time_counted": (c_int, [_P, _P, _P, _P, c_longlong, _P]),
     "fvhoe_log_count": (c_int, [_P, _P, c_longlong, c_int, _P, _P]),
     "fvhoe_log_mean": (c_int, [_P, _P, c_longlong, c_double, _P, _P]),
+    "fvhoe_log_magnitudes": (c_int, [_P, _P, _P, _P, c_longlong, _P, _P, _P, _P, c_int, c_longlong, _P, _P]),
+    "fvhoe_set_scalar_gated": (c_int, [_P, c_double, _P, _P]),
+    "fvhoe_slope_limit": (c_int, [c_int, _P, _P, _P, c_longlong, _P]),
+    "fvhoe_central_difference": (c_int, [_P, _P, c_longlong, c_longlong, c_longlong, c_double, c_int, _P]),
+    "fvhoe_compute_fluxes": (c_int, [_P, _P, _P, c_int, c_longlong, c_int, c_int, _P]),
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
     "fvhoe_stage_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),
     "fvhoe_stage": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, _P, _P]),
@@ -215,29 +222,86 @@ class OverlapStreams:
 
 
 def _destroy_events(library, *events):
+    if _capture_depth > 0:  # destroying an event inside an open capture is not allowed: later
+        _deferred.append(lambda: _destroy_events(library, *events))
+        return
     for e in events:
         library.fvhoe_event_destroy(c_void_p(e))
 
 
+# Releases asked for while a graph capture is open, run when the outermost capture closes.  Destroying a
+# graph, an event or a stream during a capture aborts the process, whether the owner dies by reference
+# count or in a collection.
+_capture_depth = 0
+_deferred: list = []
+
+
+def capturing() -> bool:
+    return _capture_depth > 0
+
+
+def defer_release(obj) -> bool:
+    """Keep obj alive until the open capture closes (True), or do nothing when no capture is open (False)."""
+    if _capture_depth > 0:
+        _deferred.append(obj)
+        return True
+    return False
+
+
+def _drain_deferred() -> None:
+    while _deferred:
+        item = _deferred.pop()
+        if callable(item) and not isinstance(item, GraphHolder):
+            item()
+        del item
+
+
+class GraphHolder:
+    """Owner of a captured graph (and whatever else must outlive it): when the holder is released while a
+    capture is open, the graph is handed to the deferral list instead of being destroyed on the spot."""
+
+    __slots__ = ("graph", "extra")
+
+    def __init__(self, graph, extra=None):
+        self.graph = graph
+        self.extra = extra
+
+    def replay(self):
+        self.graph.replay()
+
+    def __del__(self):
+        g, self.graph = self.graph, None
+        if g is not None:
+            defer_release(g)
+
+
 class capture_guard:
-    """Context for a HIP graph capture: dead objects that hold device resources (solvers in reference
-    cycles with their own graphs and tensors) must not be finalised while a capture is open -- destroying
-    a graph or releasing memory is not allowed during a capture and aborts the process.  Collects them
-    on entry and keeps Python's cyclic collector off until the capture is closed."""
+    """Context for a HIP graph capture.  Dead objects that hold device resources (solvers in reference
+    cycles with their own graphs and tensors) must not be finalised while a capture is open: destroying
+    a graph or an event during a capture aborts the process.  On entry: collect, and keep Python's cyclic
+    collector off until the capture closes; graphs (GraphHolder) and events (OverlapStreams) released in
+    between by reference count are kept on a deferral list and released when the outermost capture
+    closes."""
 
     def __enter__(self):
         import gc
 
+        global _capture_depth
         gc.collect()
         self._was_enabled = gc.isenabled()
         gc.disable()
+        _capture_depth += 1
         return self
 
     def __exit__(self, *exc):
         import gc
 
+        global _capture_depth
+        _capture_depth -= 1
         if self._was_enabled:
             gc.enable()
+        if _capture_depth == 0:
+            _drain_deferred()
         return False
 
 

@@ -12,7 +12,7 @@ the device once per (ghost width, time), and read there by the fill kernel.
 
 from __future__ import annotations
 
-from dataclasses import dataclass
+from dataclasses import dataclass, field
 from typing import Tuple, Union
 
 import numpy as np
@@ -31,6 +31,10 @@ class BoundaryCondition:
     y_value: object = None
     z_value: object = None
     slab_coords: dict = None
+    # accepted for the reference's signature (fvhoe/boundary_conditions.py:189-190): its array manager and
+    # variable map; here slabs live in device tensors and the variable order is fixed
+    array_manager: object = field(default_factory=lambda: None)
+    hydro_state: object = field(default_factory=lambda: None)
 
     def __post_init__(self):
         for d in "xyz":
@@ -51,6 +55,7 @@ class BoundaryCondition:
             self.slab_buffer_size = (self.slab_coords["xl"][0].shape[0], self.slab_coords["yl"][0].shape[1],
                                      self.slab_coords["zl"][0].shape[2])
         self._slab_cache = {}
+        self.version = 0  # bumped by reset_value: holders of device slabs fetch them again
 
     # ------------------------------------------------------------- settings
     def codes(self):
@@ -80,6 +85,7 @@ class BoundaryCondition:
             raise ValueError(f"Invalid pos '{pos}'")
         setattr(self, f"{dim}_value", (lv, rv))
         self._slab_cache = {}
+        self.version += 1
 
     def trim_slabs(self, gw: Tuple[int, int, int], axis: int, pos: str):
         """Slab-centre coordinates of side (axis, pos) for ghost widths gw, cut from the

@@ -627,6 +627,45 @@ int launch_face_flux_p(int p, const T* w, T* out, int n0, int n1, int n2, int so
   return -1;
 }
 
+// minmod / moncen of two difference arrays (fvhoe/slope_limiting.py:56-84), elementwise
+__global__ void k_limit(int limiter, const double* __restrict__ dl, const double* __restrict__ dr,
+                        double* __restrict__ out, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    out[i] = limiter == LIM_MINMOD ? minmod(dl[i], dr[i]) : moncen(dl[i], dr[i]);
+}
+
+// 0.5 (u[l+1] - u[l-1]) along the middle axis of u viewed as (A, L, B), divided by h when asked
+// (fvhoe/fv.py:280-296)
+__global__ void k_central_diff(const double* __restrict__ u, double* __restrict__ out, long long A, long long L,
+                               long long B, double h, int has_h) {
+  const long long Lo = L - 2, n = A * Lo * B;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const long long b = i % B, r = i / B, l = r % Lo, a = r / Lo;
+    const long long s = (a * L + l) * B + b;
+    double d = 0.5 * (u[s + 2 * B] - u[s]);
+    if (has_h) d = d / h;
+    out[i] = d;
+  }
+}
+
+// physical Euler flux along dim from primitives w and conservatives u (fvhoe/hydro.py:213-246)
+__global__ void k_fluxes(const double* __restrict__ w, const double* __restrict__ u, double* __restrict__ out,
+                         int nv, long long nc, int dim, int include_pressure) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < nc; i += (long long)gridDim.x * blockDim.x) {
+    const double v = w[(1 + dim) * nc + i];
+    out[i] = v * w[i];
+    for (int q = 1; q < 4; ++q) out[q * nc + i] = v * u[q * nc + i];
+    double E = v * u[4 * nc + i];
+    if (include_pressure) {
+      const double P = w[4 * nc + i];
+      out[(1 + dim) * nc + i] = out[(1 + dim) * nc + i] + P;
+      E = E + v * P;
+    }
+    out[4 * nc + i] = E;
+    for (int q = 5; q < nv; ++q) out[q * nc + i] = v * u[q * nc + i];
+  }
+}
+
 __global__ void k_muscl(const double* __restrict__ u, double* __restrict__ left, double* __restrict__ right, int nv,
                         int n0, int n1, int n2, int axis, int limiter) {
   const int m0 = axis == 1 ? n0 - 2 : n0, m1 = axis == 2 ? n1 - 2 : n1, m2 = axis == 3 ? n2 - 2 : n2;
@@ -1077,6 +1116,39 @@ __global__ void k_log_mean(double* __restrict__ dst, const int* __restrict__ cnt
 }
 
 __global__ void k_set_scalar(double* p, double v) { p[0] = v; }
+__global__ void k_set_scalar_gated(double* p, double v, const double* __restrict__ gate) {
+  if (!(gate && gate[0] == 0.0)) p[0] = v;
+}
+
+// dense violation-magnitude logs from the fused stage's slots: out = base (or 0), then one slot after the
+// other out[cell] = out[cell] + magnitude (a cell appears at most once per slot), then out / divisor
+__global__ void k_mag_base(double* __restrict__ nad_out, double* __restrict__ pad_out,
+                           const double* __restrict__ nad_base, const double* __restrict__ pad_base, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    nad_out[i] = nad_base ? nad_base[i] : 0.0;
+    pad_out[i] = pad_base ? pad_base[i] : 0.0;
+  }
+}
+__global__ void k_mag_scatter(double* __restrict__ nad_out, double* __restrict__ pad_out,
+                              const int* __restrict__ count, const int* __restrict__ idx,
+                              const double* __restrict__ nad, const double* __restrict__ pad, long long n,
+                              long long cap) {
+  const int m = (long long)count[0] < cap ? count[0] : (int)cap;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const int c = idx[e];
+    if (c < 0 || c >= n) continue;  // entries always name interior cells; a bad slot drops out
+    nad_out[c] = nad_out[c] + nad[e];
+    pad_out[c] = pad_out[c] + pad[e];
+  }
+}
+__global__ void k_mag_divide(double* __restrict__ nad_out, double* __restrict__ pad_out, long long n,
+                             const double* __restrict__ divisor) {
+  const double d = divisor[0];
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    nad_out[i] = nad_out[i] / d;
+    pad_out[i] = pad_out[i] / d;
+  }
+}
 
 __global__ void k_hit_target(double* dt_step, const double* dt_stage, const double* t, double target, int has_target) {
   const double dt = dt_stage[0];
@@ -1429,6 +1501,32 @@ int fvhoe_face_fluxes_f32(const float* w, float* out, int n0, int n1, int n2, in
   return face_fluxes_impl<float>(w, out, n0, n1, n2, axis, p, solver, gamma, csq_floor, stream);
 }
 
+int fvhoe_slope_limit(int limiter, const double* du_left, const double* du_right, double* out, long long n,
+                      void* stream) {
+  if (!du_left || !du_right || !out || n < 0 || (limiter != LIM_MINMOD && limiter != LIM_MONCEN))
+    return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  k_limit<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(limiter, du_left, du_right, out, n);
+  return launch_status();
+}
+
+int fvhoe_central_difference(const double* u, double* out, long long outer, long long len, long long inner,
+                             double h, int has_h, void* stream) {
+  if (!u || !out || outer < 0 || len < 2 || inner < 0) return FVHOE_ERR_ARG;
+  if (outer * (len - 2) * inner == 0) return FVHOE_OK;
+  k_central_diff<<<grid_for(outer * (len - 2) * inner), TPB, 0, (hipStream_t)stream>>>(u, out, outer, len, inner, h,
+                                                                                        has_h);
+  return launch_status();
+}
+
+int fvhoe_compute_fluxes(const double* w, const double* u, double* out, int nv, long long ncell, int dim,
+                         int include_pressure, void* stream) {
+  if (!w || !u || !out || nv < 5 || ncell < 0 || dim < 0 || dim > 2) return FVHOE_ERR_ARG;
+  if (ncell == 0) return FVHOE_OK;
+  k_fluxes<<<grid_for(ncell), TPB, 0, (hipStream_t)stream>>>(w, u, out, nv, ncell, dim, include_pressure);
+  return launch_status();
+}
+
 int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
                 void* stream) {
   if (!u || !left || !right || axis < 1 || axis > 3 || limiter < 0 || limiter > 2) return FVHOE_ERR_ARG;
@@ -1637,6 +1735,29 @@ int fvhoe_log_mean(double* mean, const int* count, long long n, double divisor, 
   if (!mean || !count || n < 0) return FVHOE_ERR_ARG;
   if (n == 0) return FVHOE_OK;
   k_log_mean<<<grid_for(n), TPB, 0, (hipStream_t)stream>>>(mean, count, n, divisor, gate);
+  return launch_status();
+}
+
+int fvhoe_log_magnitudes(double* nad_out, double* pad_out, const double* nad_base, const double* pad_base,
+                         long long n, const int* count, const int* idx, const double* nad, const double* pad,
+                         int nslots, long long cap, const double* divisor, void* stream) {
+  if (!nad_out || !pad_out || n < 0 || nslots < 0 || cap < 0 || (nslots > 0 && (!count || !idx || !nad || !pad)))
+    return FVHOE_ERR_ARG;
+  if ((nad_base == nullptr) != (pad_base == nullptr) || n > 2147483647LL || cap > 2147483647LL) return FVHOE_ERR_ARG;
+  if (n == 0) return FVHOE_OK;
+  hipStream_t s = (hipStream_t)stream;
+  k_mag_base<<<grid_for(n), TPB, 0, s>>>(nad_out, pad_out, nad_base, pad_base, n);
+  const int gs = grid_for(cap > 0 ? cap : 1);
+  for (int k = 0; k < nslots; ++k)
+    k_mag_scatter<<<gs, TPB, 0, s>>>(nad_out, pad_out, count + k, idx + k * cap, nad + k * cap, pad + k * cap, n,
+                                     cap);
+  if (divisor) k_mag_divide<<<grid_for(n), TPB, 0, s>>>(nad_out, pad_out, n, divisor);
+  return launch_status();
+}
+
+int fvhoe_set_scalar_gated(double* p, double v, const double* gate, void* stream) {
+  if (!p) return FVHOE_ERR_ARG;
+  k_set_scalar_gated<<<1, 1, 0, (hipStream_t)stream>>>(p, v, gate);
   return launch_status();
 }
 

@@ -297,7 +297,9 @@ __device__ __forceinline__ T minmod(T dl, T dr) {
 }
 
 template <typename T>
-__device__ __forceinline__ T dsign(T x) { return x > T(0) ? T(1) : (x < T(0) ? T(-1) : x); }
+__device__ __forceinline__ T dsign(T x) {  // np.sign: +0 for either zero, NaN stays NaN
+  return x > T(0) ? T(1) : (x < T(0) ? T(-1) : (x == x ? T(0) : x));
+}
 
 template <typename T>
 __device__ __forceinline__ T moncen(T dl, T dr) {

@@ -796,23 +796,55 @@ struct DetectP {
   double eps, pad_lo, pad_hi, sed_eps;
 };
 
+// Violation magnitudes for the trouble log (fvhoe/slope_limiting.py:225-239): every cell whose NAD or PAD
+// test flags is appended to this stage's slot as (cell, NAD magnitude, PAD magnitude).  NAD magnitude:
+// -min over all five variables of the NAD indicator where NAD flags (after SED and the NAD_vars mask),
+// else 0; PAD magnitude: -(PAD indicator of the last PAD key) where PAD flags, else 0.
+struct MagLog {
+  int* count;          // entries in this stage's slot (cleared by k_stage_reduce)
+  int* idx;
+  double* nad;
+  double* pad;
+  const double* gate;  // the step's dt: a step at its target time (dt = 0) appends nothing
+};
+__device__ __forceinline__ bool mag_on(const MagLog& m) { return m.idx && !(m.gate && m.gate[0] == 0.0); }
+
+// append the lanes' flagged cells to the slot: one atomic per wave
+__device__ __forceinline__ void mag_append(const MagLog& m, bool f, int cell, double nad, double pad) {
+  const unsigned long long b = __ballot(f);
+  if (b == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(m.count, (int)__popcll(b));
+  base = __shfl(base, 0, 64);
+  if (f) {
+    const int e = base + (int)__popcll(b & ((1ull << lane) - 1ull));
+    m.idx[e] = cell;
+    m.nad[e] = nad;
+    m.pad[e] = pad;
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0, int NW1,
                                                const T* __restrict__ WS, int NS0, int NS1, Geo g,
                                                const double* __restrict__ range_mm, DetectP a,
                                                signed char* __restrict__ trouble, int* __restrict__ count,
                                                int* __restrict__ list, const signed char* __restrict__ pre,
-                                               int* __restrict__ fp_count) {
+                                               int* __restrict__ fp_count, MagLog ml) {
   const int nc = g.n0 * g.n1;
   const bool two_d = g.ge1 > 0;
   const int c = blockIdx.x * TPB + threadIdx.x;
   int t = 0, fp = 0;
+  bool mflag = false;
+  double mnad = 0.0, mpad = 0.0;
   if (c < nc) {
     const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
     const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
     const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
     const int NWC = NW0 * NW1, NSC = NS0 * NS1;
     bool tvv[5];
+    double imin = 0.0;  // min over the variables of the NAD indicator
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
       const T* wv = W + v * NWC;
@@ -846,6 +878,7 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
         tv = ind < 0.0;
       }
       tvv[v] = tv && ((a.nad_vars >> v) & 1);
+      imin = v == 0 ? ind : dmin(imin, ind);
     }
     bool nad = false;
     for (int v = 0; v < 5; ++v) {
@@ -859,10 +892,15 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
       nad = nad || tv;
     }
     bool pad = false;
+    double pind = 0.0;
     if (a.pad_var >= 0) {
       const double x = (double)WS[a.pad_var * NSC + sb];
-      pad = dmin(x - a.pad_lo, a.pad_hi - x) < 0.0;
+      pind = dmin(x - a.pad_lo, a.pad_hi - x);
+      pad = pind < 0.0;
     }
+    mflag = nad || pad;
+    mnad = nad ? -imin : 0.0;
+    mpad = pad ? -pind : 0.0;
     t = (a.force || pad || nad) ? 1 : 0;
     if (pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
       const int pf = pre[c];
@@ -895,6 +933,7 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
   }
   __syncthreads();
   if (t) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
+  if (mag_on(ml)) mag_append(ml, mflag, c, mnad, mpad);
 }
 
 // 2-D detection over strips of R rows (x) per thread, one column (y) per
@@ -907,13 +946,14 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
                                                      const double* __restrict__ range_mm, DetectP a,
                                                      signed char* __restrict__ trouble, int* __restrict__ count,
                                                      int* __restrict__ list, const signed char* __restrict__ pre,
-                                                     int* __restrict__ fp_count, int jblocks) {
+                                                     int* __restrict__ fp_count, int jblocks, MagLog ml) {
   const int tid = threadIdx.x;
   const int j = (blockIdx.x % jblocks) * TPB + tid, i0 = (blockIdx.x / jblocks) * R;
   const bool col = j < g.n1;
   const int NWC = NW0 * NW1, NSC = NS0 * NS1;
   const int jc = col ? j : g.n1 - 1;  // idle lanes read a valid column, their results are dropped
   unsigned nadbits[R];  // bit v: NAD test of variable v (masked by nad_vars)
+  double imin[R];       // min over the variables of the NAD indicator
 #pragma unroll
   for (int k = 0; k < R; ++k) nadbits[k] = 0u;
 #pragma unroll
@@ -935,20 +975,27 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
       const double m = dmin(dmin(hm[k], hm[k + 1]), hm[k + 2]);
       const double uc = (double)WS[v * NSC + (i + 3) * NS1 + jc + 3];
       bool tv;
+      double ind;
       if (a.nad_range == 0) {
         const double rng = a.nad_mode == 0 ? rhi - rlo : M - m;
-        tv = dmin(uc - m, M - uc) < -a.eps * rng;
+        ind = dmin(uc - m, M - uc);
+        tv = ind < -a.eps * rng;
       } else {
         const double lo = a.nad_mode == 0 ? (1.0 - a.eps) * rlo : (1.0 - a.eps) * m;
         const double hi = a.nad_mode == 0 ? (1.0 + a.eps) * rhi : (1.0 + a.eps) * M;
-        tv = dmin(uc - lo, hi - uc) < 0.0;
+        ind = dmin(uc - lo, hi - uc);
+        tv = ind < 0.0;
       }
       if (tv && ((a.nad_vars >> v) & 1)) nadbits[k] |= 1u << v;
+      imin[k] = v == 0 ? ind : dmin(imin[k], ind);
     }
   }
-  unsigned tbits = 0u, fpbits = 0u;
+  unsigned tbits = 0u, fpbits = 0u, mbits = 0u;
+  double mnad[R], mpad[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
+    mnad[k] = 0.0;
+    mpad[k] = 0.0;
     const int i = i0 + k;
     if (!col || i >= g.n0) continue;
     const int sb = (i + 3) * NS1 + j + 3;
@@ -964,10 +1011,15 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
       nad = nad || tv;
     }
     bool pad = false;
+    double pind = 0.0;
     if (a.pad_var >= 0) {
       const double x = (double)WS[a.pad_var * NSC + sb];
-      pad = dmin(x - a.pad_lo, a.pad_hi - x) < 0.0;
+      pind = dmin(x - a.pad_lo, a.pad_hi - x);
+      pad = pind < 0.0;
     }
+    if (nad || pad) mbits |= 1u << k;
+    mnad[k] = nad ? -imin[k] : 0.0;
+    mpad[k] = pad ? -pind : 0.0;
     int t = (a.force || pad || nad) ? 1 : 0;
     const int c = i * g.n1 + j;
     if (pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
@@ -1003,6 +1055,10 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
 #pragma unroll
   for (int k = 0; k < R; ++k)
     if ((tbits >> k) & 1u) list[pos++] = (i0 + k) * g.n1 + j;
+  if (mag_on(ml)) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) mag_append(ml, (mbits >> k) & 1u, (i0 + k) * g.n1 + j, mnad[k], mpad[k]);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,7 +1258,7 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
                                                        double* range, double cfl_h, int fixed_on, double fixed_dt,
                                                        double* dt, double* dt_step, const double* t, double target,
                                                        int has_target, const double* target_ptr, int* count,
-                                                       int reset_pre) {
+                                                       int reset_pre, int* mag_count, const double* mag_gate) {
   __shared__ unsigned long long part[16];
   const int tid = threadIdx.x, k = blockIdx.x;
   if (k == 0 && tid == 0) {  // this stage's flag counters (the previous stage's consumers ran before)
@@ -1224,6 +1280,8 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
       dt[0] = d;
       const double tg = target_ptr ? target_ptr[0] : target;  // a device target serves every replay
       if (dt_step) dt_step[0] = (has_target && tg - t[0] < d) ? tg - t[0] : d;
+      // this stage's magnitude slot, after the step's dt is known (a step at its target keeps the slot)
+      if (mag_count && !(mag_gate && mag_gate[0] == 0.0)) mag_count[0] = 0;
     } else {
       range[k - 1] = from_ordered_bits(x);
     }
@@ -1311,6 +1369,7 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
       return FVHOE_ERR_ARG;
   }
   if (dirichlet && pred) return FVHOE_ERR_UNSUPPORTED;  // the predictor's features map ghosts into the interior
+  if (c->mag_idx && (!c->mag_count || !c->mag_nad || !c->mag_pad || c->mag_slot < 0)) return FVHOE_ERR_ARG;
   L->dirichlet = dirichlet;
   L->two_d = ay;
   L->p = c->p[0];
@@ -1439,11 +1498,18 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
   RkArgs rk{-1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   const bool set_step = rkf && rkf->set_dt_step;
+  MagLog ml{};
+  if (apost && c->mag_idx) {
+    const size_t off = (size_t)c->mag_slot * (size_t)L.n0 * (size_t)L.n1;
+    ml = MagLog{c->mag_count + c->mag_slot, c->mag_idx + off, c->mag_nad + off, c->mag_pad + off,
+                rkf ? rkf->dt_step : nullptr};
+  }
   if (rkf && rkf->kind >= 0) rk = RkArgs{rkf->kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
   k_stage_reduce<<<NPART, 1024, 0, s>>>(partials, L.prims_blocks, range, c->cfl * hmin, c->fixed_dt_on, c->fixed_dt,
                                     dt_stage, set_step ? rkf->dt_step : nullptr, set_step ? rkf->t : nullptr,
                                     set_step ? rkf->target : 0.0, set_step ? rkf->has_target : 0,
-                                    set_step ? rkf->target_ptr : nullptr, count, use_pred && pred->infer);
+                                    set_step ? rkf->target_ptr : nullptr, count, use_pred && pred->infer,
+                                    ml.idx ? ml.count : nullptr, ml.gate);
   if (use_pred && pred->infer) {  // the network on this stage's W, features formed on load
     hipStream_t ps = s;
     if (overlap) {  // fork: the network runs beside the rest of this stage
@@ -1525,10 +1591,10 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       TimedLaunch tl(TK_DETECT, s);
       if (TWO_D)
         k_detect_strip<T, DR><<<gridd2, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count,
-                                                      list, preflags, count + 2, jblocks);
+                                                      list, preflags, count + 2, jblocks, ml);
       else
         k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list,
-                                        preflags, count + 2);
+                                        preflags, count + 2, ml);
     }
     {
       TimedLaunch tl(TK_FALLBACK, s);

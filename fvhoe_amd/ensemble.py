@@ -119,8 +119,8 @@ class Ensemble:
                           _lib.ptr(self.u), _lib.ptr(self.t), _lib.ptr(self.steps), float(target),
                           self._wsp(), _lib.ctypes.c_void_p(s.cuda_stream))
         torch.cuda.current_stream().wait_stream(s)
-        self._graph = (g, float(target))
-        return g
+        self._graph = (_lib.GraphHolder(g), float(target))
+        return self._graph[0]
 
     def run(self, T: float, graph: bool = True, check_every: int = 64, max_steps: int = 10 ** 7) -> int:
         """Advance every instance to time T; returns the number of ensemble steps taken."""

@@ -9,9 +9,14 @@ float32.
 
 from __future__ import annotations
 
+from functools import lru_cache
 from itertools import product
+from typing import Callable, Tuple
+
+import numpy as np
 
 from fvhoe_amd import _lib
+from fvhoe_amd import initial_conditions as _ics
 from fvhoe_amd.initial_conditions import uniform_mesh
 from fvhoe_amd.stencils import get_stencil_size
 
@@ -43,6 +48,52 @@ def fv_uniform_meshgen(n, x=(0, 1), y=(0, 1), z=(0, 1), slab_thickness=(0, 0, 0)
                 ext.append((lo - h[e] * g[e], hi + h[e] * g[e]))
         slabs[dim + pos] = tuple(uniform_mesh(tuple(cells), *ext))
     return inner, slabs
+
+
+@lru_cache(maxsize=None)
+def get_symmetric_slices(ndim: int, nslices: int, axis: int) -> list:
+    """The nslices shifted views along axis, left to right: cut (i, -(nslices-1)+i) for i < nslices
+    (fvhoe/fv.py:88-103); index tuples for arrays of rank ndim.  The sweep kernels index the same
+    windows directly."""
+    from fvhoe_amd.hydro import HydroState
+
+    if nslices < 1:
+        raise ValueError(f"{nslices=}")
+    hs = HydroState(ndim=ndim)
+    return [hs(axis=axis, cut=(i, -(nslices - 1) + i)) for i in range(nslices)]
+
+
+@lru_cache(maxsize=10)
+def get_legendre_quadrature(p: int) -> Tuple[np.ndarray, np.ndarray]:
+    """Gauss-Legendre points and weights on [-1/2, 1/2] with ceil((p+1)/2) points (fvhoe/fv.py:227-240)."""
+    points, weights = np.polynomial.legendre.leggauss(-(-(p + 1) // 2))
+    return points / 2, weights / 2
+
+
+def fv_average(f: Callable, x, y, z, h, p=(0, 0, 0)):
+    """Cell averages of f(x, y, z) by tensor Gauss-Legendre quadrature (fvhoe/fv.py:243-277); x, y, z are
+    NumPy meshes or float64 device tensors."""
+    return _ics.fv_average(f, x, y, z, h, p)
+
+
+def second_order_central_difference(u, axis: int, h: float = None):
+    """0.5 (u[i+1] - u[i-1]) along axis, divided by h when given (fvhoe/fv.py:280-296); u of any rank,
+    the result shorter by 2 along axis."""
+    x, was_np = _lib.as_device(u)
+    shape = tuple(int(s) for s in x.shape)
+    if not -len(shape) <= axis < len(shape):
+        raise IndexError(f"axis {axis} is out of bounds for array of dimension {len(shape)}")
+    axis = axis % len(shape)
+    if shape[axis] < 2:
+        raise ValueError(f"axis {axis} of length {shape[axis]} is shorter than the 3-point stencil")
+    outer = int(np.prod(shape[:axis], dtype=np.int64))
+    inner = int(np.prod(shape[axis + 1:], dtype=np.int64))
+    out_shape = list(shape)
+    out_shape[axis] -= 2
+    out = _lib.empty(out_shape)
+    _lib.call("fvhoe_central_difference", _lib.ptr(x), _lib.ptr(out), outer, shape[axis], inner,
+              float(h) if h is not None else 1.0, int(h is not None), _lib.stream())
+    return _lib.back(out, was_np)
 
 
 def _as4(u):

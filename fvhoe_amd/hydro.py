@@ -94,6 +94,29 @@ def compute_sound_speed(w, gamma: float, csq_floor: float):
     return _lib.back(out, was_np)
 
 
+def compute_fluxes(hs: HydroState, w, u, gamma: float, dim: str, include_pressure: bool = True):
+    """Physical flux along dim from primitives w and conservatives u (fvhoe/hydro.py:213-246):
+    rho v, m v (+ P on the dim component), E v (+ v P), passives advected; gamma is not used."""
+    if dim not in ("x", "y", "z"):
+        raise ValueError(f"Variable 'v{dim}' not found.")
+    x, was_np = _lib.as_device(w)
+    y, _ = _lib.as_device(u)
+    if tuple(x.shape) != tuple(y.shape) or x.shape[0] < 5:
+        raise ValueError(f"w and u must share a (nvars >= 5, ...) shape, got {tuple(x.shape)}, {tuple(y.shape)}")
+    if x.shape[0] != hs.nvars:
+        raise ValueError(f"{x.shape[0]} variables for a HydroState of {hs.nvars}")
+    out = _lib.empty(x.shape)
+    _lib.call("fvhoe_compute_fluxes", _lib.ptr(x), _lib.ptr(y), _lib.ptr(out), x.shape[0], _ncell(x),
+              "xyz".index(dim), int(bool(include_pressure)), _lib.stream())
+    return _lib.back(out, was_np)
+
+
+def advection_dt(h: float, vx: float = 0.0, vy: float = 0.0, vz: float = 0.0, CFL: float = 0.8) -> float:
+    """Time step of an advection problem at Courant number CFL (fvhoe/hydro.py:249-267): scalar host
+    arithmetic, CFL h / (vx + vy + vz)."""
+    return CFL * h / (vx + vy + vz)
+
+
 def hydro_dt_device(w, h: float, ndim: int, CFL: float, gamma: float, csq_floor: float):
     """hydro_dt with the result left on the device (1-element tensor)."""
     x, _ = _lib.as_device_bulk(w)

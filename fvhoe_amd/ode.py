@@ -29,7 +29,8 @@ class ODE(ABC):
 
     TIMES_CAP = 256  # most steps replayed between two host synchronisations in integrate
 
-    def __init__(self, u0, progress_bar: bool = False):
+    def __init__(self, u0, progress_bar: bool = True, cupy: bool = False):
+        # cupy: accepted for the reference's signature; the state is always a device tensor here
         self.t = 0.0
         self.timestamps = [0.0]
         self.step_count = 0
@@ -47,6 +48,26 @@ class ODE(ABC):
         # steps that moved since the last reset, and their end times (the first TIMES_CAP of them)
         self._steps_dev = _lib.zeros((1,), dtype=_lib.torch().int64)
         self._times_dev = _lib.zeros((self.TIMES_CAP,))
+
+    def get_commit_details(self) -> dict:
+        """Hash, author, date and branch of the package's git commit (fvhoe/ode.py:61-92)."""
+        from fvhoe_amd.solver import commit_details
+
+        return commit_details()
+
+    def progress_bar_action(self, action: str, T: float = None):
+        """Progress bar over simulated time: "setup", "update", "cleanup" (fvhoe/ode.py:201-216)."""
+        if not self.print_progress_bar:
+            return
+        if action == "setup":
+            from tqdm import tqdm
+
+            self.progress_bar = tqdm(total=T, bar_format="{l_bar}{bar}| {n_fmt}/{total_fmt} [{elapsed}]")
+        elif action == "update":
+            self.progress_bar.n = self.t
+            self.progress_bar.refresh()
+        elif action == "cleanup":
+            self.progress_bar.close()
 
     # ------------------------------------------------------------ to override
     @abstractmethod
@@ -217,7 +238,7 @@ class ODE(ABC):
                 self.f_evaluation_count = fe
             for k, v in saved.items():
                 setattr(self, k, v)
-            self._graphs[key] = (g, per_step)
+            self._graphs[key] = (_lib.GraphHolder(g), per_step)
         return self._graphs[key]
 
     def _chunkable(self) -> bool:
@@ -317,6 +338,7 @@ class ODE(ABC):
         downbeats = sorted(set([0] + list(downbeats) + [T]))[1:]
         target = downbeats.pop(0) if exact else None
         chunked = target is not None and not log_every_step and self._chunkable()
+        self.progress_bar_action(action="setup", T=T)
         self.snapshot()
         t0 = time.time()
         while self.t < T:
@@ -324,6 +346,7 @@ class ODE(ABC):
                 self._run_steps(target, self._chunk_length(target))
             else:
                 self.take_step(target_time=target)
+            self.progress_bar_action(action="update")
             if self.t == target or self.t >= T or log_every_step:
                 ts = time.time()
                 self.snapshot()
@@ -332,6 +355,7 @@ class ODE(ABC):
                     target = downbeats.pop(0)
         _lib.torch().cuda.synchronize()
         self.timer["TOTAL"] += time.time() - t0
+        self.progress_bar_action(action="cleanup")
         if save:
             self.write_snapshots(overwrite)
 

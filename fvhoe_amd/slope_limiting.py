@@ -28,6 +28,28 @@ def MUSCL_interpolations(u, axis: int, limiter: Optional[str] = "minmod"):
     return _lib.back(left, was_np), _lib.back(right, was_np)
 
 
+def _limit(limiter: int, du_left, du_right):
+    t = _lib.torch()
+    a, was_np = _lib.as_device(du_left)
+    b, _ = _lib.as_device(du_right)
+    if tuple(a.shape) != tuple(b.shape):
+        a, b = (x.contiguous() for x in t.broadcast_tensors(a, b))
+    out = _lib.empty(a.shape)
+    _lib.call("fvhoe_slope_limit", limiter, _lib.ptr(a), _lib.ptr(b), _lib.ptr(out), out.numel(), _lib.stream())
+    return _lib.back(out, was_np)
+
+
+def minmod(du_left, du_right):
+    """clip(dR / dL, 0, 1) dL, with dL moved away from 0 by 1e-16 (fvhoe/slope_limiting.py:56-70)."""
+    return _limit(_lib.LIMITER["minmod"], du_left, du_right)
+
+
+def moncen(du_left, du_right):
+    """sign(dC) min(2|dL|, 2|dR|, |dC|) where dL dR >= 0, else 0; dC = (dL + dR) / 2
+    (fvhoe/slope_limiting.py:73-84)."""
+    return _limit(_lib.LIMITER["moncen"], du_left, du_right)
+
+
 def pad_spec(PAD_bounds) -> Tuple[int, float, float]:
     """The PAD loop overwrites its indicator per key, so only the last key counts
     (fvhoe/slope_limiting.py:230-237)."""
@@ -157,6 +179,26 @@ def _convex_2d(t2):
     _shift_max(ox, th[2:-1, 2:-2], 1.0, 0, th[1:-2, 2:-2])
     _shift_max(oy, th[2:-2, 2:-1], 1.0, 0, th[2:-2, 1:-2])
     return ox, oy
+
+
+def convex_1d_broadcast_to_troubled_interfaces(trouble):
+    """Face weights theta in {0, 0.25, 0.75, 1} of the convex 1-D broadcast (fvhoe/slope_limiting.py:387-407):
+    trouble of shape (nx + 4,) (two ghost cells per side) -> theta of shape (nx + 1,)."""
+    x, was_np = _lib.as_device(trouble)
+    if x.dim() != 1 or x.shape[0] < 4:
+        raise ValueError(f"expected trouble of shape (nx + 4,), got {tuple(x.shape)}")
+    out = _convex_1d(x.reshape(-1, 1, 1).contiguous()).reshape(-1)
+    return _lib.back(out, was_np)
+
+
+def convex_2d_broadcast_to_troubled_interfaces(trouble):
+    """Face weights of the convex 2-D broadcast (fvhoe/slope_limiting.py:410-442): trouble of shape
+    (nx + 4, ny + 4) -> (theta_x (nx + 1, ny), theta_y (nx, ny + 1))."""
+    x, was_np = _lib.as_device(trouble)
+    if x.dim() != 2 or min(x.shape) < 4:
+        raise ValueError(f"expected trouble of shape (nx + 4, ny + 4), got {tuple(x.shape)}")
+    ox, oy = _convex_2d(x.reshape(x.shape[0], x.shape[1], 1).contiguous())
+    return _lib.back(ox[..., 0], was_np), _lib.back(oy[..., 0], was_np)
 
 
 def broadcast_to_troubled_interfaces(trouble, dims: str = None, convex: bool = False, periodic_x: bool = False,

@@ -30,12 +30,51 @@ from fvhoe_amd.fv import (conservative_interpolation, fv_uniform_meshgen, interp
                           interpolate_fv_averages, sweep_chain)
 from fvhoe_amd.fv import transverse_reconstruction
 from fvhoe_amd.hydro import HydroState, compute_primitives, hydro_dt_device
-from fvhoe_amd.initial_conditions import (InitialCondition, device_mesh, fv_average, primitives_to_conservatives,
-                                          uniform_mesh)
+from fvhoe_amd.initial_conditions import (InitialCondition, Square, device_mesh, fv_average,
+                                          primitives_to_conservatives, uniform_mesh)
 from fvhoe_amd.ode import ODE
 from fvhoe_amd.slope_limiting import (MUSCL_interpolations, broadcast_to_troubled_interfaces, detect_troubled_cells,
                                       nad_mask, pad_spec)
 from fvhoe_amd.stencils import get_stencil_size
+
+
+MAG_SLOTS = 4  # stages whose violation magnitudes the fused stage keeps in slots (RK4's four)
+
+
+def _jsonable(v) -> bool:
+    if v is None or isinstance(v, (bool, int, float, str)):
+        return not isinstance(v, float) or np.isfinite(v)
+    if isinstance(v, (tuple, list)):
+        return all(_jsonable(x) for x in v)
+    if isinstance(v, dict):
+        return all(isinstance(k, str) and _jsonable(x) for k, x in v.items())
+    return False
+
+
+def timing_report(timer: dict) -> str:
+    """Category / seconds table of the run's timer (the layout of fvhoe/timer.py:78-110)."""
+    cats = list(timer)
+    times = [f"{float(timer[c]):.2f}" for c in cats]
+    wc = max([len(c) for c in cats] or [len("Category")])
+    wt = max([len(t) for t in times] + [len("Time (s)")])
+    out = f"{'Category':<{wc}} {'Time (s)':<{wt}}\n" + "-" * (wc + wt + 1) + "\n"
+    return out + "".join(f"{c:<{wc}} {t:>{wt}}\n" for c, t in zip(cats, times))
+
+
+def commit_details() -> dict:
+    """The package's git commit (hash, author, date, branch), or the error git reported."""
+    import subprocess
+
+    repo = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    try:
+        r = subprocess.run(["git", "-C", repo, "log", "-1", "--pretty=format:%H|%an|%ai|%D"],
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, check=True)
+    except (OSError, subprocess.CalledProcessError) as e:
+        return {"error": f"An error occurred: {getattr(e, 'stderr', str(e)) or e}".strip()}
+    parts = r.stdout.strip().split("|")
+    parts += [""] * (4 - len(parts))
+    branch = parts[3].split(",")[0].strip().split()[-1] if parts[3].strip() else None
+    return {"commit_hash": parts[0], "author_name": parts[1], "commit_date": parts[2], "branch_name": branch}
 
 
 class _TroubleLog(dict):
@@ -53,13 +92,15 @@ class _TroubleLog(dict):
         if key == "mean trouble" and s._count_pending:
             _lib.call("fvhoe_log_mean", _lib.ptr(out), _lib.ptr(s._trouble_count), out.numel(), 1.0, None,
                       _lib.stream())
+        elif key in ("mean NAD mag", "mean PAD mag") and s._mag is not None:
+            s._mag_materialize()
         return out
 
 
 class EulerSolver(ODE):
     def __init__(
         self,
-        w0: InitialCondition = None,
+        w0: InitialCondition = Square(),
         w0_passives: Dict[str, Callable] = None,
         nx: int = 1, ny: int = 1, nz: int = 1,
         px: int = 0, py: int = 0, pz: int = 0,
@@ -99,7 +140,7 @@ class EulerSolver(ODE):
         ic_device: bool = False,
     ):
         if w0 is None:
-            raise ValueError("w0 (an InitialCondition) is required")
+            w0 = Square()
         if riemann_solver not in _lib.RIEMANN:
             raise TypeError(f"Invalid Riemann solver {riemann_solver}")
         if slope_limiter not in _lib.LIMITER:
@@ -138,6 +179,9 @@ class EulerSolver(ODE):
         # as every class of fvhoe_amd.initial_conditions is); the mesh holds the same doubles either way
         Xi, Yi, Zi = device_mesh(self.n, x, y, z) if ic_device else (self.X, self.Y, self.Z)
         # self.u0: pointwise conservative IC f(x, y, z) where the IC defines one (fvhoe/solver.py:278-299)
+        self.conservative_ic, self.fv_ic = conservative_ic, fv_ic
+        # self.w0: the pointwise primitive IC where the IC gives primitives (fvhoe/solver.py:281-299)
+        self.w0 = ic if not conservative_ic else NotImplemented
         if conservative_ic and fv_ic:
             self.u0 = NotImplemented
             u0 = ic(Xi, Yi, Zi)
@@ -147,7 +191,7 @@ class EulerSolver(ODE):
             point = ic if conservative_ic else (lambda x, y, z: primitives_to_conservatives(ic(x, y, z), gamma))
             self.u0 = point
             u0 = fv_average(point, Xi, Yi, Zi, self.h, self.p)
-        super().__init__(u0, progress_bar=progress_bar)
+        super().__init__(u0, progress_bar=progress_bar, cupy=cupy)
         self.f_evaluation_count = 0
 
         # boundaries (fvhoe/solver.py:314-362): slab kinds read their ghost values at the centres of a
@@ -178,7 +222,7 @@ class EulerSolver(ODE):
         self.log = _TroubleLog(self, {k: _lib.zeros((nx_, ny_, nz_)) for k in ("mean trouble", "mean NAD mag",
                                                                                "mean PAD mag")})
         self._zero_like_log = _lib.zeros((nx_, ny_, nz_))
-        self._log_written = {}  # log arrays that have received magnitudes (the fused stage logs flags only)
+        self._log_written = {}  # log arrays that have received magnitudes from the composed stage
         self._trouble_count = _lib.zeros((nx_, ny_, nz_), dtype=_lib.torch().int32)  # flags counted since a reset
         self._count_pending = False
         self._one = _lib.zeros((1,))
@@ -197,6 +241,19 @@ class EulerSolver(ODE):
                                       "without a-posteriori limiting or slab boundaries, on the composed stage")
         self._ws = None
         self._trouble = _lib.empty(self.n, dtype=_lib.torch().int8) if a_posteriori_slope_limiting else None
+        # violation magnitudes of the fused stage: the check appends (cell, NAD magnitude, PAD magnitude) of
+        # its flagged cells to one slot per stage; reading the log turns the slots of the current
+        # accumulation into the reference's dense running sums (or means, after a reset)
+        self._mag = None
+        self._mag_used, self._mag_closed, self._mag_base_dense, self._mag_acc = 0, False, False, None
+        if self.fused and a_posteriori_slope_limiting:
+            torch_, nc = _lib.torch(), int(np.prod(self.n))
+            self._mag = dict(count=_lib.zeros((MAG_SLOTS,), dtype=torch_.int32),
+                             idx=_lib.empty((MAG_SLOTS, nc), dtype=torch_.int32), nad=_lib.empty((MAG_SLOTS, nc)),
+                             pad=_lib.empty((MAG_SLOTS, nc)), div=_lib.zeros((1,)))
+            c = self._cfg
+            c.mag_count, c.mag_idx = self._mag["count"].data_ptr(), self._mag["idx"].data_ptr()
+            c.mag_nad, c.mag_pad = self._mag["nad"].data_ptr(), self._mag["pad"].data_ptr()
         # learned troubled-cell predictor (fvhoe_amd.predictor): pre-flagged cells get MUSCL fluxes before
         # the candidate is formed; the plain check still judges every cell
         self.predictor = predictor
@@ -253,6 +310,8 @@ class EulerSolver(ODE):
         _lib.call("fvhoe_stage_ghost_depth", _lib.ctypes.byref(c), _lib.ctypes.byref(depth))
         gw = (depth.value if self.xdim else 0, depth.value if self.ydim else 0, 0)
         slabs = self.bc.device_slabs(gw, 5, self.t)
+        self._slab_refs = slabs  # the kernels (and captured graphs) read these tensors
+        self._slab_version = self.bc.version
         for k in range(6):
             c.slab[k] = slabs[k].data_ptr() if slabs[k] is not None else None
         c.slab_g = depth.value
@@ -267,10 +326,52 @@ class EulerSolver(ODE):
         if self.fused and self.bc.time_dependent() and self._slab_t != self.t:
             self._set_stage_slabs(self._cfg)
 
+    def _fused_stage_prologue(self) -> None:
+        """Host bookkeeping before a fused stage: boundary values changed by bc.reset_value are fetched
+        again (captured steps read the old slabs, so they are dropped), and the stage's magnitude slot is
+        chosen."""
+        if self.bc.has_slabs() and getattr(self, "_slab_version", self.bc.version) != self.bc.version:
+            self._set_stage_slabs(self._cfg)
+            self._graphs = {}
+        if self._mag is None:
+            return
+        if self._mag_closed:  # the first stage after a reset starts a new sum (the reference assigns)
+            self._mag_used, self._mag_closed, self._mag_base_dense = 0, False, False
+        if self._mag_used == MAG_SLOTS:  # a long run of stages without a reset: fold the slots first
+            self._mag_fold()
+        self._cfg.mag_slot = self._mag_used
+        self._mag_used += 1
+
+    def _mag_call(self, nad_out, pad_out, divisor):
+        m = self._mag
+        base = self._mag_acc if self._mag_base_dense else (None, None)
+        nc = m["idx"].shape[1]
+        _lib.call("fvhoe_log_magnitudes", _lib.ptr(nad_out), _lib.ptr(pad_out), _lib.ptr(base[0]), _lib.ptr(base[1]),
+                  nc, _lib.ptr(m["count"]), _lib.ptr(m["idx"]), _lib.ptr(m["nad"]), _lib.ptr(m["pad"]),
+                  self._mag_used, nc, _lib.ptr(divisor), _lib.stream())
+
+    def _mag_fold(self) -> None:
+        """Partial sums of the filled slots into the dense accumulator; the slots are free again."""
+        if self._mag_acc is None:
+            self._mag_acc = (_lib.empty(self.n), _lib.empty(self.n))
+        self._mag_call(self._mag_acc[0], self._mag_acc[1], None)
+        self._mag_base_dense, self._mag_used = True, 0
+
+    def _mag_materialize(self) -> None:
+        """The log's "mean NAD mag" / "mean PAD mag" from the slots: the running sums since the last
+        reset, or their means when a reset closed them (the divisor is the device scalar the reset set)."""
+        if self._mag_used == 0 and not self._mag_base_dense and not self._mag_closed:
+            return  # no stage since construction: the log holds its initial zeros
+        lg = self.log
+        self._mag_call(dict.__getitem__(lg, "mean NAD mag"), dict.__getitem__(lg, "mean PAD mag"),
+                       self._mag["div"] if self._mag_closed else None)
+
     # ------------------------------------------------------------------- stage
     def stage(self, u, k_out, dt_out) -> None:
         """One RHS evaluation into k_out, this stage's dt into dt_out[0] (device)."""
         self.f_evaluation_count += 1
+        if self.fused:
+            self._fused_stage_prologue()
         if self.predictor is not None:
             _lib.call("fvhoe_stage_predicted", _lib.ctypes.byref(self._cfg), _lib.ptr(u), _lib.ptr(k_out),
                       _lib.ptr(dt_out), _lib.ptr(self._trouble), self._workspace(), None,
@@ -329,6 +430,7 @@ class EulerSolver(ODE):
                 raise NotImplementedError("the composed stage takes a host target time")
             return self._composed_stage_rk(u, k_out, dt_out, kind, out, u0, k0, k1, k2, dt_step, set_dt_step,
                                            t_dev, target)
+        self._fused_stage_prologue()
         rk = _lib.FvhoeRkFuse()
         rk.kind = kind
         rk.out, rk.u0 = out.data_ptr(), u0.data_ptr()
@@ -404,7 +506,7 @@ class EulerSolver(ODE):
         w = compute_primitives(self.hydro_state, centres, self.gamma)
         return interpolate_fv_averages(w, p) if fv_average else w
 
-    def hydrofluxes(self, u, p, slope_limiter: str = None, views: bool = False):
+    def hydrofluxes(self, u, p, slope_limiter: str = None, timer_prefix: str = "", views: bool = False):
         """dt (device), (F, G, H) (fvhoe/solver.py:456-651).  views=True returns the trimmed fluxes as views
         into the untrimmed arrays (for the strided divergence), else contiguous copies."""
         if slope_limiter is None:
@@ -577,6 +679,10 @@ class EulerSolver(ODE):
                     if k == "mean trouble" or not self._log_written.get(k, False):
                         continue  # never written since construction: dividing zeros changes nothing
                     log(lg[k], None, 2, float(self.trouble_counter))
+                if self._mag is not None:  # the fused stage's magnitudes: the mean is formed when read
+                    _lib.call("fvhoe_set_scalar_gated", _lib.ptr(self._mag["div"]), float(self.trouble_counter), gp,
+                              _lib.stream())
+                    self._mag_closed = True
                 self.trouble_counter = 0
             return
         t8 = trouble if trouble.dtype == _lib.torch().int8 else trouble.to(_lib.torch().int8).contiguous()
@@ -630,14 +736,35 @@ class EulerSolver(ODE):
         os.makedirs(d)
         with open(os.path.join(d, "arrs.pkl"), "wb") as f:
             pickle.dump(self.snapshots, f)
-        attrs = {k: v for k, v in self.__dict__.items()
-                 if isinstance(v, (int, float, str, bool, tuple, type(None))) and not k.startswith("_")}
-        attrs["PAD"] = {k: [{np.inf: "inf", -np.inf: "-inf"}.get(x, x) for x in v] for k, v in self.PAD.items()}
-        attrs["bc"] = self.bc.to_dict()
         with open(os.path.join(d, "attrs.json"), "w") as f:
-            json.dump(dict(sorted(attrs.items(), key=lambda kv: (kv[0].lower(), kv[0]))), f, indent=4, default=str)
+            json.dump(self.snapshot_attrs(), f, indent=4)
         with open(os.path.join(d, "timing.txt"), "w") as f:
-            f.write("".join(f"{k}: {v:.6f} s\n" for k, v in self.timer.items()))
+            f.write(timing_report(self.timer))
+
+    def snapshot_attrs(self) -> dict:
+        """The run's settings as written to attrs.json (fvhoe/solver.py:909-943): every attribute that JSON
+        can hold (arrays, tensors, functions and the snapshots themselves left out), the initial condition
+        by its function name, the boundary conditions, timer and device arrays as dictionaries, infinite PAD
+        bounds as strings, keys sorted case-insensitively."""
+        skip = ("progress_bar", "timestamps", "snapshots", "w0_cell_centers_cache")
+        attrs = {}
+        for k, v in self.__dict__.items():
+            if k.startswith("_") or k in skip:
+                continue
+            if k == "w0":
+                attrs[k] = getattr(v, "__name__", "__name__ not found")
+            elif k == "bc":
+                attrs[k] = v.to_dict()
+            elif k == "timer":
+                attrs[k] = {c: round(float(x), 2) for c, x in v.items()}
+            elif k == "PAD":
+                attrs[k] = {q: [{np.inf: "inf", -np.inf: "-inf"}.get(x, x) for x in b] for q, b in v.items()}
+            elif _jsonable(v):
+                attrs[k] = v
+        # the device arrays the run holds (the reference's array manager lists its arrays by name)
+        attrs["am"] = {"names": ["u", "mean trouble", "mean NAD mag", "mean PAD mag"]}
+        attrs["commit_details"] = commit_details()
+        return {key: attrs[key] for key in sorted(attrs, key=lambda x: (x.lower(), x))}
 
     def run(self, *args, **kwargs):
         """RK order matched to max(p), capped at RK4 (fvhoe/solver.py:955-974)."""

@@ -117,6 +117,23 @@ int fvhoe_stencil_sweep_xypair_f32(const float* u, float* out, int nv, int n0, i
 int fvhoe_face_fluxes(const double* w, double* out, int n0, int n1, int n2, int axis, int p, int solver, double gamma,
                       double csq_floor, void* stream);
 
+/* minmod (limiter FVHOE_LIM_MINMOD) and moncen (FVHOE_LIM_MONCEN) of two
+ * difference arrays of n values (fvhoe/slope_limiting.py:56-84). */
+int fvhoe_slope_limit(int limiter, const double* du_left, const double* du_right, double* out, long long n,
+                      void* stream);
+
+/* second_order_central_difference (fvhoe/fv.py:280-296) of u viewed as
+ * (outer, len, inner) along its middle axis: out (outer, len - 2, inner) =
+ * 0.5 * (u[l + 2] - u[l]), then / h when has_h. */
+int fvhoe_central_difference(const double* u, double* out, long long outer, long long len, long long inner,
+                             double h, int has_h, void* stream);
+
+/* compute_fluxes (fvhoe/hydro.py:213-246): the physical flux along dim (0/1/2
+ * = x/y/z) of (nv, ncell) primitives w and conservatives u, with or without
+ * the pressure terms; passives (nv > 5) are advected. */
+int fvhoe_compute_fluxes(const double* w, const double* u, double* out, int nv, long long ncell, int dim,
+                         int include_pressure, void* stream);
+
 /* MUSCL_interpolations with minmod / moncen / no limiter (fvhoe/slope_limiting.py:24-84) */
 int fvhoe_muscl(const double* u, double* left, double* right, int nv, int n0, int n1, int n2, int axis, int limiter,
                 void* stream);
@@ -240,6 +257,19 @@ int fvhoe_log_troubles_gated(double* mean, const signed char* trouble, long long
  * of 0/1 values are exact).  gate (device, may be NULL) as above. */
 int fvhoe_log_count(int* count, const signed char* trouble, long long n, int mode, const double* gate, void* stream);
 int fvhoe_log_mean(double* mean, const int* count, long long n, double divisor, const double* gate, void* stream);
+/* The "mean NAD mag" / "mean PAD mag" arrays of the trouble log
+ * (fvhoe/solver.py:794-833) from the fused stage's magnitude slots
+ * (FvhoeStageConfig.mag_*; cap entries per slot): nad_out / pad_out (n
+ * cells) = base + slot 0 + slot 1 + ... + slot nslots-1, added in slot
+ * order cell by cell, then divided by *divisor when divisor is not NULL.
+ * base: nad_base / pad_base, or zeros when they are NULL.  A cell missing
+ * from a slot adds 0, so the sums and the quotient are the reference's
+ * running double sums bit for bit (magnitudes are >= 0). */
+int fvhoe_log_magnitudes(double* nad_out, double* pad_out, const double* nad_base, const double* pad_base,
+                         long long n, const int* count, const int* idx, const double* nad, const double* pad,
+                         int nslots, long long cap, const double* divisor, void* stream);
+/* *p = v unless *gate (device, may be NULL) is 0. */
+int fvhoe_set_scalar_gated(double* p, double v, const double* gate, void* stream);
 
 /* device scalars: *p = v; *dt_step = _hit_target(t, dt_stage, target) (fvhoe/ode.py:11-22); *t += *dt */
 int fvhoe_set_scalar(double* p, double v, void* stream);
@@ -286,6 +316,20 @@ typedef struct FvhoeStageConfig {
   const double* slab[6];
   int slab_g;
   int slab_pad[3];
+  /* Violation magnitudes for the trouble log (fvhoe/slope_limiting.py:225-239,
+   * fvhoe/solver.py:794-833).  With mag_idx set, an a-posteriori stage
+   * appends (cell, NAD magnitude, PAD magnitude) for every cell whose NAD
+   * or PAD test flags to slot mag_slot: mag_count[mag_slot] entries at
+   * mag_idx / mag_nad / mag_pad + mag_slot * n0 * n1 (each slot holds
+   * n0 * n1 entries, every cell at most once).  The stage clears its slot
+   * first.  Under a FvhoeRkFuse a step whose *dt_step is 0 (a step replayed
+   * at its target time) leaves the slot as it was.  fvhoe_log_magnitudes
+   * turns slots into the log's dense arrays.  mag_idx NULL: nothing logged. */
+  int* mag_count;
+  int* mag_idx;
+  double* mag_nad;
+  double* mag_pad;
+  int mag_slot;
 } FvhoeStageConfig;
 
 /* 1 if fvhoe_stage handles cfg (1-D along x, or 2-D in x-y, nvars = 5,

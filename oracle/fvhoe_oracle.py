@@ -194,9 +194,12 @@ def interpolate_fv_averages(u: np.ndarray, p=(0, 0, 0)) -> np.ndarray:
     return transverse_reconstruction(out, p[2], 3)
 
 
-def second_order_central_difference(u: np.ndarray, axis: int) -> np.ndarray:
-    """0.5 * (u[2:] - u[:-2]) along axis (fvhoe/fv.py:280-296, h=None)."""
-    return 0.5 * (_shift(u, axis, 2, 0) - _shift(u, axis, 0, 2))
+def second_order_central_difference(u: np.ndarray, axis: int, h: Optional[float] = None) -> np.ndarray:
+    """0.5 * (u[2:] - u[:-2]) along axis, then / h when h is given (fvhoe/fv.py:280-296)."""
+    out = 0.5 * (_shift(u, axis, 2, 0) - _shift(u, axis, 0, 2))
+    if h is not None:
+        out = out / h
+    return out
 
 
 # ---------------------------------------------------------------------------

@@ -16,6 +16,7 @@ from __future__ import annotations
 
 import hashlib
 import os
+import re
 import sys
 
 import numpy as np
@@ -261,6 +262,144 @@ def run_detection():
     print(f"detection: {i} combos", flush=True)
 
 
+# ------------------------------------------------------------------ snapshots
+
+
+def _log_slices(snap):
+    """The three trouble-log arrays of a snapshot: (5, n...) broadcasts of one (n...) array."""
+    return {"trouble": np.asarray(snap["trouble"])[0], "nad_mag": np.asarray(snap["NAD violation magnitude"])[0],
+            "pad_mag": np.asarray(snap["PAD violation magnitude"])[0]}
+
+
+def run_snapshots():
+    """What EulerSolver.snapshot records (fvhoe/solver.py:840-874), trouble-log means included, over whole
+    integrate() calls, and the file layout of write_snapshots (fvhoe/solver.py:889-953)."""
+    import json
+    import tempfile
+
+    P = ("periodic", "periodic")
+    O = ("outflow", "outflow")
+    rec = {}
+
+    def keep(name, s):
+        rec[f"{name}/t"] = np.array([sn["t"] for sn in s.snapshots])
+        rec[f"{name}/timestamps"] = np.array(s.timestamps)
+        for i, sn in enumerate(s.snapshots):
+            rec[f"{name}/w{i}"] = np.asarray(sn["w"])
+            for k, v in _log_slices(sn).items():
+                rec[f"{name}/{k}{i}"] = v
+        print(f"snapshots {name}: {len(s.snapshots)} at t={[round(float(sn['t']), 6) for sn in s.snapshots]} "
+              f"max NAD {float(rec[f'{name}/nad_mag{len(s.snapshots) - 1}'].max()):.3g} "
+              f"max PAD {float(rec[f'{name}/pad_mag{len(s.snapshots) - 1}'].max()):.3g}", flush=True)
+
+    # Sod N=256, p=3, SSP-RK3 to t=0.245 with two downbeats
+    u0 = ics.sod_shock_tube(256, 3)
+    s = make_solver(u0, (256, 1, 1), (3, 0, 0), (O, P, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    s.ssprk3(0.245, downbeats=[0.1, 0.2])
+    rec["sod/u0"] = u0
+    keep("sod", s)
+
+    # a double rarefaction at p=7 whose candidate pressure goes negative (PAD magnitudes), 6 steps
+    n = 64
+    x = (np.arange(n) + 0.5) / n
+    w = np.zeros((5, n, 1, 1))
+    w[0] = 1.0
+    w[1] = np.where(x < 0.5, -5.0, 5.0)[:, None, None]
+    w[4] = 0.05
+    u0 = ics.primitives_to_conservatives(w, 1.4)
+    s = make_solver(u0, (n, 1, 1), (7, 0, 0), (O, P, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    s.ssprk3(n=6)
+    rec["rarefaction/u0"] = u0
+    keep("rarefaction", s)
+
+    # four quadrants 64^2, 3 SSP-RK3 steps
+    u0 = ics.four_quadrants(64, seed=200)
+    s = make_solver(u0, (64, 64, 1), (3, 3, 0), (O, O, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True)
+    s.ssprk3(n=3)
+    rec["quad64/u0"] = u0
+    keep("quad64", s)
+
+    # Kelvin-Helmholtz 64^2, 2 RK4 steps (four stages per step), NAD local and absolute
+    u0 = ics.kelvin_helmholtz(64, seed=3)
+    s = make_solver(u0, (64, 64, 1), (3, 3, 0), (P, P, P), CFL=0.8, riemann_solver="hllc",
+                    a_posteriori_slope_limiting=True, NAD=1e-3, NAD_mode="local", NAD_range="absolute")
+    s.rk4(n=2)
+    rec["kh64_rk4/u0"] = u0
+    keep("kh64_rk4", s)
+
+    # write_snapshots: the attrs.json keys, the snapshot dictionary keys and the timing.txt categories.  A
+    # pointwise IC: with conservative_ic and fv_ic the reference's attrs.json step raises TypeError
+    # (its u0 attribute is NotImplemented, which json cannot write)
+    from fvhoe.initial_conditions import Shock1D
+
+    s = EulerSolver(w0=Shock1D(), nx=32, px=3, gamma=1.4, bc=BoundaryCondition(x="outflow"), CFL=0.8,
+                    riemann_solver="hllc", a_posteriori_slope_limiting=True, progress_bar=False)
+    with tempfile.TemporaryDirectory() as d:
+        sd = os.path.join(d, "run")
+        s.ssprk3(0.05, snapshot_dir=sd)
+        with open(os.path.join(sd, "attrs.json")) as f:
+            attrs = json.load(f)
+        with open(os.path.join(sd, "timing.txt")) as f:
+            timing = f.read().splitlines()
+        files = sorted(os.listdir(sd))
+    rec["io/attrs_keys"] = np.array(list(attrs.keys()))
+    rec["io/files"] = np.array(files)
+    rec["io/snapshot_keys"] = np.array(list(s.snapshots[-1].keys()))
+    rec["io/timing_lines"] = np.array(timing)
+    rec["io/n_snapshots"] = np.int64(len(s.snapshots))
+    np.savez_compressed(os.path.join(OUT, "snapshots.npz"), **rec)
+
+
+# ----------------------------------------------------------------- signatures
+
+
+SIGNATURE_MODULES = ("fv", "stencils", "hydro", "riemann_solvers", "solver", "boundary_conditions", "slope_limiting",
+                     "smooth_extrema_detection", "ode")
+
+
+def _sig(obj):
+    import inspect
+
+    try:
+        sig = inspect.signature(obj)
+    except (TypeError, ValueError):
+        return None
+    out = []
+    for p in sig.parameters.values():
+        d = None if p.default is inspect.Parameter.empty else re.sub(r" at 0x[0-9a-fA-F]+", "", repr(p.default))
+        out.append([p.name, str(p.kind), d])
+    return out
+
+
+def run_signatures():
+    """Public functions and classes (with their methods) of the reference modules on the per-stage path:
+    names, parameter names and kinds, default values by repr, written to signatures.json."""
+    import importlib
+    import inspect
+    import json
+
+    rec = {}
+    for m in SIGNATURE_MODULES:
+        mod = importlib.import_module(f"fvhoe.{m}")
+        entries = {}
+        for name, obj in vars(mod).items():
+            if name.startswith("_") or getattr(obj, "__module__", None) != mod.__name__:
+                continue
+            if inspect.isfunction(obj) or (callable(obj) and inspect.isfunction(getattr(obj, "__wrapped__", None))):
+                entries[name] = {"kind": "function", "params": _sig(obj)}
+            elif inspect.isclass(obj):
+                methods = {k: _sig(v) for k, v in vars(obj).items()
+                           if inspect.isfunction(v) and (not k.startswith("_") or k in ("__init__", "__call__"))}
+                entries[name] = {"kind": "class", "methods": methods}
+        rec[m] = entries
+    with open(os.path.join(OUT, "signatures.json"), "w") as f:
+        json.dump(rec, f, indent=1, sort_keys=True)
+    print(f"signatures: {sum(len(v) for v in rec.values())} names", flush=True)
+
+
 def run_initial_conditions():
     """The reference solver's initial conservative averages (fvhoe/solver.py:255-306) and its
     pointwise primitive IC at the cell centres, per IC class."""
@@ -284,8 +423,16 @@ if __name__ == "__main__":
     if sys.argv[1:] == ["ics"]:
         run_initial_conditions()
         sys.exit(0)
+    if sys.argv[1:] == ["snapshots"]:
+        run_snapshots()
+        sys.exit(0)
+    if sys.argv[1:] == ["signatures"]:
+        run_signatures()
+        sys.exit(0)
     build_stage_cases()
     run_stage_cases()
     run_detection()
     run_trajectories()
+    run_snapshots()
+    run_signatures()
     run_initial_conditions()

@@ -47,6 +47,8 @@ def _compare(a, b, tag):
     for k, (sa, sb) in enumerate(zip(a.snapshots, b.snapshots)):
         assert_bits_equal(sb["w"], sa["w"], f"{tag} snapshot {k}")
         assert_bits_equal(sb["trouble"], sa["trouble"], f"{tag} snapshot {k} trouble")
+        for key in ("NAD violation magnitude", "PAD violation magnitude"):
+            assert_bits_equal(sb[key], sa[key], f"{tag} snapshot {k} {key}")
     assert_bits_equal(b.u.cpu().numpy(), a.u.cpu().numpy(), f"{tag} u")
     if a.predictor_stats is not None:
         assert np.array_equal(a.predictor_stats.cpu().numpy(), b.predictor_stats.cpu().numpy()), tag

@@ -222,3 +222,49 @@ def test_ghost_fill_index_maps(kind):
         if flip:
             want[1] = -want[1]
         assert np.array_equal(out[:, j, 0, 0], want), (kind, j)
+
+
+# ------------------------------------------------- central difference, fluxes
+@pytest.mark.parametrize("axis", [0, 1, 2, 3])
+def test_central_difference_of_a_linear_field_is_its_slope(axis):
+    """0.5 (u[i+1] - u[i-1]) / h of u = a + b x_i (x_i = i h, all values dyadic) is b exactly; without h it
+    is b h.  A wrong shift, sign or scale fails."""
+    h, a, b = 0.125, 3.0, -1.5
+    shape = [4, 5, 6, 7]
+    x = np.arange(shape[axis]) * h
+    u = np.broadcast_to((a + b * x).reshape([-1 if k == axis else 1 for k in range(4)]), shape).copy()
+    d = O.second_order_central_difference(u, axis, h)
+    want = list(shape)
+    want[axis] -= 2
+    assert d.shape == tuple(want)
+    assert np.all(d == b)
+    assert np.all(O.second_order_central_difference(u, axis) == b * h)
+
+
+@pytest.mark.parametrize("dim", ["x", "y", "z"])
+def test_physical_flux_closed_forms(dim):
+    """Euler fluxes (Toro 2009, eq. 3.3): at rest only the pressure pushes the normal momentum; in motion
+    F = (rho v, rho v vx + P dx, rho v vy + P dy, rho v vz + P dz, v (E + P)) up to rounding; without
+    pressure every variable is advected."""
+    d = O.AXES[dim]
+    rng = np.random.default_rng(31)
+    w = np.empty((6, 40))
+    w[0] = 0.5 + rng.random(40)
+    w[1:4] = 0.0
+    w[4] = 0.2 + rng.random(40)
+    w[5] = rng.random(40)
+    u = O.compute_conservatives(w, GAMMA)
+    u[5] = w[0] * w[5]
+    f = O.compute_fluxes(w, u, dim)
+    for q in range(6):
+        assert np.all(f[q] == (w[4] if q == d else 0.0)), q
+    w[1:4] = rng.standard_normal((3, 40))
+    u = O.compute_conservatives(w, GAMMA)
+    u[5] = w[0] * w[5]
+    f = O.compute_fluxes(w, u, dim)
+    v, rho, P, E = w[d], w[0], w[4], u[4]
+    want = [rho * v] + [rho * v * w[k] + (P if k == d else 0.0) for k in (1, 2, 3)] + [v * (E + P), v * u[5]]
+    for q in range(6):
+        assert np.allclose(f[q], want[q], rtol=1e-14, atol=1e-14), q
+    g = O.compute_fluxes(w, u, dim, include_pressure=False)
+    assert np.array_equal(g[0], v * rho) and all(np.array_equal(g[q], v * u[q]) for q in range(1, 6))
