@@ -16,7 +16,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfvhoe_hip.so")
 
-OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH = 0, 1, 2, 3
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_LAUNCH, ERR_TOO_LARGE = 0, 1, 2, 3, 4
+STATIC_SCRATCH_BYTES = 64 << 20  # FVHOE_STATIC_SCRATCH_BYTES: the fixed device scratch of fvhoe_stage_rk_static
 STENCIL = {"l": 0, "c": 1, "r": 2, "t": 3}
 RIEMANN = {"advection_upwind": 0, "llf": 1, "hllc": 2}
 LIMITER = {None: 0, "minmod": 1, "moncen": 2}
@@ -97,6 +98,7 @@ _SIGNATURES = {
     "fvhoe_log_magnitudes": (c_int, [_P, _P, _P, _P, c_longlong, _P, _P, _P, _P, c_int, c_longlong, _P, _P]),
     "fvhoe_set_scalar_gated": (c_int, [_P, c_double, _P, _P]),
     "fvhoe_slope_limit": (c_int, [c_int, _P, _P, _P, c_longlong, _P]),
+    "fvhoe_stage_rk_static": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P, _P, POINTER(FvhoeRkFuse), _P]),
     "fvhoe_central_difference": (c_int, [_P, _P, c_longlong, c_longlong, c_longlong, c_double, c_int, _P]),
     "fvhoe_compute_fluxes": (c_int, [_P, _P, _P, c_int, c_longlong, c_int, c_int, _P]),
     "fvhoe_stage_supported": (c_int, [POINTER(FvhoeStageConfig)]),
@@ -174,6 +176,9 @@ def check(status: int, what: str):
         raise ValueError(f"{what}: invalid argument")
     if status == ERR_UNSUPPORTED:
         raise NotImplementedError(f"{what}: unsupported configuration")
+    if status == ERR_TOO_LARGE:
+        raise ValueError(f"{what}: the problem needs more than the library's fixed device scratch of "
+                         f"{STATIC_SCRATCH_BYTES} bytes")
     raise RuntimeError(f"{what}: kernel launch failed")
 
 

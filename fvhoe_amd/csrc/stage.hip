@@ -22,6 +22,7 @@
 // In the FP32 form the sweeps, Riemann solves and W/F/G/WS storage are float;
 // the conservative-to-primitive pressure recovery, the NAD/PAD/SED tests, the
 // divergence, the candidate and the RK accumulation stay FP64.
+#include <mutex>
 #include <type_traits>
 
 #include "fvhoe_common.h"
@@ -1784,6 +1785,57 @@ int fvhoe_stage(const FvhoeStageConfig* cfg, const double* u, double* dudt, doub
                 void* workspace, void* stream) {
   if (!dudt) return FVHOE_ERR_ARG;
   return fvhoe_stage_rk(cfg, u, dudt, dt_stage, trouble, workspace, nullptr, stream);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// The stage without a caller workspace: its W / F / G / WS arrays, flags, cell
+// list, reduction partials and counters live in a file-scope device array of
+// FVHOE_STATIC_SCRATCH_BYTES.  Each GPU the library runs on holds its own copy
+// (allocated when the code object loads, kept until the process ends); host
+// code reaches it only through the per-device address hipGetSymbolAddress
+// returns.  Nothing in it must start at zero: every stage writes what it reads
+// (k_stage_reduce clears the stage's counters).  All calls on one GPU share the
+// array, so calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+namespace {
+
+static __device__ __align__(256) char scratch[FVHOE_STATIC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// this device's address of the scratch array, looked up once per device with that device current
+int scratch_for_current_device(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return FVHOE_ERR_LAUNCH;
+  if (dev < 0 || dev >= MAX_DEVICES) return FVHOE_ERR_UNSUPPORTED;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return FVHOE_ERR_LAUNCH;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FVHOE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fvhoe_stage_rk_static(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                          signed char* trouble, const FvhoeRkFuse* rk, void* stream) {
+  Layout L;
+  const int st = make_layout(cfg, &L);
+  if (st != FVHOE_OK) return st;
+  if (L.total > (size_t)FVHOE_STATIC_SCRATCH_BYTES) return FVHOE_ERR_TOO_LARGE;
+  void* ws = nullptr;
+  const int sst = scratch_for_current_device(&ws);
+  if (sst != FVHOE_OK) return sst;
+  return stage_entry(cfg, u, dudt, dt_stage, trouble, ws, rk, nullptr, stream);
 }
 
 }  // extern "C"

@@ -28,7 +28,8 @@ enum {
   FVHOE_OK = 0,
   FVHOE_ERR_ARG = 1,
   FVHOE_ERR_UNSUPPORTED = 2,
-  FVHOE_ERR_LAUNCH = 3
+  FVHOE_ERR_LAUNCH = 3,
+  FVHOE_ERR_TOO_LARGE = 4  /* the problem needs more than the fixed device scratch of fvhoe_stage_rk_static */
 };
 
 /* stencil kinds, Riemann solvers, limiters, boundary kinds */
@@ -388,6 +389,21 @@ int fvhoe_euler_equation_rk_f32(const float* F, const long long* fst, const floa
 
 int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
                    signed char* trouble, void* workspace, const FvhoeRkFuse* rk, void* stream);
+
+/* fvhoe_stage_rk without a workspace argument: the stage's scratch (the
+ * arrays fvhoe_stage_workspace_bytes counts) is a fixed device array of
+ * FVHOE_STATIC_SCRATCH_BYTES inside the library, one per GPU, held until the
+ * process ends, so the call allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
+ * Returns FVHOE_ERR_TOO_LARGE when cfg needs more scratch than that (with
+ * a-posteriori limiting it fits 2-D grids up to 633 x 633 cells in FP64 and
+ * 884 x 884 in FP32 at p=3, 1-D grids up to about 535,000 cells at p=5).  Every call on
+ * one GPU shares the array: calls that overlap on one GPU (concurrent streams
+ * or threads) race.  Results equal fvhoe_stage_rk's bit for bit. */
+#define FVHOE_STATIC_SCRATCH_BYTES (64ull << 20)
+int fvhoe_stage_rk_static(const FvhoeStageConfig* cfg, const double* u, double* dudt, double* dt_stage,
+                          signed char* trouble, const FvhoeRkFuse* rk, void* stream);
 
 /* Bench instrumentation: with timing on, every fused-stage kernel launch is
  * bracketed by hipEvents on its stream.  fvhoe_stage_timing_read waits for
