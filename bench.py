@@ -311,42 +311,75 @@ def run_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(n: int, stages: int):
-    """NumPy oracle, `stages` stages of KH n^2 on one core."""
+ONE_THREAD = ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_child(kind: str, n: int, stages: int) -> None:
+    """Runs in a child process started with the thread variables at 1 before NumPy loads: pins itself to
+    one core, times the NumPy oracle and prints one JSON line."""
+    cores = sorted(os.sched_getaffinity(0))
+    core = cores[0]
+    os.sched_setaffinity(0, {core})
+    assert all(os.environ.get(k) == "1" for k in ONE_THREAD), "thread variables must be 1 before NumPy loads"
     import numpy as np
 
     import oracle as O
-    from fvhoe_amd import initial_conditions as ics
 
-    u = ics.kelvin_helmholtz(n, seed=0)
-    s = O.Scheme(n=(n, n, 1), p=(3, 3, 0), gamma=1.4, a_posteriori=True)
+    if kind == "kh":
+        from fvhoe_amd import initial_conditions as ics
+
+        u = ics.kelvin_helmholtz(n, seed=0)
+        s = O.Scheme(n=(n, n, 1), p=(3, 3, 0), gamma=1.4, a_posteriori=True)
+        cells, what = n * n * stages, f"{stages} a-posteriori FP64 stages of Kelvin-Helmholtz {n}^2 p=3 HLLC"
+    else:
+        X, Y, Z = O.fv_uniform_meshgen((n, n, n))
+        one = np.ones_like(X)
+        u = O.compute_conservatives(np.stack([1 + 0.5 * np.sin(2 * np.pi * (X + Y + Z)), one, one, one, one]), 1.4)
+        s = O.Scheme(n=(n, n, n), p=(7, 7, 7), gamma=1.4, riemann_solver="llf")
+        stages = 1
+        cells, what = n ** 3, (f"1 FP64 stage of the density wave at {n}^3 p=7 LLF (the cell-centre sample of the "
+                               f"point-value IC)")
     t0 = time.perf_counter()
     for _ in range(stages):
         O.hydrodynamics(s, u)
     dt = time.perf_counter() - t0
-    return {"value": round(n * n * stages / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"{stages} a-posteriori FP64 stages of Kelvin-Helmholtz {n}^2 p=3 HLLC (NumPy oracle), "
-                      f"{dt:.1f} s; {os.cpu_count()} logical cores visible, 1 used",
-            "numpy": np.__version__}
+    print(json.dumps({"value": round(cells / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
+                      "sample": f"{what} (NumPy oracle), {dt:.1f} s on core {core}; {len(cores)} cores in the "
+                                f"process's affinity set, {os.cpu_count()} visible, 1 used",
+                      "affinity": [core], "cpu_model": cpu_model(), "seconds": round(dt, 2),
+                      "threads": {k: os.environ.get(k) for k in ONE_THREAD}, "numpy": np.__version__}), flush=True)
+
+
+def _cpu_baseline_run(kind: str, n: int, stages: int):
+    """The CPU baseline in a child process: one thread (OMP / OpenBLAS / MKL at 1 before NumPy loads),
+    pinned to one core."""
+    import subprocess
+
+    env = dict(os.environ, **{k: "1" for k in ONE_THREAD})
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", kind, str(n), str(stages)],
+                       env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(n: int, stages: int):
+    """NumPy oracle, `stages` stages of KH n^2 on one pinned core."""
+    return _cpu_baseline_run("kh", n, stages)
 
 
 def cpu_baseline_wave3d(n: int):
-    """NumPy oracle, one FP64 stage of the 3-D density wave at n^3 (p=7, LLF, no limiter) on one core."""
-    import numpy as np
-
-    import oracle as O
-
-    X, Y, Z = O.fv_uniform_meshgen((n, n, n))
-    one = np.ones_like(X)
-    u = O.compute_conservatives(np.stack([1 + 0.5 * np.sin(2 * np.pi * (X + Y + Z)), one, one, one, one]), 1.4)
-    s = O.Scheme(n=(n, n, n), p=(7, 7, 7), gamma=1.4, riemann_solver="llf")
-    t0 = time.perf_counter()
-    O.hydrodynamics(s, u)
-    dt = time.perf_counter() - t0
-    return {"value": round(n ** 3 / dt / 1e6, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": "port",
-            "sample": f"1 FP64 stage of the density wave at {n}^3 p=7 LLF (NumPy oracle; the cell-centre "
-                      f"sample of the point-value IC), {dt:.1f} s; {os.cpu_count()} logical cores visible, 1 used",
-            "numpy": np.__version__}
+    """NumPy oracle, one FP64 stage of the 3-D density wave at n^3 (p=7, LLF, no limiter) on one pinned core."""
+    return _cpu_baseline_run("wave3d", n, 1)
 
 
 def run_form(n, form, steps, warmup, u0):
@@ -507,4 +540,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-baseline-child":
+        cpu_baseline_child(sys.argv[2], int(sys.argv[3]), int(sys.argv[4]))
+    else:
+        main()

@@ -43,3 +43,63 @@ def test_kelvin_helmholtz_fp32_l1():
     print(f"[kh256 fp32] 10 steps L1(rho)={d:.3e} t64={ref.t:.6e} t32={f32.t:.6e}", flush=True)
     assert d < KH_L1_BOUND
     assert abs(f32.t - ref.t) / ref.t < 1e-5
+
+
+# The FP32 form across degrees and boundary kinds on grids large enough for the interior tiles: 5 SSP-RK3
+# steps of a four-quadrant state at 128^2, L1(rho) against the FP64 run.  "reflective-free": reflective x,
+# free y; "dirichlet": constant Dirichlet x, outflow y (the slab-reading sources of every kernel).
+FP32_SWEEP_L1_BOUND = 1e-5
+_C = np.array([1.1, 0.2, -0.1, 0.0, 2.5])
+
+
+def _bc(kind):
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+
+    if kind == "periodic":
+        return BoundaryCondition(x="periodic", y="periodic")
+    if kind == "reflective-free":
+        return BoundaryCondition(x="reflective", y="free")
+    return BoundaryCondition(x="dirichlet", y="outflow", x_value=(_C, _C))
+
+
+@pytest.mark.parametrize("p", [1, 2, 5, 8])
+@pytest.mark.parametrize("kind", ["periodic", "reflective-free", "dirichlet"])
+def test_fp32_degrees_and_boundaries(p, kind):
+    from gpu_helpers import ArrayIC
+    from fvhoe_amd.solver import EulerSolver
+
+    n = 128
+    u0 = ics.four_quadrants(n, seed=9)
+
+    def mk(bulk):
+        return EulerSolver(w0=ArrayIC(u0), nx=n, ny=n, px=p, py=p, gamma=1.4, bc=_bc(kind), conservative_ic=True,
+                           fv_ic=True, riemann_solver="hllc", a_posteriori_slope_limiting=True, progress_bar=False,
+                           bulk_precision=bulk)
+
+    ref, f32 = mk("fp64"), mk("fp32")
+    assert ref.fused and f32.fused
+    ref.ssprk3(n=5)
+    f32.ssprk3(n=5)
+    d = (f32.u[0] - ref.u[0]).abs().mean().item()
+    print(f"[fp32 p={p} {kind}] 5 steps L1(rho)={d:.3e}", flush=True)
+    assert np.isfinite(d) and d < FP32_SWEEP_L1_BOUND
+
+
+def test_fp64_p5_stage_128_matches_oracle():
+    """An FP64 p=5 stage on a grid with interior tiles (the fast path of the tiled prims kernel) against
+    the oracle, bit for bit."""
+    import oracle as Orc
+    from gpu_helpers import assert_bits_equal
+
+    n = 128
+    u0 = ics.four_quadrants(n, seed=10)
+    s = solver_from(u0, (n, n, 1), (5, 5, 0), (O, P, P), riemann_solver="hllc", a_posteriori_slope_limiting=True)
+    assert s.fused
+    dt, dudt = s.hydrodynamics(u0)
+    log = Orc.StageLog()
+    dto, dudto = Orc.hydrodynamics(Orc.Scheme(n=(n, n, 1), p=(5, 5, 0), gamma=1.4, a_posteriori=True,
+                                              bc=(O, P, P)), u0, log)
+    print(f"[fp64 p5 128] dt={dt!r} troubled {int(np.asarray(log.trouble).sum())}", flush=True)
+    assert dt == dto
+    assert_bits_equal(dudt, dudto, "dudt")
+    assert np.array_equal(s._trouble.cpu().numpy().astype(bool), np.asarray(log.trouble).astype(bool).reshape(n, n, 1))

@@ -27,7 +27,9 @@ def _theta(seed):
     return th
 
 
-@pytest.mark.parametrize("n0,n1", [(16, 16), (37, 23), (64, 48)])
+# 512 x 512 is 585 output tiles of 16 x 28: more tiles than the persistent grid's 512 blocks, so blocks
+# loop over several tiles
+@pytest.mark.parametrize("n0,n1", [(16, 16), (37, 23), (64, 48), (512, 512)])
 def test_inference_matches_oracle(n0, n1):
     rng = np.random.default_rng(n0 * 100 + n1)
     th = _theta(n0)
@@ -193,10 +195,11 @@ def _call_predicted(s, u, infer, overlap):
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32"])
-def test_overlapped_inference(form):
+@pytest.mark.parametrize("n", [48, 512])
+def test_overlapped_inference(form, n):
     """A call that runs the network on the side stream is the plain stage bit for
-    bit; the next call applies the same flags as the non-overlapped sequence."""
-    n = 48
+    bit; the next call applies the same flags as the non-overlapped sequence (at 512^2 the side
+    stream's 256 blocks each walk several of the 585 tiles)."""
     u0 = ics.four_quadrants(n, seed=5)
     kw = dict(riemann_solver="hllc", a_posteriori_slope_limiting=True, bulk_precision=form)
     m = TroubledCellPredictor(_theta(6), logit_threshold=0.0)
