@@ -247,3 +247,31 @@ def test_training_reduces_loss_on_solver_flags():
     hist = train(m, X, Y, iters=300, batch=4, lr=3e-3, log_every=50)
     print(f"[train] loss {hist[0][1]:.4f} -> {hist[-1][1]:.4f}", flush=True)
     assert hist[-1][1] < 0.6 * hist[0][1]
+
+
+# What the trained network gains on held-out four-quadrant problems (seeds 200-249 were never trained on):
+# the check's flags left to the second pass, relative to the FP32 run without the predictor, stay below
+# one (the network moves fallbacks into the first pass), and the learned form's density differs from the
+# plain FP64 run by less than LEARNED_L1_BOUND while its error against a 2x finer FP64 run stays within
+# 1.5x of plain FP64's.
+LEARNED_L1_BOUND = 1e-2
+MATCHED_ERROR_RATIO = 1.5
+
+
+def test_trained_predictor_gain_on_held_out_seeds():
+    from fvhoe_amd.train_predictor import evaluate
+
+    rows = evaluate(TroubledCellPredictor.load(), seeds=[201, 202, 205], n=128, t_end=0.05, log=lambda *a: None,
+                    ref_seeds=3)
+    done = [r for r in rows if "second_pass_share" in r]
+    for r in rows:
+        print("[gain] " + " ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}" for k, v in r.items()),
+              flush=True)
+    assert len(done) >= 2, rows
+    for r in done:
+        assert r["check_flags_fp32"] > 0
+        assert r["second_pass_share"] < 1.0, r
+        assert r["l1_density_learned_fp32_vs_plain"] < LEARNED_L1_BOUND, r
+        if "err_ratio_learned_fp32" in r:
+            assert r["err_ratio_learned_fp32"] <= MATCHED_ERROR_RATIO, r
+    assert sum("err_ratio_learned_fp32" in r for r in done) >= 1
