@@ -289,8 +289,9 @@ def run_ensemble(a):
         if form.startswith("learned"):
             continue  # the predictor network is two-dimensional
         e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision=form)
-        tune = e.autotune(a.ensemble_t, steps=10)
-        print(f"# ensemble {form}: {sum(e.n)} cells, autotune {tune} -> tile {e.tile}", file=sys.stderr, flush=True)
+        tune = {f"{k[0]}/{k[1]}": round(v, 4) for k, v in e.autotune(a.ensemble_t, steps=10).items()}
+        print(f"# ensemble {form}: {sum(e.n)} cells, autotune {tune} -> tile {e.tile}, "
+              f"{'persistent' if e.persistent else 'chain'}", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         steps = e.run(a.ensemble_t)
@@ -298,7 +299,8 @@ def run_ensemble(a):
         wall = time.perf_counter() - t0
         cu = e.cell_updates()
         forms[form] = {"value": round(cu / wall / 1e6, 2), "seconds": round(wall, 3), "ensemble_steps": steps,
-                       "cell_updates": cu, "tile": e.tile, "autotune_ms_per_step": tune,
+                       "cell_updates": cu, "tile": e.tile, "stage_form": "persistent" if e.persistent else "chain",
+                       "autotune_ms_per_step": tune,
                        "t_min": float(e.times().min()), "t_max": float(e.times().max())}
         print(f"# ensemble {form}: {wall:.2f}s, {steps} steps", file=sys.stderr, flush=True)
     head = forms[list(forms)[-1]]

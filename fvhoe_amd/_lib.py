@@ -120,6 +120,9 @@ _SIGNATURES = {
                                      _P, _P]),
     "fvhoe_ensemble_step": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P, c_double,
                                     _P, _P]),
+    "fvhoe_ensemble_step_persistent": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P,
+                                               c_double, _P, _P]),
+    "fvhoe_ensemble_status": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, POINTER(c_int)]),
     "fvhoe_event_create": (c_int, [POINTER(c_void_p)]),
     "fvhoe_event_destroy": (c_int, [_P]),
     "fvhoe_stage_predicted_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),

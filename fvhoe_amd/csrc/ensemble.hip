@@ -95,25 +95,23 @@ __device__ __forceinline__ double ustar_at(const double* __restrict__ u, const T
   return (f && v == 1) ? -x : x;
 }
 
-__global__ void k_ens_reset(Ens e) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_ens_reset(const Ens& e, int i) {
   if (i >= e.count) return;
   e.acc[i * NACC] = 0ull;
   for (int k = 1; k < NACC; ++k) e.acc[i * NACC + k] = (k & 1) ? ~0ull : 0ull;
   e.cnt[i] = 0;
 }
+__global__ void k_ens_reset(Ens e) { d_ens_reset(e, blockIdx.x * blockDim.x + threadIdx.x); }
 
 // W (CAND = false, from u; ghost width GE) or WS (CAND = true, from the candidate; ghost width 3)
 template <typename T, int P, int TILE, bool CAND>
-__global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restrict__ tiles,
-                                                   const double* __restrict__ u, const T* __restrict__ F,
-                                                   T* __restrict__ out) {
+__device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double* __restrict__ u,
+                                            const T* __restrict__ F, T* __restrict__ out) {
   constexpr int RC = e_reach(P), RT = e_reach(P), GE = e_gext(P), G = e_ghosts(P);
   constexpr int G0 = CAND ? 3 : GE;
   constexpr int HU = TILE + 2 * (RC + RT), HC = TILE + 2 * RT;
   __shared__ T SU[5][HU];
   __shared__ T SC[5][HC];
-  const int2 tl = tiles[blockIdx.x];
   const int i = tl.x;
   if (!e.active[i]) return;
   const int n = e.n[i], NO = n + 2 * G0;
@@ -197,10 +195,16 @@ __global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restric
     else atomicMin(&e.acc[i * NACC + tid], x);
   }
 }
+template <typename T, int P, int TILE, bool CAND>
+__global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restrict__ tiles,
+                                                   const double* __restrict__ u, const T* __restrict__ F,
+                                                   T* __restrict__ out) {
+  d_ens_prims<T, P, TILE, CAND>(e, tiles[blockIdx.x], u, F, out);
+}
 
 // per-instance dt, range; first stage: active flag and the step's dt
-__global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, double target) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_ens_scalars(const Ens& e, int i, int first, const double* __restrict__ t,
+                                              double target) {
   if (i >= e.count) return;
   if (first) e.active[i] = t[i] < target ? 1 : 0;
   if (!e.active[i]) return;
@@ -210,13 +214,14 @@ __global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, do
   if (first) e.dt_step[i] = (target - t[i] < d) ? target - t[i] : d;
   for (int k = 0; k < 10; ++k) e.range[i * 10 + k] = from_obits(a[1 + k]);
 }
+__global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, double target) {
+  d_ens_scalars(e, blockIdx.x * blockDim.x + threadIdx.x, first, t, target);
+}
 
 // faces fa in [0, n] of each instance: l/r values of W, Riemann solver
 template <typename T, int P, int TILE>
-__global__ void __launch_bounds__(TILE) k_ens_flux(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
-                                                  T* __restrict__ F) {
+__device__ __forceinline__ void d_ens_flux(const Ens& e, int2 tl, const T* __restrict__ W, T* __restrict__ F) {
   constexpr int GE = e_gext(P), EL = reach(P);
-  const int2 tl = tiles[blockIdx.x];
   const int i = tl.x;
   if (!e.active[i]) return;
   const int n = e.n[i], fa = tl.y + threadIdx.x;
@@ -242,12 +247,16 @@ __global__ void __launch_bounds__(TILE) k_ens_flux(Ens e, const int2* __restrict
     Fi[q * (n + 1) + fa] = T(0) + x;
   }
 }
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_flux(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
+                                                  T* __restrict__ F) {
+  d_ens_flux<T, P, TILE>(e, tiles[blockIdx.x], W, F);
+}
 
 template <typename T, int P, int TILE>
-__global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
-                                                    const T* __restrict__ WS, int* __restrict__ list) {
+__device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __restrict__ W, const T* __restrict__ WS,
+                                             int* __restrict__ list) {
   constexpr int GE = e_gext(P);
-  const int2 tl = tiles[blockIdx.x];
   const int i = tl.x;
   if (!e.active[i]) return;
   const int n = e.n[i], c = tl.y + threadIdx.x;
@@ -314,6 +323,11 @@ __global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restri
   __syncthreads();
   if (t) list[e.off_u[i] + bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
 }
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
+                                                    const T* __restrict__ WS, int* __restrict__ list) {
+  d_ens_detect<T, P, TILE>(e, tiles[blockIdx.x], W, WS, list);
+}
 
 // MUSCL p=1 flux at face fa of instance i from the stage input (fvhoe/solver.py:712-717)
 template <typename T>
@@ -345,10 +359,8 @@ __device__ void ens_fallback_flux(const Ens& e, const double* __restrict__ ui, i
 }
 
 template <typename T, int TILE>
-__global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __restrict__ tiles,
-                                                      const double* __restrict__ u, const int* __restrict__ list,
-                                                      T* __restrict__ F) {
-  const int2 tl = tiles[blockIdx.x];
+__device__ __forceinline__ void d_ens_fallback(const Ens& e, int2 tl, const double* __restrict__ u,
+                                               const int* __restrict__ list, T* __restrict__ F) {
   const int i = tl.x;
   if (!e.active[i]) return;
   const int n = e.n[i], cnt = e.cnt[i];
@@ -364,15 +376,20 @@ __global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __rest
     for (int q = 0; q < 5; ++q) Fi[q * (n + 1) + fa] = Fv[q];
   }
 }
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __restrict__ tiles,
+                                                      const double* __restrict__ u, const int* __restrict__ list,
+                                                      T* __restrict__ F) {
+  d_ens_fallback<T, TILE>(e, tiles[blockIdx.x], u, list, F);
+}
 
 // divergence (fvhoe/solver.py:744-750) and the SSP-RK3 combination of this stage
 // (fvhoe/ode.py:275-279): stage 0 -> k0, s = u + dt k0; stage 1 -> k1,
 // s = u + 0.25 dt k0 + 0.25 dt k1; stage 2 -> u + (1/6) dt (k0 + k1 + 4 k2).
 template <typename T, int TILE>
-__global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restrict__ tiles, const T* __restrict__ F,
-                                                   int stage, double* __restrict__ u, double* __restrict__ s,
-                                                   double* __restrict__ k0, double* __restrict__ k1) {
-  const int2 tl = tiles[blockIdx.x];
+__device__ __forceinline__ void d_ens_final(const Ens& e, int2 tl, const T* __restrict__ F, int stage,
+                                            double* __restrict__ u, double* __restrict__ s, double* __restrict__ k0,
+                                            double* __restrict__ k1) {
   const int i = tl.x;
   if (!e.active[i]) return;
   const int n = e.n[i], c = tl.y + threadIdx.x;
@@ -396,13 +413,95 @@ __global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restric
     }
   }
 }
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restrict__ tiles, const T* __restrict__ F,
+                                                   int stage, double* __restrict__ u, double* __restrict__ s,
+                                                   double* __restrict__ k0, double* __restrict__ k1) {
+  d_ens_final<T, TILE>(e, tiles[blockIdx.x], F, stage, u, s, k0, k1);
+}
 
-__global__ void k_ens_advance(Ens e, double* __restrict__ t, int* __restrict__ steps) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void d_ens_advance(const Ens& e, int i, double* __restrict__ t, int* __restrict__ steps) {
   if (i < e.count && e.active[i]) {
     t[i] = t[i] + e.dt_step[i];
     if (steps) steps[i] += 1;
   }
+}
+__global__ void k_ens_advance(Ens e, double* __restrict__ t, int* __restrict__ steps) {
+  d_ens_advance(e, blockIdx.x * blockDim.x + threadIdx.x, t, steps);
+}
+
+// ---------------------------------------------------------------------------
+// The whole RK stage in one persistent kernel: the blocks (all resident at once: the host sizes the grid
+// from the occupancy of this kernel) walk each phase's tiles in turn, with a grid-wide barrier between
+// phases where one phase needs another's results (per-instance reductions before dt, the flag list
+// before the fallback, ...).  The barrier waits a bounded time: a block that waits too long sets the
+// error word and goes on, so the kernel always ends (the host reports the error).
+// ---------------------------------------------------------------------------
+struct GridBar {
+  unsigned* count;  // blocks arrived at the current barrier
+  unsigned* gen;    // barrier generation
+  int* err;         // set when a wait timed out
+  unsigned nblocks;
+};
+
+constexpr long long BAR_SPIN_LIMIT = 1ll << 24;  // ~1 s of polling before the error word is set
+
+__device__ __forceinline__ void grid_sync(const GridBar& b) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();  // this block's writes are visible to every XCD before it arrives
+    const unsigned g = __hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned arrived = __hip_atomic_fetch_add(b.count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (arrived == b.nblocks - 1) {
+      __hip_atomic_store(b.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(b.gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long long spins = 0;
+      while (__hip_atomic_load(b.gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > BAR_SPIN_LIMIT) {
+          __hip_atomic_fetch_or(b.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+      }
+    }
+    __threadfence();  // and sees every other block's writes after it
+  }
+  __syncthreads();
+}
+
+struct StageTiles {
+  const int2 *tw, *tf, *ts, *tc;
+  int nw, nf, ns, nc;
+};
+
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_stage(Ens e, StageTiles tl, GridBar bar, int st, const double* in,
+                                                   double* u, double* sv, double* k0, double* k1, T* W, T* F, T* WS,
+                                                   int* list, double* t, int* nsteps, double target) {
+  const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+  for (int i = b * TILE + (int)threadIdx.x; i < e.count; i += nb * TILE) d_ens_reset(e, i);
+  grid_sync(bar);
+  for (int k = b; k < tl.nw; k += nb) d_ens_prims<T, P, TILE, false>(e, tl.tw[k], in, nullptr, W);
+  grid_sync(bar);
+  for (int i = b * TILE + (int)threadIdx.x; i < e.count; i += nb * TILE) d_ens_scalars(e, i, st == 0, t, target);
+  grid_sync(bar);
+  for (int k = b; k < tl.nf; k += nb) d_ens_flux<T, P, TILE>(e, tl.tf[k], W, F);
+  if (e.apost) {
+    grid_sync(bar);
+    for (int k = b; k < tl.ns; k += nb) d_ens_prims<T, P, TILE, true>(e, tl.ts[k], in, F, WS);
+    grid_sync(bar);
+    for (int k = b; k < tl.nc; k += nb) {
+      __syncthreads();  // the block's compaction words of the previous tile have been read
+      d_ens_detect<T, P, TILE>(e, tl.tc[k], W, WS, list);
+    }
+    grid_sync(bar);
+    for (int k = b; k < tl.nc; k += nb) d_ens_fallback<T, TILE>(e, tl.tc[k], in, list, F);
+  }
+  grid_sync(bar);
+  for (int k = b; k < tl.nc; k += nb) d_ens_final<T, TILE>(e, tl.tc[k], F, st, u, sv, k0, k1);
+  if (st == 2)  // the clock moves once per step (no instance's final reads t)
+    for (int i = b * TILE + (int)threadIdx.x; i < e.count; i += nb * TILE) d_ens_advance(e, i, t, nsteps);
 }
 
 // ------------------------------------------------------------------ host side
@@ -414,7 +513,7 @@ struct Plan {
   long long cells, wcells, fcells, scells;
   int tw, tf, ts, tc;  // tiles of each kind
   size_t o_n, o_h, o_ou, o_ow, o_of, o_os, o_dts, o_dtp, o_act, o_acc, o_rng, o_cnt, o_tw, o_tf, o_ts, o_tc;
-  size_t o_W, o_F, o_WS, o_list, o_k0, o_k1, o_s, total;
+  size_t o_bar, o_W, o_F, o_WS, o_list, o_k0, o_k1, o_s, total;
 };
 
 int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan* P) {
@@ -463,6 +562,7 @@ int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan
   P->o_tf = o; o = a256(o + 8 * (size_t)P->tf);
   P->o_ts = o; o = a256(o + 8 * (size_t)P->ts);
   P->o_tc = o; o = a256(o + 8 * (size_t)P->tc);
+  P->o_bar = o; o = a256(o + 16);  // grid-barrier words of the persistent stage (zeroed by the setup copy)
   P->o_W = o; o = a256(o + esz * 5 * (size_t)P->wcells);
   P->o_F = o; o = a256(o + esz * 5 * (size_t)P->fcells);
   P->o_WS = o; o = a256(o + esz * 5 * (size_t)P->scells);
@@ -510,9 +610,25 @@ Ens make_ens(const FvhoeStageConfig* c, const Plan& P, char* ws) {
   return e;
 }
 
+// blocks of the persistent stage that are resident at once on this device (all of them must be)
+template <typename T, int P, int TILE>
+int persistent_blocks() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (cached[dev] == 0) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ens_stage<T, P, TILE>, TILE, 0) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 0;
+    cached[dev] = per_cu * cus;
+  }
+  return cached[dev];
+}
+
 template <typename T, int P, int TILE>
 int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-             hipStream_t s) {
+             hipStream_t s, bool persistent) {
   Ens e = make_ens(c, PL, ws);
   const int2* tw = (const int2*)(ws + PL.o_tw);
   const int2* tf = (const int2*)(ws + PL.o_tf);
@@ -526,6 +642,21 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
   double* k1 = (double*)(ws + PL.o_k1);
   double* sv = (double*)(ws + PL.o_s);
   const int gi = (PL.count + 255) / 256;
+  if (persistent) {
+    int nb = persistent_blocks<T, P, TILE>();
+    if (nb < 1) return FVHOE_ERR_LAUNCH;
+    const int most = PL.tw > PL.ts ? PL.tw : PL.ts;
+    if (nb > most) nb = most;
+    unsigned* bw = (unsigned*)(ws + PL.o_bar);
+    GridBar bar{bw, bw + 1, (int*)(bw + 2), (unsigned)nb};
+    StageTiles tls{tw, tf, ts, tc, PL.tw, PL.tf, PL.ts, PL.tc};
+    for (int st = 0; st < 3; ++st) {
+      const double* in = st == 0 ? u : sv;
+      k_ens_stage<T, P, TILE><<<nb, TILE, 0, s>>>(e, tls, bar, st, in, u, sv, k0, k1, W, F, WS, list, t, nsteps,
+                                                  target);
+    }
+    return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
+  }
   for (int st = 0; st < 3; ++st) {
     const double* in = st == 0 ? u : sv;
     k_ens_reset<<<gi, 256, 0, s>>>(e);
@@ -545,31 +676,45 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
 
 template <typename T, int TILE>
 int by_p(int p, const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-         hipStream_t s) {
+         hipStream_t s, bool pers) {
   switch (p) {
-    case 1: return run_step<T, 1, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 2: return run_step<T, 2, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 3: return run_step<T, 3, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 4: return run_step<T, 4, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 5: return run_step<T, 5, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 6: return run_step<T, 6, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 7: return run_step<T, 7, TILE>(c, PL, u, t, nsteps, target, ws, s);
-    case 8: return run_step<T, 8, TILE>(c, PL, u, t, nsteps, target, ws, s);
+    case 1: return run_step<T, 1, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 2: return run_step<T, 2, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 3: return run_step<T, 3, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 4: return run_step<T, 4, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 5: return run_step<T, 5, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 6: return run_step<T, 6, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 7: return run_step<T, 7, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 8: return run_step<T, 8, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
   }
   return FVHOE_ERR_UNSUPPORTED;
 }
 
 template <typename T>
 int by_tile(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-            hipStream_t s) {
+            hipStream_t s, bool pers) {
   switch (PL.tile) {
-    case 64: return by_p<T, 64>(PL.p, c, PL, u, t, nsteps, target, ws, s);
-    case 128: return by_p<T, 128>(PL.p, c, PL, u, t, nsteps, target, ws, s);
-    case 256: return by_p<T, 256>(PL.p, c, PL, u, t, nsteps, target, ws, s);
+    case 64: return by_p<T, 64>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
+    case 128: return by_p<T, 128>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
+    case 256: return by_p<T, 256>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
   }
   return FVHOE_ERR_ARG;
 }
 
+}  // namespace
+
+namespace {
+int ensemble_step(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u, double* t, int* steps,
+                  double target, void* workspace, void* stream, bool persistent) {
+  Plan P;
+  const int st = make_plan(cfg, count, n, tile, &P);
+  if (st != FVHOE_OK) return st;
+  if (!u || !t || !workspace || ((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  return P.fp32 ? by_tile<float>(cfg, P, u, t, steps, target, ws, s, persistent)
+                 : by_tile<double>(cfg, P, u, t, steps, target, ws, s, persistent);
+}
 }  // namespace
 
 extern "C" {
@@ -629,16 +774,26 @@ int fvhoe_ensemble_setup(const FvhoeStageConfig* cfg, int count, const int* n, c
   return FVHOE_OK;
 }
 
+
 int fvhoe_ensemble_step(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u, double* t,
                         int* steps, double target, void* workspace, void* stream) {
+  return ensemble_step(cfg, count, n, tile, u, t, steps, target, workspace, stream, false);
+}
+
+int fvhoe_ensemble_step_persistent(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u,
+                                   double* t, int* steps, double target, void* workspace, void* stream) {
+  return ensemble_step(cfg, count, n, tile, u, t, steps, target, workspace, stream, true);
+}
+
+int fvhoe_ensemble_status(const FvhoeStageConfig* cfg, int count, const int* n, int tile, const void* workspace,
+                          int* error_word) {
   Plan P;
   const int st = make_plan(cfg, count, n, tile, &P);
   if (st != FVHOE_OK) return st;
-  if (!u || !t || !workspace || ((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
-  char* ws = (char*)workspace;
-  hipStream_t s = (hipStream_t)stream;
-  return P.fp32 ? by_tile<float>(cfg, P, u, t, steps, target, ws, s)
-                 : by_tile<double>(cfg, P, u, t, steps, target, ws, s);
+  if (!workspace || !error_word) return FVHOE_ERR_ARG;
+  if (hipMemcpy(error_word, (const char*)workspace + P.o_bar + 8, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    return FVHOE_ERR_LAUNCH;
+  return FVHOE_OK;
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ class Ensemble:
                  slope_limiter: str = "minmod", NAD: float = 1e-2, NAD_mode: str = "global",
                  NAD_range: str = "relative", NAD_vars: tuple = None, PAD: dict = None, SED: bool = True,
                  SED_tolerance: float = 1e-10, csq_floor: float = 1e-10, bulk_precision: str = "fp64",
-                 tile: int = 128):
+                 tile: int = 128, persistent: bool = False):
         torch = _lib.torch()
         if bulk_precision not in ("fp64", "fp32"):
             raise ValueError(f"bulk_precision must be 'fp64' or 'fp32', got {bulk_precision!r}")
@@ -80,6 +80,7 @@ class Ensemble:
         self._graph = None
         self._ws = None
         self.tile = None
+        self.persistent = bool(persistent)  # one persistent kernel per RK stage instead of a chain of launches
         self.set_tile(tile)
 
     # ----------------------------------------------------------------- set-up
@@ -100,12 +101,26 @@ class Ensemble:
         return _lib.ctypes.c_void_p(base + (-base) % 256)
 
     # ------------------------------------------------------------------- steps
+    def _entry(self) -> str:
+        # one persistent kernel per RK stage, or the chain of seven launches per stage
+        return "fvhoe_ensemble_step_persistent" if self.persistent else "fvhoe_ensemble_step"
+
     def step(self, target: float) -> None:
         """One SSP-RK3 step of every instance whose time is below target (no host sync)."""
-        _lib.call("fvhoe_ensemble_step", _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
+        _lib.call(self._entry(), _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
                   _lib.ptr(self.u), _lib.ptr(self.t), _lib.ptr(self.steps), float(target), self._wsp(),
                   _lib.stream())
         self.step_count += 1
+
+    def check(self) -> None:
+        """Raise if a persistent stage's grid barrier timed out (its results are then not valid)."""
+        if not self.persistent:
+            return
+        err = _lib.ctypes.c_int(0)
+        _lib.call("fvhoe_ensemble_status", _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
+                  self._wsp(), _lib.ctypes.byref(err))
+        if err.value:
+            raise RuntimeError("a grid barrier of the persistent ensemble stage timed out (blocks not all resident)")
 
     def _capture(self, target: float):
         torch = _lib.torch()
@@ -115,7 +130,7 @@ class Ensemble:
         with _lib.capture_guard(), torch.cuda.stream(s):
             # recorded, not run: the state is unchanged by the capture
             with torch.cuda.graph(g, stream=s):
-                _lib.call("fvhoe_ensemble_step", _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
+                _lib.call(self._entry(), _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
                           _lib.ptr(self.u), _lib.ptr(self.t), _lib.ptr(self.steps), float(target),
                           self._wsp(), _lib.ctypes.c_void_p(s.cuda_stream))
         torch.cuda.current_stream().wait_stream(s)
@@ -136,35 +151,42 @@ class Ensemble:
                     self.step(T)
                 steps += 1
             tmin = float(self.t.min().item())
+            self.check()
             if not np.isfinite(tmin) or tmin >= T:
                 break
         torch.cuda.synchronize()
+        self.check()
         self.step_count += steps if graph else 0
         return steps
 
-    def autotune(self, T: float, steps: int = 20, tiles=TILES) -> dict:
-        """Time `steps` captured steps for each tile size from the current state (restored
-        afterwards) and keep the fastest; returns {tile: ms per step}."""
+    def autotune(self, T: float, steps: int = 20, tiles=TILES, persistent=(False, True)) -> dict:
+        """Time `steps` captured steps for each tile size and each stage form (the chain of launches,
+        the persistent kernel) from the current state (restored afterwards) and keep the fastest;
+        returns {(tile, "persistent" | "chain"): ms per step}."""
         torch = _lib.torch()
         u0, t0, k0 = self.u.clone(), self.t.clone(), self.steps.clone()
         res = {}
-        for tile in tiles:
-            self.set_tile(tile)
-            g = self._capture(T)
-            g.replay()
-            torch.cuda.synchronize()
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            for _ in range(steps):
+        for pers in persistent:
+            self.persistent = bool(pers)
+            for tile in tiles:
+                self.set_tile(tile)
+                g = self._capture(T)
                 g.replay()
-            b.record()
-            torch.cuda.synchronize()
-            res[tile] = a.elapsed_time(b) / steps
-            self.u.copy_(u0)
-            self.t.copy_(t0)
-            self.steps.copy_(k0)
+                torch.cuda.synchronize()
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                for _ in range(steps):
+                    g.replay()
+                b.record()
+                torch.cuda.synchronize()
+                self.check()
+                res[(tile, "persistent" if pers else "chain")] = a.elapsed_time(b) / steps
+                self.u.copy_(u0)
+                self.t.copy_(t0)
+                self.steps.copy_(k0)
         best = min(res, key=res.get)
-        self.set_tile(best)
+        self.persistent = best[1] == "persistent"
+        self.set_tile(best[0])
         return res
 
     # ------------------------------------------------------------------ access

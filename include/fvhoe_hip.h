@@ -521,6 +521,16 @@ int fvhoe_ensemble_setup(const FvhoeStageConfig* scheme, int count, const int* n
  * kernels, so it can be captured in a graph. */
 int fvhoe_ensemble_step(const FvhoeStageConfig* scheme, int count, const int* n, int tile, double* u, double* t,
                         int* steps, double target, void* workspace, void* stream);
+/* The same step with each RK stage as one persistent kernel (three launches
+ * per step instead of 22): the grid is as many blocks as are resident at
+ * once, the phases of the stage are separated by grid-wide barriers whose
+ * waits are bounded; a wait that times out sets the error word that
+ * fvhoe_ensemble_status reads (0: no error) instead of hanging.  Results
+ * equal fvhoe_ensemble_step's bit for bit.  Captured in a graph like it. */
+int fvhoe_ensemble_step_persistent(const FvhoeStageConfig* scheme, int count, const int* n, int tile, double* u,
+                                   double* t, int* steps, double target, void* workspace, void* stream);
+int fvhoe_ensemble_status(const FvhoeStageConfig* scheme, int count, const int* n, int tile, const void* workspace,
+                          int* error_word);
 
 #ifdef __cplusplus
 }

@@ -88,7 +88,38 @@ def test_autotune_keeps_results():
     ref.run(T, graph=False)
     e = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=128)
     timings = e.autotune(T, steps=3)
-    assert set(timings) == {64, 128, 256} and e.tile in timings
+    assert {k[0] for k in timings} == {64, 128, 256}
+    assert (e.tile, "persistent" if e.persistent else "chain") in timings
     e.run(T)
     for i in range(len(ns)):
         assert_bits_equal(e.state(i), ref.state(i), f"instance {i}")
+
+
+@pytest.mark.parametrize("tile", [64, 256])
+@pytest.mark.parametrize("bulk", ["fp64", "fp32"])
+def test_persistent_stage_equals_chain(tile, bulk):
+    """One persistent kernel per RK stage (grid barriers between its phases) against the chain of launches,
+    bit for bit, eagerly and as graph replays, past the target time too."""
+    ns = [64, 97, 300, 203, 1200]
+    u0 = _instances(ns, seed=11)
+    T = 0.06
+    a = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=tile, bulk_precision=bulk)
+    b = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=tile, bulk_precision=bulk, persistent=True)
+    c = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=tile, bulk_precision=bulk, persistent=True)
+    sa = a.run(T, graph=False, check_every=4)
+    sb = b.run(T, graph=False, check_every=4)
+    sc = c.run(T, graph=True, check_every=4)
+    print(f"[ensemble persistent tile={tile} {bulk}] steps {sa} {sb} {sc}", flush=True)
+    for i in range(len(ns)):
+        assert_bits_equal(b.state(i), a.state(i), f"eager instance {i}")
+        assert_bits_equal(c.state(i), a.state(i), f"graph instance {i}")
+    assert np.array_equal(a.times(), b.times()) and np.array_equal(a.times(), c.times())
+    assert np.array_equal(a.steps.cpu().numpy(), c.steps.cpu().numpy())
+
+
+def test_autotune_picks_a_stage_form():
+    e = Ensemble(_instances([256, 512, 1024], seed=12), x=(-5, 5), p=5, riemann_solver="hllc")
+    res = e.autotune(0.05, steps=3)
+    assert set(res) == {(t, f) for t in (64, 128, 256) for f in ("chain", "persistent")}
+    best = min(res, key=res.get)
+    assert (e.tile, "persistent" if e.persistent else "chain") == best
