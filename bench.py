@@ -278,8 +278,9 @@ def run_wave3d(a):
 
 
 def run_ensemble(a):
-    """512 Shu-Osher-type 1-D instances, N ~ U{256..8192}, p=5 HLLC a-posteriori, SSP-RK3 to t=1.8,
-    every step one captured graph over all instances; tile size from the autotuner."""
+    """512 Shu-Osher-type 1-D instances, N ~ U{256..8192}, p=5 HLLC a-posteriori, SSP-RK3 to t=1.8, "ic"
+    boundaries, every step one captured graph over all instances; tile size and stage form (chain of
+    launches or one persistent kernel per stage) from the autotuner."""
     import torch
 
     from fvhoe_amd.ensemble import shu_osher_ensemble_solver
@@ -308,7 +309,7 @@ def run_ensemble(a):
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32" if list(forms)[-1] == "fp32" else "f64", "data": "synthetic",
            "config": {"workload": f"shu_osher_ensemble_{a.ensemble_count}_p5_hllc_aposteriori_ssprk3_t{a.ensemble_t}",
-                      "form": list(forms)[-1], "parallelism": "single GPU"},
+                      "boundary": "ic", "form": list(forms)[-1], "parallelism": "single GPU"},
            "forms": forms}
     print(json.dumps(out), flush=True)
 

@@ -74,20 +74,45 @@ struct Ens {
   // detection
   int nad_mode, nad_range, nad_vars, pad_var, sed, force;
   double nad_eps, pad_lo, pad_hi, sed_eps;
+  // ghost values of "dirichlet" / "ic" sides: (count, 5, slab_g) per side, layer k of the lo side is cell
+  // k - slab_g, of the hi side cell n + k
+  const double* slab_lo;
+  const double* slab_hi;
+  int slab_g;
 };
 
-// u of instance i at (possibly ghost) cell c, BC applied (fvhoe/boundary_conditions.py:306-402)
-__device__ __forceinline__ double u_at(const double* __restrict__ u, int n, int v, int c, int lo, int hi) {
+// ghost value of a slab side of instance i, or nothing (returns false) for a cell that no slab covers
+__device__ __forceinline__ bool slab_at(const Ens& e, int i, int n, int v, int c, double* x) {
+  if (c < 0 && e.bc_lo == BC_DIRICHLET) {
+    *x = e.slab_lo[((long long)i * 5 + v) * e.slab_g + (c + e.slab_g)];
+    return true;
+  }
+  if (c >= n && e.bc_hi == BC_DIRICHLET) {
+    *x = e.slab_hi[((long long)i * 5 + v) * e.slab_g + (c - n)];
+    return true;
+  }
+  return false;
+}
+
+// u of instance i at (possibly ghost) cell c, BC applied (fvhoe/boundary_conditions.py:306-402); slab sides
+// give their ghost values directly (fvhoe/boundary_conditions.py:334-348)
+__device__ __forceinline__ double u_at(const Ens& e, int i, const double* __restrict__ u, int n, int v, int c,
+                                       int lo, int hi) {
+  double g;
+  if (slab_at(e, i, n, v, c, &g)) return g;
   bool f;
   const int s = bc_map(c, n, n, lo, hi, &f);
   const double x = u[v * n + s];
   return (f && v == 1) ? -x : x;
 }
 
-// candidate u + dt L(F) at a (possibly ghost) cell (fvhoe/solver.py:678); reflective ghosts negate it whole
+// candidate u + dt L(F) at a (possibly ghost) cell (fvhoe/solver.py:678); reflective ghosts negate it whole;
+// the candidate's ghosts on slab sides are the slab values (the boundary fill of the candidate state)
 template <typename T>
-__device__ __forceinline__ double ustar_at(const double* __restrict__ u, const T* __restrict__ F, int n, double cx,
-                                           double dt, int v, int c, int lo, int hi) {
+__device__ __forceinline__ double ustar_at(const Ens& e, int i, const double* __restrict__ u, const T* __restrict__ F,
+                                           int n, double cx, double dt, int v, int c, int lo, int hi) {
+  double gv;
+  if (slab_at(e, i, n, v, c, &gv)) return gv;
   bool f;
   const int s = bc_map(c, n, n, lo, hi, &f);
   const double L = 0.0 + cx * ((double)F[v * (n + 1) + s + 1] - (double)F[v * (n + 1) + s]);
@@ -126,7 +151,8 @@ __device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double*
     const int c = min(c0 - RC - RT + k, n + span - 1);
 #pragma unroll
     for (int v = 0; v < 5; ++v)
-      SU[v][k] = T(CAND ? ustar_at<T>(ui, Fi, n, cx, dt, v, c, e.bc_lo, e.bc_hi) : u_at(ui, n, v, c, e.bc_lo, e.bc_hi));
+      SU[v][k] = T(CAND ? ustar_at<T>(e, i, ui, Fi, n, cx, dt, v, c, e.bc_lo, e.bc_hi)
+                        : u_at(e, i, ui, n, v, c, e.bc_lo, e.bc_hi));
   }
   __syncthreads();
   for (int k = tid; k < HC; k += TILE) {
@@ -331,13 +357,13 @@ __global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restri
 
 // MUSCL p=1 flux at face fa of instance i from the stage input (fvhoe/solver.py:712-717)
 template <typename T>
-__device__ void ens_fallback_flux(const Ens& e, const double* __restrict__ ui, int n, int fa, T F[5]) {
+__device__ void ens_fallback_flux(const Ens& e, int i, const double* __restrict__ ui, int n, int fa, T F[5]) {
   T w[4][5];
 #pragma unroll
   for (int k = 0; k < 4; ++k) {
     double uu[5], wd[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) uu[q] = 0.0 + u_at(ui, n, q, fa - 2 + k, e.bc_lo, e.bc_hi);
+    for (int q = 0; q < 5; ++q) uu[q] = 0.0 + u_at(e, i, ui, n, q, fa - 2 + k, e.bc_lo, e.bc_hi);
     prims_from_cons(uu, wd, e.gamma);
 #pragma unroll
     for (int q = 0; q < 5; ++q) w[k][q] = T(0.0 + wd[q]);
@@ -371,7 +397,7 @@ __device__ __forceinline__ void d_ens_fallback(const Ens& e, int2 tl, const doub
     if (j >= cnt) continue;
     const int fa = list[e.off_u[i] + j] + (k & 1);
     T Fv[5];
-    ens_fallback_flux<T>(e, ui, n, fa, Fv);
+    ens_fallback_flux<T>(e, i, ui, n, fa, Fv);
 #pragma unroll
     for (int q = 0; q < 5; ++q) Fi[q * (n + 1) + fa] = Fv[q];
   }
@@ -523,7 +549,13 @@ int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan
     return FVHOE_ERR_UNSUPPORTED;
   if (c->fixed_dt_on || c->bulk_fp32 < 0 || c->bulk_fp32 > 1) return FVHOE_ERR_UNSUPPORTED;
   if (c->riemann < 0 || c->riemann > 2 || c->limiter < 0 || c->limiter > 3) return FVHOE_ERR_ARG;
-  if (c->bc[0] < 0 || c->bc[0] > 3 || c->bc[1] < 0 || c->bc[1] > 3) return FVHOE_ERR_UNSUPPORTED;  // no slabs
+  for (int k = 0; k < 2; ++k) {
+    if (c->bc[k] < 0 || c->bc[k] > BC_DIRICHLET) return FVHOE_ERR_ARG;
+    // slab sides: (count, 5, slab_g) ghost values, deep enough for every ghost the stage reads
+    if (c->bc[k] == BC_DIRICHLET &&
+        (!c->slab[k] || c->slab_g < e_gext(c->p[0]) + 2 * e_reach(c->p[0])))
+      return FVHOE_ERR_ARG;
+  }
   P->count = count;
   P->p = c->p[0];
   P->tile = tile;
@@ -607,6 +639,9 @@ Ens make_ens(const FvhoeStageConfig* c, const Plan& P, char* ws) {
   e.pad_lo = c->pad_lo;
   e.pad_hi = c->pad_hi;
   e.sed_eps = c->sed_eps;
+  e.slab_lo = c->bc[0] == BC_DIRICHLET ? c->slab[0] : nullptr;
+  e.slab_hi = c->bc[1] == BC_DIRICHLET ? c->slab[1] : nullptr;
+  e.slab_g = c->slab_g;
   return e;
 }
 

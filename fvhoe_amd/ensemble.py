@@ -21,6 +21,7 @@ import numpy as np
 
 from fvhoe_amd import _lib
 from fvhoe_amd.boundary_conditions import BoundaryCondition
+from fvhoe_amd.fv import fv_uniform_meshgen
 from fvhoe_amd.slope_limiting import nad_mask, pad_spec
 
 TILES = (64, 128, 256)
@@ -34,7 +35,7 @@ class Ensemble:
                  slope_limiter: str = "minmod", NAD: float = 1e-2, NAD_mode: str = "global",
                  NAD_range: str = "relative", NAD_vars: tuple = None, PAD: dict = None, SED: bool = True,
                  SED_tolerance: float = 1e-10, csq_floor: float = 1e-10, bulk_precision: str = "fp64",
-                 tile: int = 128, persistent: bool = False):
+                 tile: int = 128, persistent: bool = False, ic: Sequence = None, slab_buffer_size: int = 30):
         torch = _lib.torch()
         if bulk_precision not in ("fp64", "fp32"):
             raise ValueError(f"bulk_precision must be 'fp64' or 'fp32', got {bulk_precision!r}")
@@ -45,9 +46,7 @@ class Ensemble:
         self.count = len(arrs)
         self.x_domain = x
         self.h = [(x[1] - x[0]) / n for n in self.n]
-        self.bc = BoundaryCondition(x=bc.x) if bc is not None else BoundaryCondition(x="outflow")
-        if self.bc.has_slabs():
-            raise NotImplementedError("the ensemble takes periodic / mirror / free boundaries")
+        self.bc = BoundaryCondition(x=bc.x, x_value=bc.x_value) if bc is not None else BoundaryCondition(x="outflow")
         c = _lib.FvhoeStageConfig()
         c.n[:] = [1, 1, 1]
         c.p[:] = [p, 0, 0]
@@ -68,6 +67,7 @@ class Ensemble:
         c.fixed_dt_on = 0
         c.fixed_dt = 0.0
         c.bulk_fp32 = int(bulk_precision == "fp32")
+        self._slabs = self._make_slabs(c, p, ic, slab_buffer_size)
         self._cfg = c
         self._n_c = (_lib.ctypes.c_int * self.count)(*self.n)
         self._h_c = (_lib.ctypes.c_double * self.count)(*self.h)
@@ -84,6 +84,47 @@ class Ensemble:
         self.set_tile(tile)
 
     # ----------------------------------------------------------------- set-up
+    def _make_slabs(self, c, p: int, ic, buffer: int):
+        """Ghost values of "dirichlet" / "ic" sides (fvhoe/boundary_conditions.py:18-57, 334-348), per
+        instance, as deep as the stage reads: the values at the ghost cell centres of each instance's own
+        slab buffer (fvhoe/fv.py:12-86, the mesh EulerSolver builds), an "ic" side from that instance's
+        pointwise conservative initial condition ic[i](x, y, z), a "dirichlet" side from its constant or
+        callable value at t = 0 (time-dependent values are not supported here)."""
+        kinds = self.bc.x
+        if not any(k in ("dirichlet", "ic") for k in kinds):
+            return None
+        if any(k == "dirichlet" and callable(v) for k, v in zip(kinds, self.bc.x_value)):
+            raise NotImplementedError("the ensemble takes constant Dirichlet values")
+        if "ic" in kinds and (ic is None or len(ic) != self.count):
+            raise ValueError("'ic' boundaries need the pointwise initial condition of every instance (ic=[...])")
+        depth = _lib.ctypes.c_int(0)
+        scheme = _lib.FvhoeStageConfig()
+        scheme.p[:] = [p, 0, 0]
+        _lib.call("fvhoe_stage_ghost_depth", _lib.ctypes.byref(scheme), _lib.ctypes.byref(depth))
+        g = depth.value
+        if g > buffer:
+            raise ValueError(f"Cannot apply bc to region of thickness {g} with a buffer of thickness {buffer}")
+        torch = _lib.torch()
+        out = []
+        for j, pos in enumerate("lr"):
+            if kinds[j] not in ("dirichlet", "ic"):
+                out.append(None)
+                continue
+            vals = np.empty((self.count, 5, g))
+            for i in range(self.count):
+                _, coords = fv_uniform_meshgen((self.n[i], 1, 1), self.x_domain, (0, 1), (0, 1),
+                                               slab_thickness=(buffer, 0, 0))
+                values = list(self.bc.x_value)
+                if kinds[j] == "ic":
+                    values[j] = ic[i]
+                bci = BoundaryCondition(x=kinds, x_value=tuple(values), slab_coords=coords)
+                vals[i] = bci.slab_values((g, 0, 0), 0, pos, 5, 0.0).reshape(5, g)
+            out.append(torch.from_numpy(vals).to(_lib.device()))
+        for k, sl in enumerate(out):
+            c.slab[k] = sl.data_ptr() if sl is not None else None
+        c.slab_g = g
+        return out  # the device arrays the kernels read stay alive with the ensemble
+
     def set_tile(self, tile: int) -> None:
         if tile not in TILES:
             raise ValueError(f"tile must be one of {TILES}")
@@ -205,11 +246,19 @@ class Ensemble:
 
 
 def shu_osher_ensemble_solver(count: int = 512, seed: int = 0, nmin: int = 256, nmax: int = 8192, p: int = 5,
-                              bulk_precision: str = "fp64", tile: int = 128, **kw) -> Ensemble:
+                              bulk_precision: str = "fp64", tile: int = 128, boundary: str = "ic", **kw) -> Ensemble:
     """The Shu-Osher-type ensemble on [-5, 5]: N ~ U{nmin..nmax}, eps ~ U[0.1, 0.3],
-    k ~ U[3, 7] per instance (seeded), outflow boundaries, p=5 HLLC a-posteriori."""
+    k ~ U[3, 7] per instance (seeded), p=5 HLLC a-posteriori.  boundary "ic": the ghost cells take the
+    initial condition at their centres, as the reference runs Shu-Osher (tests/test_boundary_conditions.py:
+    146-224); "outflow": mirror ghosts."""
     from fvhoe_amd import initial_conditions as ics
 
     insts = ics.shu_osher_ensemble(count, seed=seed, nmin=nmin, nmax=nmax)
     u0 = [ics.shu_osher_instance(n, e, k) for n, e, k in insts]
+    if boundary == "ic":
+        point = [ics.ShuOsherInstance(eps=e, k=k).build_ic() for _, e, k in insts]
+        kw["bc"] = BoundaryCondition(x="ic")
+        kw["ic"] = [(lambda f: (lambda x, y, z: ics.primitives_to_conservatives(f(x, y, z), 1.4)))(f) for f in point]
+    elif boundary != "outflow":
+        raise ValueError(f"boundary must be 'ic' or 'outflow', got {boundary!r}")
     return Ensemble(u0, x=(-5.0, 5.0), p=p, bulk_precision=bulk_precision, tile=tile, **kw)

@@ -484,6 +484,20 @@ def four_quadrants(n: int = 1024, seed: int = 0, gamma: float = 1.4) -> np.ndarr
     return primitives_to_conservatives(w, gamma)
 
 
+class ShuOsherInstance(InitialCondition):
+    """The Shu-Osher-type tube of the ensemble workload on [-5, 5] as a pointwise primitive IC: left
+    (x < -4) (3.857143, 2.629369, 10.33333), right rho = 1 + eps sin(k x), u = 0, P = 1."""
+
+    def base_ic(self, eps=0.2, k=5.0):
+        def f(x, y=None, z=None):
+            xp = xp_of(x)
+            left = x < -4
+            return _fill(xp, x.shape, x, xp.where(left, 3.857143, 1 + eps * xp.sin(k * x)),
+                         xp.where(left, 2.629369, 0.0), 0.0, 0.0, xp.where(left, 10.33333, 1.0))
+
+        return f
+
+
 def shu_osher_instance(n: int, eps: float, k: float, gamma: float = 1.4) -> np.ndarray:
     """Shu-Osher-type tube on [-5,5]: left (x<-4) (3.857143, 2.629369, 10.33333);
     right rho = 1 + eps sin(k x), u = 0, P = 1 (cell-centre values)."""

@@ -123,3 +123,35 @@ def test_autotune_picks_a_stage_form():
     assert set(res) == {(t, f) for t in (64, 128, 256) for f in ("chain", "persistent")}
     best = min(res, key=res.get)
     assert (e.tile, "persistent" if e.persistent else "chain") == best
+
+
+@pytest.mark.parametrize("kinds", [("ic", "ic"), ("outflow", "ic"), ("dirichlet", "free")], ids=["ic-ic", "out-ic",
+                                                                                                   "dir-free"])
+def test_slab_boundaries_match_single_runs(kinds):
+    """"ic" and constant "dirichlet" sides (the reference runs Shu-Osher with "ic" boundaries,
+    tests/test_boundary_conditions.py:146-224): every instance equals its single fused run with the same
+    boundaries, bit for bit."""
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+    from fvhoe_amd.solver import EulerSolver
+
+    c = np.array([3.5, 8.0, 0.0, 0.0, 30.0])
+    vals = tuple(c if k == "dirichlet" else None for k in kinds)
+    rng = np.random.default_rng(14)
+    ns = [80, 257, 640]
+    solvers = []
+    for n in ns:
+        w0 = ics.ShuOsherInstance(eps=float(rng.uniform(0.1, 0.3)), k=float(rng.uniform(3, 7)))
+        s = EulerSolver(w0=w0, nx=n, px=5, x=(-5, 5), gamma=1.4, riemann_solver="hllc",
+                        a_posteriori_slope_limiting=True, progress_bar=False,
+                        bc=BoundaryCondition(x=kinds, x_value=vals))
+        assert s.fused
+        solvers.append(s)
+    T = 0.05
+    e = Ensemble([s.u.cpu().numpy() for s in solvers], x=(-5, 5), p=5, riemann_solver="hllc", tile=64,
+                 bc=BoundaryCondition(x=kinds, x_value=vals), ic=[s.u0 for s in solvers])
+    e.run(T, graph=True, check_every=4)
+    for i, s in enumerate(solvers):
+        s.ssprk3(T)
+        print(f"[ensemble {kinds}] instance {i} n={ns[i]} t={s.t!r} steps={s.step_count}", flush=True)
+        assert s.t == e.times()[i]
+        assert_bits_equal(e.state(i), s.u.cpu().numpy().reshape(5, ns[i]), f"instance {i}")

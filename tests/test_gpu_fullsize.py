@@ -91,16 +91,33 @@ def test_density_wave_128_p7_composed_stage():
     _stage_vs_oracle(s, sch, u0, "wave128 p7")
 
 
-def test_ensemble_512_against_single_runs():
-    e = shu_osher_ensemble_solver(512, seed=0)
+@pytest.mark.parametrize("boundary", ["ic", "outflow"])
+def test_ensemble_512_against_single_runs(boundary):
+    """Sampled instances of the 512-run ensemble against single fused runs with the same boundaries: "ic"
+    ghosts (the initial condition at the ghost centres, as the reference runs Shu-Osher) or outflow."""
+    import torch
+
+    from fvhoe_amd.boundary_conditions import BoundaryCondition
+    from fvhoe_amd.solver import EulerSolver
+
+    e = shu_osher_ensemble_solver(512, seed=0, boundary=boundary)
+    insts = ics.shu_osher_ensemble(512, seed=0)
     u0 = [e.state(i).copy() for i in range(e.count)]
     steps = 4
     for _ in range(steps):
         e.step(1e9)
     for i in (0, 17, 100, 255, 400, 511):
         n = int(e.n[i])
-        s = solver_from(u0[i].reshape(5, n, 1, 1), (n, 1, 1), (5, 0, 0), (OUT, P, P), x=(-5, 5),
-                        riemann_solver="hllc", a_posteriori_slope_limiting=True)
+        if boundary == "ic":
+            # the single run's "ic" ghosts come from its pointwise IC; its cells start from the ensemble's
+            # cell-centre values
+            s = EulerSolver(w0=ics.ShuOsherInstance(eps=insts[i][1], k=insts[i][2]), nx=n, px=5, x=(-5, 5),
+                            gamma=1.4, riemann_solver="hllc", a_posteriori_slope_limiting=True, progress_bar=False,
+                            bc=BoundaryCondition(x="ic"))
+            s.u.copy_(torch.from_numpy(u0[i].reshape(s.u.shape)))
+        else:
+            s = solver_from(u0[i].reshape(5, n, 1, 1), (n, 1, 1), (5, 0, 0), (OUT, P, P), x=(-5, 5),
+                            riemann_solver="hllc", a_posteriori_slope_limiting=True)
         assert s.fused
         s.integrator = "ssprk3"
         for _ in range(steps):
