@@ -42,14 +42,15 @@ BYTES_PER_CELL = {
     "prims": 40 + 40,             # read u, write W
     "flux_x": 40 + 40,            # read W, write F
     "flux_y": 40 + 40,            # read W, write G
-    "prims_candidate": 120 + 40,  # read u, F, G, write WS
-    "detect": 80 + 1,             # read W, WS, write flags
+    # 2-D: the candidate tile also runs the troubled-cell test (NAD over W's 3x3, PAD) on its cells
+    "prims_candidate": 120 + 40 + 40 + 1,  # read u, F, G, write WS; read W, write flags
+    "detect": 0,                  # the SED pass over the cells whose NAD test flagged: scales with them only
     "fallback": 0,                # work and traffic scale with the flagged cells only
     "final": 80 + 40 + 80,        # read F, G, write dudt; RK combine reads u0 (+ k) and writes the next state
     "predict": 32 + 8 + 8 + 1,    # features: read W (rho, vx, vy, P), write 4 bf16; network: read them, write flags
 }
 # bytes of the arrays stored as the bulk type (W, F, G, WS) per cell for each kernel, FP64
-BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 120, "detect": 80, "fallback": 0,
+BULK_BYTES = {"prims": 40, "flux_x": 80, "flux_y": 80, "prims_candidate": 160, "detect": 0, "fallback": 0,
               "final": 80, "predict": 32}
 STAGE_BYTES_APOST_FP64 = 40 * (5 / 3 + 1) + 120  # 226.7 B per cell-stage
 BF16_PEAK_TFLOPS = 2500.0

@@ -11,9 +11,11 @@
 //   flux<D>: W -> c-sweep across the face, l/r sweeps along it, Riemann,
 //            t-sweep across it = F (D=0) or G (D=1) (fvhoe/solver.py:519-649).
 //   prims again on u* = u + dt*L(F,G), formed on load -> WS (candidate
-//            primitive averages, fvhoe/solver.py:678-687).
+//            primitive averages, fvhoe/solver.py:678-687); in 2-D the same
+//            tile runs the NAD/PAD tests of its cells from W and WS.
 //   detect : NAD/PAD/SED flags from W and WS (fvhoe/slope_limiting.py:87-244)
-//            and a ballot compaction of the flagged cells.
+//            and a ballot compaction of the flagged cells; in 2-D only for the
+//            cells whose NAD test flagged (the SED reads WS beyond the tile).
 //   fallback: MUSCL p=1 fluxes at the faces of flagged cells, recomputed
 //            locally from the stage input and written over F/G
 //            (fvhoe/solver.py:710-731).
@@ -133,6 +135,10 @@ constexpr int TPB = 256;
 #endif
 #ifndef FVHOE_FLUX_NT_F32
 #define FVHOE_FLUX_NT_F32 512
+#endif
+// 2-D: the troubled-cell test inside the candidate tile (1) or its own pass over W and WS (0)
+#ifndef FVHOE_CAND_CHECK
+#define FVHOE_CAND_CHECK 1
 #endif
 #ifndef FVHOE_DETECT_ROWS
 #define FVHOE_DETECT_ROWS 2
@@ -322,11 +328,117 @@ __device__ __forceinline__ unsigned long long red_op(int k, unsigned long long a
 }
 
 // ---------------------------------------------------------------------------
+// troubled-cell tests (fvhoe/slope_limiting.py:87-244), in FP64
+// ---------------------------------------------------------------------------
+struct DetectP {
+  int nad_mode, nad_range, nad_vars, pad_var, sed, force;
+  double eps, pad_lo, pad_hi, sed_eps;
+};
+
+// Violation magnitudes for the trouble log (fvhoe/slope_limiting.py:225-239): every cell whose NAD or PAD
+// test flags is appended to this stage's slot as (cell, NAD magnitude, PAD magnitude).  NAD magnitude:
+// -min over all five variables of the NAD indicator where NAD flags (after SED and the NAD_vars mask),
+// else 0; PAD magnitude: -(PAD indicator of the last PAD key) where PAD flags, else 0.
+struct MagLog {
+  int* count;          // entries in this stage's slot (cleared by k_stage_reduce)
+  int* idx;
+  double* nad;
+  double* pad;
+  const double* gate;  // the step's dt: a step at its target time (dt = 0) appends nothing
+};
+__device__ __forceinline__ bool mag_on(const MagLog& m) { return m.idx && !(m.gate && m.gate[0] == 0.0); }
+
+// append the lanes' flagged cells to the slot: one atomic per wave
+__device__ __forceinline__ void mag_append(const MagLog& m, bool f, int cell, double nad, double pad) {
+  const unsigned long long b = __ballot(f);
+  if (b == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0) base = atomicAdd(m.count, (int)__popcll(b));
+  base = __shfl(base, 0, 64);
+  if (f) {
+    const int e = base + (int)__popcll(b & ((1ull << lane) - 1ull));
+    m.idx[e] = cell;
+    m.nad[e] = nad;
+    m.pad[e] = pad;
+  }
+}
+
+// The troubled-cell test fused into the 2-D candidate tile: the tile's candidate values (WS) are still in
+// registers and the stage's W over the tile's cells plus one ring comes through LDS.  A cell whose NAD
+// test flags while the smooth-extrema detector is on goes to a pending list for k_detect_list (its SED
+// reads candidate values up to 3 cells beyond the tile); every other cell is decided in the tile: flag,
+// fallback list, pre-flag count and the trouble log's magnitudes, with k_detect_strip's operations.
+template <typename T>
+struct CheckArgs {
+  const T* __restrict__ W;
+  int NW0, NW1, wg0, wg1;  // W's extents and ghost widths
+  const double* __restrict__ range;
+  DetectP dp;
+  signed char* __restrict__ trouble;
+  int* __restrict__ count;       // cells in the fallback list
+  int* __restrict__ list;
+  int* __restrict__ pend_count;  // cells left to k_detect_list
+  int* __restrict__ pend;
+  const signed char* __restrict__ pre;
+  int* __restrict__ fp_count;
+  MagLog ml;
+};
+
+// Append up to K cells per thread to two lists (bit k of sel0 / sel1 selects cell[k]): wave ballots, a
+// block prefix over the waves and one atomic per list and block.  Every thread of the block calls it.
+template <int K, int NT>
+__device__ __forceinline__ void block_append2(const int (&cell)[K], unsigned sel0, unsigned sel1, int* count0,
+                                              int* list0, int* count1, int* list1) {
+  __shared__ int wsum[NT / 64][2];
+  __shared__ int bbase[2];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  int o0[K], o1[K], n0 = 0, n1 = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const unsigned long long b0 = __ballot((sel0 >> k) & 1u), b1 = __ballot((sel1 >> k) & 1u);
+    o0[k] = n0 + (int)__popcll(b0 & lt);
+    o1[k] = n1 + (int)__popcll(b1 & lt);
+    n0 += (int)__popcll(b0);
+    n1 += (int)__popcll(b1);
+  }
+  if (lane == 0) {
+    wsum[wave][0] = n0;
+    wsum[wave][1] = n1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int a0 = 0, a1 = 0;
+    for (int w = 0; w < NT / 64; ++w) {
+      const int x0 = wsum[w][0], x1 = wsum[w][1];
+      wsum[w][0] = a0;
+      wsum[w][1] = a1;
+      a0 += x0;
+      a1 += x1;
+    }
+    bbase[0] = a0 ? atomicAdd(count0, a0) : 0;
+    bbase[1] = a1 ? atomicAdd(count1, a1) : 0;
+  }
+  __syncthreads();
+  const int p0 = bbase[0] + wsum[wave][0], p1 = bbase[1] + wsum[wave][1];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if ((sel0 >> k) & 1u) list0[p0 + o0[k]] = cell[k];
+    if ((sel1 >> k) & 1u) list1[p1 + o1[k]] = cell[k];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // prims tile: W[a][b] for W indices a in [oa, oa+TX), b in [ob, ob+TY).
 // ---------------------------------------------------------------------------
-template <typename T, int PX, int PY, int TX, int TY, typename Src, int NT>
-__global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W, int NW0, int NW1, int tiles1,
-                                             Reduce red, double gamma) {
+// The fused check's FP64 tiles ask for 6 waves per SIMD (three 512-thread tiles per CU, <= 80 VGPRs).
+#ifndef FVHOE_CHECK_WAVES_F64
+#define FVHOE_CHECK_WAVES_F64 6
+#endif
+template <typename T, int PX, int PY, int TX, int TY, typename Src, int NT, bool CHECK = false>
+__global__ void __launch_bounds__(NT, CHECK && sizeof(T) == 8 ? FVHOE_CHECK_WAVES_F64 : 1) k_prims(Src src, Geo g, T* __restrict__ W, int NW0, int NW1, int tiles1,
+                                             Reduce red, double gamma, CheckArgs<T> ck) {
   constexpr int RCX = ereach_c(PX), RTX = ereach_t(PX), RCY = ereach_c(PY), RTY = ereach_t(PY);
   constexpr int HX0 = TX + 2 * (RCX + RTX), HY0 = TY + 2 * (RCY + RTY);
   constexpr int HX1 = TX + 2 * RTX, HY1 = TY + 2 * RTY;
@@ -424,6 +536,30 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
       }
     }
   };
+  // the check's W box: tile cell (ti, tj) is interior cell (oa + ti - ge0, ob + tj - ge1) of the candidate
+  // grid; box element (r, c) is W's (oa - 1 - ge0 + wg0 + r, ob - 1 - ge1 + wg1 + c), clamped into W (the
+  // clamped elements belong to no interior cell).  Fetched after the c-sweeps, into the registers the
+  // tile's own fetches held, so the loads overlap the t-sweeps.
+  // FP32 prefetches all five variables' boxes; FP64 only the first (all five would hold 30 more VGPRs
+  // through the t-sweeps and cost occupancy), the others are fetched one variable ahead in the check
+  constexpr int BX = TX + 2, BY = TY + 2, NBE = (BX * BY + NT - 1) / NT;
+  constexpr int NPF = sizeof(T) == 8 ? 1 : 5;
+  T wbx[CHECK ? NPF : 1][CHECK ? NBE : 1];
+  auto fetch_box = [&](int v, T (&dst)[CHECK ? NBE : 1]) {
+    if constexpr (CHECK) {
+      const int r0 = oa - 1 - g.ge0 + ck.wg0, c0 = ob - 1 - g.ge1 + ck.wg1;
+      tile_for_k<BX, BY, NT>([&](int k, int e, int r, int c) {
+        const int wa = min(max(r0 + r, 0), ck.NW0 - 1), wb = min(max(c0 + c, 0), ck.NW1 - 1);
+        dst[k] = ck.W[(v * ck.NW0 + wa) * ck.NW1 + wb];
+      });
+    }
+  };
+  auto prefetch_box = [&]() {
+    if constexpr (CHECK) {
+#pragma unroll
+      for (int v = 0; v < NPF; ++v) fetch_box(v, wbx[v]);
+    }
+  };
 #if FVHOE_TILE_PIPE
   // Two buffers per sweep stage: variable v's second sweep and variable v+1's first sweep share a
   // barrier interval, one barrier per variable.  Each step writes a buffer whose last reader ran
@@ -437,6 +573,7 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
     __syncthreads();
     csweep_y(v, v & 1);
   }
+  prefetch_box();
   recover();
   put_cen(0, 0);
   __syncthreads();
@@ -457,6 +594,7 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
     csweep_y(v, 0);
     __syncthreads();
   }
+  prefetch_box();
   recover();
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -469,6 +607,120 @@ __global__ void __launch_bounds__(NT) k_prims(Src src, Geo g, T* __restrict__ W,
     // before S1 is written again
   }
 #endif
+  if constexpr (CHECK) {
+    // two W box buffers: S0's two buffers where they are large enough (free after the last t-sweep
+    // barrier), else LDS of their own
+    constexpr bool REUSE = NB == 2 && BX * BY <= HX0 * HY0;
+    __shared__ T SBx[REUSE ? 1 : 2][REUSE ? 1 : BX * BY];
+    T* const SB0 = REUSE ? &S0[0][0] : &SBx[0][0];
+    T* const SB1 = REUSE ? &S0[NB - 1][0] : &SBx[REUSE ? 0 : 1][0];
+    auto put_box = [&](const T (&src)[NBE], T* b) {
+      tile_for_k<BX, BY, NT>([&](int k, int e, int r, int c) { b[e] = src[k]; });
+    };
+    const DetectP& a = ck.dp;
+    unsigned nadb[NPT];
+    double imin[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+      nadb[q] = 0u;
+      imin[q] = 0.0;
+    }
+    put_box(wbx[0], SB0);
+    __syncthreads();
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      T nxt[NBE];  // the next variable's box, fetched before this variable's tests
+      if (v + 1 < 5 && v + 1 >= NPF) fetch_box(v + 1, nxt);
+      const T* b = (v & 1) ? SB1 : SB0;
+      const double rlo = ck.range[2 * v], rhi = ck.range[2 * v + 1];
+#pragma unroll
+      for (int q = 0; q < NPT; ++q) {
+        const int e = tid + q * NT;
+        if (e < TX * TY) {
+          const int ti = e / TY, tj = e % TY;
+          double hM[3], hm[3];
+#pragma unroll
+          for (int r = 0; r < 3; ++r) {  // rows -1, 0, 1 of the cell, each over columns -1, 0, 1
+            const T* row = b + (ti + r) * BY + tj;
+            const double x0 = (double)row[0], x1 = (double)row[1], x2 = (double)row[2];
+            hM[r] = dmax(dmax(x0, x1), x2);
+            hm[r] = dmin(dmin(x0, x1), x2);
+          }
+          const double M = dmax(dmax(hM[0], hM[1]), hM[2]);
+          const double m = dmin(dmin(hm[0], hm[1]), hm[2]);
+          const double uc = (double)wr[v][q];
+          bool tv;
+          double ind;
+          if (a.nad_range == 0) {
+            const double rng = a.nad_mode == 0 ? rhi - rlo : M - m;
+            ind = dmin(uc - m, M - uc);
+            tv = ind < -a.eps * rng;
+          } else {
+            const double lo = a.nad_mode == 0 ? (1.0 - a.eps) * rlo : (1.0 - a.eps) * m;
+            const double hi = a.nad_mode == 0 ? (1.0 + a.eps) * rhi : (1.0 + a.eps) * M;
+            ind = dmin(uc - lo, hi - uc);
+            tv = ind < 0.0;
+          }
+          if (tv && ((a.nad_vars >> v) & 1)) nadb[q] |= 1u << v;
+          imin[q] = v == 0 ? ind : dmin(imin[q], ind);
+        }
+      }
+      if (v + 1 < 5) put_box(v + 1 < NPF ? wbx[v + 1 < NPF ? v + 1 : 0] : nxt, ((v + 1) & 1) ? SB1 : SB0);
+      __syncthreads();
+    }
+    int cell[NPT];
+    unsigned lsel = 0u, psel = 0u, msel = 0u, fpbits = 0u;
+    double mnad[NPT], mpad[NPT];
+#pragma unroll
+    for (int q = 0; q < NPT; ++q) {
+      const int e = tid + q * NT;
+      const int ti = e / TY, tj = e % TY;
+      const int i = oa + ti - g.ge0, j = ob + tj - g.ge1;
+      cell[q] = i * g.n1 + j;
+      mnad[q] = 0.0;
+      mpad[q] = 0.0;
+      if (e >= TX * TY || i < 0 || i >= g.n0 || j < 0 || j >= g.n1) continue;
+      if (a.sed && nadb[q]) {  // the SED decides: k_detect_list
+        psel |= 1u << q;
+        continue;
+      }
+      const bool nad = nadb[q] != 0u;
+      bool pad = false;
+      double pind = 0.0;
+      if (a.pad_var >= 0) {
+        double x = 0.0;
+#pragma unroll
+        for (int v = 0; v < 5; ++v)
+          if (v == a.pad_var) x = (double)wr[v][q];
+        pind = dmin(x - a.pad_lo, a.pad_hi - x);
+        pad = pind < 0.0;
+      }
+      if (nad || pad) msel |= 1u << q;
+      mnad[q] = nad ? -imin[q] : 0.0;
+      mpad[q] = pad ? -pind : 0.0;
+      int t = (a.force || pad || nad) ? 1 : 0;
+      if (ck.pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
+        const int pf = ck.pre[cell[q]];
+        ck.trouble[cell[q]] = (signed char)(t | pf);
+        if (pf && !t) fpbits |= 1u << q;
+        t = t && !pf;
+      } else {
+        ck.trouble[cell[q]] = (signed char)t;
+      }
+      if (t) lsel |= 1u << q;
+    }
+    if (ck.pre) {
+      int f = __popc(fpbits);
+      for (int off = 32; off > 0; off >>= 1) f += __shfl_down(f, off, 64);
+      if ((tid & 63) == 0 && f) atomicAdd(ck.fp_count, f);
+    }
+    block_append2<NPT, NT>(cell, lsel, psel, ck.count, ck.list, ck.pend_count, ck.pend);
+    if (mag_on(ck.ml)) {
+#pragma unroll
+      for (int q = 0; q < NPT; ++q) mag_append(ck.ml, (msel >> q) & 1u, cell[q], mnad[q], mpad[q]);
+    }
+    return;
+  }
   if (!red.partials) return;
   // partial reductions over this tile's cells of the dt box and the range box
   double m = 0.0;
@@ -792,38 +1044,74 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
 // ---------------------------------------------------------------------------
 // troubled-cell flags from W (3^d max/min) and WS (candidate, SED), in FP64
 // ---------------------------------------------------------------------------
-struct DetectP {
-  int nad_mode, nad_range, nad_vars, pad_var, sed, force;
-  double eps, pad_lo, pad_hi, sed_eps;
-};
-
-// Violation magnitudes for the trouble log (fvhoe/slope_limiting.py:225-239): every cell whose NAD or PAD
-// test flags is appended to this stage's slot as (cell, NAD magnitude, PAD magnitude).  NAD magnitude:
-// -min over all five variables of the NAD indicator where NAD flags (after SED and the NAD_vars mask),
-// else 0; PAD magnitude: -(PAD indicator of the last PAD key) where PAD flags, else 0.
-struct MagLog {
-  int* count;          // entries in this stage's slot (cleared by k_stage_reduce)
-  int* idx;
-  double* nad;
-  double* pad;
-  const double* gate;  // the step's dt: a step at its target time (dt = 0) appends nothing
-};
-__device__ __forceinline__ bool mag_on(const MagLog& m) { return m.idx && !(m.gate && m.gate[0] == 0.0); }
-
-// append the lanes' flagged cells to the slot: one atomic per wave
-__device__ __forceinline__ void mag_append(const MagLog& m, bool f, int cell, double nad, double pad) {
-  const unsigned long long b = __ballot(f);
-  if (b == 0ull) return;
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (lane == 0) base = atomicAdd(m.count, (int)__popcll(b));
-  base = __shfl(base, 0, 64);
-  if (f) {
-    const int e = base + (int)__popcll(b & ((1ull << lane) - 1ull));
-    m.idx[e] = cell;
-    m.nad[e] = nad;
-    m.pad[e] = pad;
+// The tests of one interior cell c: t = flag before pre-flags, mflag / mnad / mpad for the trouble log.
+template <typename T>
+__device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int NWC, const T* __restrict__ WS,
+                                           int NS1, int NSC, const Geo& g, bool two_d,
+                                           const double* __restrict__ range_mm, const DetectP& a, int c,
+                                           bool* mflag, double* mnad, double* mpad) {
+  const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
+  const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
+  const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
+  bool tvv[5];
+  double imin = 0.0;  // min over the variables of the NAD indicator
+#pragma unroll
+  for (int v = 0; v < 5; ++v) {
+    const T* wv = W + v * NWC;
+    double M = (double)wv[wb], m = M;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di) {
+      if (two_d) {
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+          const double x = (double)wv[wb + di * NW1 + dj];
+          M = dmax(M, x);
+          m = dmin(m, x);
+        }
+      } else {
+        const double x = (double)wv[wb + di * NW1];
+        M = dmax(M, x);
+        m = dmin(m, x);
+      }
+    }
+    const double uc = (double)WS[v * NSC + sb];
+    double ind;
+    bool tv;
+    if (a.nad_range == 0) {
+      const double rng = a.nad_mode == 0 ? range_mm[2 * v + 1] - range_mm[2 * v] : M - m;
+      ind = dmin(uc - m, M - uc);
+      tv = ind < -a.eps * rng;
+    } else {
+      const double lo = a.nad_mode == 0 ? (1.0 - a.eps) * range_mm[2 * v] : (1.0 - a.eps) * m;
+      const double hi = a.nad_mode == 0 ? (1.0 + a.eps) * range_mm[2 * v + 1] : (1.0 + a.eps) * M;
+      ind = dmin(uc - lo, hi - uc);
+      tv = ind < 0.0;
+    }
+    tvv[v] = tv && ((a.nad_vars >> v) & 1);
+    imin = v == 0 ? ind : dmin(imin, ind);
   }
+  bool nad = false;
+  for (int v = 0; v < 5; ++v) {
+    bool tv = tvv[v];
+    if (tv && a.sed && !nad) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
+      const T* sv = WS + v * NSC;
+      double alpha = sed_alpha_1d([&](int o) { return (double)sv[sb + o * NS1]; }, a.sed_eps);
+      if (two_d) alpha = dmin(alpha, sed_alpha_1d([&](int o) { return (double)sv[sb + o]; }, a.sed_eps));
+      tv = alpha < 1.0;
+    }
+    nad = nad || tv;
+  }
+  bool pad = false;
+  double pind = 0.0;
+  if (a.pad_var >= 0) {
+    const double x = (double)WS[a.pad_var * NSC + sb];
+    pind = dmin(x - a.pad_lo, a.pad_hi - x);
+    pad = pind < 0.0;
+  }
+  *mflag = nad || pad;
+  *mnad = nad ? -imin : 0.0;
+  *mpad = pad ? -pind : 0.0;
+  return (a.force || pad || nad) ? 1 : 0;
 }
 
 template <typename T>
@@ -834,75 +1122,12 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
                                                int* __restrict__ list, const signed char* __restrict__ pre,
                                                int* __restrict__ fp_count, MagLog ml) {
   const int nc = g.n0 * g.n1;
-  const bool two_d = g.ge1 > 0;
   const int c = blockIdx.x * TPB + threadIdx.x;
   int t = 0, fp = 0;
   bool mflag = false;
   double mnad = 0.0, mpad = 0.0;
   if (c < nc) {
-    const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
-    const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
-    const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
-    const int NWC = NW0 * NW1, NSC = NS0 * NS1;
-    bool tvv[5];
-    double imin = 0.0;  // min over the variables of the NAD indicator
-#pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const T* wv = W + v * NWC;
-      double M = (double)wv[wb], m = M;
-#pragma unroll
-      for (int di = -1; di <= 1; ++di) {
-        if (two_d) {
-#pragma unroll
-          for (int dj = -1; dj <= 1; ++dj) {
-            const double x = (double)wv[wb + di * NW1 + dj];
-            M = dmax(M, x);
-            m = dmin(m, x);
-          }
-        } else {
-          const double x = (double)wv[wb + di * NW1];
-          M = dmax(M, x);
-          m = dmin(m, x);
-        }
-      }
-      const double uc = (double)WS[v * NSC + sb];
-      double ind;
-      bool tv;
-      if (a.nad_range == 0) {
-        const double rng = a.nad_mode == 0 ? range_mm[2 * v + 1] - range_mm[2 * v] : M - m;
-        ind = dmin(uc - m, M - uc);
-        tv = ind < -a.eps * rng;
-      } else {
-        const double lo = a.nad_mode == 0 ? (1.0 - a.eps) * range_mm[2 * v] : (1.0 - a.eps) * m;
-        const double hi = a.nad_mode == 0 ? (1.0 + a.eps) * range_mm[2 * v + 1] : (1.0 + a.eps) * M;
-        ind = dmin(uc - lo, hi - uc);
-        tv = ind < 0.0;
-      }
-      tvv[v] = tv && ((a.nad_vars >> v) & 1);
-      imin = v == 0 ? ind : dmin(imin, ind);
-    }
-    bool nad = false;
-    for (int v = 0; v < 5; ++v) {
-      bool tv = tvv[v];
-      if (tv && a.sed && !nad) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
-        const T* sv = WS + v * NSC;
-        double alpha = sed_alpha_1d([&](int o) { return (double)sv[sb + o * NS1]; }, a.sed_eps);
-        if (two_d) alpha = dmin(alpha, sed_alpha_1d([&](int o) { return (double)sv[sb + o]; }, a.sed_eps));
-        tv = alpha < 1.0;
-      }
-      nad = nad || tv;
-    }
-    bool pad = false;
-    double pind = 0.0;
-    if (a.pad_var >= 0) {
-      const double x = (double)WS[a.pad_var * NSC + sb];
-      pind = dmin(x - a.pad_lo, a.pad_hi - x);
-      pad = pind < 0.0;
-    }
-    mflag = nad || pad;
-    mnad = nad ? -imin : 0.0;
-    mpad = pad ? -pind : 0.0;
-    t = (a.force || pad || nad) ? 1 : 0;
+    t = detect_cell(W, NW1, NW0 * NW1, WS, NS1, NS0 * NS1, g, g.ge1 > 0, range_mm, a, c, &mflag, &mnad, &mpad);
     if (pre) {  // pre-flagged cells already carry MUSCL fluxes: list only the others
       const int pf = pre[c];
       trouble[c] = (signed char)(t | pf);
@@ -916,25 +1141,47 @@ __global__ void __launch_bounds__(TPB) k_detect(const T* __restrict__ W, int NW0
     const unsigned long long bf = __ballot(fp);
     if ((threadIdx.x & 63) == 0 && bf) atomicAdd(fp_count, (int)__popcll(bf));
   }
-  // compaction of the flagged cells: wave ballot, block prefix, one atomic per block
-  __shared__ int wbase[TPB / 64];
-  __shared__ int bbase;
-  const unsigned long long ball = __ballot(t);
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  if (lane == 0) wbase[wave] = __popcll(ball);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int tot = 0;
-    for (int w = 0; w < TPB / 64; ++w) {
-      const int x = wbase[w];
-      wbase[w] = tot;
-      tot += x;
-    }
-    bbase = tot ? atomicAdd(count, tot) : 0;
-  }
-  __syncthreads();
-  if (t) list[bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
+  const int cell[1] = {c};
+  block_append2<1, TPB>(cell, (unsigned)t, 0u, count, list, count, list);
   if (mag_on(ml)) mag_append(ml, mflag, c, mnad, mpad);
+}
+
+// The cells the fused candidate check left pending (NAD flagged with the SED on), decided as k_detect
+// decides them.  The block loop's trip count is uniform over the block (block_append2 has barriers).
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_detect_list(const int* __restrict__ pend_count, const int* __restrict__ pend,
+                                                    const T* __restrict__ W, int NW0, int NW1,
+                                                    const T* __restrict__ WS, int NS0, int NS1, Geo g,
+                                                    const double* __restrict__ range_mm, DetectP a,
+                                                    signed char* __restrict__ trouble, int* __restrict__ count,
+                                                    int* __restrict__ list, const signed char* __restrict__ pre,
+                                                    int* __restrict__ fp_count, MagLog ml) {
+  const int n = pend_count[0];
+  for (int base = blockIdx.x * TPB; base < n; base += gridDim.x * TPB) {
+    const int k = base + (int)threadIdx.x;
+    int t = 0, fp = 0, c = 0;
+    bool mflag = false;
+    double mnad = 0.0, mpad = 0.0;
+    if (k < n) {
+      c = pend[k];
+      t = detect_cell(W, NW1, NW0 * NW1, WS, NS1, NS0 * NS1, g, g.ge1 > 0, range_mm, a, c, &mflag, &mnad, &mpad);
+      if (pre) {
+        const int pf = pre[c];
+        trouble[c] = (signed char)(t | pf);
+        fp = pf && !t;
+        t = t && !pf;
+      } else {
+        trouble[c] = (signed char)t;
+      }
+    }
+    if (pre) {
+      const unsigned long long bf = __ballot(fp);
+      if ((threadIdx.x & 63) == 0 && bf) atomicAdd(fp_count, (int)__popcll(bf));
+    }
+    const int cell[1] = {c};
+    block_append2<1, TPB>(cell, (unsigned)t, 0u, count, list, count, list);
+    if (mag_on(ml)) mag_append(ml, mflag, c, mnad, mpad);
+  }
 }
 
 // 2-D detection over strips of R rows (x) per thread, one column (y) per
@@ -1265,6 +1512,7 @@ __global__ void __launch_bounds__(1024) k_stage_reduce(const unsigned long long*
   if (k == 0 && tid == 0) {  // this stage's flag counters (the previous stage's consumers ran before)
     count[0] = 0;
     count[2] = 0;
+    count[3] = 0;
     if (reset_pre) count[1] = 0;
   }
   unsigned long long x = k == 0 ? 0ull : ((k & 1) ? ~0ull : 0ull);
@@ -1324,7 +1572,7 @@ struct TimedLaunch {
 
 struct Layout {
   int two_d, p, fp32, G, GE, n0, n1, NW0, NW1, NS0, NS1, prims_blocks, pred, dirichlet;
-  size_t offW, offF, offG, offWS, offFlags, offList, offPart, offScal, offPre, offPreList, offFeat, total;
+  size_t offW, offF, offG, offWS, offFlags, offList, offPend, offPart, offScal, offPre, offPreList, offFeat, total;
 };
 
 size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1397,6 +1645,8 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offWS = o; o = align256(o + (c->a_posteriori ? esz * 5 * (size_t)L->NS0 * L->NS1 : 0));
   L->offFlags = o; o = align256(o + (c->a_posteriori ? (size_t)L->n0 * L->n1 : 0));
   L->offList = o; o = align256(o + (c->a_posteriori ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
+  // the cells the fused 2-D candidate check leaves to the SED pass
+  L->offPend = o; o = align256(o + (FVHOE_CAND_CHECK && ay && c->a_posteriori && c->sed ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
   if (L->two_d && (L->fp32 ? use_strip<float>(L->p) : use_strip<double>(L->p))) {
     L->prims_blocks = strip_blocks(L->p, L->fp32 ? strip_rows<float>() : strip_rows<double>(), L->NW0, L->NW1);
   } else if (L->two_d) {
@@ -1493,7 +1743,8 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
           make_src<SU>(u, sl), g, W, L.NW0, L.NW1, strip_cwins(P, L.NW1), red, c->gamma);
     } else {
       const int t0 = (L.NW0 + TX - 1) / TX, t1 = (L.NW1 + TY - 1) / TY;
-      k_prims<T, P, PY, TX, TY, SU, NTILE><<<t0 * t1, NTILE, 0, s>>>(make_src<SU>(u, sl), g, W, L.NW0, L.NW1, t1, red, c->gamma);
+      k_prims<T, P, PY, TX, TY, SU, NTILE><<<t0 * t1, NTILE, 0, s>>>(make_src<SU>(u, sl), g, W, L.NW0, L.NW1, t1, red,
+                                                                     c->gamma, CheckArgs<T>{});
     }
   }
   const double hmin = c->h[0] < c->h[1] ? (c->h[0] < c->h[2] ? c->h[0] : c->h[2]) : (c->h[1] < c->h[2] ? c->h[1] : c->h[2]);
@@ -1577,25 +1828,36 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
     gs.ge1 = TWO_D ? 3 : 0;
     const int t0 = (L.NS0 + CX - 1) / CX, t1 = (L.NS1 + CY - 1) / CY;
     Reduce none{};
+    DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
+               c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
+    constexpr bool FUSED_CHECK = TWO_D && FVHOE_CAND_CHECK;
+    int* pend = (int*)(ws + L.offPend);
     {
       TimedLaunch tl(TK_PRIMS_CAND, s);
       // the LDS tiles form each candidate value once per tile (the strips would form it (R + 2H) / R times)
-      k_prims<T, P, PY, CX, CY, SUS, NCAND><<<t0 * t1, NCAND, 0, s>>>(
-          make_srcstar<SUS>(u, F, G, dt_stage, sl), gs, WS, L.NS0, L.NS1, t1, none, c->gamma);
+      CheckArgs<T> ck{};
+      if (FUSED_CHECK)
+        ck = CheckArgs<T>{W, L.NW0, L.NW1, g.ge0, g.ge1, range, dp, flags, count, list, count + 3, pend,
+                          preflags, count + 2, ml};
+      k_prims<T, P, PY, CX, CY, SUS, NCAND, FUSED_CHECK><<<t0 * t1, NCAND, 0, s>>>(
+          make_srcstar<SUS>(u, F, G, dt_stage, sl), gs, WS, L.NS0, L.NS1, t1, none, c->gamma, ck);
     }
-    DetectP dp{c->nad_mode, c->nad_range, c->nad_vars, c->pad_var, c->sed, c->force_trouble,
-               c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
     const int gridd = (nc + TPB - 1) / TPB;
     constexpr int DR = FVHOE_DETECT_ROWS;  // rows per thread of the 2-D detection
     const int jblocks = (L.n1 + TPB - 1) / TPB, gridd2 = jblocks * ((L.n0 + DR - 1) / DR);
     {
       TimedLaunch tl(TK_DETECT, s);
-      if (TWO_D)
+      if (FUSED_CHECK) {
+        if (c->sed)  // the cells whose NAD test flagged: the SED reads WS beyond the tiles
+          k_detect_list<T><<<512, TPB, 0, s>>>(count + 3, pend, W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp,
+                                               flags, count, list, preflags, count + 2, ml);
+      } else if (TWO_D) {
         k_detect_strip<T, DR><<<gridd2, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count,
                                                       list, preflags, count + 2, jblocks, ml);
-      else
+      } else {
         k_detect<T><<<gridd, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count, list,
                                         preflags, count + 2, ml);
+      }
     }
     {
       TimedLaunch tl(TK_FALLBACK, s);

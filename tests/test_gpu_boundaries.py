@@ -174,12 +174,16 @@ def test_ic_equals_constant_dirichlet_shu_osher(dim, p, N=100, gamma=1.4, T=1.8)
     assert O.l1err(Pd, Pi) < 1e-14
 
 
-def test_slab_boundaries_refuse_the_predictor_and_the_ensemble():
+def test_slab_boundaries_refuse_the_predictor_and_the_ensemble_checks_its_inputs():
+    """The predictor's features map ghosts into the interior, so slab sides refuse it; the ensemble takes
+    "ic" sides only with every instance's pointwise IC, and constant Dirichlet values only."""
     from fvhoe_amd.ensemble import Ensemble
     from fvhoe_amd.predictor import TroubledCellPredictor
 
     bc = BoundaryCondition(x="ic", y="outflow")
     with pytest.raises(NotImplementedError):
         _solver((32, 32, 1), (3, 3, 0), bc, predictor=TroubledCellPredictor.load())
-    with pytest.raises(NotImplementedError):
+    with pytest.raises(ValueError, match="pointwise initial condition"):
         Ensemble([np.ones((5, 32))], bc=BoundaryCondition(x="ic"))
+    with pytest.raises(NotImplementedError, match="constant Dirichlet"):
+        Ensemble([np.ones((5, 32))], bc=BoundaryCondition(x="dirichlet", x_value=(lambda x, y, z, t: x, None)))
