@@ -6,6 +6,7 @@
 // reference bit for bit.  Division and sqrt are the compiler's IEEE
 // correctly rounded FP64 operations.
 #pragma once
+#include <type_traits>
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -134,17 +135,38 @@ __device__ __forceinline__ void prims_from_cons(const T u[5], T w[5], T gamma) {
   w[4] = (gamma - T(1)) * (u[4] - T(0.5) * msum);
 }
 
-// total energy of compute_conservatives: P/(gamma-1) + 0.5*sum(m*v), m = rho*v
-template <typename T>
-__device__ __forceinline__ T energy(const T w[5], T gamma) {
-  const T mx = w[0] * w[1], my = w[0] * w[2], mz = w[0] * w[3];
-  return w[4] / (gamma - T(1)) + T(0.5) * (((mx * w[1]) + (my * w[2])) + (mz * w[3]));
+// Arithmetic of the Riemann solvers.  FAST (the fused stage's FP32 bulk form only): float quotients as a
+// product with the hardware reciprocal (v_rcp_f32, 1 ulp) and square roots as v_sqrt_f32 (1 ulp) instead
+// of the correctly rounded multi-instruction sequences, and fused multiply-adds in the products' sums.
+// Every FP64 path, and the composed FP32 form that reproduces the oracle's float32 arithmetic, keeps
+// FAST = false: IEEE quotients and square roots, no contraction (the library's -ffp-contract=off).
+template <bool FAST, typename T>
+__device__ __forceinline__ T q_div(T a, T b) {
+  if constexpr (FAST && std::is_same<T, float>::value) return a * __builtin_amdgcn_rcpf(b);
+  else return a / b;
+}
+template <bool FAST, typename T>
+__device__ __forceinline__ T q_sqrt(T x) {
+  if constexpr (FAST && std::is_same<T, float>::value) return __builtin_amdgcn_sqrtf(x);
+  else return sqrt(x);
+}
+template <bool FAST, typename T>
+__device__ __forceinline__ T q_fma(T a, T b, T c) {  // a * b + c
+  if constexpr (FAST && std::is_same<T, float>::value) return __builtin_fmaf(a, b, c);
+  else return a * b + c;
 }
 
-template <typename T>
+// total energy of compute_conservatives: P/(gamma-1) + 0.5*sum(m*v), m = rho*v
+template <bool FAST = false, typename T>
+__device__ __forceinline__ T energy(const T w[5], T gamma) {
+  const T mx = w[0] * w[1], my = w[0] * w[2], mz = w[0] * w[3];
+  return q_div<FAST>(w[4], gamma - T(1)) + T(0.5) * q_fma<FAST>(mz, w[3], q_fma<FAST>(my, w[2], mx * w[1]));
+}
+
+template <bool FAST = false, typename T>
 __device__ __forceinline__ T sound_speed(T rho, T P, T gamma, T csq_floor) {
-  const T csq = gamma * P / rho;
-  return sqrt(csq > csq_floor ? csq : csq_floor);
+  const T csq = q_div<FAST>(gamma * P, rho);
+  return q_sqrt<FAST>(csq > csq_floor ? csq : csq_floor);
 }
 
 // max / min as the hardware v_max / v_min: for non-NaN operands they equal
@@ -170,12 +192,12 @@ __device__ __forceinline__ T np_max(T a, T b) { return a != a ? a : (a > b ? a :
 enum { RS_UPWIND = 0, RS_LLF = 1, RS_HLLC = 2 };
 
 // HLLC "David variation", fvhoe/riemann_solvers.py:110-183
-template <int D, typename T>
+template <int D, bool FAST = false, typename T>
 __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor,
                                      T* v_godunov = nullptr) {
-  const T El = energy(wl, gamma), Er = energy(wr, gamma);
-  const T cl = sound_speed(wl[0], wl[4], gamma, csq_floor);
-  const T cr = sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const T El = energy<FAST>(wl, gamma), Er = energy<FAST>(wr, gamma);
+  const T cl = sound_speed<FAST>(wl[0], wl[4], gamma, csq_floor);
+  const T cr = sound_speed<FAST>(wr[0], wr[4], gamma, csq_floor);
   const T cmax = dmax(cl, cr);
   const T vl = wl[D + 1], vr = wr[D + 1];
   const T rhol = wl[0], rhor = wr[0], Pl = wl[4], Pr = wr[4];
@@ -183,18 +205,37 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T F[5], T gam
   const T Sr = dmax(vl, vr) + cmax;
   const T rcL = rhol * (vl - Sl);
   const T rcR = rhor * (Sr - vr);
-  const T vs = (rcL * vl + rcR * vr + Pl - Pr) / (rcL + rcR);
-  const T Ps = (rcL * Pr + rcR * Pl + rcL * rcR * (vl - vr)) / (rcL + rcR);
+  T vs, Ps;
+  if constexpr (FAST && std::is_same<T, float>::value) {  // one reciprocal for both quotients
+    const T inv = __builtin_amdgcn_rcpf(rcL + rcR);
+    vs = (q_fma<FAST>(rcL, vl, rcR * vr) + Pl - Pr) * inv;
+    Ps = q_fma<FAST>(rcL * rcR, vl - vr, q_fma<FAST>(rcL, Pr, rcR * Pl)) * inv;
+  } else {
+    vs = (rcL * vl + rcR * vr + Pl - Pr) / (rcL + rcR);
+    Ps = (rcL * Pr + rcR * Pl + rcL * rcR * (vl - vr)) / (rcL + rcR);
+  }
   T r, v, P, E;
   if (Sl > T(0)) {
     r = rhol; v = vl; P = Pl; E = El;
   } else if (vs > T(0)) {
-    r = rhol * (Sl - vl) / (Sl - vs);
-    E = ((Sl - vl) * El - Pl * vl + Ps * vs) / (Sl - vs);
+    if constexpr (FAST && std::is_same<T, float>::value) {
+      const T inv = __builtin_amdgcn_rcpf(Sl - vs);
+      r = rhol * (Sl - vl) * inv;
+      E = q_fma<FAST>(Ps, vs, q_fma<FAST>(Sl - vl, El, -(Pl * vl))) * inv;
+    } else {
+      r = rhol * (Sl - vl) / (Sl - vs);
+      E = ((Sl - vl) * El - Pl * vl + Ps * vs) / (Sl - vs);
+    }
     v = vs; P = Ps;
   } else if (Sr > T(0)) {
-    r = rhor * (Sr - vr) / (Sr - vs);
-    E = ((Sr - vr) * Er - Pr * vr + Ps * vs) / (Sr - vs);
+    if constexpr (FAST && std::is_same<T, float>::value) {
+      const T inv = __builtin_amdgcn_rcpf(Sr - vs);
+      r = rhor * (Sr - vr) * inv;
+      E = q_fma<FAST>(Ps, vs, q_fma<FAST>(Sr - vr, Er, -(Pr * vr))) * inv;
+    } else {
+      r = rhor * (Sr - vr) / (Sr - vs);
+      E = ((Sr - vr) * Er - Pr * vr + Ps * vs) / (Sr - vs);
+    }
     v = vs; P = Ps;
   } else {
     r = rhor; v = vr; P = Pr; E = Er;
@@ -204,7 +245,7 @@ __device__ __forceinline__ void hllc(const T wl[5], const T wr[5], T F[5], T gam
 #pragma unroll
   for (int k = 1; k < 4; ++k) {
     if (k == D + 1) {
-      F[k] = rv * v + P;
+      F[k] = q_fma<FAST>(rv, v, P);
     } else {
       const T up = v > T(0) ? wl[k] : (v < T(0) ? wr[k] : T(0));
       F[k] = rv * up;
@@ -229,25 +270,25 @@ __device__ __forceinline__ void phys_flux(const T w[5], const T u[5], T F[5], bo
   }
 }
 
-template <typename T>
+template <bool FAST = false, typename T>
 __device__ __forceinline__ void cons_from_prims(const T w[5], T u[5], T gamma) {
   u[0] = w[0];
   u[1] = w[0] * w[1];
   u[2] = w[0] * w[2];
   u[3] = w[0] * w[3];
-  u[4] = w[4] / (gamma - T(1)) + T(0.5) * (((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]));
+  u[4] = q_div<FAST>(w[4], gamma - T(1)) + T(0.5) * q_fma<FAST>(u[3], w[3], q_fma<FAST>(u[2], w[2], u[1] * w[1]));
 }
 
 // local Lax-Friedrichs (Toro p.331; fvhoe/riemann_solvers.py:68-88)
-template <int D, typename T>
+template <int D, bool FAST = false, typename T>
 __device__ __forceinline__ void llf(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor) {
   T ul[5], ur[5], Fl[5], Fr[5];
-  cons_from_prims(wl, ul, gamma);
-  cons_from_prims(wr, ur, gamma);
+  cons_from_prims<FAST>(wl, ul, gamma);
+  cons_from_prims<FAST>(wr, ur, gamma);
   phys_flux<D>(wl, ul, Fl, true);
   phys_flux<D>(wr, ur, Fr, true);
-  const T sl = fabs(wl[D + 1]) + sound_speed(wl[0], wl[4], gamma, csq_floor);
-  const T sr = fabs(wr[D + 1]) + sound_speed(wr[0], wr[4], gamma, csq_floor);
+  const T sl = fabs(wl[D + 1]) + sound_speed<FAST>(wl[0], wl[4], gamma, csq_floor);
+  const T sr = fabs(wr[D + 1]) + sound_speed<FAST>(wr[0], wr[4], gamma, csq_floor);
   const T hs = T(0.5) * dmax(sl, sr);
 #pragma unroll
   for (int k = 0; k < 5; ++k) F[k] = T(0.5) * (Fl[k] + Fr[k]) - hs * (ur[k] - ul[k]);
@@ -275,10 +316,10 @@ __device__ __forceinline__ void riemann(int solver, const T wl[5], const T wr[5]
 }
 
 // compile-time solver choice
-template <int RS, int D, typename T>
+template <int RS, int D, bool FAST = false, typename T>
 __device__ __forceinline__ void riemann_t(const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor) {
-  if constexpr (RS == RS_HLLC) hllc<D>(wl, wr, F, gamma, csq_floor);
-  else if constexpr (RS == RS_LLF) llf<D>(wl, wr, F, gamma, csq_floor);
+  if constexpr (RS == RS_HLLC) hllc<D, FAST>(wl, wr, F, gamma, csq_floor);
+  else if constexpr (RS == RS_LLF) llf<D, FAST>(wl, wr, F, gamma, csq_floor);
   else upwind<D>(wl, wr, F, gamma);
 }
 

@@ -136,12 +136,53 @@ constexpr int TPB = 256;
 #ifndef FVHOE_FLUX_NT_F32
 #define FVHOE_FLUX_NT_F32 512
 #endif
+// 2-D fluxes: one kernel for both face axes from one W box (FVHOE_FLUX_XY = 1), tile of A (x) x B (y)
+// face positions on NT threads per arithmetic type, or one kernel per axis (0)
+#ifndef FVHOE_FLUX_XY
+#define FVHOE_FLUX_XY 1
+#endif
+// (KH 2048^2 p=3, both axes per launch: FP64 32 x 16 on 256 threads holding the box in registers 202 us
+// against 104 + 103 for the two per-axis kernels, 212 fetching it again for the y faces, 214 at 16 x 16;
+// FP32 16 x 32 on 256 threads fetching it again 114 us against 58 + 61, 119 for 32 x 32 on 512 holding it)
+#ifndef FVHOE_FXY_RELOAD_F64
+#define FVHOE_FXY_RELOAD_F64 0
+#endif
+#ifndef FVHOE_FXY_RELOAD_F32
+#define FVHOE_FXY_RELOAD_F32 1
+#endif
+#ifndef FVHOE_FXY64_A
+#define FVHOE_FXY64_A 32
+#endif
+#ifndef FVHOE_FXY64_B
+#define FVHOE_FXY64_B 16
+#endif
+#ifndef FVHOE_FXY64_NT
+#define FVHOE_FXY64_NT 256
+#endif
+#ifndef FVHOE_FXY32_A
+#define FVHOE_FXY32_A 16
+#endif
+#ifndef FVHOE_FXY32_B
+#define FVHOE_FXY32_B 32
+#endif
+#ifndef FVHOE_FXY32_NT
+#define FVHOE_FXY32_NT 256
+#endif
 // 2-D: the troubled-cell test inside the candidate tile (1) or its own pass over W and WS (0)
 #ifndef FVHOE_CAND_CHECK
 #define FVHOE_CAND_CHECK 1
 #endif
 #ifndef FVHOE_DETECT_ROWS
 #define FVHOE_DETECT_ROWS 2
+#endif
+
+// FP32 bulk form: fused multiply-adds in the stencil sweeps (FVHOE_F32_FMA) and the Riemann solvers'
+// approximate reciprocal / square root (FVHOE_F32_FAST); 0 gives the correctly rounded operations
+#ifndef FVHOE_F32_FMA
+#define FVHOE_F32_FMA 1
+#endif
+#ifndef FVHOE_F32_FAST
+#define FVHOE_F32_FAST 1
 #endif
 
 // effective reach of the c and t stencils (outermost non-zero tap)
@@ -152,12 +193,13 @@ __host__ __device__ constexpr int flux_ghosts(int p) {
 }
 
 // stencil sweep in arithmetic type T (weights rounded to T once, at compile time)
+// (FP64: a product and a sum per tap in the reference's order; FP32 bulk form: one fused multiply-add)
 template <int KIND, int P, typename T, typename Get>
 __device__ __forceinline__ T sweepT(Get&& x) {
   T acc = T(0);
   static_for<ntaps(P)>([&](auto I) {
     constexpr double w = weight(KIND, P, decltype(I)::value);
-    if constexpr (w != 0.0) acc = acc + T(w) * x(decltype(I)::value);
+    if constexpr (w != 0.0) acc = q_fma<FVHOE_F32_FMA>(T(w), x(decltype(I)::value), acc);
   });
   return acc;
 }
@@ -777,7 +819,8 @@ __device__ __forceinline__ T sweepL(Get&& x) {
   T acc = T(0);
   static_for<ntaps(P)>([&](auto I) {
     constexpr double w = weight(KIND, P, decltype(I)::value);
-    if constexpr (w != 0.0) acc = acc + T(w) * x(std::integral_constant<int, decltype(I)::value - reach(P)>{});
+    if constexpr (w != 0.0)
+      acc = q_fma<FVHOE_F32_FMA>(T(w), x(std::integral_constant<int, decltype(I)::value - reach(P)>{}), acc);
   });
   return acc;
 }
@@ -1007,7 +1050,7 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
         wl[q] = sweepT<KIND_R, PN, T>([&](int t) { return X[q][(fr + t) * HT1 + c]; });
         wr[q] = sweepT<KIND_L, PN, T>([&](int t) { return X[q][(fr + 1 + t) * HT1 + c]; });
       }
-      riemann_t<RS, D>(wl, wr, res[k], gamma, csq_floor);  // the solver is a compile-time choice
+      riemann_t<RS, D, FVHOE_F32_FAST && std::is_same<T, float>::value>(wl, wr, res[k], gamma, csq_floor);  // the solver is a compile-time choice
     }
   }
   __syncthreads();
@@ -1039,6 +1082,124 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
       });
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// x and y faces of one tile from one W box (2-D, both axes of degree P): the
+// box W[i0 - LO, i0 + A + HI) x [j0 - LO, j0 + B + HI) is fetched once into
+// registers and staged twice, once per face axis, through k_flux's steps (the
+// same operations per face, so F and G are identical to k_flux's).  x faces
+// fa in [i0, i0 + A) for rows [j0, j0 + B); y faces fb in [j0, j0 + B) for
+// rows [i0, i0 + A); the tiles cover (n0 + 1) x (n1 + 1) face positions.
+// ---------------------------------------------------------------------------
+template <typename T, int P, int A, int B, int NT, int RS>
+__global__ void __launch_bounds__(NT) k_flux_xy(const T* __restrict__ W, int NW0, int NW1, Geo g, T* __restrict__ F,
+                                               T* __restrict__ Gout, int tilesB, double gamma_d, double csq_floor_d) {
+  constexpr int EL = reach(P), RC = ereach_c(P), RT = ereach_t(P);
+  constexpr int LO = 1 + EL > RC + RT ? 1 + EL : RC + RT;  // box cells before the tile on each axis
+  constexpr int HI = EL > RC + RT ? EL : RC + RT;          // and after it
+  constexpr int BX = A + LO + HI, BY = B + LO + HI;
+  constexpr int NE = (BX * BY + NT - 1) / NT;
+  // per face axis D: TA faces x TB rows, HN normal cells, HT0 / HT1 transverse cells before / after the c-sweep
+  constexpr int HN_0 = A + 1 + 2 * EL, HT1_0 = B + 2 * RT, HS_0 = (B + 2 * (RC + RT)) | 1;
+  constexpr int HN_1 = B + 1 + 2 * EL, HT1_1 = A + 2 * RT, HS_1 = (A + 2 * (RC + RT)) | 1;
+  constexpr int XSZ = HN_0 * HT1_0 > HN_1 * HT1_1 ? HN_0 * HT1_0 : HN_1 * HT1_1;
+  constexpr int SSZ = HN_0 * HS_0 > HN_1 * HS_1 ? HN_0 * HS_0 : HN_1 * HS_1;
+  __shared__ T X[5][XSZ];
+  __shared__ T S0[SSZ];
+  const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int i0 = (tile / tilesB) * A, j0 = (tile % tilesB) * B;
+  const int tid = threadIdx.x;
+  const int wr0 = i0 - LO + g.ge0, wc0 = j0 - LO + g.ge1;  // W index of box element (0, 0); >= 0 as ge >= LO
+
+  // the box, y (W's contiguous axis) fastest over the threads; clamped reads feed unused outputs only.
+  // RELOAD: fetched again for the second axis (the block's own lines, hot in the caches) instead of held
+  // in registers through the first axis
+  constexpr bool RELOAD = sizeof(T) == 8 ? FVHOE_FXY_RELOAD_F64 : FVHOE_FXY_RELOAD_F32;
+  T pre[5][NE];
+  auto fetch = [&]() {
+    tile_for_k<BX, BY, NT>([&](int k, int e, int r, int c) {
+      const int a = min(wr0 + r, NW0 - 1), b = min(wc0 + c, NW1 - 1);
+#pragma unroll
+      for (int v = 0; v < 5; ++v) pre[v][k] = W[(v * NW0 + a) * NW1 + b];
+    });
+  };
+  fetch();
+
+  auto faces = [&](auto dc) {
+    constexpr int D = decltype(dc)::value;
+    constexpr int TA = D == 0 ? A : B, TB = D == 0 ? B : A;
+    constexpr int HN = TA + 1 + 2 * EL, HT0 = TB + 2 * (RC + RT), HT1 = TB + 2 * RT, HS = HT0 | 1;
+    constexpr int NR = (TA * HT1 + NT - 1) / NT;  // Riemann problems per thread
+    // S0[nrm * HS + trn]: normal cell f0 - 1 - EL + nrm, transverse cell b0 - RC - RT + trn, i.e. box
+    // element (nrm + ON, trn + OT) along (normal, transverse)
+    constexpr int ON = LO - 1 - EL, OT = LO - (RC + RT);
+    const int f0 = D == 0 ? i0 : j0, b0 = D == 0 ? j0 : i0;
+    const int nN = D == 0 ? g.n0 : g.n1, nT = D == 0 ? g.n1 : g.n0;
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      tile_for_k<BX, BY, NT>([&](int k, int e, int r, int c) {
+        const int nrm = (D == 0 ? r : c) - ON, trn = (D == 0 ? c : r) - OT;
+        if (nrm >= 0 && nrm < HN && trn >= 0 && trn < HT0) S0[nrm * HS + trn] = pre[v][k];
+      });
+      __syncthreads();
+      tile_for<HN, HT1, NT>([&](int e, int r, int c) {
+        const T x = sweepT<KIND_C, P, T>([&](int t) { return S0[r * HS + c + RC + t - reach(P)]; });
+        X[v][e] = T(0) + x;  // degree-0 z sweep precedes/follows (fvhoe/solver.py:524,541)
+      });
+      __syncthreads();
+    }
+    T res[NR][5];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int e = tid + k * NT;
+      if (e < TA * HT1) {
+        const int fr = e / HT1, c = e % HT1;
+        T wl[5], wr[5];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          wl[q] = sweepT<KIND_R, P, T>([&](int t) { return X[q][(fr + t) * HT1 + c]; });
+          wr[q] = sweepT<KIND_L, P, T>([&](int t) { return X[q][(fr + 1 + t) * HT1 + c]; });
+        }
+        riemann_t<RS, D, FVHOE_F32_FAST && std::is_same<T, float>::value>(wl, wr, res[k], gamma, csq_floor);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int e = tid + k * NT;
+      if (e < TA * HT1) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) X[q][e] = res[k][q];
+      }
+    }
+    __syncthreads();
+    const int nF0 = D == 0 ? g.n0 + 1 : g.n0, nF1 = D == 0 ? g.n1 : g.n1 + 1;
+    T* const out = D == 0 ? F : Gout;
+    for (int q = 0; q < 5; ++q) {
+      T* Fq = out + q * nF0 * nF1;
+      if (D == 0) {
+        tile_for<TA, TB, NT>([&](int e, int fr, int c) {
+          const int fa = f0 + fr, b = b0 + c;
+          if (fa > nN || b >= nT) return;
+          const T x = sweepT<KIND_T, P, T>([&](int t) { return X[q][fr * HT1 + c + RT + t - reach(P)]; });
+          Fq[fa * nF1 + b] = T(0) + x;  // degree-0 z sweep (fvhoe/solver.py:633)
+        });
+      } else {
+        tile_for<TB, TA, NT>([&](int e, int c, int fr) {
+          const int fa = f0 + fr, b = b0 + c;
+          if (fa > nN || b >= nT) return;
+          const T x = sweepT<KIND_T, P, T>([&](int t) { return X[q][fr * HT1 + c + RT + t - reach(P)]; });
+          Fq[b * nF1 + fa] = T(0) + x;  // (fvhoe/solver.py:639)
+        });
+      }
+    }
+    __syncthreads();  // S0 and X are restaged by the next face axis
+  };
+  faces(std::integral_constant<int, 0>{});
+  if (RELOAD) fetch();
+  faces(std::integral_constant<int, 1>{});
 }
 
 // ---------------------------------------------------------------------------
@@ -1793,6 +1954,15 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       constexpr int TA0 = FA, TB0 = FB, TA1 = F64 ? FVHOE_FLUX64_YA : FVHOE_FLUX32_YA,
                     TB1 = F64 ? FVHOE_FLUX64_YB : FVHOE_FLUX32_YB;
       constexpr int NF = F64 ? FVHOE_FLUX_NT_F64 : FVHOE_FLUX_NT_F32;
+      if constexpr (FVHOE_FLUX_XY) {
+        constexpr int XA = F64 ? FVHOE_FXY64_A : FVHOE_FXY32_A, XB = F64 ? FVHOE_FXY64_B : FVHOE_FXY32_B;
+        constexpr int XNT = F64 ? FVHOE_FXY64_NT : FVHOE_FXY32_NT;
+        const int ta = (L.n0 + 1 + XA - 1) / XA, tb = (L.n1 + 1 + XB - 1) / XB;
+        TimedLaunch tl(TK_FLUX_X, s);  // both axes, timed as one
+        k_flux_xy<T, P, XA, XB, XNT, RS><<<ta * tb, XNT, 0, s>>>(W, L.NW0, L.NW1, g, F, G, tb, c->gamma,
+                                                                 c->csq_floor);
+        return;
+      }
       const int tx = (L.n0 + 1 + TA0 - 1) / TA0, ty = (L.n1 + TB0 - 1) / TB0;
       {
         TimedLaunch tl(TK_FLUX_X, s);

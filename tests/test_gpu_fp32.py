@@ -17,6 +17,10 @@ P, O = ("periodic", "periodic"), ("outflow", "outflow")
 # L1 bounds on density (mean |rho_fp32 - rho_ref|), measured values are printed
 SOD_L1_BOUND = 1e-5
 KH_L1_BOUND = 1e-5
+# relative difference of the simulated time after 10 CFL steps: the CFL maximum sits on single cells, and a
+# cell whose trouble flag differs between the forms (borderline NAD tests) takes a MUSCL instead of a
+# high-order update there, which moves that step's dt by up to ~1e-3 relative (3.5e-4 measured at KH 256^2)
+KH_TIME_REL_BOUND = 1e-3
 
 
 def test_sod_fp32_l1():
@@ -42,7 +46,7 @@ def test_kelvin_helmholtz_fp32_l1():
     d = (f32.u[0] - ref.u[0]).abs().mean().item()
     print(f"[kh256 fp32] 10 steps L1(rho)={d:.3e} t64={ref.t:.6e} t32={f32.t:.6e}", flush=True)
     assert d < KH_L1_BOUND
-    assert abs(f32.t - ref.t) / ref.t < 1e-5
+    assert abs(f32.t - ref.t) / ref.t < KH_TIME_REL_BOUND
 
 
 # The FP32 form across degrees and boundary kinds on grids large enough for the interior tiles: 5 SSP-RK3

@@ -250,10 +250,13 @@ def test_training_reduces_loss_on_solver_flags():
 
 
 # What the trained network gains on held-out four-quadrant problems (seeds 200-249 were never trained on):
-# the check's flags left to the second pass, relative to the FP32 run without the predictor, stay below
-# one (the network moves fallbacks into the first pass), and the learned form's density differs from the
-# plain FP64 run by less than LEARNED_L1_BOUND while its error against a 2x finer FP64 run stays within
-# 1.5x of plain FP64's.
+# the check's flags left to the second pass, relative to the FP32 run without the predictor, summed over
+# the seeds, stay below SECOND_PASS_SHARE_BOUND (the network moves fallbacks into the first pass); on a
+# seed where it pre-flags almost nothing the share is one up to the few cells whose borderline tests
+# differ between runs (SEED_SHARE_SLACK); the learned form's density differs from the plain FP64 run by
+# less than LEARNED_L1_BOUND while its error against a 2x finer FP64 run stays within 1.5x of plain FP64's.
+SECOND_PASS_SHARE_BOUND = 0.95
+SEED_SHARE_SLACK = 1e-3
 LEARNED_L1_BOUND = 1e-2
 MATCHED_ERROR_RATIO = 1.5
 
@@ -268,9 +271,12 @@ def test_trained_predictor_gain_on_held_out_seeds():
         print("[gain] " + " ".join(f"{k}={v:.4g}" if isinstance(v, float) else f"{k}={v}" for k, v in r.items()),
               flush=True)
     assert len(done) >= 2, rows
+    share = sum(r["second_pass"] for r in done) / sum(r["check_flags_fp32"] for r in done)
+    print(f"[gain] second_pass_share over the seeds {share:.4f}", flush=True)
+    assert share < SECOND_PASS_SHARE_BOUND
     for r in done:
         assert r["check_flags_fp32"] > 0
-        assert r["second_pass_share"] < 1.0, r
+        assert r["second_pass_share"] < 1.0 + SEED_SHARE_SLACK, r
         assert r["l1_density_learned_fp32_vs_plain"] < LEARNED_L1_BOUND, r
         if "err_ratio_learned_fp32" in r:
             assert r["err_ratio_learned_fp32"] <= MATCHED_ERROR_RATIO, r
