@@ -36,12 +36,21 @@ __host__ __device__ constexpr int e_gext(int p) { return e_ghosts(p) > 3 ? e_gho
 
 constexpr int NACC = 11;  // [0] max wave speed (raw bits), [1 + 2v] / [2 + 2v] ordered min / max of variable v
 
+// FP32 bulk form: fused multiply-adds in the sweeps, the Riemann solvers' approximate reciprocal / square
+// root and one reciprocal of rho in the FP64 pressure recovery (FVHOE_ENS_F32_FAST = 0: correctly rounded
+// operations throughout); FP64 keeps the reference's operations
+#ifndef FVHOE_ENS_F32_FAST
+#define FVHOE_ENS_F32_FAST 1
+#endif
+template <typename T>
+constexpr bool ens_fast() { return FVHOE_ENS_F32_FAST && std::is_same<T, float>::value; }
+
 template <int KIND, int P, typename T, typename Get>
 __device__ __forceinline__ T esweep(Get&& x) {
   T acc = T(0);
   static_for<ntaps(P)>([&](auto I) {
     constexpr double w = weight(KIND, P, decltype(I)::value);
-    if constexpr (w != 0.0) acc = acc + T(w) * x(decltype(I)::value);
+    if constexpr (w != 0.0) acc = q_fma<ens_fast<T>()>(T(w), x(decltype(I)::value), acc);
   });
   return acc;
 }
@@ -164,7 +173,7 @@ __device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double*
       const T sy = esweep<KIND_C, 0, T>([&](int) { return sx; });
       uu[v] = (double)(T(0) + sy);
     }
-    prims_from_cons(uu, ww, e.gamma);
+    prims_from_cons<ens_fast<T>()>(uu, ww, e.gamma);
 #pragma unroll
     for (int v = 0; v < 5; ++v) SC[v][k] = T(ww[v]);
   }
@@ -265,7 +274,7 @@ __device__ __forceinline__ void d_ens_flux(const Ens& e, int2 tl, const T* __res
     wl[q] = esweep<KIND_R, P, T>([&](int t) { return xc(fa - 1 - EL + t); });
     wr[q] = esweep<KIND_L, P, T>([&](int t) { return xc(fa - EL + t); });
   }
-  riemann<0>(e.riemann, wl, wr, res, T(e.gamma), T(e.csq_floor));
+  riemann<0, ens_fast<T>()>(e.riemann, wl, wr, res, T(e.gamma), T(e.csq_floor));
   T* Fi = F + 5 * e.off_f[i];
 #pragma unroll
   for (int q = 0; q < 5; ++q) {

@@ -124,13 +124,22 @@ __device__ __forceinline__ double sweep(Get&& x) {
 // T is the arithmetic type: double for the reference-exact path, float for
 // the FP32 bulk form (where pressure recovery still runs in double).
 // ---------------------------------------------------------------------------
-template <typename T>
+// ONE_DIV (the FP32 bulk forms, whose recovery still runs in FP64): the velocities as products with one
+// reciprocal of rho instead of three quotients
+template <bool ONE_DIV = false, typename T>
 __device__ __forceinline__ void prims_from_cons(const T u[5], T w[5], T gamma) {
   const T rho = u[0];
   w[0] = rho;
-  w[1] = u[1] / rho;
-  w[2] = u[2] / rho;
-  w[3] = u[3] / rho;
+  if constexpr (ONE_DIV) {
+    const T inv = T(1) / rho;
+    w[1] = u[1] * inv;
+    w[2] = u[2] * inv;
+    w[3] = u[3] * inv;
+  } else {
+    w[1] = u[1] / rho;
+    w[2] = u[2] / rho;
+    w[3] = u[3] / rho;
+  }
   const T msum = ((u[1] * w[1]) + (u[2] * w[2])) + (u[3] * w[3]);
   w[4] = (gamma - T(1)) * (u[4] - T(0.5) * msum);
 }
@@ -307,11 +316,11 @@ __device__ __forceinline__ void upwind(const T wl[5], const T wr[5], T F[5], T g
   for (int k = 0; k < 5; ++k) F[k] = v > T(0) ? Fl[k] : (v < T(0) ? Fr[k] : T(0));
 }
 
-template <int D, typename T>
+template <int D, bool FAST = false, typename T>
 __device__ __forceinline__ void riemann(int solver, const T wl[5], const T wr[5], T F[5], T gamma, T csq_floor,
                                         T* v_godunov = nullptr) {
-  if (solver == RS_HLLC) hllc<D>(wl, wr, F, gamma, csq_floor, v_godunov);
-  else if (solver == RS_LLF) llf<D>(wl, wr, F, gamma, csq_floor);
+  if (solver == RS_HLLC) hllc<D, FAST>(wl, wr, F, gamma, csq_floor, v_godunov);
+  else if (solver == RS_LLF) llf<D, FAST>(wl, wr, F, gamma, csq_floor);
   else upwind<D>(wl, wr, F, gamma);
 }
 

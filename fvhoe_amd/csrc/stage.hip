@@ -572,7 +572,7 @@ __global__ void __launch_bounds__(NT, CHECK && sizeof(T) == 8 ? FVHOE_CHECK_WAVE
         double uu[5], ww[5];
 #pragma unroll
         for (int q = 0; q < 5; ++q) uu[q] = (double)cen[q][k];
-        prims_from_cons(uu, ww, gamma);
+        prims_from_cons<FVHOE_F32_FAST && std::is_same<T, float>::value>(uu, ww, gamma);
 #pragma unroll
         for (int q = 0; q < 5; ++q) cen[q][k] = T(ww[q]);
       }
@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(256, sizeof(T) == 8 ? FVHOE_STRIP_WAVES_F64 : 
     double uu[5], ww[5];
 #pragma unroll
     for (int q = 0; q < 5; ++q) uu[q] = (double)C[q][k];
-    prims_from_cons(uu, ww, gamma);
+    prims_from_cons<FVHOE_F32_FAST && std::is_same<T, float>::value>(uu, ww, gamma);
 #pragma unroll
     for (int q = 0; q < 5; ++q) C[q][k] = T(ww[q]);
   }
