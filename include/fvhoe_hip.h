@@ -507,7 +507,11 @@ int fvhoe_stage_predicted(const FvhoeStageConfig* cfg, const double* u, double* 
  * bc[0..1], riemann, a-posteriori options, bulk_fp32; n[1] = n[2] = 1,
  * p[1] = p[2] = 0; n[0] and h are per instance).  tile: 64, 128 or 256 cells
  * per workgroup.  u: the instances' conservative states, instance-major, each
- * (5, n[i]) contiguous; t: per-instance times (device).
+ * (5, n[i]) contiguous; t: per-instance times (device).  A "dirichlet" / "ic"
+ * side (bc[k] = FVHOE_BC_DIRICHLET, fvhoe/boundary_conditions.py:18-57,334-348)
+ * reads slab[k]: device values (count, 5, slab_g), layer j of the low side
+ * being cell j - slab_g and of the high side cell n[i] + j of instance i;
+ * slab_g at least fvhoe_stage_ghost_depth's depth for p[0].
  * ---------------------------------------------------------------------- */
 int fvhoe_ensemble_workspace_bytes(const FvhoeStageConfig* scheme, int count, const int* n, int tile,
                                    unsigned long long* bytes);
