@@ -39,6 +39,11 @@ constexpr int NACC = 11;  // [0] max wave speed (raw bits), [1 + 2v] / [2 + 2v] 
 // FP32 bulk form: fused multiply-adds in the sweeps, the Riemann solvers' approximate reciprocal / square
 // root and one reciprocal of rho in the FP64 pressure recovery (FVHOE_ENS_F32_FAST = 0: correctly rounded
 // operations throughout); FP64 keeps the reference's operations
+// the chain's fused passes: W with its tile's faces, and the candidate with its cells' tests (1), or the
+// separate W / F / WS / test passes of the persistent stage (0)
+#ifndef FVHOE_ENS_FUSED
+#define FVHOE_ENS_FUSED 1
+#endif
 #ifndef FVHOE_ENS_F32_FAST
 #define FVHOE_ENS_F32_FAST 1
 #endif
@@ -137,27 +142,28 @@ __device__ __forceinline__ void d_ens_reset(const Ens& e, int i) {
 }
 __global__ void k_ens_reset(Ens e) { d_ens_reset(e, blockIdx.x * blockDim.x + threadIdx.x); }
 
-// W (CAND = false, from u; ghost width GE) or WS (CAND = true, from the candidate; ghost width 3)
-template <typename T, int P, int TILE, bool CAND>
-__device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double* __restrict__ u,
-                                            const T* __restrict__ F, T* __restrict__ out) {
-  constexpr int RC = e_reach(P), RT = e_reach(P), GE = e_gext(P), G = e_ghosts(P);
+// Primitive fv averages of one tile: W (CAND = false, from u; ghost width GE) or WS (CAND = true, from the
+// candidate; ghost width 3), outputs a in [tl.y, tl.y + TILE) of the instance's ghost-extended grid.
+// XL / XR: outputs computed before / after the tile as well and kept in LDS (SO) for the caller.
+template <typename T, int P, int TILE, bool CAND, int XL, int XR>
+__device__ __forceinline__ void ens_prims_tile(const Ens& e, int i, int a0, const double* __restrict__ u,
+                                               const T* __restrict__ F, T (&SO)[5][TILE + XL + XR]) {
+  constexpr int RC = e_reach(P), RT = e_reach(P), GE = e_gext(P);
   constexpr int G0 = CAND ? 3 : GE;
-  constexpr int HU = TILE + 2 * (RC + RT), HC = TILE + 2 * RT;
+  constexpr int NOUT = TILE + XL + XR;
+  constexpr int HU = NOUT + 2 * (RC + RT), HC = NOUT + 2 * RT;
   __shared__ T SU[5][HU];
   __shared__ T SC[5][HC];
-  const int i = tl.x;
-  if (!e.active[i]) return;
-  const int n = e.n[i], NO = n + 2 * G0;
+  const int n = e.n[i];
   const double cx = -(1.0 / e.h[i]);
-  const double dt = e.dt_stage[i];
+  const double dt = CAND ? e.dt_stage[i] : 0.0;
   const double* ui = u + 5 * e.off_u[i];
   const T* Fi = CAND ? F + 5 * e.off_f[i] : nullptr;
-  const int c0 = tl.y - G0;  // first cell of the tile
+  const int c0 = a0 - XL - G0;  // cell of the first output
   const int tid = threadIdx.x;
   const int span = G0 + RC + RT;  // cells beyond [-span, n + span) feed discarded outputs only
   for (int k = tid; k < HU; k += TILE) {
-    const int c = min(c0 - RC - RT + k, n + span - 1);
+    const int c = max(min(c0 - RC - RT + k, n + span - 1), -span);
 #pragma unroll
     for (int v = 0; v < 5; ++v)
       SU[v][k] = T(CAND ? ustar_at<T>(e, i, ui, Fi, n, cx, dt, v, c, e.bc_lo, e.bc_hi)
@@ -178,17 +184,65 @@ __device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double*
     for (int v = 0; v < 5; ++v) SC[v][k] = T(ww[v]);
   }
   __syncthreads();
+  for (int k = tid; k < NOUT; k += TILE) {
+#pragma unroll
+    for (int v = 0; v < 5; ++v) {
+      const T tx = esweep<KIND_T, P, T>([&](int t) { return SC[v][k + RT + t - reach(P)]; });
+      const T ty = esweep<KIND_T, 0, T>([&](int) { return tx; });
+      SO[v][k] = T(0) + ty;
+    }
+  }
+  __syncthreads();
+}
+
+// W (CAND = false) or WS (CAND = true) of one tile to global memory; W also feeds the per-instance CFL
+// and NAD-range reductions, and with FLUX the tile's faces fa = a - GE (a in the tile, 0 <= fa <= n) are
+// solved from the tile's W with EL + 1 / EL more outputs on each side (F written, as k_ens_flux does)
+template <typename T, int P, int TILE, bool CAND, bool FLUX = false>
+__device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double* __restrict__ u,
+                                            const T* __restrict__ F, T* __restrict__ out,
+                                            T* __restrict__ Fout = nullptr) {
+  constexpr int GE = e_gext(P), G = e_ghosts(P), EL = reach(P);
+  constexpr int G0 = CAND ? 3 : GE;
+  constexpr int XL = FLUX ? EL + 1 : 0, XR = FLUX ? EL : 0;
+  __shared__ T SO[5][TILE + XL + XR];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  ens_prims_tile<T, P, TILE, CAND, XL, XR>(e, i, tl.y, u, F, SO);
+  const int n = e.n[i], NO = n + 2 * G0;
+  const int tid = threadIdx.x;
   const int a = tl.y + tid;  // output index (cell a - G0)
   const bool valid = a < NO;
   T w[5];
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    const T tx = esweep<KIND_T, P, T>([&](int t) { return SC[v][tid + RT + t - reach(P)]; });
-    const T ty = esweep<KIND_T, 0, T>([&](int) { return tx; });
-    w[v] = T(0) + ty;
+    w[v] = SO[v][tid + XL];
     if (valid) out[5 * (CAND ? e.off_s[i] : e.off_w[i]) + v * NO + a] = w[v];
   }
   if (CAND) return;
+  if constexpr (FLUX) {
+    const int fa = a - GE;
+    if (fa >= 0 && fa <= n) {
+      T wl[5], wr[5], res[5];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        // face fa: cells fa - 1 - EL .. fa + EL = tile outputs tid + XL - 1 - EL .. (fvhoe/solver.py:522-568)
+        auto xc = [&](int k) {
+          const T x = esweep<KIND_C, 0, T>([&](int) { return SO[q][k]; });
+          return T(0) + x;
+        };
+        wl[q] = esweep<KIND_R, P, T>([&](int t) { return xc(tid + XL - 1 - EL + t); });
+        wr[q] = esweep<KIND_L, P, T>([&](int t) { return xc(tid + XL - EL + t); });
+      }
+      riemann<0, ens_fast<T>()>(e.riemann, wl, wr, res, T(e.gamma), T(e.csq_floor));
+      T* Fi = Fout + 5 * e.off_f[i];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        const T x = esweep<KIND_T, 0, T>([&](int) { return res[q]; });
+        Fi[q * (n + 1) + fa] = T(0) + x;
+      }
+    }
+  }
   // per-instance CFL max over cells [-G, n+G) and NAD range over [-3, n+3)
   const int c = a - GE;
   unsigned long long val[NACC];
@@ -230,11 +284,11 @@ __device__ __forceinline__ void d_ens_prims(const Ens& e, int2 tl, const double*
     else atomicMin(&e.acc[i * NACC + tid], x);
   }
 }
-template <typename T, int P, int TILE, bool CAND>
+template <typename T, int P, int TILE, bool CAND, bool FLUX = false>
 __global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restrict__ tiles,
                                                    const double* __restrict__ u, const T* __restrict__ F,
-                                                   T* __restrict__ out) {
-  d_ens_prims<T, P, TILE, CAND>(e, tiles[blockIdx.x], u, F, out);
+                                                   T* __restrict__ out, T* __restrict__ Fout) {
+  d_ens_prims<T, P, TILE, CAND, FLUX>(e, tiles[blockIdx.x], u, F, out, Fout);
 }
 
 // per-instance dt, range; first stage: active flag and the step's dt
@@ -288,58 +342,56 @@ __global__ void __launch_bounds__(TILE) k_ens_flux(Ens e, const int2* __restrict
   d_ens_flux<T, P, TILE>(e, tiles[blockIdx.x], W, F);
 }
 
-template <typename T, int P, int TILE>
-__device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __restrict__ W, const T* __restrict__ WS,
-                                             int* __restrict__ list) {
+// NAD / PAD / SED tests of interior cell c of instance i (fvhoe/slope_limiting.py:87-244); ws(v, o) is the
+// candidate's primitive average of variable v at cell c + o (-3 <= o <= 3)
+template <typename T, int P, typename WsAt>
+__device__ __forceinline__ int ens_detect_cell(const Ens& e, int i, int n, int c, const T* __restrict__ W,
+                                               WsAt&& ws) {
   constexpr int GE = e_gext(P);
-  const int i = tl.x;
-  if (!e.active[i]) return;
-  const int n = e.n[i], c = tl.y + threadIdx.x;
-  int t = 0;
-  if (c < n) {
-    const T* Wi = W + 5 * e.off_w[i];
-    const T* Si = WS + 5 * e.off_s[i];
-    const int NW = n + 2 * GE, NS = n + 6;
-    const double* rg = e.range + 10 * i;
-    bool tvv[5];
+  const T* Wi = W + 5 * e.off_w[i];
+  const int NW = n + 2 * GE;
+  const double* rg = e.range + 10 * i;
+  bool tvv[5];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-      const T* wv = Wi + v * NW + c + GE;
-      double M = (double)wv[0], m = M;
+  for (int v = 0; v < 5; ++v) {
+    const T* wv = Wi + v * NW + c + GE;
+    double M = (double)wv[0], m = M;
 #pragma unroll
-      for (int d = -1; d <= 1; ++d) {
-        const double x = (double)wv[d];
-        M = dmax(M, x);
-        m = dmin(m, x);
-      }
-      const double uc = (double)Si[v * NS + c + 3];
-      bool tv;
-      if (e.nad_range == 0) {
-        const double rng = e.nad_mode == 0 ? rg[2 * v + 1] - rg[2 * v] : M - m;
-        tv = dmin(uc - m, M - uc) < -e.nad_eps * rng;
-      } else {
-        const double lo = e.nad_mode == 0 ? (1.0 - e.nad_eps) * rg[2 * v] : (1.0 - e.nad_eps) * m;
-        const double hi = e.nad_mode == 0 ? (1.0 + e.nad_eps) * rg[2 * v + 1] : (1.0 + e.nad_eps) * M;
-        tv = dmin(uc - lo, hi - uc) < 0.0;
-      }
-      tvv[v] = tv && ((e.nad_vars >> v) & 1);
+    for (int d = -1; d <= 1; ++d) {
+      const double x = (double)wv[d];
+      M = dmax(M, x);
+      m = dmin(m, x);
     }
-    bool nad = false;
-    for (int v = 0; v < 5; ++v) {
-      bool tv = tvv[v];
-      if (tv && e.sed && !nad) {  // smooth-extrema detector only where NAD flags
-        const T* sv = Si + v * NS + c + 3;
-        tv = sed_alpha_1d([&](int o) { return (double)sv[o]; }, e.sed_eps) < 1.0;
-      }
-      nad = nad || tv;
+    const double uc = (double)ws(v, 0);
+    bool tv;
+    if (e.nad_range == 0) {
+      const double rng = e.nad_mode == 0 ? rg[2 * v + 1] - rg[2 * v] : M - m;
+      tv = dmin(uc - m, M - uc) < -e.nad_eps * rng;
+    } else {
+      const double lo = e.nad_mode == 0 ? (1.0 - e.nad_eps) * rg[2 * v] : (1.0 - e.nad_eps) * m;
+      const double hi = e.nad_mode == 0 ? (1.0 + e.nad_eps) * rg[2 * v + 1] : (1.0 + e.nad_eps) * M;
+      tv = dmin(uc - lo, hi - uc) < 0.0;
     }
-    bool pad = false;
-    if (e.pad_var >= 0) {
-      const double x = (double)Si[e.pad_var * NS + c + 3];
-      pad = dmin(x - e.pad_lo, e.pad_hi - x) < 0.0;
-    }
-    t = (e.force || pad || nad) ? 1 : 0;
+    tvv[v] = tv && ((e.nad_vars >> v) & 1);
   }
+  bool nad = false;
+  for (int v = 0; v < 5; ++v) {
+    bool tv = tvv[v];
+    if (tv && e.sed && !nad)  // smooth-extrema detector only where NAD flags
+      tv = sed_alpha_1d([&](int o) { return (double)ws(v, o); }, e.sed_eps) < 1.0;
+    nad = nad || tv;
+  }
+  bool pad = false;
+  if (e.pad_var >= 0) {
+    const double x = (double)ws(e.pad_var, 0);
+    pad = dmin(x - e.pad_lo, e.pad_hi - x) < 0.0;
+  }
+  return (e.force || pad || nad) ? 1 : 0;
+}
+
+// the flagged cells of the tile appended to the instance's list: wave ballot, block prefix, one atomic
+template <int TILE>
+__device__ __forceinline__ void ens_list_append(const Ens& e, int i, int t, int c, int* __restrict__ list) {
   __shared__ int wbase[TILE / 64];
   __shared__ int bbase;
   const unsigned long long ball = __ballot(t);
@@ -358,6 +410,46 @@ __device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __r
   __syncthreads();
   if (t) list[e.off_u[i] + bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
 }
+
+template <typename T, int P, int TILE>
+__device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __restrict__ W, const T* __restrict__ WS,
+                                             int* __restrict__ list) {
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], c = tl.y + threadIdx.x;
+  int t = 0;
+  if (c < n) {
+    const T* Si = WS + 5 * e.off_s[i];
+    const int NS = n + 6;
+    t = ens_detect_cell<T, P>(e, i, n, c, W, [&](int v, int o) { return Si[v * NS + c + 3 + o]; });
+  }
+  ens_list_append<TILE>(e, i, t, c, list);
+}
+
+// The candidate tile and the tests of its cells in one pass: WS of the tile's cells and 3 more on each
+// side (the SED's reach) stays in LDS, so WS is never stored (the chain's form; the persistent stage keeps
+// d_ens_prims<CAND> + d_ens_detect, with the same arithmetic)
+template <typename T, int P, int TILE>
+__device__ __forceinline__ void d_ens_cand_detect(const Ens& e, int2 tl, const double* __restrict__ u,
+                                                  const T* __restrict__ F, const T* __restrict__ W,
+                                                  int* __restrict__ list) {
+  __shared__ T SO[5][TILE + 6];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  // outputs: candidate grid (3 ghosts) indices tl.y .. tl.y + TILE + 5 = cells tl.y - 3 .. tl.y + TILE + 2
+  ens_prims_tile<T, P, TILE, true, 0, 6>(e, i, tl.y, u, F, SO);
+  const int n = e.n[i], c = tl.y + threadIdx.x;
+  int t = 0;
+  if (c < n) t = ens_detect_cell<T, P>(e, i, n, c, W, [&](int v, int o) { return SO[v][threadIdx.x + 3 + o]; });
+  ens_list_append<TILE>(e, i, t, c, list);
+}
+template <typename T, int P, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_cand_detect(Ens e, const int2* __restrict__ tiles,
+                                                         const double* __restrict__ u, const T* __restrict__ F,
+                                                         const T* __restrict__ W, int* __restrict__ list) {
+  d_ens_cand_detect<T, P, TILE>(e, tiles[blockIdx.x], u, F, W, list);
+}
+
 template <typename T, int P, int TILE>
 __global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
                                                     const T* __restrict__ WS, int* __restrict__ list) {
@@ -704,12 +796,21 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
   for (int st = 0; st < 3; ++st) {
     const double* in = st == 0 ? u : sv;
     k_ens_reset<<<gi, 256, 0, s>>>(e);
-    k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W);
+    // W with the faces of each W tile solved in the same pass (FVHOE_ENS_FUSED), or W then F
+    if (FVHOE_ENS_FUSED) {
+      k_ens_prims<T, P, TILE, false, true><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W, F);
+    } else {
+      k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W, nullptr);
+    }
     k_ens_scalars<<<gi, 256, 0, s>>>(e, st == 0, t, target);
-    k_ens_flux<T, P, TILE><<<PL.tf, TILE, 0, s>>>(e, tf, W, F);
+    if (!FVHOE_ENS_FUSED) k_ens_flux<T, P, TILE><<<PL.tf, TILE, 0, s>>>(e, tf, W, F);
     if (c->a_posteriori) {
-      k_ens_prims<T, P, TILE, true><<<PL.ts, TILE, 0, s>>>(e, ts, in, F, WS);
-      k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
+      if (FVHOE_ENS_FUSED) {  // the candidate and the tests of each interior tile in one pass
+        k_ens_cand_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, F, W, list);
+      } else {
+        k_ens_prims<T, P, TILE, true><<<PL.ts, TILE, 0, s>>>(e, ts, in, F, WS, nullptr);
+        k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
+      }
       k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F);
     }
     k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1);
