@@ -9,10 +9,12 @@ path (bit-identical to the reference), the FP32 bulk form (FP32 sweeps and
 Riemann solves; FP64 pressure recovery, trouble tests and RK accumulation),
 and the learned form (FP32 bulk plus the troubled-cell predictor, run once
 per step, whose pre-flagged cells get MUSCL fluxes before the candidate; the
-plain check still runs on every cell), also timed with the network on a side
-stream beside each step's first stage ("learned-overlap", the headline).
+plain check still runs on every cell, the headline), also timed with the
+network on a side stream beside each step's first stage ("learned-overlap":
+no faster, since the stage's kernels already occupy every CU, and it slows the
+candidate kernel it shares the chip with).
 
-    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32,learned,learned-overlap] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n N] [--forms fp64,fp32,learned-overlap,learned] [--no-cpu]
 
 Prints one JSON line.  `value` is Mcell-updates/s per RK stage (cells x
 stages / s) of the last form listed.  `roofline` is for the kernel that takes
@@ -78,7 +80,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--forms", default="fp64,fp32,learned,learned-overlap",
+    ap.add_argument("--forms", default="fp64,fp32,learned-overlap,learned",
                     help="comma list of forms to time (fp64, fp32, learned, learned-overlap = the network on a "
                          "side stream beside each step's first stage); the last one is the headline")
     ap.add_argument("--no-cpu", action="store_true")

@@ -103,17 +103,17 @@ __global__ void __launch_bounds__(256) k_sweep_slide(const T* __restrict__ u, T*
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
   const int m0 = axis == 1 ? ma : n0, m1 = axis == 2 ? ma : n1, m2 = axis == 3 ? ma : n2;
   int v, i, j, k, a0;
-  if (!ZAX) {  // fast index: k over n2; slow: (v, other axis, block of R along the sweep axis)
-    // blocks of neighbouring windows (which share K - 1 input rows) on one XCD
+  if (!ZAX) {  // threads over the (other axis, z) plane, z fastest; blocks: window along the axis fastest,
+    // then the plane, then v (neighbouring windows, which share K - 1 input rows, on one XCD)
     const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);
-    k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
-    if (k >= n2) return;
-    unsigned rest = blk / (unsigned)nblk_fast;
-    const int ab = (int)(rest % (unsigned)nb_axis);
-    rest /= (unsigned)nb_axis;
+    const int ab = (int)(blk % (unsigned)nb_axis);
+    const unsigned rest = blk / (unsigned)nb_axis;
     const int no = axis == 1 ? n1 : n0;
-    const int o = (int)(rest % (unsigned)no);
-    v = (int)(rest / (unsigned)no);
+    const int t = (int)(rest % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
+    v = (int)(rest / (unsigned)nblk_fast);
+    if (t >= no * n2) return;
+    const int o = t / n2;
+    k = t - o * n2;
     a0 = ab * R;
     i = axis == 1 ? a0 : o;
     j = axis == 1 ? o : a0;
@@ -163,10 +163,11 @@ int launch_slide(const T* u, T* out, int nv, int n0, int n1, int n2, int axis, h
   const int na = axis == 1 ? n0 : axis == 2 ? n1 : n2, ma = na - (K - 1);
   if (axis != 3) {
     constexpr int R = FVHOE_SLIDE_R;
-    const int nbk = (n2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? n1 : n0;
-    const long long blocks = (long long)nbk * nb * no * nv;
-    if (blocks >= (1ll << 31)) return -1;
-    k_sweep_slide<T, KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbk, nb);
+    const long long plane = (long long)(axis == 1 ? n1 : n0) * n2;
+    const int nbp = (int)((plane + 255) / 256), nb = (ma + R - 1) / R;
+    const long long blocks = (long long)nbp * nb * nv;
+    if (blocks >= (1ll << 31) || plane >= (1ll << 31)) return -1;
+    k_sweep_slide<T, KIND, P, R, false><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, axis, nbp, nb);
   } else {
     constexpr int R = FVHOE_SLIDE_RZ;
     const int nb = (ma + R - 1) / R;
@@ -327,8 +328,12 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
 // register window of the five variables over R consecutive faces, so w is
 // read once and only the fluxes are written; the sums and the solver are the
 // separate kernels' operations in the same order (bit-identical).
+// waves per SIMD the face-flux kernel is compiled for (1: the compiler's choice)
+#ifndef FVHOE_FACE_WAVES
+#define FVHOE_FACE_WAVES 1
+#endif
 template <typename T, int P, int D, int R, bool ZAX, int RS>
-__global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* __restrict__ out, int n0, int n1,
+__global__ void __launch_bounds__(256, FVHOE_FACE_WAVES) k_face_flux(const T* __restrict__ w, T* __restrict__ out, int n0, int n1,
                                                    int n2, int nblk_fast, int nb_axis, double gamma_d,
                                                    double csq_floor_d) {
   const T gamma = T(gamma_d), csq_floor = T(csq_floor_d);
@@ -338,12 +343,14 @@ __global__ void __launch_bounds__(256) k_face_flux(const T* __restrict__ w, T* _
   const int m0 = axis == 1 ? mf : n0, m1 = axis == 2 ? mf : n1, m2 = axis == 3 ? mf : n2;
   int i, j, k, a0;
   if (!ZAX) {
-    const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour windows on one XCD
-    k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
-    if (k >= n2) return;
-    const unsigned rest = blk / (unsigned)nblk_fast;
-    const int ab = (int)(rest % (unsigned)nb_axis), o = (int)(rest / (unsigned)nb_axis);
-    if (o >= (axis == 1 ? n1 : n0)) return;
+    // threads over the plane of the two other indices, z fastest (so no wave idles at the end of a z row);
+    // the windows along the axis fastest across blocks, neighbours (sharing K - 1 rows) on one XCD
+    const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int ab = (int)(blk % (unsigned)nb_axis);
+    const int t = (int)(blk / (unsigned)nb_axis) * 256 + (int)threadIdx.x;
+    if (t >= (axis == 1 ? n1 : n0) * n2) return;
+    const int o = t / n2;
+    k = t - o * n2;
     a0 = ab * R;
     i = axis == 1 ? a0 : o;
     j = axis == 1 ? o : a0;
@@ -453,9 +460,16 @@ __global__ void __launch_bounds__(256) k_sweep_zpair(const T* __restrict__ u, T*
   }
 }
 
+// outputs per thread along the strided axis of the z-pair kernel
+// (3-D wave 256^3 p=7, per launch FP64 / FP32: R = 4: 317-479 / -, 8: 282-424 / 155-232, 16: 268-395 /
+// 141-222 us)
+#ifndef FVHOE_ZPAIR_R
+#define FVHOE_ZPAIR_R 16
+#endif
+
 template <typename T, int KIND, int P, bool ZFIRST>
 int launch_zpair(const T* u, T* out, int nv, int n0, int n1, int n2, int axis, hipStream_t st) {
-  constexpr int K = ntaps(P), R = 8, VC = 64 - (K - 1);
+  constexpr int K = ntaps(P), R = FVHOE_ZPAIR_R, VC = 64 - (K - 1);
   const int ns = axis == 1 ? n0 : n1, ms = ns - (K - 1), m2 = n2 - (K - 1);
   const int nzw = (m2 + VC - 1) / VC, nsb = (ms + R - 1) / R, nother = axis == 1 ? n1 : n0;
   const long long waves = (long long)nzw * nsb * nother * nv;
@@ -491,13 +505,15 @@ template <typename T, int KIND, int P, int R, int RB>
 __global__ void __launch_bounds__(256) k_sweep_xypair(const T* __restrict__ u, T* __restrict__ out, int n0, int n1,
                                                       int n2, int afirst, int nblk_fast, int nbA, int nbB) {
   constexpr int K = ntaps(P), NA = R + K - 1, NB = RB + K - 1;
-  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour runs on one XCD
-  const int k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
-  if (k >= n2) return;
-  unsigned rest = blk / (unsigned)nblk_fast;
-  const int ia = (int)(rest % (unsigned)nbA);
-  rest /= (unsigned)nbA;
-  const int ib = (int)(rest % (unsigned)nbB), v = (int)(rest / (unsigned)nbB);
+  // threads over (v, z), z fastest; blocks: run along A fastest (neighbour runs on one XCD), then B, then
+  // the (v, z) chunk
+  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int ia = (int)(blk % (unsigned)nbA);
+  unsigned rest = blk / (unsigned)nbA;
+  const int ib = (int)(rest % (unsigned)nbB);
+  const int t = (int)(rest / (unsigned)nbB) * 256 + (int)threadIdx.x;
+  if (t >= nblk_fast * n2) return;  // nblk_fast carries nv here
+  const int v = t / n2, k = t - (t / n2) * n2;
   const bool a_is_x = afirst == 1;
   const int nA = a_is_x ? n0 : n1, nB = a_is_x ? n1 : n0, mA = nA - (K - 1), mB = nB - (K - 1);
   const int m0 = n0 - (K - 1), m1 = n1 - (K - 1);
@@ -543,22 +559,32 @@ __global__ void __launch_bounds__(256) k_sweep_xypair(const T* __restrict__ u, T
   }
 }
 
-// outputs per thread along the first and the second axis of the x-y pair kernel
+// outputs per thread along the first and the second axis of the x-y pair kernel, FP64 and FP32 (3-D wave
+// 256^3 p=7, per launch FP64 / FP32: 4 x 16: 656-676 / 363, 4 x 32: 638-639 / 343-361, 8 x 16: 627-657 /
+// 316, 8 x 8: 733-769 / 340-345, 2 x 32: 790-809 us)
 #ifndef FVHOE_XY_R
 #define FVHOE_XY_R 4
 #endif
 #ifndef FVHOE_XY_RB
-#define FVHOE_XY_RB 16
+#define FVHOE_XY_RB 32
+#endif
+#ifndef FVHOE_XY32_R
+#define FVHOE_XY32_R 8
+#endif
+#ifndef FVHOE_XY32_RB
+#define FVHOE_XY32_RB 16
 #endif
 
 template <typename T, int KIND, int P>
 int launch_xypair(const T* u, T* out, int nv, int n0, int n1, int n2, int afirst, hipStream_t st) {
-  constexpr int K = ntaps(P), R = FVHOE_XY_R, RB = FVHOE_XY_RB;
+  constexpr bool F64 = sizeof(T) == 8;
+  constexpr int K = ntaps(P), R = F64 ? FVHOE_XY_R : FVHOE_XY32_R, RB = F64 ? FVHOE_XY_RB : FVHOE_XY32_RB;
   const int nA = afirst == 1 ? n0 : n1, nB = afirst == 1 ? n1 : n0;
-  const int nbk = (n2 + 255) / 256, nbA = (nA - (K - 1) + R - 1) / R, nbB = (nB - (K - 1) + RB - 1) / RB;
-  const long long blocks = (long long)nbk * nbA * nbB * nv;
+  const int nbk = (int)(((long long)nv * n2 + 255) / 256), nbA = (nA - (K - 1) + R - 1) / R,
+            nbB = (nB - (K - 1) + RB - 1) / RB;
+  const long long blocks = (long long)nbk * nbA * nbB;
   if (blocks >= (1ll << 31)) return -1;
-  k_sweep_xypair<T, KIND, P, R, RB><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, afirst, nbk, nbA, nbB);
+  k_sweep_xypair<T, KIND, P, R, RB><<<(unsigned)blocks, 256, 0, st>>>(u, out, n0, n1, n2, afirst, nv, nbA, nbB);
   return 0;
 }
 
@@ -588,10 +614,9 @@ int launch_face_flux_rs(const T* w, T* out, int n0, int n1, int n2, double gamma
   const int axis = D + 1, na = axis == 1 ? n0 : axis == 2 ? n1 : n2, mf = na - K;
   const int nb = (mf + R - 1) / R;
   if (axis != 3) {
-    const int nbk = (n2 + 255) / 256, no = axis == 1 ? n1 : n0;
-    const long long blocks = (long long)nbk * nb * no;
-    if (blocks >= (1ll << 31)) return -1;
-    k_face_flux<T, P, D, R, false, RS><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, nbk, nb, gamma,
+    const long long plane = (long long)(axis == 1 ? n1 : n0) * n2, blocks = (plane + 255) / 256 * nb;
+    if (blocks >= (1ll << 31) || plane >= (1ll << 31)) return -1;
+    k_face_flux<T, P, D, R, false, RS><<<(unsigned)blocks, 256, 0, st>>>(w, out, n0, n1, n2, 0, nb, gamma,
                                                                        csq_floor);
   } else {
     const long long threads = (long long)nb * n1 * n0;
@@ -732,14 +757,15 @@ __global__ void __launch_bounds__(256) k_bc_sweep_slide(const double* __restrict
   const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
   const int na = axis == 1 ? N0 : N1, ma = na - (K - 1);
   const int m0 = axis == 1 ? ma : N0, m1 = axis == 2 ? ma : N1;
-  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);  // neighbour windows on one XCD
-  const int k = (int)(blk % (unsigned)nblk_fast) * 256 + (int)threadIdx.x;
-  if (k >= N2) return;
-  unsigned rest = blk / (unsigned)nblk_fast;
-  const int ab = (int)(rest % (unsigned)nb_axis);
-  rest /= (unsigned)nb_axis;
+  // threads over the (other axis, z) plane, z fastest; blocks: window along the axis fastest (neighbours on
+  // one XCD), then the plane, then v
+  const unsigned blk = (unsigned)xcd_tile((int)blockIdx.x, (int)gridDim.x);
+  const int ab = (int)(blk % (unsigned)nb_axis);
+  const unsigned rest = blk / (unsigned)nb_axis;
   const int no = axis == 1 ? N1 : N0;
-  const int o = (int)(rest % (unsigned)no), v = (int)(rest / (unsigned)no);
+  const int t = (int)(rest % (unsigned)nblk_fast) * 256 + (int)threadIdx.x, v = (int)(rest / (unsigned)nblk_fast);
+  if (t >= no * N2) return;
+  const int o = t / N2, k = t - (t / N2) * N2;
   const int a0 = ab * R;
   const int dS = axis == 1 ? 0 : 1, dO = axis == 1 ? 1 : 0;  // sweep axis and the other strided axis
   bool fk, fo;
@@ -838,6 +864,48 @@ __global__ void k_euler_eq_rk(const T* __restrict__ F, const T* __restrict__ G, 
     if (dudt) dudt[idx] = rk.kind == FVHOE_RK_SSPRK3_STAGE3_SUM ? rk.k0[idx] + acc : acc;
     rk.out[idx] = rk_combine(rk, (int)idx, acc);
   }
+}
+
+// The divergence (and, with RK, the stage's RK combination) for grids whose contiguous axis holds at
+// least a wave: blocks of 64 lanes along z x 4 waves along y, the (v, x) pair from blockIdx.z, so the
+// row offsets are wave-uniform and no cell index is divided (the grid-stride kernels above split a
+// linear index per cell).  The same operations in the same order as those kernels.
+template <typename T, bool RK>
+__global__ void __launch_bounds__(256) k_euler_eq_rows(const T* __restrict__ F, const T* __restrict__ G,
+                                                       const T* __restrict__ H, double* __restrict__ dudt, int n0,
+                                                       int n1, int n2, int active, double cx, double cy, double cz,
+                                                       Strides3 fs, Strides3 gs, Strides3 hs, RkArgs rk) {
+  const int k = (int)blockIdx.x * 64 + ((int)threadIdx.x & 63);
+  const int j = (int)blockIdx.y * 4 + ((int)threadIdx.x >> 6);
+  if (k >= n2 || j >= n1) return;
+  const int vi = (int)blockIdx.z, v = vi / n0, i = vi - v * n0;
+  const long long idx = ((long long)vi * n1 + j) * n2 + k;
+  double acc = 0.0;
+  if (active & 1) {
+    const long long b = v * fs.v + i * fs.a + j * fs.b + k;
+    acc = acc + cx * ((double)F[b + fs.a] - (double)F[b]);
+  }
+  if (active & 2) {
+    const long long b = v * gs.v + i * gs.a + j * gs.b + k;
+    acc = acc + cy * ((double)G[b + gs.b] - (double)G[b]);
+  }
+  if (active & 4) {
+    const long long b = v * hs.v + i * hs.a + j * hs.b + k;
+    acc = acc + cz * ((double)H[b + 1] - (double)H[b]);
+  }
+  if (RK) {
+    if (dudt) dudt[idx] = rk.kind == FVHOE_RK_SSPRK3_STAGE3_SUM ? rk.k0[idx] + acc : acc;
+    rk.out[idx] = rk_combine(rk, (int)idx, acc);
+  } else {
+    dudt[idx] = acc;
+  }
+}
+
+// whether the row kernel applies: a z row of at least one wave and a grid within the launch limits
+inline bool rows_grid(int nv, int n0, int n1, int n2, dim3* g) {
+  if (n2 < 64 || (long long)nv * n0 > 65535 || (n1 + 3) / 4 > 65535) return false;
+  *g = dim3((unsigned)((n2 + 63) / 64), (unsigned)((n1 + 3) / 4), (unsigned)(nv * n0));
+  return true;
 }
 
 __global__ void k_init_minmax(unsigned long long* acc, int nv) {
@@ -1304,8 +1372,13 @@ int euler_equation_impl(const T* F, const long long* fst, const T* G, const long
   if (gst) gs = Strides3{gst[0], gst[1], gst[2]};
   if (hst) hs = Strides3{hst[0], hst[1], hst[2]};
   const long long total = (long long)nv * n0 * n1 * n2;
-  k_euler_eq<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy, cz,
-                                                                  fs, gs, hs);
+  dim3 rg;
+  if (rows_grid(nv, n0, n1, n2, &rg))
+    k_euler_eq_rows<T, false><<<rg, 256, 0, (hipStream_t)stream>>>(F, G, H, dudt, n0, n1, n2, active, cx, cy, cz, fs,
+                                                                   gs, hs, RkArgs{});
+  else
+    k_euler_eq<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy,
+                                                                    cz, fs, gs, hs);
   return launch_status();
 }
 
@@ -1314,10 +1387,11 @@ int launch_bc_slide(const double* u, T* out, const BcArgs& a, int nv, int axis, 
   constexpr int K = ntaps(P), R = FVHOE_SLIDE_R;
   const int N0 = a.n[0] + 2 * a.g[0], N1 = a.n[1] + 2 * a.g[1], N2 = a.n[2] + 2 * a.g[2];
   const int na = axis == 1 ? N0 : N1, ma = na - (K - 1);
-  const int nbk = (N2 + 255) / 256, nb = (ma + R - 1) / R, no = axis == 1 ? N1 : N0;
-  const long long blocks = (long long)nbk * nb * no * nv;
-  if (blocks >= (1ll << 31)) return -1;
-  k_bc_sweep_slide<T, KIND, P, R><<<(unsigned)blocks, 256, 0, st>>>(u, out, a, axis, nbk, nb);
+  const long long plane = (long long)(axis == 1 ? N1 : N0) * N2;
+  const int nbp = (int)((plane + 255) / 256), nb = (ma + R - 1) / R;
+  const long long blocks = (long long)nbp * nb * nv;
+  if (blocks >= (1ll << 31) || plane >= (1ll << 31)) return -1;
+  k_bc_sweep_slide<T, KIND, P, R><<<(unsigned)blocks, 256, 0, st>>>(u, out, a, axis, nbp, nb);
   return 0;
 }
 
@@ -1383,8 +1457,13 @@ int euler_equation_rk_impl(const T* F, const long long* fst, const T* G, const l
   if (gst) gs = Strides3{gst[0], gst[1], gst[2]};
   if (hst) hs = Strides3{hst[0], hst[1], hst[2]};
   const RkArgs rk{kind, rkf->out, rkf->u0, rkf->k0, rkf->k1, rkf->k2, rkf->dt_step};
-  k_euler_eq_rk<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx, cy,
-                                                                     cz, fs, gs, hs, rk);
+  dim3 rg;
+  if (rows_grid(nv, n0, n1, n2, &rg))
+    k_euler_eq_rows<T, true><<<rg, 256, 0, (hipStream_t)stream>>>(F, G, H, dudt, n0, n1, n2, active, cx, cy, cz, fs,
+                                                                  gs, hs, rk);
+  else
+    k_euler_eq_rk<T><<<grid_for(total), TPB, 0, (hipStream_t)stream>>>(F, G, H, dudt, nv, n0, n1, n2, active, cx,
+                                                                       cy, cz, fs, gs, hs, rk);
   return launch_status();
 }
 

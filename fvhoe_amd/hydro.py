@@ -117,11 +117,11 @@ def advection_dt(h: float, vx: float = 0.0, vy: float = 0.0, vz: float = 0.0, CF
     return CFL * h / (vx + vy + vz)
 
 
-def hydro_dt_device(w, h: float, ndim: int, CFL: float, gamma: float, csq_floor: float):
-    """hydro_dt with the result left on the device (1-element tensor)."""
+def hydro_dt_device(w, h: float, ndim: int, CFL: float, gamma: float, csq_floor: float, out=None):
+    """hydro_dt with the result left on the device (1-element tensor; into out when given)."""
     x, _ = _lib.as_device_bulk(w)
     t = _lib.torch()
-    dt = _lib.empty((1,))
+    dt = _lib.empty((1,)) if out is None else out
     scratch = _lib.empty((1,), dtype=t.int64)
     _lib.call("fvhoe_hydro_dt" + _lib.sfx(x), _lib.ptr(x), _ncell(x), float(h), int(ndim), float(CFL), float(gamma),
               float(csq_floor), _lib.ptr(dt), _lib.ptr(scratch), _lib.stream())

@@ -383,11 +383,10 @@ class EulerSolver(ODE):
             if self._trouble is not None:
                 self._log_troubles_device(self._trouble)
         else:
-            dt, (F, G, H) = self.hydrofluxes(u, self.p, views=not self.a_posteriori_slope_limiting)
+            dt, (F, G, H) = self.hydrofluxes(u, self.p, views=not self.a_posteriori_slope_limiting, dt_out=dt_out)
             if self.a_posteriori_slope_limiting:
                 F, G, H = self.revise_fluxes(u, F, G, H, dt, self.force_trouble)
             self._euler_equation_into(F, G, H, k_out)
-            dt_out.copy_(dt)
 
     def _fused_rk(self) -> bool:
         # the composed stage without a-posteriori limiting ends in the divergence + RK combination kernel
@@ -401,8 +400,7 @@ class EulerSolver(ODE):
     def _composed_stage_rk(self, u, k_out, dt_out, kind, out, u0, k0, k1, k2, dt_step, set_dt_step, t_dev, target):
         """The composed stage (no a-posteriori limiting) with the divergence and the RK combination in one
         kernel; the step's dt is clipped to the target first, as the separate combine kernels take it."""
-        dt, (F, G, H) = self.hydrofluxes(u, self.p, views=True)
-        dt_out.copy_(dt)
+        _, (F, G, H) = self.hydrofluxes(u, self.p, views=True, dt_out=dt_out)
         if set_dt_step:
             _lib.call("fvhoe_hit_target", _lib.ptr(dt_step), _lib.ptr(dt_out), _lib.ptr(t_dev),
                       float(target) if target is not None else 0.0, int(target is not None), _lib.stream())
@@ -506,9 +504,11 @@ class EulerSolver(ODE):
         w = compute_primitives(self.hydro_state, centres, self.gamma)
         return interpolate_fv_averages(w, p) if fv_average else w
 
-    def hydrofluxes(self, u, p, slope_limiter: str = None, timer_prefix: str = "", views: bool = False):
+    def hydrofluxes(self, u, p, slope_limiter: str = None, timer_prefix: str = "", views: bool = False,
+                    dt_out=None):
         """dt (device), (F, G, H) (fvhoe/solver.py:456-651).  views=True returns the trimmed fluxes as views
-        into the untrimmed arrays (for the strided divergence), else contiguous copies."""
+        into the untrimmed arrays (for the strided divergence), else contiguous copies; dt_out (1-element
+        device tensor) receives dt when given."""
         if slope_limiter is None:
             limit = False
         elif max(p) <= 1:
@@ -520,9 +520,9 @@ class EulerSolver(ODE):
         gw = max(cm + 1, 2 * cm)
         w_bc = self.interpolate_primitives_from_conservatives(u, p, tuple(gw if a else 0 for a in self.active))
         if self.fixed_dt is None:
-            dt = hydro_dt_device(w_bc, min(self.h), self.ndim, self.CFL, self.gamma, self.csq_floor)
+            dt = hydro_dt_device(w_bc, min(self.h), self.ndim, self.CFL, self.gamma, self.csq_floor, out=dt_out)
         else:
-            dt = _lib.empty((1,))
+            dt = _lib.empty((1,)) if dt_out is None else dt_out
             dt.fill_(float(self.fixed_dt))
         order = {0: (3, 2), 1: (1, 3), 2: (2, 1)}
         tr_order = {0: (2, 3), 1: (1, 3), 2: (1, 2)}

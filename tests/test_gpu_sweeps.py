@@ -72,7 +72,8 @@ def test_composed_3d_stage_with_long_z(p, rs):
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("axis", [1, 2])
 def test_fused_fill_and_sweep(axis, f32):
-    """fvhoe_bc_stencil_sweep == fill + sweep bit for bit, every non-slab kind on every side."""
+    """fvhoe_bc_stencil_sweep == fill + sweep bit for bit, every non-slab kind on every side, and both
+    equal the oracle's fill followed by its sweep (in float32 after the fill for the FP32 form)."""
     from fvhoe_amd.boundary_conditions import BoundaryCondition
 
     torch = _lib.torch()
@@ -92,13 +93,17 @@ def test_fused_fill_and_sweep(axis, f32):
                 assert torch.equal(got.view(torch.int32), ref.view(torch.int32)), (kinds, p, axis)
             else:
                 assert_bits_equal(got.cpu().numpy(), ref.cpu().numpy(), f"{kinds} p={p} axis={axis}")
+            filled = O.apply_bc(u, g, kinds)
+            want = O.conservative_interpolation(filled.astype(np.float32) if f32 else filled, p, axis, "c")
+            assert np.array_equal(got.cpu().numpy().view(np.uint8), want.view(np.uint8)), (kinds, p, axis, "oracle")
 
 
 @pytest.mark.parametrize("f32", [False, True], ids=["f64", "f32"])
 @pytest.mark.parametrize("pos", ["c", "t"])
 @pytest.mark.parametrize("first", [1, 2], ids=["x-y", "y-x"])
 def test_xypair_sweeps(first, pos, f32):
-    """fvhoe_stencil_sweep_xypair == two single sweeps bit for bit."""
+    """fvhoe_stencil_sweep_xypair == two single sweeps == the oracle's two sweeps, bit for bit (the
+    oracle's float32 sweeps for the FP32 form)."""
     rng = np.random.default_rng(7 + first)
     u = rng.standard_normal((5, 29, 27, 33))
     x = _lib.as_device(u)[0]
@@ -115,3 +120,6 @@ def test_xypair_sweeps(first, pos, f32):
             assert _lib.torch().equal(pair.view(_lib.torch().int32), ref.view(_lib.torch().int32)), (pos, p, first)
         else:
             assert_bits_equal(pair.cpu().numpy(), ref.cpu().numpy(), f"{pos} p={p} first={first}")
+        uo = u.astype(np.float32) if f32 else u
+        want = _oracle_sweep(_oracle_sweep(uo, pos, p, first), pos, p, 3 - first)
+        assert np.array_equal(pair.cpu().numpy().view(np.uint8), want.view(np.uint8)), (pos, p, first, "oracle")
