@@ -285,16 +285,28 @@ def run_wave3d(a):
 def run_ensemble(a):
     """512 Shu-Osher-type 1-D instances, N ~ U{256..8192}, p=5 HLLC a-posteriori, SSP-RK3 to t=1.8, "ic"
     boundaries, every step one captured graph over all instances; tile size and stage form (chain of
-    launches or one persistent kernel per stage) from the autotuner."""
+    launches or one persistent kernel per stage) from the autotuner.  Forms: fp64, fp32 and learned (the
+    FP32 form with the troubled-cell predictor on the runs seen as planar 2-D states, the chain of
+    launches); with learned, an untimed FP32 run counts the plain check's flags for the second-pass
+    share, and the final densities of the two are compared."""
+    import numpy as np
     import torch
 
     from fvhoe_amd.ensemble import shu_osher_ensemble_solver
+    from fvhoe_amd.predictor import TroubledCellPredictor
 
-    forms = {}
+    forms, finals = {}, {}
+    names = []
     for form in a.forms.split(","):
-        if form.startswith("learned"):
-            continue  # the predictor network is two-dimensional
-        e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision=form)
+        f = "learned" if form.startswith("learned") else form
+        if f not in names:
+            names.append(f)
+    for form in names:
+        if form == "learned":
+            e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision="fp32",
+                                          predictor=TroubledCellPredictor.load())
+        else:
+            e = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision=form)
         tune = {f"{k[0]}/{k[1]}": round(v, 4) for k, v in e.autotune(a.ensemble_t, steps=10).items()}
         print(f"# ensemble {form}: {sum(e.n)} cells, autotune {tune} -> tile {e.tile}, "
               f"{'persistent' if e.persistent else 'chain'}", file=sys.stderr, flush=True)
@@ -308,11 +320,35 @@ def run_ensemble(a):
                        "cell_updates": cu, "tile": e.tile, "stage_form": "persistent" if e.persistent else "chain",
                        "autotune_ms_per_step": tune,
                        "t_min": float(e.times().min()), "t_max": float(e.times().max())}
+        if form == "learned":
+            st = e.predictor_stats.cpu().numpy()
+            forms[form]["predictor"] = {"preflagged": int(st[0]), "second_pass": int(st[1]),
+                                        "preflag_unflagged": int(st[2]),
+                                        "logit_threshold": e.predictor.logit_threshold}
+        if form in ("fp32", "learned"):
+            finals[form] = [e.state(i)[0] for i in range(e.count)]
         print(f"# ensemble {form}: {wall:.2f}s, {steps} steps", file=sys.stderr, flush=True)
+        del e
+        torch.cuda.empty_cache()
+    if "learned" in forms:
+        # the plain FP32 check's flags over the same run (untimed), and the learned form's density
+        # difference from the plain FP32 run at the end time
+        c = shu_osher_ensemble_solver(a.ensemble_count, seed=0, bulk_precision="fp32", count_troubles=True)
+        c.run(a.ensemble_t)
+        plain_flags = int(c.predictor_stats[1].item())
+        if "fp32" not in finals:
+            finals["fp32"] = [c.state(i)[0] for i in range(c.count)]
+        del c
+        pr = forms["learned"]["predictor"]
+        pr["check_flags_without_predictor"] = plain_flags
+        pr["second_pass_share"] = round(pr["second_pass"] / max(plain_flags, 1), 4)
+        pr["preflag_per_check_flag"] = round(pr["preflagged"] / max(plain_flags, 1), 4)
+        d = [float(np.abs(x - y).mean()) for x, y in zip(finals["learned"], finals["fp32"])]
+        forms["learned"]["l1_density_vs_fp32"] = {"mean": float(np.mean(d)), "max": float(np.max(d))}
     head = forms[list(forms)[-1]]
     out = {"metric": "Mcell-updates/s per RK stage (ensemble)", "value": head["value"], "unit": "Mcell-updates/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-           "dtype": "f32" if list(forms)[-1] == "fp32" else "f64", "data": "synthetic",
+           "dtype": "f64" if list(forms)[-1] == "fp64" else "f32", "data": "synthetic",
            "config": {"workload": f"shu_osher_ensemble_{a.ensemble_count}_p5_hllc_aposteriori_ssprk3_t{a.ensemble_t}",
                       "boundary": "ic", "form": list(forms)[-1], "parallelism": "single GPU"},
            "forms": forms}

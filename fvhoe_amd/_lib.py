@@ -56,6 +56,7 @@ class FvhoePredict(ctypes.Structure):
 
 PRED_NPARAMS = 8433
 PRED_PACKED_BYTES = 24576
+PRED_PLANAR_BYTES = 4608
 PRED_HALO = 6
 
 _P = c_void_p
@@ -108,6 +109,7 @@ _SIGNATURES = {
     "fvhoe_stage_timing": (c_int, [c_int]),
     "fvhoe_stage_timing_read": (c_int, [POINTER(c_double), POINTER(c_longlong)]),
     "fvhoe_pred_pack": (c_int, [_P, _P, _P]),
+    "fvhoe_pred_pack_planar": (c_int, [_P, _P, _P]),
     "fvhoe_pred_infer": (c_int, [_P, c_int, c_int, _P, c_float, _P, _P, _P]),
     "fvhoe_pred_features": (c_int, [POINTER(FvhoeStageConfig), _P, _P, _P]),
     "fvhoe_pred_conv_forward": (c_int, [_P, _P, _P, _P, c_int, c_int, c_int, c_int, c_int, c_int, _P]),
@@ -123,6 +125,10 @@ _SIGNATURES = {
     "fvhoe_ensemble_step_persistent": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P,
                                                c_double, _P, _P]),
     "fvhoe_ensemble_status": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, POINTER(c_int)]),
+    "fvhoe_ensemble_step_predicted": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P,
+                                              c_double, _P, POINTER(FvhoePredict), _P]),
+    "fvhoe_ensemble_pred_logits": (c_int, [POINTER(FvhoeStageConfig), c_int, POINTER(c_int), c_int, _P, _P, _P, _P,
+                                           _P, _P]),
     "fvhoe_event_create": (c_int, [POINTER(c_void_p)]),
     "fvhoe_event_destroy": (c_int, [_P]),
     "fvhoe_stage_predicted_workspace_bytes": (c_int, [POINTER(FvhoeStageConfig), POINTER(c_ulonglong)]),

@@ -25,6 +25,7 @@
 
 #include "fvhoe_common.h"
 #include "fvhoe_hip.h"
+#include "predictor_infer.h"
 
 using namespace fvhoe;
 
@@ -93,6 +94,10 @@ struct Ens {
   const double* slab_lo;
   const double* slab_hi;
   int slab_g;
+  // learned stage: the step's pre-flags (per cell), and per instance the pre-flagged cells the check does
+  // not flag (null without a predictor)
+  const signed char* pre;
+  int* fpc;
 };
 
 // ghost value of a slab side of instance i, or nothing (returns false) for a cell that no slab covers
@@ -138,6 +143,7 @@ __device__ __forceinline__ void d_ens_reset(const Ens& e, int i) {
   if (i >= e.count) return;
   e.acc[i * NACC] = 0ull;
   for (int k = 1; k < NACC; ++k) e.acc[i * NACC + k] = (k & 1) ? ~0ull : 0ull;
+  if (e.fpc) e.fpc[i] = 0;
   e.cnt[i] = 0;
 }
 __global__ void k_ens_reset(Ens e) { d_ens_reset(e, blockIdx.x * blockDim.x + threadIdx.x); }
@@ -411,6 +417,16 @@ __device__ __forceinline__ void ens_list_append(const Ens& e, int i, int t, int 
   if (t) list[e.off_u[i] + bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
 }
 
+// With a predictor, a pre-flagged cell already has MUSCL fluxes at its faces: the check lists only the
+// others, and counts the pre-flagged cells it does not flag (one atomic per wave)
+__device__ __forceinline__ int ens_pre_filter(const Ens& e, int i, int n, int c, int t) {
+  if (!e.pre) return t;
+  const int pf = c < n ? e.pre[e.off_u[i] + c] : 0;
+  const unsigned long long b = __ballot(pf && !t);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(&e.fpc[i], (int)__popcll(b));
+  return t && !pf;
+}
+
 template <typename T, int P, int TILE>
 __device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __restrict__ W, const T* __restrict__ WS,
                                              int* __restrict__ list) {
@@ -423,6 +439,7 @@ __device__ __forceinline__ void d_ens_detect(const Ens& e, int2 tl, const T* __r
     const int NS = n + 6;
     t = ens_detect_cell<T, P>(e, i, n, c, W, [&](int v, int o) { return Si[v * NS + c + 3 + o]; });
   }
+  t = ens_pre_filter(e, i, n, c, t);
   ens_list_append<TILE>(e, i, t, c, list);
 }
 
@@ -441,6 +458,7 @@ __device__ __forceinline__ void d_ens_cand_detect(const Ens& e, int2 tl, const d
   const int n = e.n[i], c = tl.y + threadIdx.x;
   int t = 0;
   if (c < n) t = ens_detect_cell<T, P>(e, i, n, c, W, [&](int v, int o) { return SO[v][threadIdx.x + 3 + o]; });
+  t = ens_pre_filter(e, i, n, c, t);
   ens_list_append<TILE>(e, i, t, c, list);
 }
 template <typename T, int P, int TILE>
@@ -487,10 +505,11 @@ __device__ void ens_fallback_flux(const Ens& e, int i, const double* __restrict_
 
 template <typename T, int TILE>
 __device__ __forceinline__ void d_ens_fallback(const Ens& e, int2 tl, const double* __restrict__ u,
-                                               const int* __restrict__ list, T* __restrict__ F) {
+                                               const int* __restrict__ list, T* __restrict__ F,
+                                               const int* __restrict__ counts = nullptr) {
   const int i = tl.x;
   if (!e.active[i]) return;
-  const int n = e.n[i], cnt = e.cnt[i];
+  const int n = e.n[i], cnt = counts ? counts[i] : e.cnt[i];
   const double* ui = u + 5 * e.off_u[i];
   T* Fi = F + 5 * e.off_f[i];
   for (int k = threadIdx.x; k < 2 * TILE; k += TILE) {
@@ -506,8 +525,8 @@ __device__ __forceinline__ void d_ens_fallback(const Ens& e, int2 tl, const doub
 template <typename T, int TILE>
 __global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __restrict__ tiles,
                                                       const double* __restrict__ u, const int* __restrict__ list,
-                                                      T* __restrict__ F) {
-  d_ens_fallback<T, TILE>(e, tiles[blockIdx.x], u, list, F);
+                                                      T* __restrict__ F, const int* __restrict__ counts) {
+  d_ens_fallback<T, TILE>(e, tiles[blockIdx.x], u, list, F, counts);
 }
 
 // divergence (fvhoe/solver.py:744-750) and the SSP-RK3 combination of this stage
@@ -547,6 +566,11 @@ __global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restric
   d_ens_final<T, TILE>(e, tiles[blockIdx.x], F, stage, u, s, k0, k1);
 }
 
+__global__ void k_ens_zero(int* __restrict__ x, int count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[i] = 0;
+}
+
 __device__ __forceinline__ void d_ens_advance(const Ens& e, int i, double* __restrict__ t, int* __restrict__ steps) {
   if (i < e.count && e.active[i]) {
     t[i] = t[i] + e.dt_step[i];
@@ -555,6 +579,179 @@ __device__ __forceinline__ void d_ens_advance(const Ens& e, int i, double* __res
 }
 __global__ void k_ens_advance(Ens e, double* __restrict__ t, int* __restrict__ steps) {
   d_ens_advance(e, blockIdx.x * blockDim.x + threadIdx.x, t, steps);
+}
+
+// ---------------------------------------------------------------------------
+// Troubled-cell predictor on the 1-D runs: the 2-D network applied to each run seen as a 2-D state that
+// does not vary along y.  On such an input every valid 5x5 cross-correlation acts as a 1-D one along x
+// with the kernel summed over its y taps, so the network reduces to three 1-D convolutions of 5 taps
+// (4 -> 16 -> 16 -> 1 channels) with the summed kernels (fvhoe_pred_pack_planar).  One block per tile of
+// TILE cells of an instance: the features of the TILE + 12 cells around it (the 2-D features' definition:
+// (rho, vx, vy, P) of W, ghosts mapped into the interior, scaled by the instance's NAD range) are staged
+// in LDS as bfloat16; layers 1 and 2 run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation (A = the 16
+// output channels' weights x 32 of K, B = 16 consecutive pixels x 32 of K, as the 2-D kernel), layer 3 as
+// 80 FP32 products per cell.  Flags (logit > threshold) go to the per-cell pre-flag array and to the
+// instance's pre-flag list.
+// ---------------------------------------------------------------------------
+template <typename T, int TILE>
+__device__ __forceinline__ void d_ens_pred(const Ens& e, int2 tl, const T* __restrict__ W, int GE,
+                                           const unsigned char* __restrict__ pw, float thr,
+                                           signed char* __restrict__ pre, int* __restrict__ plist,
+                                           int* __restrict__ pcnt, float* __restrict__ logits) {
+  using namespace fvhoe::pred;
+  constexpr int NG2 = (TILE + 4 + 15) / 16, NG1 = NG2 + 1;  // 16-pixel groups of layers 2 and 1
+  constexpr int NX = 16 * NG1 + 4;                         // feature pixels staged (cells c0 - 6 + p)
+  constexpr int NWV = TILE / 64;
+  __shared__ __attribute__((aligned(16))) unsigned short xin[NX * 4];
+  __shared__ __attribute__((aligned(16))) unsigned short a1[16 * NG1 * 16];
+  __shared__ __attribute__((aligned(16))) unsigned short a2[16 * NG2 * 16];
+  __shared__ float w3[C2 * KS + 1];
+  __shared__ double sc[8];
+  const int i = tl.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], c0 = tl.y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, mrow = lane & 15, kb = lane >> 4;
+  if (tid < 4) {  // per-variable (min, 1 / (max - min)) of the instance's NAD range, 0 where max = min
+    const int v = tid == 3 ? 4 : tid;
+    const double lo = e.range[10 * i + 2 * v], rng = e.range[10 * i + 2 * v + 1] - lo;
+    sc[2 * tid] = lo;
+    sc[2 * tid + 1] = rng > 0.0 ? 1.0 / rng : 0.0;
+  }
+  for (int k = tid; k < C2 * KS + 1; k += TILE) w3[k] = k < C2 * KS ? ((const float*)(pw + QW3))[k] : *(const float*)(pw + QB3);
+  __syncthreads();
+  const T* Wi = W + 5 * e.off_w[i];
+  const int NW = n + 2 * GE;
+  for (int p = tid; p < NX; p += TILE) {
+    const int c = c0 - HALO + p;
+    unsigned short f[4] = {0, 0, 0, 0};
+    if (c < n + HALO) {  // beyond the extended run (the last tile's overhang): zero
+      bool flip;
+      const int src = bc_map(c, n, n, e.bc_lo == BC_DIRICHLET ? BC_OUTFLOW : e.bc_lo,
+                             e.bc_hi == BC_DIRICHLET ? BC_OUTFLOW : e.bc_hi, &flip);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int v = k == 3 ? 4 : k;
+        double x = (double)Wi[v * NW + src + GE];
+        if (flip && v == 1) x = -x;
+        const double inv = sc[2 * k + 1];
+        f[k] = f2bf(inv != 0.0 ? (float)((x - sc[2 * k]) * inv) : 0.0f);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) xin[p * 4 + k] = f[k];
+  }
+  __syncthreads();
+  // layer 1: group g = a1 pixels 16g .. 16g + 15 (cells c0 - 4 + p); lane (mrow, kb): taps 2kb, 2kb + 1
+  const bf16x8 wa1 = *(const bf16x8*)(pw + QW1 + 2 * (mrow * 32 + kb * 8));
+  float bias1[4], bias2[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    bias1[r] = ((const float*)(pw + QB1))[4 * kb + r];
+    bias2[r] = ((const float*)(pw + QB2))[4 * kb + r];
+  }
+  for (int g = wave; g < NG1; g += NWV) {
+    const unsigned short* px = &xin[(16 * g + mrow + 2 * kb) * 4];
+    u16x4 lo = {0, 0, 0, 0}, hi = {0, 0, 0, 0};
+    if (kb < 3) lo = *(const u16x4*)px;        // tap 2kb (tap 4 for kb = 2)
+    if (kb < 2) hi = *(const u16x4*)(px + 4);  // tap 2kb + 1
+    const u16x8 b = {lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa1, __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    const bf16x4v v = {(__bf16)fmaxf(acc[0] + bias1[0], 0.f), (__bf16)fmaxf(acc[1] + bias1[1], 0.f),
+                       (__bf16)fmaxf(acc[2] + bias1[2], 0.f), (__bf16)fmaxf(acc[3] + bias1[3], 0.f)};
+    *(bf16x4v*)&a1[(16 * g + mrow) * 16 + 4 * kb] = v;
+  }
+  __syncthreads();
+  // layer 2: K step k = taps 2k + kb/2, channels 8 (kb % 2) .. + 7 (tap 5 zero)
+  bf16x8 wa2[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) wa2[k] = *(const bf16x8*)(pw + QW2 + 2 * ((k * 16 + mrow) * 32 + kb * 8));
+  const u16x8 zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int g = wave; g < NG2; g += NWV) {
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const int t = 2 * k + (kb >> 1);
+      u16x8 b = zero8;
+      if (t < KS) b = *(const u16x8*)&a1[(16 * g + mrow + t) * 16 + 8 * (kb & 1)];
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(wa2[k], __builtin_bit_cast(bf16x8, b), acc, 0, 0, 0);
+    }
+    const bf16x4v v = {(__bf16)fmaxf(acc[0] + bias2[0], 0.f), (__bf16)fmaxf(acc[1] + bias2[1], 0.f),
+                       (__bf16)fmaxf(acc[2] + bias2[2], 0.f), (__bf16)fmaxf(acc[3] + bias2[3], 0.f)};
+    *(bf16x4v*)&a2[(16 * g + mrow) * 16 + 4 * kb] = v;
+  }
+  __syncthreads();
+  // layer 3: cell c0 + tid from a2 pixels tid .. tid + 4
+  float z = w3[C2 * KS];
+#pragma unroll
+  for (int t = 0; t < KS; ++t) {
+    const u16x8 lo8 = *(const u16x8*)&a2[(tid + t) * 16];
+    const u16x8 hi8 = *(const u16x8*)&a2[(tid + t) * 16 + 8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      z = z + w3[c * KS + t] * bf2f(lo8[c]);
+      z = z + w3[(c + 8) * KS + t] * bf2f(hi8[c]);
+    }
+  }
+  const int c = c0 + tid;
+  const int hit = (c < n && z > thr) ? 1 : 0;
+  if (c < n) {
+    pre[e.off_u[i] + c] = (signed char)hit;
+    if (logits) logits[e.off_u[i] + c] = z;
+  }
+  // the flagged cells appended to the instance's pre-flag list: wave ballot, block prefix, one atomic
+  __shared__ int wbase[NWV];
+  __shared__ int bbase;
+  const unsigned long long ball = __ballot(hit);
+  if (lane == 0) wbase[wave] = __popcll(ball);
+  __syncthreads();
+  if (tid == 0) {
+    int tot = 0;
+    for (int w = 0; w < NWV; ++w) {
+      const int x = wbase[w];
+      wbase[w] = tot;
+      tot += x;
+    }
+    bbase = tot ? atomicAdd(&pcnt[i], tot) : 0;
+  }
+  __syncthreads();
+  if (hit) plist[e.off_u[i] + bbase + wbase[wave] + __popcll(ball & ((1ull << lane) - 1ull))] = c;
+}
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_pred(Ens e, const int2* __restrict__ tiles, const T* __restrict__ W,
+                                                  int GE, const unsigned char* __restrict__ pw, float thr,
+                                                  signed char* __restrict__ pre, int* __restrict__ plist,
+                                                  int* __restrict__ pcnt, float* __restrict__ logits) {
+  d_ens_pred<T, TILE>(e, tiles[blockIdx.x], W, GE, pw, thr, pre, plist, pcnt, logits);
+}
+
+// per-stage counters of the learned stage, summed over the instances that moved: [0] pre-flagged cells,
+// [1] cells the check listed (the second fallback pass), [2] pre-flagged cells the check did not flag
+__global__ void k_ens_pred_stats(Ens e, const int* __restrict__ pcnt, long long* __restrict__ stats) {
+  long long a = 0, b = 0, c = 0;
+  for (int i = threadIdx.x; i < e.count; i += blockDim.x) {
+    if (!e.active[i] || e.dt_step[i] == 0.0) continue;
+    a += pcnt ? pcnt[i] : 0;
+    b += e.cnt[i];
+    c += e.fpc ? e.fpc[i] : 0;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  __shared__ long long part[4][3];
+  if ((threadIdx.x & 63) == 0) {
+    part[threadIdx.x >> 6][0] = a;
+    part[threadIdx.x >> 6][1] = b;
+    part[threadIdx.x >> 6][2] = c;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    long long x = 0;
+    for (int w = 0; w < (int)(blockDim.x / 64); ++w) x += part[w][threadIdx.x];
+    stats[threadIdx.x] += x;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -640,7 +837,7 @@ struct Plan {
   long long cells, wcells, fcells, scells;
   int tw, tf, ts, tc;  // tiles of each kind
   size_t o_n, o_h, o_ou, o_ow, o_of, o_os, o_dts, o_dtp, o_act, o_acc, o_rng, o_cnt, o_tw, o_tf, o_ts, o_tc;
-  size_t o_bar, o_W, o_F, o_WS, o_list, o_k0, o_k1, o_s, total;
+  size_t o_bar, o_W, o_F, o_WS, o_list, o_k0, o_k1, o_s, o_pre, o_plist, o_pcnt, o_fpc, total;
 };
 
 int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan* P) {
@@ -703,6 +900,11 @@ int make_plan(const FvhoeStageConfig* c, int count, const int* n, int tile, Plan
   P->o_k0 = o; o = a256(o + 8 * 5 * (size_t)P->cells);
   P->o_k1 = o; o = a256(o + 8 * 5 * (size_t)P->cells);
   P->o_s = o; o = a256(o + 8 * 5 * (size_t)P->cells);
+  // the learned stage: pre-flags per cell, the pre-flag list, per-instance counts
+  P->o_pre = o; o = a256(o + (size_t)P->cells);
+  P->o_plist = o; o = a256(o + 4 * (size_t)P->cells);
+  P->o_pcnt = o; o = a256(o + 4 * (size_t)count);
+  P->o_fpc = o; o = a256(o + 4 * (size_t)count);
   P->total = o;
   return FVHOE_OK;
 }
@@ -764,8 +966,27 @@ int persistent_blocks() {
 
 template <typename T, int P, int TILE>
 int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-             hipStream_t s, bool persistent) {
+             hipStream_t s, bool persistent, const FvhoePredict* pred, float* logits_only) {
   Ens e = make_ens(c, PL, ws);
+  signed char* pre = (signed char*)(ws + PL.o_pre);
+  int* plist = (int*)(ws + PL.o_plist);
+  int* pcnt = (int*)(ws + PL.o_pcnt);
+  const bool learned = pred && pred->infer;
+  if (pred) e.fpc = (int*)(ws + PL.o_fpc);
+  if (learned) e.pre = pre;
+  if (logits_only) {  // the network on the stage input u alone (its W, range and features): logits per cell
+    const int2* tw = (const int2*)(ws + PL.o_tw);
+    const int2* tc = (const int2*)(ws + PL.o_tc);
+    T* W = (T*)(ws + PL.o_W);
+    const int gi = (PL.count + 255) / 256;
+    k_ens_reset<<<gi, 256, 0, s>>>(e);
+    k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, u, nullptr, W, nullptr);
+    k_ens_scalars<<<gi, 256, 0, s>>>(e, 1, t, target);
+    k_ens_zero<<<gi, 256, 0, s>>>(pcnt, PL.count);
+    k_ens_pred<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, PL.GE, (const unsigned char*)pred->packed,
+                                               pred->logit_threshold, pre, plist, pcnt, logits_only);
+    return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
+  }
   const int2* tw = (const int2*)(ws + PL.o_tw);
   const int2* tf = (const int2*)(ws + PL.o_tf);
   const int2* ts = (const int2*)(ws + PL.o_ts);
@@ -779,6 +1000,7 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
   double* sv = (double*)(ws + PL.o_s);
   const int gi = (PL.count + 255) / 256;
   if (persistent) {
+    if (pred) return FVHOE_ERR_UNSUPPORTED;  // the learned stage runs as the chain of launches
     int nb = persistent_blocks<T, P, TILE>();
     if (nb < 1) return FVHOE_ERR_LAUNCH;
     const int most = PL.tw > PL.ts ? PL.tw : PL.ts;
@@ -804,6 +1026,15 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
     }
     k_ens_scalars<<<gi, 256, 0, s>>>(e, st == 0, t, target);
     if (!FVHOE_ENS_FUSED) k_ens_flux<T, P, TILE><<<PL.tf, TILE, 0, s>>>(e, tf, W, F);
+    if (learned && c->a_posteriori) {
+      if (st == 0) {  // one inference per step, on the step's input; its flags serve the three stages
+        k_ens_zero<<<gi, 256, 0, s>>>(pcnt, PL.count);
+        k_ens_pred<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, PL.GE, (const unsigned char*)pred->packed,
+                                                   pred->logit_threshold, pre, plist, pcnt, nullptr);
+      }
+      // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate is formed
+      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, plist, F, pcnt);
+    }
     if (c->a_posteriori) {
       if (FVHOE_ENS_FUSED) {  // the candidate and the tests of each interior tile in one pass
         k_ens_cand_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, F, W, list);
@@ -811,7 +1042,8 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
         k_ens_prims<T, P, TILE, true><<<PL.ts, TILE, 0, s>>>(e, ts, in, F, WS, nullptr);
         k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
       }
-      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F);
+      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F, nullptr);
+      if (pred && pred->stats) k_ens_pred_stats<<<1, 256, 0, s>>>(e, learned ? pcnt : nullptr, (long long*)pred->stats);
     }
     k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1);
   }
@@ -821,27 +1053,27 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
 
 template <typename T, int TILE>
 int by_p(int p, const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-         hipStream_t s, bool pers) {
+         hipStream_t s, bool pers, const FvhoePredict* pr, float* lo) {
   switch (p) {
-    case 1: return run_step<T, 1, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 2: return run_step<T, 2, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 3: return run_step<T, 3, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 4: return run_step<T, 4, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 5: return run_step<T, 5, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 6: return run_step<T, 6, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 7: return run_step<T, 7, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
-    case 8: return run_step<T, 8, TILE>(c, PL, u, t, nsteps, target, ws, s, pers);
+    case 1: return run_step<T, 1, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 2: return run_step<T, 2, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 3: return run_step<T, 3, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 4: return run_step<T, 4, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 5: return run_step<T, 5, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 6: return run_step<T, 6, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 7: return run_step<T, 7, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 8: return run_step<T, 8, TILE>(c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
   }
   return FVHOE_ERR_UNSUPPORTED;
 }
 
 template <typename T>
 int by_tile(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int* nsteps, double target, char* ws,
-            hipStream_t s, bool pers) {
+            hipStream_t s, bool pers, const FvhoePredict* pr, float* lo) {
   switch (PL.tile) {
-    case 64: return by_p<T, 64>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
-    case 128: return by_p<T, 128>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
-    case 256: return by_p<T, 256>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers);
+    case 64: return by_p<T, 64>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 128: return by_p<T, 128>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
+    case 256: return by_p<T, 256>(PL.p, c, PL, u, t, nsteps, target, ws, s, pers, pr, lo);
   }
   return FVHOE_ERR_ARG;
 }
@@ -850,15 +1082,17 @@ int by_tile(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, int
 
 namespace {
 int ensemble_step(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u, double* t, int* steps,
-                  double target, void* workspace, void* stream, bool persistent) {
+                  double target, void* workspace, void* stream, bool persistent, const FvhoePredict* pred = nullptr,
+                  float* logits_only = nullptr) {
   Plan P;
   const int st = make_plan(cfg, count, n, tile, &P);
   if (st != FVHOE_OK) return st;
   if (!u || !t || !workspace || ((uintptr_t)workspace & 255) != 0) return FVHOE_ERR_ARG;
+  if (pred && pred->infer && (!pred->packed || !cfg->a_posteriori)) return FVHOE_ERR_ARG;
   char* ws = (char*)workspace;
   hipStream_t s = (hipStream_t)stream;
-  return P.fp32 ? by_tile<float>(cfg, P, u, t, steps, target, ws, s, persistent)
-                 : by_tile<double>(cfg, P, u, t, steps, target, ws, s, persistent);
+  return P.fp32 ? by_tile<float>(cfg, P, u, t, steps, target, ws, s, persistent, pred, logits_only)
+                 : by_tile<double>(cfg, P, u, t, steps, target, ws, s, persistent, pred, logits_only);
 }
 }  // namespace
 
@@ -928,6 +1162,24 @@ int fvhoe_ensemble_step(const FvhoeStageConfig* cfg, int count, const int* n, in
 int fvhoe_ensemble_step_persistent(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u,
                                    double* t, int* steps, double target, void* workspace, void* stream) {
   return ensemble_step(cfg, count, n, tile, u, t, steps, target, workspace, stream, true);
+}
+
+int fvhoe_ensemble_step_predicted(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u,
+                                  double* t, int* steps, double target, void* workspace, const FvhoePredict* pred,
+                                  void* stream) {
+  if (!pred) return FVHOE_ERR_ARG;
+  return ensemble_step(cfg, count, n, tile, u, t, steps, target, workspace, stream, false, pred);
+}
+
+int fvhoe_ensemble_pred_logits(const FvhoeStageConfig* cfg, int count, const int* n, int tile, double* u,
+                               double* t, void* workspace, const void* packed, float* logits, void* stream) {
+  if (!packed || !logits) return FVHOE_ERR_ARG;
+  FvhoePredict pr{};
+  pr.packed = packed;
+  pr.logit_threshold = 0.0f;
+  pr.infer = 1;
+  return ensemble_step(cfg, count, n, tile, u, t, nullptr, __builtin_huge_val(), workspace, stream, false, &pr,
+                       logits);
 }
 
 int fvhoe_ensemble_status(const FvhoeStageConfig* cfg, int count, const int* n, int tile, const void* workspace,

@@ -22,6 +22,7 @@ constexpr int OW1 = 0, OB1 = OW1 + C1 * C0 * KK, OW2 = OB1 + C1, OB2 = OW2 + C2 
               OB3 = OW3 + C2 * KK, NPARAM = OB3 + 1;
 static_assert(NPARAM == FVHOE_PRED_NPARAMS, "parameter count");
 static_assert(PACKED <= FVHOE_PRED_PACKED_BYTES, "packed size");
+static_assert(QPACKED == FVHOE_PRED_PLANAR_BYTES, "planar packed size");
 
 __device__ __forceinline__ float bfround(float f) { return bf2f(f2bf(f)); }
 
@@ -49,6 +50,42 @@ __global__ void k_pack(const float* __restrict__ th, unsigned char* __restrict__
     } else if (i < N1 + N2 + N3) {
       const int j = i - N1 - N2;
       w3s[j] = bf2f(f2bf(th[OW3 + j]));
+    } else {
+      const int j = i - N1 - N2 - N3;
+      if (j < 16) b1[j] = th[OB1 + j];
+      else if (j < 32) b2[j - 16] = th[OB2 + j - 16];
+      else b3[0] = th[OB3];
+    }
+  }
+}
+
+// The planar network's weights: each kernel summed over its y taps (dx, the second image axis) in float32
+// in tap order from the bfloat16-rounded weights, rounded to bfloat16 again for the MFMA layers 1 and 2,
+// kept in float32 for layer 3.
+__global__ void k_pack_planar(const float* __restrict__ th, unsigned char* __restrict__ out) {
+  unsigned short* w1s = (unsigned short*)(out + QW1);
+  unsigned short* w2s = (unsigned short*)(out + QW2);
+  float* w3s = (float*)(out + QW3);
+  float* b1 = (float*)(out + QB1);
+  float* b2 = (float*)(out + QB2);
+  float* b3 = (float*)(out + QB3);
+  auto ysum = [&](int base) {  // base: the weight's (n, c, t, dx = 0) offset
+    float acc = 0.0f;
+    for (int dx = 0; dx < KS; ++dx) acc = acc + bf2f(f2bf(th[base + dx]));
+    return acc;
+  };
+  constexpr int N1 = 16 * 32, N2 = 3 * 16 * 32, N3 = C2 * KS;
+  for (int i = threadIdx.x + blockIdx.x * blockDim.x; i < N1 + N2 + N3 + 33; i += blockDim.x * gridDim.x) {
+    if (i < N1) {  // [n][k]: tap k/4, channel k%4
+      const int n = i / 32, k = i % 32, t = k / 4, c = k % 4;
+      w1s[i] = t < KS ? f2bf(ysum(OW1 + (n * C0 + c) * KK + t * KS)) : (unsigned short)0;
+    } else if (i < N1 + N2) {  // [pair][n][k]: tap 2 pair + k/16, channel k%16
+      const int j = i - N1, pair = j / 512, n = (j / 32) % 16, k = j % 32;
+      const int t = 2 * pair + k / 16, c = k % 16;
+      w2s[j] = t < KS ? f2bf(ysum(OW2 + (n * C1 + c) * KK + t * KS)) : (unsigned short)0;
+    } else if (i < N1 + N2 + N3) {  // [c][t]
+      const int j = i - N1 - N2, c = j / KS, t = j % KS;
+      w3s[j] = ysum(OW3 + c * KK + t * KS);
     } else {
       const int j = i - N1 - N2 - N3;
       if (j < 16) b1[j] = th[OB1 + j];
@@ -199,6 +236,12 @@ extern "C" {
 int fvhoe_pred_pack(const float* theta, void* packed, void* stream) {
   if (!theta || !packed) return FVHOE_ERR_ARG;
   k_pack<<<40, 256, 0, (hipStream_t)stream>>>(theta, (unsigned char*)packed);
+  return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
+}
+
+int fvhoe_pred_pack_planar(const float* theta, void* packed, void* stream) {
+  if (!theta || !packed) return FVHOE_ERR_ARG;
+  k_pack_planar<<<8, 256, 0, (hipStream_t)stream>>>(theta, (unsigned char*)packed);
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 

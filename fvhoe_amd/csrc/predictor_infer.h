@@ -44,6 +44,13 @@ constexpr int PW2 = PW1 + 5 * 16 * 32 * 2;    // bf16 [5 dy][3 pair][16 n][32 k]
 constexpr int PW3 = PW2 + 5 * 3 * 16 * 32 * 2;  // f32 [16][25] (bf16-rounded values)
 constexpr int PB1 = PW3 + C2 * KK * 4, PB2 = PB1 + 64, PB3 = PB2 + 64, PACKED = PB3 + 64;
 
+// packed weights of the planar network (the network on 1-D runs seen as 2-D states constant along y,
+// where each 5x5 kernel acts as its sum over the y taps): bf16 [16 n][32 k] layer 1 (tap k/4, channel k%4,
+// k < 20), bf16 [3 pair][16 n][32 k] layer 2 (tap 2 pair + k/16, channel k%16, tap 5 zero), f32 [16][5]
+// layer 3, then the biases
+constexpr int QW1 = 0, QW2 = QW1 + 16 * 32 * 2, QW3 = QW2 + 3 * 16 * 32 * 2, QB1 = QW3 + C2 * KS * 4,
+              QB2 = QB1 + 64, QB3 = QB2 + 64, QPACKED = QB3 + 64;
+
 // LDS carve-up of the inference kernel; the layer-3 tap sums G alias xin and a1
 constexpr int GT = 13;  // taps per layer-3 half
 constexpr int SM_XIN = 0, SM_A1 = SM_XIN + L0Y * L0X * C0 * 2, SM_A2 = SM_A1 + L1Y * L1X * C1 * 2,

@@ -11,6 +11,13 @@ single-run result bit for bit; bulk_precision="fp32" is the FP32 bulk form.
 
 The tile size (cells per workgroup) is chosen by `autotune`, which times the
 captured step for each candidate on the ensemble itself.
+
+With `predictor=` (a TroubledCellPredictor) every step runs the learned
+stage: the 2-D network applied to each run seen as a 2-D state constant
+along y (which acts as a 1-D network whose kernels are the 2-D kernels summed
+over their y taps) pre-flags cells on the step's input; they get MUSCL
+fluxes before the candidate in each stage, and the plain check still runs on
+every cell (`fvhoe_ensemble_step_predicted`).
 """
 
 from __future__ import annotations
@@ -35,7 +42,8 @@ class Ensemble:
                  slope_limiter: str = "minmod", NAD: float = 1e-2, NAD_mode: str = "global",
                  NAD_range: str = "relative", NAD_vars: tuple = None, PAD: dict = None, SED: bool = True,
                  SED_tolerance: float = 1e-10, csq_floor: float = 1e-10, bulk_precision: str = "fp64",
-                 tile: int = 128, persistent: bool = False, ic: Sequence = None, slab_buffer_size: int = 30):
+                 tile: int = 128, persistent: bool = False, ic: Sequence = None, slab_buffer_size: int = 30,
+                 predictor=None, count_troubles: bool = False):
         torch = _lib.torch()
         if bulk_precision not in ("fp64", "fp32"):
             raise ValueError(f"bulk_precision must be 'fp64' or 'fp32', got {bulk_precision!r}")
@@ -81,6 +89,26 @@ class Ensemble:
         self._ws = None
         self.tile = None
         self.persistent = bool(persistent)  # one persistent kernel per RK stage instead of a chain of launches
+        # learned stage: the planar network's packed weights; counters [pre-flagged cells, cells the check
+        # listed, pre-flagged cells the check did not flag], summed over stages (also without a predictor
+        # when count_troubles, for the second-pass share of a learned run)
+        self.predictor = predictor
+        self._pred = None
+        if predictor is not None or count_troubles:
+            if predictor is not None and not a_posteriori_slope_limiting:
+                raise ValueError("the troubled-cell predictor needs a-posteriori slope limiting")
+            if persistent:
+                raise NotImplementedError("the learned stage runs as the chain of launches")
+            self.predictor_stats = _lib.zeros((3,), dtype=torch.int64)
+            pr = _lib.FvhoePredict()
+            if predictor is not None:
+                self._planar = _lib.empty((_lib.PRED_PLANAR_BYTES,), dtype=torch.uint8)
+                _lib.call("fvhoe_pred_pack_planar", _lib.ptr(predictor.theta), _lib.ptr(self._planar), _lib.stream())
+                pr.packed = self._planar.data_ptr()
+                pr.logit_threshold = float(predictor.logit_threshold)
+                pr.infer = 1
+            pr.stats = self.predictor_stats.data_ptr()
+            self._pred = pr
         self.set_tile(tile)
 
     # ----------------------------------------------------------------- set-up
@@ -142,16 +170,30 @@ class Ensemble:
         return _lib.ctypes.c_void_p(base + (-base) % 256)
 
     # ------------------------------------------------------------------- steps
-    def _entry(self) -> str:
-        # one persistent kernel per RK stage, or the chain of seven launches per stage
-        return "fvhoe_ensemble_step_persistent" if self.persistent else "fvhoe_ensemble_step"
+    def _call_step(self, target: float, stream) -> None:
+        args = (_lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile, _lib.ptr(self.u), _lib.ptr(self.t),
+                _lib.ptr(self.steps), float(target), self._wsp())
+        if self._pred is not None:  # the learned stage (or the plain one with its counters)
+            _lib.call("fvhoe_ensemble_step_predicted", *args, _lib.ctypes.byref(self._pred), stream)
+        elif self.persistent:  # one persistent kernel per RK stage
+            _lib.call("fvhoe_ensemble_step_persistent", *args, stream)
+        else:  # the chain of launches per stage
+            _lib.call("fvhoe_ensemble_step", *args, stream)
 
     def step(self, target: float) -> None:
         """One SSP-RK3 step of every instance whose time is below target (no host sync)."""
-        _lib.call(self._entry(), _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
-                  _lib.ptr(self.u), _lib.ptr(self.t), _lib.ptr(self.steps), float(target), self._wsp(),
-                  _lib.stream())
+        self._call_step(target, _lib.stream())
         self.step_count += 1
+
+    def predictor_logits(self):
+        """The planar network's logits on the current states (sum n_i floats, instance-major)."""
+        if self.predictor is None:
+            raise ValueError("no predictor")
+        out = _lib.empty((int(self.offsets[-1]),), dtype=_lib.torch().float32)
+        _lib.call("fvhoe_ensemble_pred_logits", _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
+                  _lib.ptr(self.u), _lib.ptr(self.t), self._wsp(), _lib.ptr(self._planar), _lib.ptr(out),
+                  _lib.stream())
+        return out
 
     def check(self) -> None:
         """Raise if a persistent stage's grid barrier timed out (its results are then not valid)."""
@@ -171,9 +213,7 @@ class Ensemble:
         with _lib.capture_guard(), torch.cuda.stream(s):
             # recorded, not run: the state is unchanged by the capture
             with torch.cuda.graph(g, stream=s):
-                _lib.call(self._entry(), _lib.ctypes.byref(self._cfg), self.count, self._n_c, self.tile,
-                          _lib.ptr(self.u), _lib.ptr(self.t), _lib.ptr(self.steps), float(target),
-                          self._wsp(), _lib.ctypes.c_void_p(s.cuda_stream))
+                self._call_step(target, _lib.ctypes.c_void_p(s.cuda_stream))
         torch.cuda.current_stream().wait_stream(s)
         self._graph = (_lib.GraphHolder(g), float(target))
         return self._graph[0]
@@ -206,6 +246,9 @@ class Ensemble:
         returns {(tile, "persistent" | "chain"): ms per step}."""
         torch = _lib.torch()
         u0, t0, k0 = self.u.clone(), self.t.clone(), self.steps.clone()
+        st0 = self.predictor_stats.clone() if self._pred is not None else None
+        if self._pred is not None:
+            persistent = (False,)  # the learned stage (and the counting one) runs as the chain
         res = {}
         for pers in persistent:
             self.persistent = bool(pers)
@@ -225,6 +268,8 @@ class Ensemble:
                 self.u.copy_(u0)
                 self.t.copy_(t0)
                 self.steps.copy_(k0)
+                if st0 is not None:
+                    self.predictor_stats.copy_(st0)
         best = min(res, key=res.get)
         self.persistent = best[1] == "persistent"
         self.set_tile(best[0])

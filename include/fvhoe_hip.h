@@ -435,6 +435,14 @@ int fvhoe_stage_timing_read(double* ms, long long* count);
 /* theta -> packed inference weights (FVHOE_PRED_PACKED_BYTES, device). */
 int fvhoe_pred_pack(const float* theta, void* packed, void* stream);
 
+/* theta -> packed weights of the planar network (FVHOE_PRED_PLANAR_BYTES,
+ * device): the network on 1-D runs seen as 2-D states constant along y,
+ * where each 5x5 kernel acts as its sum over the y taps (the sum in float32
+ * of the bfloat16 weights, rounded to bfloat16 again for layers 1 and 2).
+ * No counterpart in the reference (the predictor is new). */
+#define FVHOE_PRED_PLANAR_BYTES 4608
+int fvhoe_pred_pack_planar(const float* theta, void* packed, void* stream);
+
 /* Network on X (4, n0+12, n1+12): flags (n0*n1 int8, may be NULL) = logit >
  * logit_threshold, logits (n0*n1 float, may be NULL). */
 int fvhoe_pred_infer(const float* X, int n0, int n1, const void* packed, float logit_threshold,
@@ -535,6 +543,30 @@ int fvhoe_ensemble_step_persistent(const FvhoeStageConfig* scheme, int count, co
                                    double* t, int* steps, double target, void* workspace, void* stream);
 int fvhoe_ensemble_status(const FvhoeStageConfig* scheme, int count, const int* n, int tile, const void* workspace,
                           int* error_word);
+
+/* fvhoe_ensemble_step with the troubled-cell predictor (no counterpart in
+ * the reference): once per step, on the step's input, the planar network
+ * (pred->packed from fvhoe_pred_pack_planar) pre-flags the cells whose logit
+ * exceeds pred->logit_threshold, from features defined as the 2-D ones
+ * ((rho, vx, vy, P) of the stage input's primitive averages over each run
+ * extended by 6 cells, ghosts mapped into the interior, scaled by the run's
+ * NAD range); in each of the step's three stages those cells get MUSCL
+ * fluxes before the candidate is formed, and the plain check then runs on
+ * every cell and sends the cells it flags (the pre-flagged ones excepted) to
+ * the fallback as well.  pred->infer 0: no network, the plain step (bit for
+ * bit) with the counters.  pred->stats (3 long long, may be NULL) += per
+ * stage: pre-flagged cells, cells the check listed, pre-flagged cells the
+ * check did not flag.  The chain of launches only (not the persistent
+ * stage).  Captured in a graph like fvhoe_ensemble_step. */
+int fvhoe_ensemble_step_predicted(const FvhoeStageConfig* scheme, int count, const int* n, int tile, double* u,
+                                  double* t, int* steps, double target, void* workspace, const FvhoePredict* pred,
+                                  void* stream);
+
+/* The planar network's logits on the instances' states u (5 x n_i each,
+ * instance-major): logits (sum n_i floats, instance-major).  t: the
+ * instances' times (device, count doubles; read only). */
+int fvhoe_ensemble_pred_logits(const FvhoeStageConfig* scheme, int count, const int* n, int tile, double* u,
+                               double* t, void* workspace, const void* packed, float* logits, void* stream);
 
 #ifdef __cplusplus
 }

@@ -139,3 +139,82 @@ def features(s, u: np.ndarray) -> np.ndarray:
         rng = hi[v] - lo[v]
         out.append((we[v] - lo[v]) * (1.0 / rng) if rng > 0 else np.zeros_like(we[v]))
     return np.stack(out).astype(np.float32)
+
+
+# ---------------------------------------------------------------------------
+# The network on planar states (1-D runs of the ensemble).  A 1-D run along x
+# seen as a 2-D state that does not vary along y gives the network input that
+# does not vary along the image's second axis, and each valid 5x5
+# cross-correlation of such an input equals a 1-D one along the first axis
+# with the kernel summed over the second axis (dx); so does every later layer.
+# The planar network is that 1-D network, with the summed kernels of layers 1
+# and 2 rounded to bfloat16 once more (the inference kernel's MFMA operands)
+# and layer 3's kept in float32; activations rounded to bfloat16 between
+# layers, as in the 2-D kernel.
+# ---------------------------------------------------------------------------
+
+
+def planar_weights(theta: np.ndarray) -> dict:
+    """w1p (16, 4, 5), w2p (16, 16, 5): bf16(sum over dx of bf16(w)) with the sum in float32 in dx order;
+    w3p (16, 5): the float32 sum; biases as they are."""
+    p = unpack(np.asarray(theta, dtype=np.float32))
+    out = {}
+    for name in ("w1", "w2", "w3"):
+        w = bf16_round(p[name])
+        acc = np.zeros(w.shape[:3], dtype=np.float32)
+        for dx in range(K):
+            acc = (acc + w[..., dx]).astype(np.float32)
+        out[name + "p"] = acc if name == "w3" else bf16_round(acc)
+    out["w3p"] = out["w3p"][0]
+    for name in ("b1", "b2", "b3"):
+        out[name] = p[name].astype(np.float32)
+    return out
+
+
+def conv1d_valid(x: np.ndarray, w: np.ndarray, b: np.ndarray) -> np.ndarray:
+    """x (Cin, L), w (Cout, Cin, K) -> (Cout, L-K+1), cross-correlation, float64."""
+    L = x.shape[1] - K + 1
+    out = np.zeros((w.shape[0], L))
+    for t in range(K):
+        out += np.einsum("oc,cl->ol", w[:, :, t].astype(np.float64), x[:, t:t + L])
+    return out + b.astype(np.float64)[:, None]
+
+
+def forward_planar(theta: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Logits (L-12,) of the planar network on features x (4, L) (float64 arithmetic, input and hidden
+    activations rounded to bfloat16 as the kernel stores them)."""
+    q = planar_weights(theta)
+    act = lambda a: bf16_round(a.astype(np.float32)).astype(np.float64)  # noqa: E731
+    a0 = act(np.asarray(x, dtype=np.float64))
+    a1 = act(np.maximum(conv1d_valid(a0, q["w1p"], q["b1"]), 0))
+    a2 = act(np.maximum(conv1d_valid(a1, q["w2p"], q["b2"]), 0))
+    return conv1d_valid(a2, q["w3p"][None], q["b3"])[0]
+
+
+def features_1d(w: np.ndarray, rng_lo: np.ndarray, rng_hi: np.ndarray, bc) -> np.ndarray:
+    """Planar-network input of one 1-D run: w (5, n) primitive fv averages of the interior cells, the
+    run's NAD range (min, max per variable over its cells and 3 ghosts on each side), bc = (lo, hi)
+    boundary kinds.  (rho, vx, vy, P) at cells -6 .. n+5, ghosts mapped into the interior as the 2-D
+    features map them (periodic wrap, free = nearest cell, otherwise mirror, vx negated across a
+    reflective side), scaled to (w - min) * (1 / (max - min)) in float64 (0 where max = min), rounded to
+    float32: (4, n + 12)."""
+    n = w.shape[1]
+    c = np.arange(-HALO, n + HALO)
+    src = c.copy()
+    flip = np.zeros(c.shape, dtype=bool)
+    for side, sel, edge in ((bc[0], c < 0, 0), (bc[1], c >= n, n - 1)):
+        if side == "periodic":
+            src[sel] = c[sel] % n
+        elif side == "free":
+            src[sel] = edge
+        else:
+            src[sel] = np.where(c[sel] < 0, -1 - c[sel], 2 * n - 1 - c[sel])
+            flip[sel] = side == "reflective"
+    out = []
+    for v in (0, 1, 2, 4):
+        x = w[v][src].astype(np.float64)
+        if v == 1:
+            x = np.where(flip, -x, x)
+        r = rng_hi[v] - rng_lo[v]
+        out.append((x - rng_lo[v]) * (1.0 / r) if r > 0 else np.zeros_like(x))
+    return np.array(out).astype(np.float32)

@@ -80,3 +80,60 @@ def test_features_invariants():
                 lo, hi = w3[1].min(), w3[1].max()
                 ref = ((-w[1, :6, :] - lo) * (1.0 / (hi - lo))).astype(np.float32)
                 assert np.array_equal(mirrored[1], ref)
+
+
+def test_planar_network_is_the_2d_network_on_y_invariant_inputs():
+    """The planar network (kernels summed over the second image axis) against the 2-D oracle network on
+    inputs constant along that axis: equal up to the second bfloat16 rounding of the summed kernels (a
+    relative weight change <= 2^-9); summing over the wrong axis, or taking one column of the kernel,
+    misses by far more."""
+    rng = np.random.default_rng(0)
+    for seed in (1, 3, 7):
+        th = PO.init_params(seed)
+        th[-1] = 0.1
+        x = rng.random((4, 44)).astype(np.float32)
+        z1 = PO.forward_planar(th, x)
+        X = np.repeat(x[None, :, :, None], 13, axis=3).astype(np.float64)
+        z2 = PO.forward(th, X, rounding=True, act_bf16=True)[0][0]
+        assert z2.shape == (32, 1) and z1.shape == (32,)
+        scale = np.abs(z2).max()
+        err = np.abs(z2[:, 0] - z1).max()
+        assert err <= 2e-2 * scale, (seed, err, scale)
+        # the same network with the kernels summed over the first axis instead
+        p = PO.unpack(th.copy())
+        for name in ("w1", "w2", "w3"):
+            p[name][...] = np.swapaxes(p[name], 2, 3)
+        th_t = np.concatenate([p[k].ravel() for k, _ in PO.param_shapes()]).astype(np.float32)
+        wrong = np.abs(z2[:, 0] - PO.forward_planar(th_t, x)).max()
+        assert wrong > 10 * err, (seed, wrong, err)
+
+
+def test_planar_weights_closed_form():
+    th = PO.init_params(5)
+    q = PO.planar_weights(th)
+    p = PO.unpack(th)
+    w1 = PO.bf16_round(p["w1"]).astype(np.float64).sum(axis=3)
+    assert q["w1p"].shape == (16, 4, 5) and q["w2p"].shape == (16, 16, 5) and q["w3p"].shape == (16, 5)
+    assert np.abs(q["w1p"] - w1).max() <= 2 ** -8 * np.abs(w1).max()
+    # bfloat16 values: the low 16 bits are zero
+    assert not np.any(q["w2p"].view(np.uint32) & 0xFFFF)
+
+
+def test_features_1d_boundary_maps():
+    n = 10
+    w = np.arange(5 * n, dtype=np.float64).reshape(5, n) + 1.0
+    lo, hi = w.min(axis=1), w.max(axis=1)
+    for bc, left, right in ((("periodic", "periodic"), [n - 6 + k for k in range(6)], list(range(6))),
+                            (("free", "free"), [0] * 6, [n - 1] * 6),
+                            (("outflow", "reflective"), [5 - k for k in range(6)], [n - 1 - k for k in range(6)])):
+        X = PO.features_1d(w, lo, hi, bc)
+        assert X.shape == (4, n + 12)
+        want = lambda src: ((w[0][src] - lo[0]) / (hi[0] - lo[0])).astype(np.float32)  # noqa: E731
+        assert np.array_equal(X[0, :6], want(np.array(left))), bc
+        assert np.array_equal(X[0, -6:], want(np.array(right))), bc
+        assert np.array_equal(X[0, 6:-6], want(np.arange(n)))
+    X = PO.features_1d(w, lo, hi, ("reflective", "reflective"))
+    vx = (-w[1][[5 - k for k in range(6)]] - lo[1]) / (hi[1] - lo[1])
+    assert np.allclose(X[1, :6], vx)
+    w[2] = 0.0
+    assert not np.any(PO.features_1d(w, w.min(axis=1), w.max(axis=1), ("periodic", "periodic"))[2])
