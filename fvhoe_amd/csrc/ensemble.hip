@@ -1,26 +1,31 @@
 // Batched ensemble of independent 1-D runs of uneven size (EulerSolver along
 // x, fvhoe/solver.py:41-1001, one per instance), advanced together: every
-// SSP-RK3 step of every active instance is one sequence of seven kernels per
-// stage, each launched once for the whole ensemble over a table of
-// (instance, tile) pairs, so that the work of one launch is balanced in tiles
-// of TILE cells whatever the instance sizes.  Each instance keeps its own CFL
-// dt, time and _hit_target (fvhoe/ode.py:11-22, 183-193); an instance whose
-// time has reached the target is skipped by every kernel.
+// SSP-RK3 step of every active instance is one sequence of kernels per stage,
+// each launched once for the whole ensemble over a table of (instance, tile)
+// pairs, so that the work of one launch is balanced in tiles of TILE cells
+// whatever the instance sizes.  Each instance keeps its own CFL dt, time and
+// _hit_target (fvhoe/ode.py:11-22, 183-193); an instance whose time has
+// reached the target is skipped by every kernel.
 //
-// Kernels of one stage (the same arithmetic, in the same order, as the fused
-// 1-D stage, so the FP64 form reproduces the reference bit for bit):
-//   reset   : per-instance reduction words and fallback counters.
-//   prims   : u (BC-mapped on load) -> centres -> FP64 primitives -> fv averages
-//             W on the ghost-extended range, with per-instance CFL max and
-//             NAD range folded in by 64-bit atomics.
-//   scalars : per-instance dt (fvhoe/hydro.py:270-294), range, and in the
-//             first stage the step's dt and the active flag.
-//   flux    : l/r face values, Riemann solver (fvhoe/solver.py:519-649).
-//   prims   : again on the candidate u + dt L(F) -> WS.
-//   detect  : NAD/PAD/SED flags (fvhoe/slope_limiting.py:87-244) and a
-//             ballot compaction of the flagged cells per instance.
-//   fallback: MUSCL fluxes at the faces of flagged cells (fvhoe/solver.py:710-731).
-//   final   : divergence and the SSP-RK3 combination (fvhoe/ode.py:271-283).
+// Kernels of one stage of the chain (the same arithmetic, in the same order,
+// as the fused 1-D stage, so the FP64 form reproduces the reference bit for
+// bit):
+//   prims+flux : u (BC-mapped on load) -> centres -> FP64 primitives -> fv
+//                averages W on the ghost-extended range, with per-instance
+//                CFL max and NAD range folded in by 64-bit atomics, and the
+//                Riemann fluxes of the tile's faces from its W.
+//   scalars    : per-instance dt (fvhoe/hydro.py:270-294), range, in the first
+//                stage the step's dt and the active flag; clears the
+//                reduction words for the next stage and this stage's counters.
+//   cand+detect: the candidate u + dt L(F) -> WS in LDS and its NAD/PAD/SED
+//                flags (fvhoe/slope_limiting.py:87-244), ballot-compacted per
+//                instance.
+//   fallback   : MUSCL fluxes at the faces of flagged cells (fvhoe/solver.py:710-731).
+//   final      : divergence and the SSP-RK3 combination (fvhoe/ode.py:271-283);
+//                in the last stage also the instance's clock.
+// With the troubled-cell predictor the first stage also runs the network
+// (pre-flags) and every stage a fallback over the pre-flagged cells before
+// the candidate.  The persistent form runs a stage's phases in one kernel.
 #include <vector>
 
 #include "fvhoe_common.h"
@@ -98,6 +103,8 @@ struct Ens {
   // not flag (null without a predictor)
   const signed char* pre;
   int* fpc;
+  int* pcnt;          // pre-flagged cells per instance (cleared in the step's first stage)
+  long long* stats;   // counters of the learned stage (may be null)
 };
 
 // ghost value of a slab side of instance i, or nothing (returns false) for a cell that no slab covers
@@ -297,20 +304,29 @@ __global__ void __launch_bounds__(TILE) k_ens_prims(Ens e, const int2* __restric
   d_ens_prims<T, P, TILE, CAND, FLUX>(e, tiles[blockIdx.x], u, F, out, Fout);
 }
 
-// per-instance dt, range; first stage: active flag and the step's dt
+// per-instance dt, range; first stage: active flag and the step's dt.  With reset (the chain of launches)
+// the reduction words are cleared once read, for the next stage's W pass, and this stage's counters before
+// its check (so no separate reset launch is needed; the set-up clears the words for the first stage)
 __device__ __forceinline__ void d_ens_scalars(const Ens& e, int i, int first, const double* __restrict__ t,
-                                              double target) {
+                                              double target, int reset = 0) {
   if (i >= e.count) return;
   if (first) e.active[i] = t[i] < target ? 1 : 0;
   if (!e.active[i]) return;
-  const unsigned long long* a = e.acc + i * NACC;
+  unsigned long long* a = e.acc + i * NACC;
   const double d = (e.cfl * e.h[i]) / __longlong_as_double((long long)a[0]);
   e.dt_stage[i] = d;
   if (first) e.dt_step[i] = (target - t[i] < d) ? target - t[i] : d;
   for (int k = 0; k < 10; ++k) e.range[i * 10 + k] = from_obits(a[1 + k]);
+  if (reset) {
+    a[0] = 0ull;
+    for (int k = 1; k < NACC; ++k) a[k] = (k & 1) ? ~0ull : 0ull;
+    e.cnt[i] = 0;
+    if (e.fpc) e.fpc[i] = 0;
+    if (first && e.pcnt) e.pcnt[i] = 0;
+  }
 }
-__global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, double target) {
-  d_ens_scalars(e, blockIdx.x * blockDim.x + threadIdx.x, first, t, target);
+__global__ void k_ens_scalars(Ens e, int first, const double* __restrict__ t, double target, int reset) {
+  d_ens_scalars(e, blockIdx.x * blockDim.x + threadIdx.x, first, t, target, reset);
 }
 
 // faces fa in [0, n] of each instance: l/r values of W, Riemann solver
@@ -532,12 +548,26 @@ __global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __rest
 // divergence (fvhoe/solver.py:744-750) and the SSP-RK3 combination of this stage
 // (fvhoe/ode.py:275-279): stage 0 -> k0, s = u + dt k0; stage 1 -> k1,
 // s = u + 0.25 dt k0 + 0.25 dt k1; stage 2 -> u + (1/6) dt (k0 + k1 + 4 k2).
+// With `clock` (the chain of launches) the instance's first tile also adds this stage's counters to
+// e.stats and, in the last stage, moves the instance's clock and step count (d_ens_advance's operations).
 template <typename T, int TILE>
 __device__ __forceinline__ void d_ens_final(const Ens& e, int2 tl, const T* __restrict__ F, int stage,
                                             double* __restrict__ u, double* __restrict__ s, double* __restrict__ k0,
-                                            double* __restrict__ k1) {
+                                            double* __restrict__ k1, double* __restrict__ t = nullptr,
+                                            int* __restrict__ nsteps = nullptr, bool clock = false) {
   const int i = tl.x;
   if (!e.active[i]) return;
+  if (clock && tl.y == 0 && threadIdx.x == 0) {
+    if (e.stats && e.dt_step[i] != 0.0) {
+      if (e.pcnt) atomicAdd((unsigned long long*)&e.stats[0], (unsigned long long)e.pcnt[i]);
+      atomicAdd((unsigned long long*)&e.stats[1], (unsigned long long)e.cnt[i]);
+      if (e.fpc) atomicAdd((unsigned long long*)&e.stats[2], (unsigned long long)e.fpc[i]);
+    }
+    if (stage == 2) {
+      t[i] = t[i] + e.dt_step[i];
+      if (nsteps) nsteps[i] += 1;
+    }
+  }
   const int n = e.n[i], c = tl.y + threadIdx.x;
   if (c >= n) return;
   const double cx = -(1.0 / e.h[i]);
@@ -562,13 +592,9 @@ __device__ __forceinline__ void d_ens_final(const Ens& e, int2 tl, const T* __re
 template <typename T, int TILE>
 __global__ void __launch_bounds__(TILE) k_ens_final(Ens e, const int2* __restrict__ tiles, const T* __restrict__ F,
                                                    int stage, double* __restrict__ u, double* __restrict__ s,
-                                                   double* __restrict__ k0, double* __restrict__ k1) {
-  d_ens_final<T, TILE>(e, tiles[blockIdx.x], F, stage, u, s, k0, k1);
-}
-
-__global__ void k_ens_zero(int* __restrict__ x, int count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) x[i] = 0;
+                                                   double* __restrict__ k0, double* __restrict__ k1,
+                                                   double* __restrict__ t, int* __restrict__ nsteps) {
+  d_ens_final<T, TILE>(e, tiles[blockIdx.x], F, stage, u, s, k0, k1, t, nsteps, true);
 }
 
 __device__ __forceinline__ void d_ens_advance(const Ens& e, int i, double* __restrict__ t, int* __restrict__ steps) {
@@ -576,9 +602,6 @@ __device__ __forceinline__ void d_ens_advance(const Ens& e, int i, double* __res
     t[i] = t[i] + e.dt_step[i];
     if (steps) steps[i] += 1;
   }
-}
-__global__ void k_ens_advance(Ens e, double* __restrict__ t, int* __restrict__ steps) {
-  d_ens_advance(e, blockIdx.x * blockDim.x + threadIdx.x, t, steps);
 }
 
 // ---------------------------------------------------------------------------
@@ -723,35 +746,6 @@ __global__ void __launch_bounds__(TILE) k_ens_pred(Ens e, const int2* __restrict
                                                   signed char* __restrict__ pre, int* __restrict__ plist,
                                                   int* __restrict__ pcnt, float* __restrict__ logits) {
   d_ens_pred<T, TILE>(e, tiles[blockIdx.x], W, GE, pw, thr, pre, plist, pcnt, logits);
-}
-
-// per-stage counters of the learned stage, summed over the instances that moved: [0] pre-flagged cells,
-// [1] cells the check listed (the second fallback pass), [2] pre-flagged cells the check did not flag
-__global__ void k_ens_pred_stats(Ens e, const int* __restrict__ pcnt, long long* __restrict__ stats) {
-  long long a = 0, b = 0, c = 0;
-  for (int i = threadIdx.x; i < e.count; i += blockDim.x) {
-    if (!e.active[i] || e.dt_step[i] == 0.0) continue;
-    a += pcnt ? pcnt[i] : 0;
-    b += e.cnt[i];
-    c += e.fpc ? e.fpc[i] : 0;
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    b += __shfl_down(b, off, 64);
-    c += __shfl_down(c, off, 64);
-  }
-  __shared__ long long part[4][3];
-  if ((threadIdx.x & 63) == 0) {
-    part[threadIdx.x >> 6][0] = a;
-    part[threadIdx.x >> 6][1] = b;
-    part[threadIdx.x >> 6][2] = c;
-  }
-  __syncthreads();
-  if (threadIdx.x < 3) {
-    long long x = 0;
-    for (int w = 0; w < (int)(blockDim.x / 64); ++w) x += part[w][threadIdx.x];
-    stats[threadIdx.x] += x;
-  }
 }
 
 // ---------------------------------------------------------------------------
@@ -972,8 +966,14 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
   int* plist = (int*)(ws + PL.o_plist);
   int* pcnt = (int*)(ws + PL.o_pcnt);
   const bool learned = pred && pred->infer;
-  if (pred) e.fpc = (int*)(ws + PL.o_fpc);
-  if (learned) e.pre = pre;
+  if (pred) {
+    e.fpc = (int*)(ws + PL.o_fpc);
+    e.stats = pred->stats;
+  }
+  if (learned) {
+    e.pre = pre;
+    e.pcnt = pcnt;
+  }
   if (logits_only) {  // the network on the stage input u alone (its W, range and features): logits per cell
     const int2* tw = (const int2*)(ws + PL.o_tw);
     const int2* tc = (const int2*)(ws + PL.o_tc);
@@ -981,8 +981,7 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
     const int gi = (PL.count + 255) / 256;
     k_ens_reset<<<gi, 256, 0, s>>>(e);
     k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, u, nullptr, W, nullptr);
-    k_ens_scalars<<<gi, 256, 0, s>>>(e, 1, t, target);
-    k_ens_zero<<<gi, 256, 0, s>>>(pcnt, PL.count);
+    k_ens_scalars<<<gi, 256, 0, s>>>(e, 1, t, target, 1);
     k_ens_pred<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, PL.GE, (const unsigned char*)pred->packed,
                                                pred->logit_threshold, pre, plist, pcnt, logits_only);
     return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
@@ -1017,21 +1016,19 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
   }
   for (int st = 0; st < 3; ++st) {
     const double* in = st == 0 ? u : sv;
-    k_ens_reset<<<gi, 256, 0, s>>>(e);
+    // (the reduction words were cleared by the set-up or by the previous stage's scalars)
     // W with the faces of each W tile solved in the same pass (FVHOE_ENS_FUSED), or W then F
     if (FVHOE_ENS_FUSED) {
       k_ens_prims<T, P, TILE, false, true><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W, F);
     } else {
       k_ens_prims<T, P, TILE, false><<<PL.tw, TILE, 0, s>>>(e, tw, in, nullptr, W, nullptr);
     }
-    k_ens_scalars<<<gi, 256, 0, s>>>(e, st == 0, t, target);
+    k_ens_scalars<<<gi, 256, 0, s>>>(e, st == 0, t, target, 1);
     if (!FVHOE_ENS_FUSED) k_ens_flux<T, P, TILE><<<PL.tf, TILE, 0, s>>>(e, tf, W, F);
     if (learned && c->a_posteriori) {
-      if (st == 0) {  // one inference per step, on the step's input; its flags serve the three stages
-        k_ens_zero<<<gi, 256, 0, s>>>(pcnt, PL.count);
+      if (st == 0)  // one inference per step, on the step's input; its flags serve the three stages
         k_ens_pred<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, PL.GE, (const unsigned char*)pred->packed,
                                                    pred->logit_threshold, pre, plist, pcnt, nullptr);
-      }
       // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate is formed
       k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, plist, F, pcnt);
     }
@@ -1043,11 +1040,10 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
         k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
       }
       k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F, nullptr);
-      if (pred && pred->stats) k_ens_pred_stats<<<1, 256, 0, s>>>(e, learned ? pcnt : nullptr, (long long*)pred->stats);
     }
-    k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1);
+    // the divergence and the RK combination; the counters and, in the last stage, the clock as well
+    k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1, t, nsteps);
   }
-  k_ens_advance<<<gi, 256, 0, s>>>(e, t, nsteps);
   return hipGetLastError() == hipSuccess ? FVHOE_OK : FVHOE_ERR_LAUNCH;
 }
 
@@ -1125,6 +1121,7 @@ int fvhoe_ensemble_setup(const FvhoeStageConfig* cfg, int count, const int* n, c
   long long* of = (long long*)(host.data() + P.o_of);
   long long* os = (long long*)(host.data() + P.o_os);
   int* act = (int*)(host.data() + P.o_act);
+  unsigned long long* acc0 = (unsigned long long*)(host.data() + P.o_acc);  // the first W pass's reduction words
   int2* tw = (int2*)(host.data() + P.o_tw);
   int2* tf = (int2*)(host.data() + P.o_tf);
   int2* ts = (int2*)(host.data() + P.o_ts);
@@ -1139,6 +1136,7 @@ int fvhoe_ensemble_setup(const FvhoeStageConfig* cfg, int count, const int* n, c
     of[i] = f;
     os[i] = sn;
     act[i] = 1;
+    for (int k = 0; k < NACC; ++k) acc0[(size_t)i * NACC + k] = (k > 0 && (k & 1)) ? ~0ull : 0ull;
     a += n[i];
     b += n[i] + 2 * P.GE;
     f += n[i] + 1;
