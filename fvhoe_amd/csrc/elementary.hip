@@ -328,6 +328,12 @@ __global__ void k_riemann_faces(const double* __restrict__ wr_face, const double
 // register window of the five variables over R consecutive faces, so w is
 // read once and only the fluxes are written; the sums and the solver are the
 // separate kernels' operations in the same order (bit-identical).
+// FP64 face states formed one variable at a time (1) or from all five variables' windows at once (0)
+// (3-D wave 256^3 p=7, per launch FP64: 547-576 -> 493-532 us; FP32 keeps the five windows: its y-face
+// launch went 389 -> 444 us one variable at a time)
+#ifndef FVHOE_FACE_SPLIT
+#define FVHOE_FACE_SPLIT 1
+#endif
 // waves per SIMD the face-flux kernel is compiled for (1: the compiler's choice)
 #ifndef FVHOE_FACE_WAVES
 #define FVHOE_FACE_WAVES 1
@@ -367,28 +373,63 @@ __global__ void __launch_bounds__(256, FVHOE_FACE_WAVES) k_face_flux(const T* __
   const int s = axis == 1 ? n1 * n2 : axis == 2 ? n2 : 1;
   const int nc = n0 * n1 * n2, no = m0 * m1 * m2;
   const int in0 = (i * n1 + j) * n2 + k;
-  T win[5][NW];
-#pragma unroll
-  for (int q = 0; q < 5; ++q)
-#pragma unroll
-    for (int r = 0; r < NW; ++r) win[q][r] = a0 + r < na ? w[q * nc + in0 + r * s] : T(0);
   const int os = axis == 1 ? m1 * m2 : axis == 2 ? m2 : 1;
   const int o0 = ((axis == 1 ? a0 : i) * m1 + (axis == 2 ? a0 : j)) * m2 + (axis == 3 ? a0 : k);
+  // FP64: one variable's window at a time, reduced to its R left and right face values before the next
+  // variable's loads are issued (fewer live registers); FP32: all five windows at once
+  constexpr bool SPLIT = FVHOE_FACE_SPLIT && sizeof(T) == 8;
+  constexpr int RS5 = SPLIT ? R : 1, NW5 = SPLIT ? 1 : NW;
+  T lf[RS5][5], rf[RS5][5];
+  T win[5][NW5];
+  if constexpr (SPLIT) {
 #pragma unroll
-  for (int f = 0; f < R; ++f) {
-    if (a0 + f < mf) {
-      T left[5], right[5], F[5];
+    for (int q = 0; q < 5; ++q) {
+      T wq[NW];
 #pragma unroll
-      for (int q = 0; q < 5; ++q) {
+      for (int r = 0; r < NW; ++r) wq[r] = a0 + r < na ? w[q * nc + in0 + r * s] : T(0);
+#pragma unroll
+      for (int f = 0; f < R; ++f) {
         T a = T(0), b = T(0);
         static_for<K>([&](auto I) {
           constexpr int t = decltype(I)::value;
           constexpr double wr = weight(KIND_R, P, t), wl = weight(KIND_L, P, t);
-          if constexpr (wr != 0.0) a = a + T(wr) * win[q][f + t];
-          if constexpr (wl != 0.0) b = b + T(wl) * win[q][f + 1 + t];
+          if constexpr (wr != 0.0) a = a + T(wr) * wq[f + t];
+          if constexpr (wl != 0.0) b = b + T(wl) * wq[f + 1 + t];
         });
-        left[q] = a;
-        right[q] = b;
+        lf[f][q] = a;
+        rf[f][q] = b;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+#pragma unroll
+      for (int r = 0; r < NW; ++r) win[q][r] = a0 + r < na ? w[q * nc + in0 + r * s] : T(0);
+  }
+#pragma unroll
+  for (int f = 0; f < R; ++f) {
+    if (a0 + f < mf) {
+      T left[5], right[5], F[5];
+      if constexpr (SPLIT) {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          left[q] = lf[f][q];
+          right[q] = rf[f][q];
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) {
+          T a = T(0), b = T(0);
+          static_for<K>([&](auto I) {
+            constexpr int t = decltype(I)::value;
+            constexpr double wr = weight(KIND_R, P, t), wl = weight(KIND_L, P, t);
+            if constexpr (wr != 0.0) a = a + T(wr) * win[q][f + t];
+            if constexpr (wl != 0.0) b = b + T(wl) * win[q][f + 1 + t];
+          });
+          left[q] = a;
+          right[q] = b;
+        }
       }
       riemann_t<RS, D>(left, right, F, gamma, csq_floor);  // the solver is a compile-time choice
 #pragma unroll

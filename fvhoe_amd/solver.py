@@ -627,7 +627,7 @@ class EulerSolver(ODE):
             want = [nv, n0, n1, n2]
             want[d + 1] += 1
             if list(f.shape) != want or f.dtype != first.dtype:
-                raise ValueError(f"face fluxes of shape {tuple(f.shape)} / {f.dtype} for cells {tuple(out.shape)}")
+                raise ValueError(f"face fluxes of shape {tuple(f.shape)} / {f.dtype} for cells {tuple(shape)}")
             if f.stride(3) != 1:
                 f = f.contiguous()
             arrs.append(f)
