@@ -138,6 +138,10 @@ constexpr int TPB = 256;
 #endif
 // 2-D fluxes: one kernel for both face axes from one W box (FVHOE_FLUX_XY = 1), tile of A (x) x B (y)
 // face positions on NT threads per arithmetic type, or one kernel per axis (0)
+// occupancy floor of the FP32 two-axis flux kernel (1 = the compiler's choice)
+#ifndef FVHOE_FLUXXY_WAVES_F32
+#define FVHOE_FLUXXY_WAVES_F32 1
+#endif
 #ifndef FVHOE_FLUX_XY
 #define FVHOE_FLUX_XY 1
 #endif
@@ -478,8 +482,12 @@ __device__ __forceinline__ void block_append2(const int (&cell)[K], unsigned sel
 #ifndef FVHOE_CHECK_WAVES_F64
 #define FVHOE_CHECK_WAVES_F64 6
 #endif
+// FP32 check tiles: the compiler's occupancy unless a floor is asked for (1 = none)
+#ifndef FVHOE_CHECK_WAVES_F32
+#define FVHOE_CHECK_WAVES_F32 1
+#endif
 template <typename T, int PX, int PY, int TX, int TY, typename Src, int NT, bool CHECK = false>
-__global__ void __launch_bounds__(NT, CHECK && sizeof(T) == 8 ? FVHOE_CHECK_WAVES_F64 : 1) k_prims(Src src, Geo g, T* __restrict__ W, int NW0, int NW1, int tiles1,
+__global__ void __launch_bounds__(NT, !CHECK ? 1 : sizeof(T) == 8 ? FVHOE_CHECK_WAVES_F64 : FVHOE_CHECK_WAVES_F32) k_prims(Src src, Geo g, T* __restrict__ W, int NW0, int NW1, int tiles1,
                                              Reduce red, double gamma, CheckArgs<T> ck) {
   constexpr int RCX = ereach_c(PX), RTX = ereach_t(PX), RCY = ereach_c(PY), RTY = ereach_t(PY);
   constexpr int HX0 = TX + 2 * (RCX + RTX), HY0 = TY + 2 * (RCY + RTY);
@@ -1093,7 +1101,7 @@ __global__ void __launch_bounds__(NT) k_flux(const T* __restrict__ W, int NW0, i
 // rows [i0, i0 + A); the tiles cover (n0 + 1) x (n1 + 1) face positions.
 // ---------------------------------------------------------------------------
 template <typename T, int P, int A, int B, int NT, int RS>
-__global__ void __launch_bounds__(NT) k_flux_xy(const T* __restrict__ W, int NW0, int NW1, Geo g, T* __restrict__ F,
+__global__ void __launch_bounds__(NT, sizeof(T) == 8 ? 1 : FVHOE_FLUXXY_WAVES_F32) k_flux_xy(const T* __restrict__ W, int NW0, int NW1, Geo g, T* __restrict__ F,
                                                T* __restrict__ Gout, int tilesB, double gamma_d, double csq_floor_d) {
   constexpr int EL = reach(P), RC = ereach_c(P), RT = ereach_t(P);
   constexpr int LO = 1 + EL > RC + RT ? 1 + EL : RC + RT;  // box cells before the tile on each axis
