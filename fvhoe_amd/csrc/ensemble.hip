@@ -545,6 +545,33 @@ __global__ void __launch_bounds__(TILE) k_ens_fallback(Ens e, const int2* __rest
   d_ens_fallback<T, TILE>(e, tiles[blockIdx.x], u, list, F, counts);
 }
 
+// The chain's fallback over instances instead of tiles: blocks (i, y) walk instance i's list with a stride
+// of FVHOE_ENS_FB_BPI blocks, so a launch is one round of count x FVHOE_ENS_FB_BPI blocks whatever the
+// number of tiles (the lists hold a few cells per run after the first steps, and a grid of one block per
+// tile spent its time on blocks that only find their tile past the list).  The same faces and values as
+// d_ens_fallback.
+#ifndef FVHOE_ENS_FB_BPI
+#define FVHOE_ENS_FB_BPI 4
+#endif
+template <typename T, int TILE>
+__global__ void __launch_bounds__(TILE) k_ens_fallback_inst(Ens e, const double* __restrict__ u,
+                                                           const int* __restrict__ list, T* __restrict__ F,
+                                                           const int* __restrict__ counts) {
+  const int i = blockIdx.x;
+  if (!e.active[i]) return;
+  const int n = e.n[i], cnt = counts ? counts[i] : e.cnt[i];
+  const double* ui = u + 5 * e.off_u[i];
+  T* Fi = F + 5 * e.off_f[i];
+  const int* li = list + e.off_u[i];
+  for (int k = (int)blockIdx.y * TILE + threadIdx.x; k < 2 * cnt; k += FVHOE_ENS_FB_BPI * TILE) {
+    const int fa = li[k >> 1] + (k & 1);
+    T Fv[5];
+    ens_fallback_flux<T>(e, i, ui, n, fa, Fv);
+#pragma unroll
+    for (int q = 0; q < 5; ++q) Fi[q * (n + 1) + fa] = Fv[q];
+  }
+}
+
 // divergence (fvhoe/solver.py:744-750) and the SSP-RK3 combination of this stage
 // (fvhoe/ode.py:275-279): stage 0 -> k0, s = u + dt k0; stage 1 -> k1,
 // s = u + 0.25 dt k0 + 0.25 dt k1; stage 2 -> u + (1/6) dt (k0 + k1 + 4 k2).
@@ -1030,7 +1057,7 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
         k_ens_pred<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, PL.GE, (const unsigned char*)pred->packed,
                                                    pred->logit_threshold, pre, plist, pcnt, nullptr);
       // MUSCL fluxes at the faces of the pre-flagged cells, before the candidate is formed
-      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, plist, F, pcnt);
+      k_ens_fallback_inst<T, TILE><<<dim3(PL.count, FVHOE_ENS_FB_BPI), TILE, 0, s>>>(e, in, plist, F, pcnt);
     }
     if (c->a_posteriori) {
       if (FVHOE_ENS_FUSED) {  // the candidate and the tests of each interior tile in one pass
@@ -1039,7 +1066,7 @@ int run_step(const FvhoeStageConfig* c, const Plan& PL, double* u, double* t, in
         k_ens_prims<T, P, TILE, true><<<PL.ts, TILE, 0, s>>>(e, ts, in, F, WS, nullptr);
         k_ens_detect<T, P, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, W, WS, list);
       }
-      k_ens_fallback<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, in, list, F, nullptr);
+      k_ens_fallback_inst<T, TILE><<<dim3(PL.count, FVHOE_ENS_FB_BPI), TILE, 0, s>>>(e, in, list, F, nullptr);
     }
     // the divergence and the RK combination; the counters and, in the last stage, the clock as well
     k_ens_final<T, TILE><<<PL.tc, TILE, 0, s>>>(e, tc, F, st, u, sv, k0, k1, t, nsteps);
