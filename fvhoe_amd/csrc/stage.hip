@@ -688,16 +688,18 @@ __global__ void __launch_bounds__(NT, !CHECK ? 1 : sizeof(T) == 8 ? FVHOE_CHECK_
         const int e = tid + q * NT;
         if (e < TX * TY) {
           const int ti = e / TY, tj = e % TY;
-          double hM[3], hm[3];
+          // max / min of the 3 x 3 box in the storage type, then widened: widening is exact and monotonic,
+          // so this equals widening first (FP32: two conversions per cell instead of nine)
+          T hM[3], hm[3];
 #pragma unroll
           for (int r = 0; r < 3; ++r) {  // rows -1, 0, 1 of the cell, each over columns -1, 0, 1
             const T* row = b + (ti + r) * BY + tj;
-            const double x0 = (double)row[0], x1 = (double)row[1], x2 = (double)row[2];
+            const T x0 = row[0], x1 = row[1], x2 = row[2];
             hM[r] = dmax(dmax(x0, x1), x2);
             hm[r] = dmin(dmin(x0, x1), x2);
           }
-          const double M = dmax(dmax(hM[0], hM[1]), hM[2]);
-          const double m = dmin(dmin(hm[0], hm[1]), hm[2]);
+          const double M = (double)dmax(dmax(hM[0], hM[1]), hM[2]);
+          const double m = (double)dmin(dmin(hm[0], hm[1]), hm[2]);
           const double uc = (double)wr[v][q];
           bool tv;
           double ind;
