@@ -155,3 +155,24 @@ def test_slab_boundaries_match_single_runs(kinds):
         print(f"[ensemble {kinds}] instance {i} n={ns[i]} t={s.t!r} steps={s.step_count}", flush=True)
         assert s.t == e.times()[i]
         assert_bits_equal(e.state(i), s.u.cpu().numpy().reshape(5, ns[i]), f"instance {i}")
+
+
+def test_forced_trouble_long_runs_match_single_runs():
+    """Every cell flagged (pure MUSCL, fvhoe/solver.py:705-706) on runs long enough that an instance's
+    fallback list takes several passes of its blocks, beside a short run: bit-identical to single fused runs
+    with force_trouble."""
+    ns = [3000, 64, 1500]
+    u0 = _instances(ns, seed=21)
+    T = 0.02
+    e = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=256)
+    e._cfg.force_trouble = 1
+    steps = e.run(T, graph=True, check_every=4)
+    assert np.all(e.times() == T)
+    for i, n in enumerate(ns):
+        s = solver_from(u0[i], (n, 1, 1), (5, 0, 0), (O, P, P), x=(-5, 5), riemann_solver="hllc",
+                        a_posteriori_slope_limiting=True, force_trouble=True)
+        assert s.fused
+        s.ssprk3(T)
+        assert_bits_equal(e.state(i), s.u.cpu().numpy().reshape(5, n), f"instance {i} (n={n})")
+        assert s.t == e.times()[i]
+    print(f"[ensemble forced trouble] {steps} ensemble steps", flush=True)

@@ -78,7 +78,7 @@ def test_never_flagging_network_gives_the_plain_step():
 
 
 def test_always_flagging_network_gives_forced_trouble():
-    ns = [160, 257, 96]
+    ns = [160, 2600, 96]  # the long run's pre-flag list takes several passes of its fallback blocks
     u0 = _instances(ns, seed=5)
     forced = Ensemble(u0, x=(-5, 5), p=5, riemann_solver="hllc", tile=64)
     forced._cfg.force_trouble = 1
