@@ -494,15 +494,21 @@ __global__ void __launch_bounds__(TILE) k_ens_detect(Ens e, const int2* __restri
 template <typename T>
 __device__ void ens_fallback_flux(const Ens& e, int i, const double* __restrict__ ui, int n, int fa, T F[5]) {
   T w[4][5];
+  auto cells = [&](auto in_c) {  // in_c: the four cells are interior, read directly (no boundary map)
+    constexpr bool IN = decltype(in_c)::value;
 #pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    double uu[5], wd[5];
+    for (int k = 0; k < 4; ++k) {
+      double uu[5], wd[5];
 #pragma unroll
-    for (int q = 0; q < 5; ++q) uu[q] = 0.0 + u_at(e, i, ui, n, q, fa - 2 + k, e.bc_lo, e.bc_hi);
-    prims_from_cons(uu, wd, e.gamma);
+      for (int q = 0; q < 5; ++q)
+        uu[q] = 0.0 + (IN ? ui[q * n + fa - 2 + k] : u_at(e, i, ui, n, q, fa - 2 + k, e.bc_lo, e.bc_hi));
+      prims_from_cons(uu, wd, e.gamma);
 #pragma unroll
-    for (int q = 0; q < 5; ++q) w[k][q] = T(0.0 + wd[q]);
-  }
+      for (int q = 0; q < 5; ++q) w[k][q] = T(0.0 + wd[q]);
+    }
+  };
+  if (fa >= 2 && fa + 1 < n) cells(std::true_type{});
+  else cells(std::false_type{});
   T wl[5], wr[5];
 #pragma unroll
   for (int q = 0; q < 5; ++q) {
