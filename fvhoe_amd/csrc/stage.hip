@@ -1229,22 +1229,24 @@ __device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
     const T* wv = W + v * NWC;
-    double M = (double)wv[wb], m = M;
+    // max / min in the storage type, widened once (widening is exact and monotonic)
+    T Mt = wv[wb], mt = Mt;
 #pragma unroll
     for (int di = -1; di <= 1; ++di) {
       if (two_d) {
 #pragma unroll
         for (int dj = -1; dj <= 1; ++dj) {
-          const double x = (double)wv[wb + di * NW1 + dj];
-          M = dmax(M, x);
-          m = dmin(m, x);
+          const T x = wv[wb + di * NW1 + dj];
+          Mt = dmax(Mt, x);
+          mt = dmin(mt, x);
         }
       } else {
-        const double x = (double)wv[wb + di * NW1];
-        M = dmax(M, x);
-        m = dmin(m, x);
+        const T x = wv[wb + di * NW1];
+        Mt = dmax(Mt, x);
+        mt = dmin(mt, x);
       }
     }
+    const double M = (double)Mt, m = (double)mt;
     const double uc = (double)WS[v * NSC + sb];
     double ind;
     bool tv;
@@ -1377,12 +1379,12 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
   for (int k = 0; k < R; ++k) nadbits[k] = 0u;
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
-    double hM[R + 2], hm[R + 2];
+    T hM[R + 2], hm[R + 2];  // in the storage type, widened once per cell (exact: widening is monotonic)
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
       const int i = min(i0 - 1 + r, g.n0);  // W row of cell i (ghosts: g.ge0 >= 3)
       const T* wr = W + v * NWC + (i + g.ge0) * NW1 + jc + g.ge1;
-      const double x0 = (double)wr[-1], x1 = (double)wr[0], x2 = (double)wr[1];
+      const T x0 = wr[-1], x1 = wr[0], x2 = wr[1];
       hM[r] = dmax(dmax(x0, x1), x2);
       hm[r] = dmin(dmin(x0, x1), x2);
     }
@@ -1390,8 +1392,8 @@ __global__ void __launch_bounds__(TPB) k_detect_strip(const T* __restrict__ W, i
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       const int i = min(i0 + k, g.n0 - 1);
-      const double M = dmax(dmax(hM[k], hM[k + 1]), hM[k + 2]);
-      const double m = dmin(dmin(hm[k], hm[k + 1]), hm[k + 2]);
+      const double M = (double)dmax(dmax(hM[k], hM[k + 1]), hM[k + 2]);
+      const double m = (double)dmin(dmin(hm[k], hm[k + 1]), hm[k + 2]);
       const double uc = (double)WS[v * NSC + (i + 3) * NS1 + jc + 3];
       bool tv;
       double ind;
