@@ -425,7 +425,9 @@ struct CheckArgs {
   int* __restrict__ count;       // cells in the fallback list
   int* __restrict__ list;
   int* __restrict__ pend_count;  // cells left to k_detect_list
-  int* __restrict__ pend;
+  int* __restrict__ pend;        // with pack: cell | (NAD bits << PEND_SHIFT), the NAD indicator in pend_imin
+  double* __restrict__ pend_imin;
+  int pack;
   const signed char* __restrict__ pre;
   int* __restrict__ fp_count;
   MagLog ml;
@@ -435,7 +437,8 @@ struct CheckArgs {
 // block prefix over the waves and one atomic per list and block.  Every thread of the block calls it.
 template <int K, int NT>
 __device__ __forceinline__ void block_append2(const int (&cell)[K], unsigned sel0, unsigned sel1, int* count0,
-                                              int* list0, int* count1, int* list1) {
+                                              int* list0, int* count1, int* list1, const int* entry1 = nullptr,
+                                              const double* aux1 = nullptr, double* auxlist1 = nullptr) {
   __shared__ int wsum[NT / 64][2];
   __shared__ int bbase[2];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -471,9 +474,14 @@ __device__ __forceinline__ void block_append2(const int (&cell)[K], unsigned sel
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if ((sel0 >> k) & 1u) list0[p0 + o0[k]] = cell[k];
-    if ((sel1 >> k) & 1u) list1[p1 + o1[k]] = cell[k];
+    if ((sel1 >> k) & 1u) {  // list1 entries may carry another value and an aux double (entry1 / aux1)
+      list1[p1 + o1[k]] = entry1 ? entry1[k] : cell[k];
+      if (auxlist1) auxlist1[p1 + o1[k]] = aux1[k];
+    }
   }
 }
+// a pending-list entry carries the cell's NAD bits above this bit (grids of fewer cells than 1 << PEND_SHIFT)
+constexpr int PEND_SHIFT = 27;
 
 // ---------------------------------------------------------------------------
 // prims tile: W[a][b] for W indices a in [oa, oa+TX), b in [ob, ob+TY).
@@ -481,6 +489,11 @@ __device__ __forceinline__ void block_append2(const int (&cell)[K], unsigned sel
 // The fused check's FP64 tiles ask for 6 waves per SIMD (three 512-thread tiles per CU, <= 80 VGPRs).
 #ifndef FVHOE_CHECK_WAVES_F64
 #define FVHOE_CHECK_WAVES_F64 6
+#endif
+// the candidate check's pending cells carry their NAD bits and indicator to the SED pass (1) or the SED
+// pass repeats the NAD test (0)
+#ifndef FVHOE_PEND_PACK
+#define FVHOE_PEND_PACK 1
 #endif
 // FP32 check tiles: the compiler's occupancy unless a floor is asked for (1 = none)
 #ifndef FVHOE_CHECK_WAVES_F32
@@ -766,7 +779,14 @@ __global__ void __launch_bounds__(NT, !CHECK ? 1 : sizeof(T) == 8 ? FVHOE_CHECK_
       for (int off = 32; off > 0; off >>= 1) f += __shfl_down(f, off, 64);
       if ((tid & 63) == 0 && f) atomicAdd(ck.fp_count, f);
     }
-    block_append2<NPT, NT>(cell, lsel, psel, ck.count, ck.list, ck.pend_count, ck.pend);
+    if (ck.pack) {  // the pending cells carry their NAD bits and indicator to k_detect_list
+      int ent[NPT];
+#pragma unroll
+      for (int q = 0; q < NPT; ++q) ent[q] = (int)((unsigned)cell[q] | (nadb[q] << PEND_SHIFT));
+      block_append2<NPT, NT>(cell, lsel, psel, ck.count, ck.list, ck.pend_count, ck.pend, ent, imin, ck.pend_imin);
+    } else {
+      block_append2<NPT, NT>(cell, lsel, psel, ck.count, ck.list, ck.pend_count, ck.pend);
+    }
     if (mag_on(ck.ml)) {
 #pragma unroll
       for (int q = 0; q < NPT; ++q) mag_append(ck.ml, (msel >> q) & 1u, cell[q], mnad[q], mpad[q]);
@@ -1216,15 +1236,18 @@ __global__ void __launch_bounds__(NT, sizeof(T) == 8 ? 1 : FVHOE_FLUXXY_WAVES_F3
 // troubled-cell flags from W (3^d max/min) and WS (candidate, SED), in FP64
 // ---------------------------------------------------------------------------
 // The tests of one interior cell c: t = flag before pre-flags, mflag / mnad / mpad for the trouble log.
+// detect_nad: the NAD test of each variable (bit v of the result: variable v flags and is one of the
+// tested variables) and the min over the variables of the NAD indicator; detect_sed_pad: the smooth-
+// extrema detector over the flagged variables and the PAD test.
 template <typename T>
-__device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int NWC, const T* __restrict__ WS,
-                                           int NS1, int NSC, const Geo& g, bool two_d,
-                                           const double* __restrict__ range_mm, const DetectP& a, int c,
-                                           bool* mflag, double* mnad, double* mpad) {
+__device__ __forceinline__ unsigned detect_nad(const T* __restrict__ W, int NW1, int NWC, const T* __restrict__ WS,
+                                               int NS1, int NSC, const Geo& g, bool two_d,
+                                               const double* __restrict__ range_mm, const DetectP& a, int c,
+                                               double* imin_out) {
   const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
   const int wb = (i + g.ge0) * NW1 + (j + g.ge1);
   const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
-  bool tvv[5];
+  unsigned tvv = 0u;
   double imin = 0.0;  // min over the variables of the NAD indicator
 #pragma unroll
   for (int v = 0; v < 5; ++v) {
@@ -1260,12 +1283,22 @@ __device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int
       ind = dmin(uc - lo, hi - uc);
       tv = ind < 0.0;
     }
-    tvv[v] = tv && ((a.nad_vars >> v) & 1);
+    if (tv && ((a.nad_vars >> v) & 1)) tvv |= 1u << v;
     imin = v == 0 ? ind : dmin(imin, ind);
   }
+  *imin_out = imin;
+  return tvv;
+}
+
+template <typename T>
+__device__ __forceinline__ int detect_sed_pad(const T* __restrict__ WS, int NS1, int NSC, const Geo& g, bool two_d,
+                                              const DetectP& a, int c, unsigned tvv, double imin, bool* mflag,
+                                              double* mnad, double* mpad) {
+  const int i = c / g.n1, j = c - (c / g.n1) * g.n1;
+  const int sb = (i + 3) * NS1 + (j + (two_d ? 3 : 0));
   bool nad = false;
   for (int v = 0; v < 5; ++v) {
-    bool tv = tvv[v];
+    bool tv = (tvv >> v) & 1u;
     if (tv && a.sed && !nad) {  // smooth-extrema detector only where NAD flags (it can only clear flags)
       const T* sv = WS + v * NSC;
       double alpha = sed_alpha_1d([&](int o) { return (double)sv[sb + o * NS1]; }, a.sed_eps);
@@ -1285,6 +1318,16 @@ __device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int
   *mnad = nad ? -imin : 0.0;
   *mpad = pad ? -pind : 0.0;
   return (a.force || pad || nad) ? 1 : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ int detect_cell(const T* __restrict__ W, int NW1, int NWC, const T* __restrict__ WS,
+                                           int NS1, int NSC, const Geo& g, bool two_d,
+                                           const double* __restrict__ range_mm, const DetectP& a, int c,
+                                           bool* mflag, double* mnad, double* mpad) {
+  double imin;
+  const unsigned tvv = detect_nad(W, NW1, NWC, WS, NS1, NSC, g, two_d, range_mm, a, c, &imin);
+  return detect_sed_pad(WS, NS1, NSC, g, two_d, a, c, tvv, imin, mflag, mnad, mpad);
 }
 
 template <typename T>
@@ -1328,7 +1371,8 @@ __global__ void __launch_bounds__(TPB) k_detect_list(const int* __restrict__ pen
                                                     const double* __restrict__ range_mm, DetectP a,
                                                     signed char* __restrict__ trouble, int* __restrict__ count,
                                                     int* __restrict__ list, const signed char* __restrict__ pre,
-                                                    int* __restrict__ fp_count, MagLog ml) {
+                                                    int* __restrict__ fp_count, MagLog ml,
+                                                    const double* __restrict__ pend_imin, int pack) {
   const int n = pend_count[0];
   for (int base = blockIdx.x * TPB; base < n; base += gridDim.x * TPB) {
     const int k = base + (int)threadIdx.x;
@@ -1336,8 +1380,15 @@ __global__ void __launch_bounds__(TPB) k_detect_list(const int* __restrict__ pen
     bool mflag = false;
     double mnad = 0.0, mpad = 0.0;
     if (k < n) {
-      c = pend[k];
-      t = detect_cell(W, NW1, NW0 * NW1, WS, NS1, NS0 * NS1, g, g.ge1 > 0, range_mm, a, c, &mflag, &mnad, &mpad);
+      if (pack) {  // the candidate tile's NAD bits and indicator: only the SED and PAD are left
+        const unsigned e = (unsigned)pend[k];
+        c = (int)(e & ((1u << PEND_SHIFT) - 1u));
+        t = detect_sed_pad(WS, NS1, NS0 * NS1, g, g.ge1 > 0, a, c, e >> PEND_SHIFT, pend_imin[k], &mflag, &mnad,
+                           &mpad);
+      } else {
+        c = pend[k];
+        t = detect_cell(W, NW1, NW0 * NW1, WS, NS1, NS0 * NS1, g, g.ge1 > 0, range_mm, a, c, &mflag, &mnad, &mpad);
+      }
       if (pre) {
         const int pf = pre[c];
         trouble[c] = (signed char)(t | pf);
@@ -1819,7 +1870,10 @@ int make_layout(const FvhoeStageConfig* c, Layout* L, bool pred = false) {
   L->offFlags = o; o = align256(o + (c->a_posteriori ? (size_t)L->n0 * L->n1 : 0));
   L->offList = o; o = align256(o + (c->a_posteriori ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
   // the cells the fused 2-D candidate check leaves to the SED pass
-  L->offPend = o; o = align256(o + (FVHOE_CAND_CHECK && ay && c->a_posteriori && c->sed ? sizeof(int) * (size_t)L->n0 * L->n1 : 0));
+  // the pending list (cells the candidate check leaves to the SED pass) and, after it, their NAD indicators
+  L->offPend = o;
+  if (FVHOE_CAND_CHECK && ay && c->a_posteriori && c->sed)
+    o = align256(align256(o + sizeof(int) * (size_t)L->n0 * L->n1) + sizeof(double) * (size_t)L->n0 * L->n1);
   if (L->two_d && (L->fp32 ? use_strip<float>(L->p) : use_strip<double>(L->p))) {
     L->prims_blocks = strip_blocks(L->p, L->fp32 ? strip_rows<float>() : strip_rows<double>(), L->NW0, L->NW1);
   } else if (L->two_d) {
@@ -2014,13 +2068,15 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
                c->nad_eps, c->pad_lo, c->pad_hi, c->sed_eps};
     constexpr bool FUSED_CHECK = TWO_D && FVHOE_CAND_CHECK;
     int* pend = (int*)(ws + L.offPend);
+    double* pend_imin = (double*)(ws + align256(L.offPend + sizeof(int) * (size_t)nc));
+    const int pack = FVHOE_PEND_PACK && (size_t)nc < ((size_t)1 << PEND_SHIFT) ? 1 : 0;
     {
       TimedLaunch tl(TK_PRIMS_CAND, s);
       // the LDS tiles form each candidate value once per tile (the strips would form it (R + 2H) / R times)
       CheckArgs<T> ck{};
       if (FUSED_CHECK)
         ck = CheckArgs<T>{W, L.NW0, L.NW1, g.ge0, g.ge1, range, dp, flags, count, list, count + 3, pend,
-                          preflags, count + 2, ml};
+                          pend_imin, pack, preflags, count + 2, ml};
       k_prims<T, P, PY, CX, CY, SUS, NCAND, FUSED_CHECK><<<t0 * t1, NCAND, 0, s>>>(
           make_srcstar<SUS>(u, F, G, dt_stage, sl), gs, WS, L.NS0, L.NS1, t1, none, c->gamma, ck);
     }
@@ -2032,7 +2088,7 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       if (FUSED_CHECK) {
         if (c->sed)  // the cells whose NAD test flagged: the SED reads WS beyond the tiles
           k_detect_list<T><<<512, TPB, 0, s>>>(count + 3, pend, W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp,
-                                               flags, count, list, preflags, count + 2, ml);
+                                               flags, count, list, preflags, count + 2, ml, pend_imin, pack);
       } else if (TWO_D) {
         k_detect_strip<T, DR><<<gridd2, TPB, 0, s>>>(W, L.NW0, L.NW1, WS, L.NS0, L.NS1, g, range, dp, flags, count,
                                                       list, preflags, count + 2, jblocks, ml);

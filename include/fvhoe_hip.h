@@ -397,8 +397,8 @@ int fvhoe_stage_rk(const FvhoeStageConfig* cfg, const double* u, double* dudt, d
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, serving loops, harnesses that forbid hipMalloc).
  * Returns FVHOE_ERR_TOO_LARGE when cfg needs more scratch than that (with
- * a-posteriori limiting it fits 2-D grids up to 626 x 626 cells in FP64 and
- * 864 x 864 in FP32 at p=3, 1-D grids up to about 535,000 cells at p=5).  Every call on
+ * a-posteriori limiting it fits 2-D grids up to 612 x 612 cells in FP64 and
+ * 828 x 828 in FP32 at p=3, 1-D grids up to about 535,000 cells at p=5).  Every call on
  * one GPU shares the array: calls that overlap on one GPU (concurrent streams
  * or threads) race.  Results equal fvhoe_stage_rk's bit for bit. */
 #define FVHOE_STATIC_SCRATCH_BYTES (64ull << 20)

@@ -87,9 +87,9 @@ def test_stage_workspace_size(fp32):
         blocks = ((n + 8 + 31) // 32) ** 2
     esz = 4 if fp32 else 8
     # W, F, G, WS; flags and the fallback list; the partials; the list of cells the fused candidate check
-    # leaves to the SED pass (a-posteriori with the SED on)
+    # leaves to the SED pass with their NAD indicators (a-posteriori with the SED on)
     expect = esz * 5 * ((n + 8) ** 2 + 2 * (n + 1) * n + (n + 6) ** 2) + 5 * n * n + 88 * blocks
-    expect += 4 * n * n if c.sed else 0
+    expect += (4 + 8) * n * n if c.sed else 0
     assert expect <= b.value <= expect + 4096
 
 
