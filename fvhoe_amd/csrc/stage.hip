@@ -1611,9 +1611,20 @@ __global__ void __launch_bounds__(TPB) k_fallback(const int* __restrict__ count,
 // dudt = 0.0 + cx (F[i+1]-F[i]) + cy (G[j+1]-G[j])  (fvhoe/solver.py:744-750) in
 // FP64, then the optional RK combination.  rk.out may alias the stage input or
 // rk.u0 (pointwise).
+// stats (learned stage, may be null): thread 0 of block 0 adds the stage's counters, [0] pre-flagged cells
+// (count[1]), [1] cells the check listed (count[0]), [2] pre-flagged cells the check did not flag (count[2]),
+// unless the step is at its target time
 template <bool TWO_D, typename T>
 __global__ void __launch_bounds__(TPB) k_final(const T* __restrict__ F, const T* __restrict__ G, Geo g, double* dudt,
-                                              RkArgs rk) {
+                                              RkArgs rk, const int* __restrict__ count, long long* __restrict__ stats,
+                                              int with_pre) {
+  if (stats && blockIdx.x == 0 && threadIdx.x == 0 && !(rk.kind >= 0 && rk.dt[0] == 0.0)) {
+    if (with_pre) {  // a step at its target time counts nothing
+      stats[0] += count[1];
+      stats[2] += count[2];
+    }
+    stats[1] += count[0];
+  }
   const int nc = g.n0 * g.n1;
   const int nF = (g.n0 + 1) * g.n1, nG = g.n0 * (g.n1 + 1);
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
@@ -1627,19 +1638,6 @@ __global__ void __launch_bounds__(TPB) k_final(const T* __restrict__ F, const T*
       if (dudt) dudt[q * nc + c] = rk.kind == FVHOE_RK_SSPRK3_STAGE3_SUM ? rk.k0[q * nc + c] + acc : acc;
       if (rk.kind >= 0) rk.out[q * nc + c] = rk_combine(rk, q * nc + c, acc);
     }
-  }
-}
-
-// counters: [0] cells listed by the check, [1] pre-flagged cells, [2] pre-flagged cells the check cleared
-
-__global__ void k_pred_stats(const int* __restrict__ count, long long* __restrict__ stats, int with_pre,
-                             const double* __restrict__ gate) {
-  if (threadIdx.x == 0 && !(gate && gate[0] == 0.0)) {  // a step at its target time counts nothing
-    if (with_pre) {
-      stats[0] += count[1];
-      stats[2] += count[2];
-    }
-    stats[1] += count[0];
   }
 }
 
@@ -2101,12 +2099,12 @@ int run_stage(const FvhoeStageConfig* c, const FvhoeRkFuse* rkf, const FvhoePred
       TimedLaunch tl(TK_FALLBACK, s);
       k_fallback<TWO_D, T, SU><<<1024, TPB, 0, s>>>(count, list, make_src<SU>(u, sl), g, fp, F, G);
     }
-    if (use_pred && pred->stats)
-      k_pred_stats<<<1, 64, 0, s>>>(count, pred->stats, pre_now ? 1 : 0, rkf && rkf->kind >= 0 ? rkf->dt_step : nullptr);
   }
   {
     TimedLaunch tl(TK_FINAL, s);
-    k_final<TWO_D, T><<<gridc, TPB, 0, s>>>(F, G, g, dudt, rk);
+    // the learned stage's counters ride on the final kernel (after the fallback, which reads the same counts)
+    long long* stats = apost && use_pred ? pred->stats : nullptr;
+    k_final<TWO_D, T><<<gridc, TPB, 0, s>>>(F, G, g, dudt, rk, count, stats, pre_now ? 1 : 0);
   }
   // join: the caller's stream waits for the network before anything after this call
   if (overlap && pred->infer && hipStreamWaitEvent(s, (hipEvent_t)pred->ev_join, 0) != hipSuccess)
